@@ -1,0 +1,210 @@
+"""Throughput of the SOPHIA ISPH-DEM step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5_1m]
+
+A step is one full ISPH-DEM time step (NNPS, prep, interaction, DEM
+contact, SPH-DEM coupling, predictor, PPE solve to rtol 1e-8, pressure force,
+corrector, XSPH) over the whole particle set, inputs resident in HBM.
+The headline path is the mixed-precision one (FP32 fields, FP32 PPE operator
+inside FP64 iterative refinement) with the learned warm start when trained
+weights are available, else the previous-step pressure; it is compared with
+the plain FP64 path (FP64 fields and FP64 BiCGStab) on the same input.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _algorithmic_bytes(name: str, n_sph: int, n_rows: int, k_avg: float, n_dem: int) -> float | None:
+    """Compulsory HBM bytes per launch of a kernel group: every stream it
+    reads or writes touched once (neighbour gathers counted once per
+    particle), plus the neighbour-list / operator streams."""
+    if name == "ppe_matvec_fp32":
+        # per row: nbr index 4k + hi coefficient 4k + count 4 + row flag 4 +
+        # diag 8 + x 4 + y 4 + one dot operand 4
+        return n_sph * (8.0 * k_avg + 32.0)
+    if name == "ppe_matvec_fp64":
+        return n_sph * (12.0 * k_avg + 44.0)
+    if name == "advection":
+        # nbr 4k + x,y,z,h,pres,m,rho,por (64) + u (12) + temp (4) + ptype (4)
+        # + cm (24) + count (4); writes f (16) + pgf (12)
+        return n_sph * (4.0 * k_avg + 140.0)
+    if name == "nbr_build":
+        return n_sph * (4.0 * k_avg + 40.0)
+    if name == "ppe_assemble":
+        return n_sph * (8.0 * k_avg + 4.0 * k_avg + 150.0)
+    if name == "pressure_force":
+        return n_sph * (4.0 * k_avg + 120.0)
+    return None
+
+
+def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp_weights=None):
+    from fluidized_bed_master_amd.sophia import Sophia
+    from fluidized_bed_master_amd import abi
+    g = Sophia(params, parts, precision=precision)
+    if mlp_weights is not None:
+        g.mlp_set_params(mlp_weights)
+    stats = []
+    for _ in range(warmup):
+        stats.append(g.step(ppe_opts))
+    g.sync()
+    if timers:
+        g.timers(True)
+    t0 = time.perf_counter()
+    st_timed = []
+    for _ in range(steps):
+        st_timed.append(g.step(ppe_opts))
+    g.sync()
+    dt = time.perf_counter() - t0
+    rep = g.timers_report() if timers else {}
+    nnps = g.get_nnps(abi.SPH)
+    cap = g.params.sph_max_neighbors
+    k_avg = float(np.mean(nnps["nbr_count"]))
+    n_sph, n_dem = g.count(abi.SPH), g.count(abi.DEM)
+    g.close()
+    return dt, st_timed, rep, k_avg, n_sph, n_dem
+
+
+def cpu_baseline(target_seconds: float = 15.0):
+    """The FP64 C++/OpenMP oracle (same formulas, same BiCGStab) timed on a
+    bounded sample of the same workload kind: a 40 x 40 x 40 ISPH-DEM column."""
+    from fluidized_bed_master_amd import abi, configs
+    from oracle import oracle
+    p, parts = configs.isph_dem_column(40, 40, 40, 1.0e-3, d_dem=2.0e-3, dem_height=0.012, seed=5, dt=2.0e-6)
+    o = oracle.Oracle(p, parts)
+    opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8)
+    o.step(opts)  # first step (count 0 extras) untimed
+    n = len(parts)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(opts)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= target_seconds or steps >= 200:
+            break
+    return {
+        "value": n * steps / el,
+        "unit": "particle-steps/s",
+        "cores": oracle.get_threads(),
+        "kind": "port",
+        "sample": f"40x40x40 ISPH-DEM column ({n} particles), {steps} steps, BiCGStab FP64 rtol 1e-8, "
+                  f"{el:.1f} s",
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c5_1m", choices=["c5_1m", "c5", "c2_isph", "small"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--weights", default=str(ROOT / "fluidized_bed_master_amd" / "weights" / "ppe_mlp_c2.npy"))
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU slab decomposition is a design, not code", file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    from fluidized_bed_master_amd import abi, configs
+    gen = {"c5_1m": configs.c5_1m, "c5": configs.c5, "c2_isph": configs.c2_isph,
+           "small": lambda: configs.small_isph_dem()}[a.config]
+    params, parts = gen()
+    n_total = len(parts)
+
+    weights = None
+    if Path(a.weights).exists():
+        weights = np.load(a.weights).astype(np.float32)
+    guess = abi.GUESS_LEARNED if weights is not None else abi.GUESS_PREVIOUS
+    mixed_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=guess, rtol=1e-8,
+                              inner_rtol=1e-4)
+    fp64_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8)
+
+    dt, st, rep, k_avg, n_sph, n_dem = run_gpu(params, parts, abi.PREC_MIXED, mixed_opts, a.steps, a.warmup,
+                                               timers=True, mlp_weights=weights)
+    value = n_total * a.steps / dt
+    ms_step = dt / a.steps * 1e3
+    ppe_ms = float(np.mean([s["ms"] for s in st]))
+    iters = float(np.mean([s["iterations"] for s in st]))
+
+    # dominant kernel group and its roofline
+    dom = max(rep.items(), key=lambda kv: kv[1][1]) if rep else None
+    roofline = None
+    if dom:
+        name, (cnt, tot) = dom
+        n_rows = st[-1]["n_rows"]
+        ab = _algorithmic_bytes(name, n_sph, n_rows, k_avg, n_dem)
+        if ab is not None and cnt > 0:
+            avg_s = tot / cnt / 1e3
+            ach = ab / avg_s / 1e9
+            roofline = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                        "launches": cnt, "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": ab}
+
+    fp64 = None
+    if not a.no_fp64:
+        dt64, st64, _, _, _, _ = run_gpu(params, parts, abi.PREC_FP64, fp64_opts, a.steps, a.warmup)
+        fp64 = {"value": n_total * a.steps / dt64, "ms_per_step": dt64 / a.steps * 1e3,
+                "ppe_ms": float(np.mean([s["ms"] for s in st64])),
+                "ppe_iters": float(np.mean([s["iterations"] for s in st64]))}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a.cpu_seconds)
+
+    out = {
+        "metric": "particle-steps/s ISPH-DEM 1M; PPE ms learned+FP32 vs FP64 @rtol 1e-8",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (seeded splitmix64 column, jittered-lattice spheres)",
+        "config": {
+            "workload": f"{a.config}: ISPH-DEM column, {n_sph - int(np.sum(parts['p_type'] == 0))} fluid + "
+                        f"{int(np.sum(parts['p_type'] == 0))} wall + {n_dem} DEM, dx=1mm, h=1.3dx, "
+                        f"Wendland C2, dt={params.dt:g}",
+            "particles": n_total,
+            "ppe": "BiCGStab rtol 1e-8, FP32 operator + FP64 refinement, "
+                   + ("learned warm start" if weights is not None else "previous-step guess"),
+            "parallelism": "single GPU",
+        },
+        "ppe_ms_per_step": ppe_ms,
+        "ppe_iters_per_step": iters,
+        "fp64_path": fp64,
+        "speedup_vs_fp64_path": (value / fp64["value"]) if fp64 else None,
+        "kernel_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in sorted(rep.items())},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

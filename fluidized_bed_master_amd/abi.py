@@ -1,0 +1,228 @@
+"""ctypes view of include/sophia/abi.h and include/sophia/particles.h.
+
+The particle record dtypes are parsed from particles.h so that the C header
+stays the single source of truth for the byte layout of part1/part2/part3
+(class_Cuda_Particle_Array2.cuh:11-167 of the reference).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+INCLUDE = ROOT / "include" / "sophia"
+LIB_PATH = Path(__file__).resolve().parent / "libsophia.so"
+
+# ---------------------------------------------------------------------------
+# constants (abi.h)
+# ---------------------------------------------------------------------------
+SOPHIA_OK = 0
+SPH, DEM = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+KNL_GAUSSIAN, KNL_QUINTIC, KNL_QUARTIC, KNL_WENDLAND2, KNL_WENDLAND4, KNL_WENDLAND6 = range(6)
+PT_BOUNDARY, PT_FLUID, PT_MOVING, PT_DEM = 0, 1, 9, 1001
+PPE_JACOBI, PPE_BICGSTAB = 0, 1
+PPE_FP64, PPE_MIXED_IR = 0, 1
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
+PREDICT, CORRECT = 0, 1
+ERRORS = {
+    -1: "SOPHIA_E_ARG", -2: "SOPHIA_E_HIP", -3: "SOPHIA_E_IO", -4: "SOPHIA_E_STATE",
+    -5: "SOPHIA_E_NOMEM", -6: "SOPHIA_E_CAPACITY", -7: "SOPHIA_E_NUMERIC",
+}
+
+
+def _struct_dtype(name: str) -> np.dtype:
+    text = (INCLUDE / "particles.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    m = re.search(r"typedef struct " + name + r" \{(.*?)\} " + name + ";", text, re.S)
+    if not m:
+        raise RuntimeError(f"struct {name} not found in particles.h")
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        typ, rest = decl.split(None, 1)
+        base = np.float64 if typ == "sophia_real" else np.int32
+        for var in rest.split(","):
+            var = var.strip()
+            dims = tuple(int(d) for d in re.findall(r"\[(\d+)\]", var))
+            vname = var.split("[")[0].strip()
+            fields.append((vname, base, dims) if dims else (vname, base))
+    return np.dtype(fields, align=True)
+
+
+PART1 = _struct_dtype("part1")
+PART2 = _struct_dtype("part2")
+PART3 = _struct_dtype("part3")
+assert PART1.itemsize == 1272 and PART2.itemsize == 176 and PART3.itemsize == 496, (
+    PART1.itemsize, PART2.itemsize, PART3.itemsize)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("solver_type", C.c_int32), ("dim", C.c_int32), ("kernel_type", C.c_int32),
+        ("rho_type", C.c_int32), ("time_type", C.c_int32), ("open_boundary", C.c_int32),
+        ("flag_z_index", C.c_int32), ("freq_filt", C.c_int32), ("freq_output", C.c_int32),
+        ("fp_solve", C.c_int32), ("fv_solve", C.c_int32), ("fva_solve", C.c_int32), ("fg_solve", C.c_int32),
+        ("con_solve", C.c_int32), ("xsph_solve", C.c_int32), ("kgc_solve", C.c_int32),
+        ("delSPH_solve", C.c_int32), ("decouple_stride", C.c_int32),
+        ("min_iteration", C.c_int32), ("max_iteration", C.c_int32),
+        ("kappa", C.c_double), ("p_ref", C.c_double), ("gamma", C.c_double), ("dt", C.c_double),
+        ("time", C.c_double), ("time_end", C.c_double),
+        ("xmargin_m", C.c_double), ("xmargin_p", C.c_double), ("ymargin_m", C.c_double),
+        ("ymargin_p", C.c_double), ("zmargin_m", C.c_double), ("zmargin_p", C.c_double),
+        ("u_limit", C.c_double), ("c_xsph", C.c_double), ("soundspeed", C.c_double), ("rho0_eos", C.c_double),
+        ("drho_th", C.c_double), ("dp_th", C.c_double), ("p_relaxation", C.c_double),
+        ("rho_fluid", C.c_double), ("rho_water", C.c_double), ("h_over_dx", C.c_double),
+        ("mu_fluid", C.c_double), ("mu_water", C.c_double), ("k_fluid", C.c_double), ("k_dem", C.c_double),
+        ("cp_fluid", C.c_double), ("cp_dem", C.c_double), ("gravity", C.c_double), ("eos_b_factor", C.c_double),
+        ("delta_sph", C.c_double), ("dem_shear_modulus", C.c_double), ("dem_poisson", C.c_double),
+        ("dem_friction", C.c_double), ("dem_damping", C.c_double), ("dem_hertz_factor", C.c_double),
+        ("drag_switch", C.c_double),
+        ("eulerian_sph", C.c_int32), ("fix_xsph_position", C.c_int32), ("dem_max_contacts", C.c_int32),
+        ("sph_max_neighbors", C.c_int32), ("domain_fixed", C.c_int32), ("periodic_mask", C.c_int32),
+        ("domain_min", C.c_double * 3), ("domain_max", C.c_double * 3), ("reserved", C.c_double * 8),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
+
+
+class Domain(C.Structure):
+    _fields_ = [
+        ("x_min", C.c_double), ("x_max", C.c_double), ("y_min", C.c_double), ("y_max", C.c_double),
+        ("z_min", C.c_double), ("z_max", C.c_double), ("dcell", C.c_double),
+        ("NI", C.c_int32), ("NJ", C.c_int32), ("NK", C.c_int32), ("num_cells", C.c_int32),
+        ("n_sph", C.c_int32), ("n_dem", C.c_int32),
+    ]
+
+
+class PpeOpts(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32), ("precision", C.c_int32), ("guess", C.c_int32), ("max_iter", C.c_int32),
+        ("max_outer", C.c_int32), ("zero_mean", C.c_int32), ("rtol", C.c_double), ("inner_rtol", C.c_double),
+    ]
+
+
+class PpeStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("guess_used", C.c_int32),
+        ("r0_rel", C.c_double), ("r0_rel_learned", C.c_double), ("r_rel", C.c_double), ("ms", C.c_double),
+        ("n_rows", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+def ppe_opts(method=PPE_JACOBI, precision=PPE_FP64, guess=GUESS_PREVIOUS, rtol=1e-8, max_iter=10000,
+             max_outer=20, inner_rtol=1e-4, zero_mean=0) -> PpeOpts:
+    return PpeOpts(method, precision, guess, max_iter, max_outer, zero_mean, rtol, inner_rtol)
+
+
+_P = C.POINTER
+_VP = C.c_void_p
+_ctx = C.c_void_p
+
+# exported symbols of libsophia.so with their argument types
+SIGNATURES = {
+    "sophia_params_default": (None, [_P(Params)]),
+    "sophia_read_solv": (C.c_int, [C.c_char_p, _P(Params)]),
+    "sophia_read_input": (C.c_int, [C.c_char_p, _VP, C.c_int32, _P(C.c_int32)]),
+    "sophia_read_ptype": (C.c_int, [C.c_char_p, _VP, C.c_int32, _P(C.c_int32)]),
+    "sophia_create_from_arrays": (C.c_int, [_P(Params), _VP, C.c_int32, C.c_int32, C.c_int32, _P(_ctx)]),
+    "sophia_create": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, _P(_ctx)]),
+    "sophia_destroy": (C.c_int, [_ctx]),
+    "sophia_last_error": (C.c_char_p, []),
+    "sophia_get_domain": (C.c_int, [_ctx, _P(Domain)]),
+    "sophia_get_params": (C.c_int, [_ctx, _P(Params)]),
+    "sophia_sync": (C.c_int, [_ctx]),
+    "sophia_get_clock": (C.c_int, [_ctx, _P(C.c_int64), _P(C.c_double)]),
+    "sophia_nnps": (C.c_int, [_ctx]),
+    "sophia_prep": (C.c_int, [_ctx]),
+    "sophia_interaction": (C.c_int, [_ctx]),
+    "sophia_dem_interaction": (C.c_int, [_ctx]),
+    "sophia_sph_dem_coupling": (C.c_int, [_ctx]),
+    "sophia_time": (C.c_int, [_ctx, C.c_int32]),
+    "sophia_ppe": (C.c_int, [_ctx, _P(PpeOpts), _P(PpeStats)]),
+    "sophia_xsph": (C.c_int, [_ctx]),
+    "sophia_step": (C.c_int, [_ctx, _P(PpeOpts), _P(PpeStats)]),
+    "sophia_run": (C.c_int, [_ctx, C.c_int32, _P(PpeOpts), _P(PpeStats)]),
+    "sophia_get_nnps": (C.c_int, [_ctx, C.c_int32, _VP, _VP, _VP, _VP, _VP]),
+    "sophia_get_particles": (C.c_int, [_ctx, C.c_int32, _VP, _VP, _VP]),
+    "sophia_count": (C.c_int, [_ctx, C.c_int32, _P(C.c_int32)]),
+    "sophia_knl_eval": (C.c_int, [C.c_int32, C.c_int32, C.c_double, _VP, C.c_int32, _VP, _VP]),
+    "sophia_set_pressure": (C.c_int, [_ctx, _VP, C.c_int32]),
+    "sophia_get_ppe_system": (C.c_int, [_ctx, _VP, _VP, _VP]),
+    "sophia_ppe_apply": (C.c_int, [_ctx, _VP, _VP, C.c_int32]),
+    "sophia_mlp_param_count": (C.c_int, [_P(C.c_int32)]),
+    "sophia_mlp_set_params": (C.c_int, [_ctx, _VP, C.c_int32]),
+    "sophia_mlp_get_params": (C.c_int, [_ctx, _VP, C.c_int32]),
+    "sophia_mlp_predict": (C.c_int, [_ctx, _VP, C.c_int32]),
+    "sophia_mlp_train_step": (C.c_int, [_ctx, _VP, C.c_int32, C.c_double, _P(C.c_double)]),
+    "sophia_mlp_loss_grad": (C.c_int, [_ctx, _VP, C.c_int32, _P(C.c_double), _VP, C.c_int32]),
+    "sophia_mlp_reset_adam": (C.c_int, [_ctx]),
+    "sophia_timers_enable": (C.c_int, [_ctx, C.c_int32]),
+    "sophia_timers_report": (C.c_int, [_ctx, C.c_char_p, C.c_int32]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function declared in include/sophia/abi.h."""
+    text = (INCLUDE / "abi.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sophia_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def load_library(path: Path | None = None) -> C.CDLL:
+    """Load libsophia.so (built by fluidized_bed_master_amd.build); fails loudly if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build it with `python -m fluidized_bed_master_amd.build` "
+            "(the HIP library is required; there is no CPU fallback)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_params() -> Params:
+    p = Params()
+    load_library().sophia_params_default(C.byref(p))
+    return p
+
+
+class SophiaError(RuntimeError):
+    def __init__(self, rc: int, where: str):
+        lib = load_library()
+        msg = lib.sophia_last_error().decode(errors="replace")
+        super().__init__(f"{where} -> {ERRORS.get(rc, rc)}: {msg}")
+        self.rc = rc
+
+
+def check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise SophiaError(rc, where)
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

@@ -1,0 +1,196 @@
+"""Seeded synthetic inputs for the SPH-DEM step (no datasets are read).
+
+Random numbers come from splitmix64 (Steele, Lea & Flood 2014) so the same
+seed gives the same particles on every machine.  Particles are emitted in
+the reference's input order: walls, then fluid, then DEM spheres
+(input/p_type.txt of the reference lists 0s, then 1s, then 1001s).
+
+Workloads (sizes in metres):
+  isph_dem_column(...)   fluid lattice in a walled column (3 wall layers on
+                         the bottom and four sides) with DEM spheres on a
+                         seeded jittered lattice in the bottom region
+  c5_1m()                0.1 x 0.1 column filled to 0.1 at dx = 1 mm:
+                         1,000,000 fluid + 30,000 spheres d = 2 mm
+  c5()                   0.1 x 0.1 x 0.2 fill: 2,000,000 fluid + 60,000 spheres
+  c2_isph()              0.05 x 0.05 x 0.15 fill at dx = 1 mm: 375,000 fluid +
+                         8,000 spheres d = 3 mm (ISPH PPE on C2's particle set)
+  small_isph_dem(...)    parity-test sizes
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .abi import KNL_WENDLAND2, PART1, PT_BOUNDARY, PT_DEM, PT_FLUID, Params, default_params
+
+_MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n consecutive outputs of splitmix64 started at `seed`."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15) * np.arange(1, n + 1, dtype=np.uint64)) & _MASK
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def config_params(dx: float, dt: float, rho_f: float = 1000.0, mu_f: float = 1.0e-3, h_ratio: float = 1.3,
+                  E: float = 1.0e7, nu: float = 0.3, mu_fric: float = 0.3, con: int = 0) -> Params:
+    """Parameters of the synthetic configs (model constants set to the
+    config values; everything else at the reference defaults)."""
+    p = default_params()
+    p.solver_type = 1
+    p.dim = 3
+    p.kernel_type = KNL_WENDLAND2
+    p.kappa = 2.0
+    p.dt = dt
+    p.rho_fluid = rho_f
+    p.rho0_eos = rho_f
+    p.mu_fluid = mu_f
+    p.h_over_dx = h_ratio
+    p.dem_shear_modulus = E / (2.0 * (1.0 + nu))
+    p.dem_poisson = nu
+    p.dem_friction = mu_fric
+    p.con_solve = con
+    p.open_boundary = 0
+    p.u_limit = 40.0
+    p.sph_max_neighbors = 96
+    return p
+
+
+def _records(n: int) -> np.ndarray:
+    a = np.zeros(n, PART1)
+    a["i_type"] = 1
+    return a
+
+
+def isph_dem_column(nx: int, ny: int, nz: int, dx: float, *, wall_layers: int = 3, tank_extra: int = 2,
+                    d_dem: float = 0.0, rho_s: float = 2500.0, dem_height: float = 0.0, phi: float = 0.35,
+                    max_dem: int | None = None, h_ratio: float = 1.3, rho_f: float = 1000.0, g: float = 9.8,
+                    jitter: float = 0.0, seed: int = 5, dt: float = 2.0e-6, temp: float = 293.0,
+                    u_noise: float = 0.0, dem_u_noise: float = 0.0, dem_w_noise: float = 0.0):
+    """Walled column of fluid with DEM spheres; returns (params, part1 records)."""
+    h = h_ratio * dx
+    H = nz * dx
+    Lx, Ly = nx * dx, ny * dx
+    # walls: 3 layers below and around, up to the fill height plus tank_extra layers
+    nzw = nz + tank_extra
+    wi = np.arange(-wall_layers, nx + wall_layers)
+    wj = np.arange(-wall_layers, ny + wall_layers)
+    wk = np.arange(-wall_layers, nzw)
+    I, J, K = np.meshgrid(wi, wj, wk, indexing="ij")
+    inside = (I >= 0) & (I < nx) & (J >= 0) & (J < ny) & (K >= 0)
+    I, J, K = I[~inside], J[~inside], K[~inside]
+    walls = _records(I.size)
+    walls["x"] = (I + 0.5) * dx
+    walls["y"] = (J + 0.5) * dx
+    walls["z"] = (K + 0.5) * dx
+    walls["p_type"] = PT_BOUNDARY
+    # fluid lattice
+    fi, fj, fk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    fluid = _records(fi.size)
+    fluid["x"] = (fi.ravel() + 0.5) * dx
+    fluid["y"] = (fj.ravel() + 0.5) * dx
+    fluid["z"] = (fk.ravel() + 0.5) * dx
+    if jitter > 0:
+        nf = fi.size
+        fluid["x"] += uniform(seed * 3 + 0, nf, -jitter, jitter) * dx
+        fluid["y"] += uniform(seed * 3 + 1, nf, -jitter, jitter) * dx
+        fluid["z"] += uniform(seed * 3 + 2, nf, -jitter, jitter) * dx
+    fluid["p_type"] = PT_FLUID
+    if u_noise > 0:
+        nf = fi.size
+        fluid["ux"] = uniform(seed * 11 + 1, nf, -u_noise, u_noise)
+        fluid["uy"] = uniform(seed * 11 + 2, nf, -u_noise, u_noise)
+        fluid["uz"] = uniform(seed * 11 + 3, nf, -u_noise, u_noise) + u_noise * fluid["x"] / Lx
+    for a in (walls, fluid):
+        a["h"] = h
+        a["m"] = rho_f * dx ** 3
+        a["rho"] = rho_f
+        a["temp"] = temp
+        a["pres"] = rho_f * g * np.maximum(H - a["z"], 0.0)
+        a["DEMpor"] = 1.0
+    # DEM spheres on a jittered cubic lattice with solid fraction phi
+    dem = _records(0)
+    if d_dem > 0 and dem_height > 0:
+        s = d_dem * (math.pi / (6.0 * phi)) ** (1.0 / 3.0)
+        gx = int(Lx // s)
+        gy = int(Ly // s)
+        gz = int(dem_height // s)
+        di, dj, dk = np.meshgrid(np.arange(gx), np.arange(gy), np.arange(gz), indexing="ij")
+        nd = di.size if max_dem is None else min(di.size, max_dem)
+        dem = _records(nd)
+        amp = 0.5 * (s - d_dem) * 0.98
+        ox = (Lx - gx * s) / 2
+        oy = (Ly - gy * s) / 2
+        dem["x"] = ox + (di.ravel()[:nd] + 0.5) * s + uniform(seed * 7 + 1, nd, -amp, amp)
+        dem["y"] = oy + (dj.ravel()[:nd] + 0.5) * s + uniform(seed * 7 + 2, nd, -amp, amp)
+        dem["z"] = (dk.ravel()[:nd] + 0.5) * s + uniform(seed * 7 + 3, nd, -amp, amp)
+        r = d_dem / 2
+        dem["rad"] = r
+        dem["m"] = rho_s * 4.0 / 3.0 * math.pi * r ** 3
+        dem["ri"] = 0.4 * dem["m"] * r * r
+        dem["rho"] = rho_s
+        dem["h"] = h
+        dem["temp"] = temp
+        dem["p_type"] = PT_DEM
+        dem["dem_idx"] = np.arange(1, nd + 1)
+        if dem_u_noise > 0:
+            dem["ux"] = uniform(seed * 13 + 1, nd, -dem_u_noise, dem_u_noise)
+            dem["uy"] = uniform(seed * 13 + 2, nd, -dem_u_noise, dem_u_noise)
+            dem["uz"] = uniform(seed * 13 + 3, nd, -dem_u_noise, dem_u_noise)
+        if dem_w_noise > 0:
+            dem["wx"] = uniform(seed * 17 + 1, nd, -dem_w_noise, dem_w_noise)
+            dem["wy"] = uniform(seed * 17 + 2, nd, -dem_w_noise, dem_w_noise)
+            dem["wz"] = uniform(seed * 17 + 3, nd, -dem_w_noise, dem_w_noise)
+    parts = np.concatenate([walls, fluid, dem])
+    p = config_params(dx, dt, rho_f=rho_f, h_ratio=h_ratio)
+    return p, parts
+
+
+def small_isph_dem(n: int = 10, nz: int = 12, dx: float = 1.0e-3, d_dem: float = 2.0e-3, seed: int = 5,
+                   jitter: float = 0.02, dem_height: float = 6.0e-3, phi: float = 0.6, u_noise: float = 0.05,
+                   dem_u_noise: float = 0.05, dem_w_noise: float = 20.0):
+    """A parity-test column: n x n x nz fluid + walls + overlapping spheres
+    (phi > pi/6 puts lattice neighbours in contact), with seeded random
+    fluid and sphere velocities so that every force term is exercised."""
+    return isph_dem_column(n, n, nz, dx, d_dem=d_dem, dem_height=dem_height, seed=seed, jitter=jitter,
+                           phi=phi, u_noise=u_noise, dem_u_noise=dem_u_noise, dem_w_noise=dem_w_noise)
+
+
+def c5_1m(seed: int = 5):
+    """ISPH-DEM 1M: 0.1 x 0.1 x 0.1 fill at dx = 1 mm (1,000,000 fluid),
+    30,000 spheres d = 2 mm (jittered lattice, phi = 0.35), dt = 2e-6."""
+    dx = 1.0e-3
+    s = 2.0e-3 * (math.pi / (6.0 * 0.35)) ** (1.0 / 3.0)
+    layers = math.ceil(30000 / (int(0.1 // s) ** 2))
+    return isph_dem_column(100, 100, 100, dx, d_dem=2.0e-3, dem_height=layers * s + 1e-9, max_dem=30000,
+                           seed=seed, dt=2.0e-6)
+
+
+def c5(seed: int = 5):
+    """Full C5 geometry: 0.1 x 0.1 column filled to 0.2 (2,000,000 fluid),
+    60,000 spheres d = 2 mm, dt = 2e-6."""
+    dx = 1.0e-3
+    s = 2.0e-3 * (math.pi / (6.0 * 0.35)) ** (1.0 / 3.0)
+    layers = math.ceil(60000 / (int(0.1 // s) ** 2))
+    return isph_dem_column(100, 100, 200, dx, d_dem=2.0e-3, dem_height=layers * s + 1e-9, max_dem=60000,
+                           seed=seed, dt=2.0e-6)
+
+
+def c2_isph(seed: int = 2):
+    """C2's particle set (50 x 50 x 150 fluid at dx = 1 mm, 8000 spheres
+    d = 3 mm, rho = 2500) run through the ISPH step, dt = 5e-7."""
+    dx = 1.0e-3
+    s = 3.0e-3 * (math.pi / (6.0 * 0.35)) ** (1.0 / 3.0)
+    layers = math.ceil(8000 / (int(0.05 // s) ** 2))
+    return isph_dem_column(50, 50, 150, dx, d_dem=3.0e-3, dem_height=layers * s + 1e-9, max_dem=8000,
+                           seed=seed, dt=5.0e-7)

@@ -1,0 +1,370 @@
+// Shared declarations of the HIP SPH-DEM library (gfx950).
+//
+// Layout in HBM: every particle field is its own contiguous stream
+// (structure of arrays) in the cell-sorted order produced by NNPS, so a
+// wavefront reading field f of 64 consecutive particles issues one
+// coalesced 256 B (FP32) or 512 B (FP64) request.  Positions, smoothing
+// lengths and pressure are FP64 on every path (cell keys, neighbour
+// membership and the pressure solve are decided in FP64), and so are mass,
+// density and porosity: pressure-gradient sums sum_j V_j (p_i + p_j) gradW
+// cancel a hydrostatic p of order H/h times the result, which amplifies a
+// per-particle FP32 rounding of V_j = m_j/rho_j past 1e-5.  The remaining
+// SPH fields use the storage type R (float on the mixed path, double on the
+// FP64 path).  DEM spheres are few (<= 3% of particles) and are kept in
+// FP64 on both paths so contact detection and shear history match the
+// oracle exactly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/sophia/abi.h"
+
+namespace sophia {
+
+constexpr uint32_t kEmpty = 0xffffffffu;  // empty cell marker
+constexpr int kMaxHist = 16;              // DEM shear-history slots
+constexpr int kBlock = 256;               // default workgroup size (4 waves)
+constexpr uint32_t kNbrFlag = 0x80000000u;  // set on list entries that fail r+1e-20 < R
+
+// error reporting
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+#define HIP_TRY(call)                                                    \
+  do {                                                                   \
+    hipError_t e__ = (call);                                             \
+    if (e__ != hipSuccess) return hip_fail(e__, #call, __FILE__, __LINE__); \
+  } while (0)
+
+// Parameters passed by value to every kernel.
+struct DevParams {
+  int dim, kernel_type, kgc, delsph, rho_type;
+  int fp_solve, fv_solve, fva_solve, fg_solve, con_solve, xsph_solve;
+  int open_boundary, z_index, eulerian, fix_xsph;
+  int NI, NJ, NK, num_cells;
+  int dem_slots;
+  int filt_step;          // 1 when the Shepard filter is due this step
+  int first_step;         // time == 0
+  double x_min, x_max, y_min, y_max, z_min, z_max;
+  double kappa, dt_sph, dt_dem, u_limit, c_xsph, soundspeed, gravity;
+  double rho_fluid, rho_water, h_over_dx, mu_fluid, mu_water, k_fluid, k_dem, cp_fluid, cp_dem;
+  double dem_G, dem_nu, dem_friction, dem_damping, dem_hertz, drag_switch;
+  double eos_b, gamma, delta_sph, p_ref;
+};
+
+// ---------------------------------------------------------------------------
+// Device state.  Persistent fields travel with the particles through the
+// NNPS permutation; scratch fields are recomputed every step.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct SphView {
+  int n;
+  // persistent
+  double *x, *y, *z, *h, *pres, *m, *rho;
+  R *ux, *uy, *uz, *temp, *flt_s, *dpor;
+  int *ptype, *btype, *itype;
+  // scratch
+  double* por;  // porosity eps (FP64: pressure-gradient sums cancel against it)
+  R *dpor_prev, *rho_ref, *rho0;
+  R *cm;              // 6 * n, symmetric KGC inverse (xx, xy, xz, yy, yz, zz)
+  R *grx, *gry, *grz; // delta-SPH density gradient
+  R *fx, *fy, *fz, *fmag;   // ftotal (P3)
+  R *fpx, *fpy, *fpz;       // pressure acceleration (P3.fp)
+  R *pgx, *pgy, *pgz;       // pressure-gradient force density (P1.pgf)
+  R *dtemp, *temp0;
+  R *ux0, *uy0, *uz0;       // velocity at step start (P2.ux0)
+  R *drho;                  // WCSPH continuity rate
+  double *ppe1, *ppe2, *ppe3, *ppe4;
+  R *xsx, *xsy, *xsz, *xst; // XSPH results
+  // SPH-SPH neighbour list (slot major: nbr[k * n + i])
+  uint32_t* nbr;
+  int* nbr_cnt;
+  int nbr_cap;
+};
+
+struct DemView {
+  int n;
+  double *x, *y, *z, *ux, *uy, *uz, *wx, *wy, *wz, *m, *ri, *rad, *h, *rho, *temp;
+  double *fdx, *fdy, *fdz;     // Fd_df (persistent: only written in the fluid regime)
+  double *qsd;
+  int *ptype, *btype, *itype, *dem_idx;
+  int* hist_idx;      // kMaxHist * n
+  double* hist_ds;    // kMaxHist * 3 * n
+  int* hist_idx_new;
+  double* hist_ds_new;
+  // scratch
+  double *flt_s, *flt_sd, *flt_sd2, *por;
+  double *fax, *fay, *faz, *fbx, *fby, *fbz;
+  double *fx, *fy, *fz, *tqx, *tqy, *tqz;
+  double *ux0, *uy0, *uz0;
+  double *test6;
+};
+
+struct CellView {
+  const uint32_t* start;
+  const uint32_t* end;
+};
+
+// ---------------------------------------------------------------------------
+// cell geometry: functions_NNPS.cuh:306-390, 776-790
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint64_t spread3(uint64_t v) {
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+__host__ __device__ inline uint64_t spread2(uint64_t v) {
+  v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
+  v = (v | (v << 8)) & 0x00FF00FF00FF00FFULL;
+  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+  v = (v | (v << 2)) & 0x3333333333333333ULL;
+  v = (v | (v << 1)) & 0x5555555555555555ULL;
+  return v;
+}
+
+__device__ inline int axis_cell(double x, double mn, double mx, int N) {
+  if (mx == mn) return 0;
+  double f = floor((x - mn) / (mx - mn) * N);
+  double v = f < (double)(N - 1) ? f : (double)(N - 1);
+  int c = (int)v;
+  return c < 0 ? 0 : c;
+}
+
+__device__ inline int cell_key(const DevParams& P, int a, int b, int c) {
+  if (!P.z_index) return a + P.NI * b + P.NI * P.NJ * c;
+  if (P.dim == 2) return (int)(spread2((uint64_t)a) | (spread2((uint64_t)b) << 1));
+  return (int)(spread3((uint64_t)a) | spread3((uint64_t)b) << 1 | spread3((uint64_t)c) << 2);
+}
+
+__device__ inline void cell_of(const DevParams& P, double x, double y, double z, int& a, int& b, int& c) {
+  a = axis_cell(x, P.x_min, P.x_max, P.NI);
+  b = axis_cell(y, P.y_min, P.y_max, P.NJ);
+  c = axis_cell(z, P.z_min, P.z_max, P.NK);
+}
+
+// Exact FP64 distance: products rounded, then summed left to right, then a
+// correctly rounded sqrt -- the oracle's expression with no FMA contraction.
+__device__ inline double dist_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  double a = dx * dx;
+  double b = dy * dy;
+  double c = dz * dz;
+  double s = (a + b) + c;
+  return sqrt(s);
+}
+
+// Visit the 27 cells around a point in the reference order (z, y, x).
+template <class F>
+__device__ inline void for_cells(const DevParams& P, double x, double y, double z, const CellView& cv, F&& f) {
+  int ic, jc, kc;
+  cell_of(P, x, y, z, ic, jc, kc);
+  for (int dz = -1; dz <= 1; dz++) {
+    int c = kc + dz;
+    if (c < 0 || c > P.NK - 1) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      int b = jc + dy;
+      if (b < 0 || b > P.NJ - 1) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        int a = ic + dx;
+        if (a < 0 || a > P.NI - 1) continue;
+        int k = cell_key(P, a, b, c);
+        uint32_t s = cv.start[k];
+        if (s == kEmpty) continue;
+        uint32_t e = cv.end[k];
+        for (uint32_t j = s; j < e; j++) f((int)j);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernel functions: functions_KNL.cuh:2-173 (Wendland uses q = r / 2h)
+// ---------------------------------------------------------------------------
+template <typename T>
+__host__ __device__ inline T knl_tmpA(int kt, int dim, T tH) {
+  const T PI = (T)3.141592653;
+  T tA = 0;
+  switch (kt) {
+    case SOPHIA_KNL_GAUSSIAN:
+      tA = dim == 2 ? (T)1.0 / (PI * tH * tH) : (T)1.0 / (pow(PI, (T)1.5) * tH * tH * tH);
+      break;
+    case SOPHIA_KNL_QUINTIC:
+      tA = dim == 2 ? (T)7.0 / ((T)478.0 * PI * tH * tH) : (T)3.0 / ((T)359.0 * PI * tH * tH * tH);
+      break;
+    case SOPHIA_KNL_QUARTIC:
+      tA = dim == 2 ? (T)15.0 / ((T)7.0 * PI * tH * tH) : (T)315.0 / ((T)208.0 * PI * tH * tH * tH);
+      break;
+    case SOPHIA_KNL_WENDLAND2:
+      tA = dim == 2 ? (T)2.228169203286535 / (4 * tH * tH) : (T)3.342253804929802 / (8 * tH * tH * tH);
+      break;
+    case SOPHIA_KNL_WENDLAND4:
+      tA = dim == 2 ? (T)2.864788975654116 / (4 * tH * tH) : (T)4.923856051905513 / (8 * tH * tH * tH);
+      break;
+    case SOPHIA_KNL_WENDLAND6:
+      tA = dim == 2 ? (T)3.546881588905096 / (4 * tH * tH) : (T)6.788953041263660 / (8 * tH * tH * tH);
+      break;
+  }
+  return tA;
+}
+
+template <typename T>
+__host__ __device__ inline T ipow5(T v) { return v * v * v * v * v; }
+template <typename T>
+__host__ __device__ inline T ipow4(T v) { return v * v * v * v; }
+
+template <typename T>
+__host__ __device__ inline T knl_w(int kt, T tA, T tH, T rr) {
+  T w = 0;
+  switch (kt) {
+    case SOPHIA_KNL_GAUSSIAN: {
+      T tR = rr / tH;
+      w = tA * exp(-tR * tR);
+      break;
+    }
+    case SOPHIA_KNL_QUINTIC: {
+      T tR = rr / tH;
+      if (tR < 1) w = tA * (ipow5((T)3 - tR) - (T)6 * ipow5((T)2 - tR) + (T)15 * ipow5((T)1 - tR));
+      else if (tR < 2) w = tA * (ipow5((T)3 - tR) - (T)6 * ipow5((T)2 - tR));
+      else if (tR < 3) w = tA * ipow5((T)3 - tR);
+      break;
+    }
+    case SOPHIA_KNL_QUARTIC: {
+      T tR = rr / tH;
+      if (tR < 2) w = tA * ((T)(2.0 / 3.0) - (T)(9.0 / 8.0) * tR * tR + (T)(19.0 / 24.0) * tR * tR * tR - (T)(5.0 / 32.0) * tR * tR * tR * tR);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND2: {
+      T tR = rr / tH * (T)0.5;
+      T o = 1 - tR;
+      if (tR < 1) w = tA * o * o * o * o * (1 + 4 * tR);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND4: {
+      T tR = rr / tH * (T)0.5;
+      T o = 1 - tR;
+      if (tR < 1) w = tA * o * o * o * o * o * o * (1 + 6 * tR + (T)11.666666666666666 * tR * tR);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND6: {
+      T tR = rr / tH * (T)0.5;
+      T o = 1 - tR;
+      if (tR < 1) w = tA * o * o * o * o * o * o * o * o * (1 + 8 * tR + 25 * tR * tR + 32 * tR * tR * tR);
+      break;
+    }
+  }
+  return w;
+}
+
+template <typename T>
+__host__ __device__ inline T knl_dw(int kt, T tA, T tH, T rr) {
+  T dw = 0;
+  switch (kt) {
+    case SOPHIA_KNL_GAUSSIAN: {
+      T tR = rr / tH;
+      dw = ((T)1 / tH) * tA * (T)(-2) * tR * exp(-tR * tR);
+      break;
+    }
+    case SOPHIA_KNL_QUINTIC: {
+      T tR = rr / tH;
+      if (tR < 1) dw = ((T)1 / tH) * tA * (ipow5((T)3 - tR) + (T)30 * ipow4((T)2 - tR) - (T)75 * ipow4((T)1 - tR));
+      else if (tR < 2) dw = ((T)1 / tH) * tA * ((T)(-5) * ipow4((T)3 - tR) + (T)30 * ipow4((T)2 - tR));
+      else if (tR < 3) dw = ((T)1 / tH) * tA * ((T)(-5) * ipow4((T)3 - tR));
+      break;
+    }
+    case SOPHIA_KNL_QUARTIC: {
+      T tR = rr / tH;
+      if (tR < 2) dw = ((T)1 / tH) * tA * ((T)(-9.0 / 8.0 * 2) * tR + (T)(19.0 / 24.0 * 3.0) * tR * tR - (T)(5.0 / 32.0 * 4.0) * tR * tR * tR);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND2: {
+      T tR = rr / tH * (T)0.5;
+      T o = 1 - tR;
+      if (tR < 1) dw = ((T)1 / (2 * tH)) * tA * ((T)(-20) * tR * o * o * o);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND4: {
+      T tR = rr / tH * (T)0.5;
+      T o = 1 - tR;
+      if (tR < 1) dw = ((T)1 / (2 * tH)) * tA * ((T)(-18.666666666666668) * tR * o * o * o * o * o * (5 * tR + 1));
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND6: {
+      T tR = rr / tH * (T)0.5;
+      T o = 1 - tR;
+      if (tR < 1) dw = ((T)1 / (2 * tH)) * tA * ((T)(-22) * tR * o * o * o * o * o * o * o * (16 * tR * tR + 7 * tR + 1));
+      break;
+    }
+  }
+  return dw;
+}
+
+// KGC-corrected gradient with the symmetric inverse stored as 6 entries
+template <typename R, typename T>
+__device__ inline void kgc_apply(const DevParams& P, const R* cm, int n, int i, T& gx, T& gy, T& gz) {
+  if (P.kgc != 1) return;
+  T a = (T)cm[i], b = (T)cm[n + i], c = (T)cm[2 * n + i];
+  T d = (T)cm[3 * n + i], e = (T)cm[4 * n + i], f = (T)cm[5 * n + i];
+  T x = a * gx + b * gy + c * gz;
+  T y = b * gx + d * gy + e * gz;
+  T z = c * gx + e * gy + f * gz;
+  gx = x;
+  gy = y;
+  gz = z;
+}
+
+inline int grid_for(int n, int block = kBlock) { return (n + block - 1) / block; }
+
+// ---------------------------------------------------------------------------
+// launch wrappers implemented in the .hip files
+// ---------------------------------------------------------------------------
+struct SortScratch {
+  uint32_t *keys_a, *keys_b, *vals_a, *vals_b;
+  uint32_t* hist;   // 256 * max_tiles
+  int max_tiles;
+};
+
+int nnps_keys(hipStream_t s, const DevParams& P, int n, const double* x, const double* y, const double* z,
+              const int* itype, uint32_t* keys, uint32_t* vals);
+int radix_sort_pairs(hipStream_t s, int n, int nbits, SortScratch& sc, uint32_t** keys_out, uint32_t** vals_out);
+int cell_ranges(hipStream_t s, int n, int num_cells, const uint32_t* keys, uint32_t* start, uint32_t* end);
+struct PermuteField {
+  void* src;
+  void* dst;
+  int bytes;  // 4 or 8
+};
+int permute_fields(hipStream_t s, int n, const uint32_t* order, const PermuteField* fields, int nf);
+int fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
+int fill_zero_bytes(hipStream_t s, void* p, size_t bytes);
+
+template <typename R>
+int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt);
+
+// SPH stages
+template <typename R> int sph_mass_init(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_kgc(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dcells);
+template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv);
+template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_set_alpha_noop(hipStream_t s, const DevParams& P, const SphView<R>& v);
+
+// DEM stages
+template <typename R> int dem_porosity(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& scells, const CellView& dcells);
+int dem_contact(hipStream_t s, const DevParams& P, const DemView& d, const CellView& dcells);
+template <typename R> int dem_coupling(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& scells);
+template <typename R> int sph_dem_reaction(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dcells);
+int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
+int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
+
+int knl_eval_device(hipStream_t s, int kt, int dim, double h, const double* r, int n, double* w, double* dw);
+
+}  // namespace sophia

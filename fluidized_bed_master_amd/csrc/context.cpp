@@ -1,0 +1,1444 @@
+// Host side of the C ABI: one context owns the device state of one run and
+// drives the stage kernels in the order of SOPHIA_single_ISPH
+// (ISPH_Calc_seperate.cuh:13-503) on a single HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "learned.hpp"
+#include "ppe.hpp"
+#include "timers.hpp"
+
+static_assert(sizeof(part1) == 1272, "part1 layout must match class_Cuda_Particle_Array2.cuh:11-85");
+static_assert(sizeof(part2) == 176, "part2 layout must match class_Cuda_Particle_Array2.cuh:87-106");
+static_assert(sizeof(part3) == 496, "part3 layout must match class_Cuda_Particle_Array2.cuh:108-167");
+
+namespace sophia {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  return e == hipErrorOutOfMemory ? SOPHIA_E_NOMEM : SOPHIA_E_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// device memory bookkeeping
+// ---------------------------------------------------------------------------
+struct Arena {
+  std::vector<void*> ptrs;
+  ~Arena() { release(); }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+  }
+  template <typename T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    ptrs.push_back(q);
+    *p = (T*)q;
+    return 0;
+  }
+  void drop(void* p) {
+    auto it = std::find(ptrs.begin(), ptrs.end(), p);
+    if (it != ptrs.end()) {
+      (void)hipFree(p);
+      ptrs.erase(it);
+    }
+  }
+};
+
+#define TRY(x)            \
+  do {                    \
+    int rc__ = (x);       \
+    if (rc__) return rc__; \
+  } while (0)
+
+// persistent field with its permutation spare
+template <typename T>
+struct PField {
+  T* cur = nullptr;
+  T* spare = nullptr;
+  int alloc(Arena& a, size_t n) {
+    TRY(a.alloc(&cur, n));
+    return a.alloc(&spare, n);
+  }
+  void swap() { std::swap(cur, spare); }
+};
+
+struct BaseCtx {
+  virtual ~BaseCtx() {}
+};
+
+}  // namespace sophia
+
+using namespace sophia;
+
+// ---------------------------------------------------------------------------
+// the context
+// ---------------------------------------------------------------------------
+struct sophia_ctx {
+  virtual ~sophia_ctx() {}
+  sophia_params p{};
+  sophia_domain d{};
+  int device = 0;
+  int precision = SOPHIA_PREC_MIXED;
+  hipStream_t stream = nullptr;
+  int64_t count = 0;
+  double time = 0;
+  bool nnps_done = false;
+  bool after_corrector = false;
+  virtual int nnps() = 0;
+  virtual int prep() = 0;
+  virtual int interaction() = 0;
+  virtual int dem_interaction() = 0;
+  virtual int coupling() = 0;
+  virtual int time_stage(int phase) = 0;
+  virtual int ppe(const sophia_ppe_opts* o, sophia_ppe_stats* st) = 0;
+  virtual int xsph() = 0;
+  virtual int get_nnps(int species, int32_t* keys, int32_t* order, uint32_t* cs, uint32_t* ce, int32_t* nc) const = 0;
+  virtual int get_particles(int species, part1* p1, part2* p2, part3* p3) const = 0;
+  virtual int set_pressure(const double* p, int n) = 0;
+  virtual int get_ppe_system(double* diag, double* rhs, int32_t* row) const = 0;
+  virtual int ppe_apply(const double* x, double* y, int n) = 0;
+  virtual int mlp_predict(double* out, int n) = 0;
+  virtual int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) = 0;
+  virtual int mlp_set(const float* w, int n) = 0;
+  virtual int mlp_get(float* w, int n) const = 0;
+  virtual int mlp_reset() = 0;
+  Timers timers;
+  Timers* tm() { return timers.on ? &timers : nullptr; }
+};
+
+namespace {
+
+int ceil_log2(uint32_t v) {
+  int b = 0;
+  while ((1ull << b) < (uint64_t)v) b++;
+  return b;
+}
+
+template <typename R>
+struct Ctx : public sophia_ctx {
+  Arena arena;
+  int ns = 0, nd = 0;
+  // SPH persistent
+  PField<double> sx, sy, sz, sh, spres, sm, srho;
+  PField<R> sux, suy, suz, stemp, sflt, sdpor;
+  PField<int> sptype, sbtype, sitype;
+  // DEM persistent
+  PField<double> dx_, dy_, dz_, dux, duy, duz, dwx, dwy, dwz, dm, dri, drad, dh, drho, dtemp, dfdx, dfdy, dfdz, dqsd;
+  PField<int> dptype, dbtype, ditype, didx;
+  int* hist_idx[2] = {nullptr, nullptr};
+  double* hist_ds[2] = {nullptr, nullptr};
+  SphView<R> sv{};
+  DemView dv{};
+  // cells and sort
+  uint32_t *scs = nullptr, *sce = nullptr, *dcs = nullptr, *dce = nullptr;
+  SortScratch ssort{}, dsort{};
+  uint32_t *skeys = nullptr, *sorder = nullptr, *dkeys = nullptr, *dorder = nullptr;
+  int* maxcnt = nullptr;
+  PpeWork ppe_w;
+  bool ppe_assembled = false;
+  int n_rows = 0;
+  Mlp mlp;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~Ctx() override {
+    ppe_workspace_free(ppe_w);
+    mlp_free(mlp);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    arena.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  DevParams devparams() const {
+    DevParams P{};
+    P.dim = p.dim;
+    P.kernel_type = p.kernel_type;
+    P.kgc = p.kgc_solve;
+    P.delsph = p.delSPH_solve;
+    P.rho_type = p.rho_type;
+    P.fp_solve = p.fp_solve;
+    P.fv_solve = p.fv_solve;
+    P.fva_solve = p.fva_solve;
+    P.fg_solve = p.fg_solve;
+    P.con_solve = p.con_solve;
+    P.xsph_solve = p.xsph_solve;
+    P.open_boundary = p.open_boundary;
+    P.z_index = p.flag_z_index;
+    P.eulerian = p.eulerian_sph;
+    P.fix_xsph = p.fix_xsph_position;
+    P.NI = d.NI;
+    P.NJ = d.NJ;
+    P.NK = d.NK;
+    P.num_cells = d.num_cells;
+    P.dem_slots = std::min(kMaxHist, std::max(1, p.dem_max_contacts));
+    int stride = std::max(1, p.decouple_stride);
+    P.filt_step = (count % ((int64_t)std::max(1, p.freq_filt) * stride)) == 0;
+    P.first_step = time == 0.0;
+    P.x_min = d.x_min; P.x_max = d.x_max;
+    P.y_min = d.y_min; P.y_max = d.y_max;
+    P.z_min = d.z_min; P.z_max = d.z_max;
+    P.kappa = p.kappa;
+    P.dt_sph = stride * p.dt;
+    P.dt_dem = p.dt;
+    P.u_limit = p.u_limit;
+    P.c_xsph = p.c_xsph;
+    P.soundspeed = p.soundspeed;
+    P.gravity = p.gravity;
+    P.rho_fluid = p.rho_fluid;
+    P.rho_water = p.rho_water;
+    P.h_over_dx = p.h_over_dx;
+    P.mu_fluid = p.mu_fluid;
+    P.mu_water = p.mu_water;
+    P.k_fluid = p.k_fluid;
+    P.k_dem = p.k_dem;
+    P.cp_fluid = p.cp_fluid;
+    P.cp_dem = p.cp_dem;
+    P.dem_G = p.dem_shear_modulus;
+    P.dem_nu = p.dem_poisson;
+    P.dem_friction = p.dem_friction;
+    P.dem_damping = p.dem_damping;
+    P.dem_hertz = p.dem_hertz_factor;
+    P.drag_switch = p.drag_switch;
+    P.eos_b = p.eos_b_factor;
+    P.gamma = p.gamma;
+    P.delta_sph = p.delta_sph;
+    P.p_ref = p.p_ref;
+    return P;
+  }
+
+  bool sph_on() const { return (count % std::max(1, p.decouple_stride)) == 0; }
+
+  void refresh_views() {
+    SphView<R>& v = sv;
+    v.n = ns;
+    v.x = sx.cur; v.y = sy.cur; v.z = sz.cur; v.h = sh.cur; v.pres = spres.cur;
+    v.ux = sux.cur; v.uy = suy.cur; v.uz = suz.cur; v.m = sm.cur; v.rho = srho.cur; v.temp = stemp.cur;
+    v.flt_s = sflt.cur; v.dpor = sdpor.cur;
+    v.ptype = sptype.cur; v.btype = sbtype.cur; v.itype = sitype.cur;
+    DemView& w = dv;
+    w.n = nd;
+    w.x = dx_.cur; w.y = dy_.cur; w.z = dz_.cur; w.ux = dux.cur; w.uy = duy.cur; w.uz = duz.cur;
+    w.wx = dwx.cur; w.wy = dwy.cur; w.wz = dwz.cur; w.m = dm.cur; w.ri = dri.cur; w.rad = drad.cur;
+    w.h = dh.cur; w.rho = drho.cur; w.temp = dtemp.cur; w.fdx = dfdx.cur; w.fdy = dfdy.cur; w.fdz = dfdz.cur;
+    w.qsd = dqsd.cur; w.ptype = dptype.cur; w.btype = dbtype.cur; w.itype = ditype.cur; w.dem_idx = didx.cur;
+    w.hist_idx = hist_idx[0]; w.hist_ds = hist_ds[0];
+    w.hist_idx_new = hist_idx[1]; w.hist_ds_new = hist_ds[1];
+  }
+
+  int alloc_nbr(int cap) {
+    if (sv.nbr) arena.drop(sv.nbr);
+    sv.nbr_cap = cap;
+    TRY(arena.alloc(&sv.nbr, (size_t)cap * std::max(ns, 1)));
+    ppe_workspace_free(ppe_w);
+    TRY(ppe_workspace_alloc(ppe_w, ns, cap, precision == SOPHIA_PREC_MIXED));
+    ppe_assembled = false;
+    return 0;
+  }
+
+  int init(const sophia_params& prm, const part1* P, int n) {
+    p = prm;
+    count = 0;
+    time = prm.time;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    timers.stream = stream;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    setup_domain(P, n);
+    std::vector<int> si, di;
+    for (int i = 0; i < n; i++) (P[i].p_type <= 1000 ? si : di).push_back(i);
+    ns = (int)si.size();
+    nd = (int)di.size();
+    for (int k : si) {
+      const part1& q = P[k];
+      if (q.i_type <= 2 && !(p.open_boundary > 0 && q.buffer_type > 0) && q.p_type > 0 && q.p_type < 1000) n_rows++;
+    }
+    d.n_sph = ns;
+    d.n_dem = nd;
+    size_t NS = ns, ND = nd;
+    // SPH persistent
+    for (auto* f : {&sx, &sy, &sz, &sh, &spres, &sm, &srho}) TRY(f->alloc(arena, NS));
+    for (auto* f : {&sux, &suy, &suz, &stemp, &sflt, &sdpor}) TRY(f->alloc(arena, NS));
+    for (auto* f : {&sptype, &sbtype, &sitype}) TRY(f->alloc(arena, NS));
+    // SPH scratch
+    SphView<R>& v = sv;
+    TRY(arena.alloc(&v.por, NS));
+    for (R** f : {&v.dpor_prev, &v.rho_ref, &v.rho0, &v.grx, &v.gry, &v.grz, &v.fx, &v.fy, &v.fz, &v.fmag,
+                  &v.fpx, &v.fpy, &v.fpz, &v.pgx, &v.pgy, &v.pgz, &v.dtemp, &v.temp0, &v.ux0, &v.uy0, &v.uz0,
+                  &v.drho, &v.xsx, &v.xsy, &v.xsz, &v.xst})
+      TRY(arena.alloc(f, NS));
+    TRY(arena.alloc(&v.cm, 6 * NS));
+    for (double** f : {&v.ppe1, &v.ppe2, &v.ppe3, &v.ppe4}) TRY(arena.alloc(f, NS));
+    TRY(arena.alloc(&v.nbr_cnt, NS));
+    // DEM persistent
+    for (auto* f : {&dx_, &dy_, &dz_, &dux, &duy, &duz, &dwx, &dwy, &dwz, &dm, &dri, &drad, &dh, &drho, &dtemp, &dfdx,
+                    &dfdy, &dfdz, &dqsd})
+      TRY(f->alloc(arena, ND));
+    for (auto* f : {&dptype, &dbtype, &ditype, &didx}) TRY(f->alloc(arena, ND));
+    for (int b = 0; b < 2; b++) {
+      TRY(arena.alloc(&hist_idx[b], (size_t)kMaxHist * ND));
+      TRY(arena.alloc(&hist_ds[b], (size_t)kMaxHist * 3 * ND));
+    }
+    DemView& w = dv;
+    for (double** f : {&w.flt_s, &w.flt_sd, &w.flt_sd2, &w.por, &w.fax, &w.fay, &w.faz, &w.fbx, &w.fby, &w.fbz,
+                       &w.fx, &w.fy, &w.fz, &w.tqx, &w.tqy, &w.tqz, &w.ux0, &w.uy0, &w.uz0, &w.test6})
+      TRY(arena.alloc(f, ND));
+    // cells and sort scratch
+    size_t NC = (size_t)d.num_cells;
+    TRY(arena.alloc(&scs, NC));
+    TRY(arena.alloc(&sce, NC));
+    TRY(arena.alloc(&dcs, NC));
+    TRY(arena.alloc(&dce, NC));
+    for (auto pr : {std::make_pair(&ssort, ns), std::make_pair(&dsort, nd)}) {
+      SortScratch& sc = *pr.first;
+      size_t m = std::max(pr.second, 1);
+      TRY(arena.alloc(&sc.keys_a, m));
+      TRY(arena.alloc(&sc.keys_b, m));
+      TRY(arena.alloc(&sc.vals_a, m));
+      TRY(arena.alloc(&sc.vals_b, m));
+      sc.max_tiles = (int)((m + 4095) / 4096);
+      TRY(arena.alloc(&sc.hist, (size_t)256 * sc.max_tiles));
+    }
+    TRY(arena.alloc(&maxcnt, 1));
+    int cap = p.sph_max_neighbors > 0 ? p.sph_max_neighbors : 96;
+    TRY(alloc_nbr(cap));
+    TRY(mlp_alloc(mlp, ns, cap));
+    // upload
+    refresh_views();
+    TRY(upload(P, si, di));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+  }
+
+  // find_minmax (function_init.cuh:913-950) over all particles except the
+  // last, margins in the reference's order; dcell = 1.1*kappa*h0 with h0 the
+  // first particle's h; NI = (int)(span/dcell)+1 (ISPH.cuh:93-113)
+  void setup_domain(const part1* P, int n) {
+    if (p.domain_fixed) {
+      d.x_min = p.domain_min[0]; d.x_max = p.domain_max[0];
+      d.y_min = p.domain_min[1]; d.y_max = p.domain_max[1];
+      d.z_min = p.domain_min[2]; d.z_max = p.domain_max[2];
+    } else {
+      double a = P[0].x, b = P[0].x, c = P[0].y, e = P[0].y, f = P[0].z, g = P[0].z;
+      for (int i = 0; i < n - 1; i++) {
+        a = P[i].x < a ? P[i].x : a; b = P[i].x > b ? P[i].x : b;
+        c = P[i].y < c ? P[i].y : c; e = P[i].y > e ? P[i].y : e;
+        f = P[i].z < f ? P[i].z : f; g = P[i].z > g ? P[i].z : g;
+      }
+      a -= (b - a) * p.xmargin_m; b += (b - a) * p.xmargin_p;
+      c -= (e - c) * p.ymargin_m; e += (e - c) * p.ymargin_p;
+      f -= (g - f) * p.zmargin_m; g += (g - f) * p.zmargin_p;
+      d.x_min = a; d.x_max = b; d.y_min = c; d.y_max = e; d.z_min = f; d.z_max = g;
+    }
+    d.dcell = 1.1 * p.kappa * P[0].h;
+    d.NI = (int)((d.x_max - d.x_min) / d.dcell) + 1;
+    d.NJ = (int)((d.y_max - d.y_min) / d.dcell) + 1;
+    d.NK = (int)((d.z_max - d.z_min) / d.dcell) + 1;
+    if (p.flag_z_index == 0) {
+      d.num_cells = d.NI * d.NJ * d.NK + 1;
+    } else {
+      int m = std::max(std::max(d.NI, d.NJ), d.NK);
+      uint64_t last = p.dim == 2 ? (spread2(m - 1) | spread2(m - 1) << 1)
+                                 : (spread3(m - 1) | spread3(m - 1) << 1 | spread3(m - 1) << 2);
+      d.num_cells = (int)last + 2;
+    }
+  }
+
+  template <typename T>
+  int put(T* dst, const std::vector<T>& src) {
+    if (src.empty()) return 0;
+    HIP_TRY(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+  }
+
+  int upload(const part1* P, const std::vector<int>& si, const std::vector<int>& di) {
+    auto colD = [&](const std::vector<int>& ix, double part1::*f) {
+      std::vector<double> o(ix.size());
+      for (size_t k = 0; k < ix.size(); k++) o[k] = P[ix[k]].*f;
+      return o;
+    };
+    auto colR = [&](const std::vector<int>& ix, double part1::*f) {
+      std::vector<R> o(ix.size());
+      for (size_t k = 0; k < ix.size(); k++) o[k] = (R)(P[ix[k]].*f);
+      return o;
+    };
+    auto colI = [&](const std::vector<int>& ix, int part1::*f) {
+      std::vector<int> o(ix.size());
+      for (size_t k = 0; k < ix.size(); k++) o[k] = P[ix[k]].*f;
+      return o;
+    };
+    TRY(put(sx.cur, colD(si, &part1::x)));
+    TRY(put(sy.cur, colD(si, &part1::y)));
+    TRY(put(sz.cur, colD(si, &part1::z)));
+    TRY(put(sh.cur, colD(si, &part1::h)));
+    TRY(put(spres.cur, colD(si, &part1::pres)));
+    TRY(put(sux.cur, colR(si, &part1::ux)));
+    TRY(put(suy.cur, colR(si, &part1::uy)));
+    TRY(put(suz.cur, colR(si, &part1::uz)));
+    TRY(put(sm.cur, colD(si, &part1::m)));
+    TRY(put(srho.cur, colD(si, &part1::rho)));
+    TRY(put(stemp.cur, colR(si, &part1::temp)));
+    TRY(put(sflt.cur, colR(si, &part1::flt_s)));
+    TRY(put(sdpor.cur, colR(si, &part1::dDEMpor)));
+    TRY(put(sptype.cur, colI(si, &part1::p_type)));
+    TRY(put(sbtype.cur, colI(si, &part1::buffer_type)));
+    TRY(put(sitype.cur, colI(si, &part1::i_type)));
+    TRY(put(dx_.cur, colD(di, &part1::x)));
+    TRY(put(dy_.cur, colD(di, &part1::y)));
+    TRY(put(dz_.cur, colD(di, &part1::z)));
+    TRY(put(dux.cur, colD(di, &part1::ux)));
+    TRY(put(duy.cur, colD(di, &part1::uy)));
+    TRY(put(duz.cur, colD(di, &part1::uz)));
+    TRY(put(dwx.cur, colD(di, &part1::wx)));
+    TRY(put(dwy.cur, colD(di, &part1::wy)));
+    TRY(put(dwz.cur, colD(di, &part1::wz)));
+    TRY(put(dm.cur, colD(di, &part1::m)));
+    TRY(put(dri.cur, colD(di, &part1::ri)));
+    TRY(put(drad.cur, colD(di, &part1::rad)));
+    TRY(put(dh.cur, colD(di, &part1::h)));
+    TRY(put(drho.cur, colD(di, &part1::rho)));
+    TRY(put(dtemp.cur, colD(di, &part1::temp)));
+    TRY(put(dfdx.cur, colD(di, &part1::Fdx_df)));
+    TRY(put(dfdy.cur, colD(di, &part1::Fdy_df)));
+    TRY(put(dfdz.cur, colD(di, &part1::Fdz_df)));
+    TRY(put(dqsd.cur, colD(di, &part1::Q_sd)));
+    TRY(put(dptype.cur, colI(di, &part1::p_type)));
+    TRY(put(dbtype.cur, colI(di, &part1::buffer_type)));
+    TRY(put(ditype.cur, colI(di, &part1::i_type)));
+    TRY(put(didx.cur, colI(di, &part1::dem_idx)));
+    if (nd > 0) {
+      std::vector<int> hi((size_t)kMaxHist * nd);
+      std::vector<double> hd((size_t)kMaxHist * 3 * nd);
+      for (int k = 0; k < kMaxHist; k++)
+        for (int i = 0; i < nd; i++) {
+          const part1& q = P[di[i]];
+          hi[(size_t)k * nd + i] = (int)q.ct_idx[k];
+          for (int c = 0; c < 3; c++) hd[((size_t)k * 3 + c) * nd + i] = q.del_s[k][c];
+        }
+      TRY(put(hist_idx[0], hi));
+      TRY(put(hist_ds[0], hd));
+    }
+    return 0;
+  }
+
+  // --- NNPS ---------------------------------------------------------------
+  int sort_species(int n, double* x, double* y, double* z, int* itype, SortScratch& sc, uint32_t** keys,
+                   uint32_t** order, uint32_t* cs, uint32_t* ce) {
+    DevParams P = devparams();
+    TRY(nnps_keys(stream, P, n, x, y, z, itype, sc.keys_a, sc.vals_a));
+    int nbits = std::max(1, ceil_log2((uint32_t)d.num_cells));
+    TRY(radix_sort_pairs(stream, n, ((nbits + 7) / 8) * 8, sc, keys, order));
+    TRY(cell_ranges(stream, n, d.num_cells, *keys, cs, ce));
+    return 0;
+  }
+
+  int nnps() override {
+    const bool on = sph_on();
+    DevParams P = devparams();
+    if (on && ns > 0) {
+      TimeScope t0(tm(), "nnps_sort_permute_sph");
+      TRY(sort_species(ns, sx.cur, sy.cur, sz.cur, sitype.cur, ssort, &skeys, &sorder, scs, sce));
+      std::vector<PermuteField> f;
+      auto add = [&](auto& fld, int bytes) { f.push_back(PermuteField{fld.cur, fld.spare, bytes}); };
+      add(sx, 8); add(sy, 8); add(sz, 8); add(sh, 8); add(spres, 8); add(sm, 8); add(srho, 8);
+      const int rb = (int)sizeof(R);
+      add(sux, rb); add(suy, rb); add(suz, rb); add(stemp, rb); add(sflt, rb);
+      add(sdpor, rb);
+      add(sptype, 4); add(sbtype, 4); add(sitype, 4);
+      TRY(permute_fields(stream, ns, sorder, f.data(), (int)f.size()));
+      for (auto* fl : {&sx, &sy, &sz, &sh, &spres, &sm, &srho}) fl->swap();
+      for (auto* fl : {&sux, &suy, &suz, &stemp, &sflt, &sdpor}) fl->swap();
+      for (auto* fl : {&sptype, &sbtype, &sitype}) fl->swap();
+      refresh_views();
+      // P3 reset (ISPH_Calc_seperate.cuh:102-105)
+      for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp})
+        TRY(fill_zero_bytes(stream, a, (size_t)ns * sizeof(R)));
+      TRY(fill_zero_bytes(stream, sv.cm, (size_t)6 * ns * sizeof(R)));
+      // neighbour list with capacity check
+      for (int attempt = 0; attempt < 2; attempt++) {
+        TRY(fill_u32(stream, (uint32_t*)maxcnt, 0u, 1));
+        CellView cv{scs, sce};
+        {
+          TimeScope t1(tm(), "nbr_build");
+          TRY(build_sph_neighbors<R>(stream, P, sv, cv, maxcnt));
+        }
+        int mc = 0;
+        HIP_TRY(hipMemcpyAsync(&mc, maxcnt, 4, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (mc <= sv.nbr_cap) break;
+        if (attempt == 1) {
+          set_error("neighbour list overflow: %d > %d", mc, sv.nbr_cap);
+          return SOPHIA_E_CAPACITY;
+        }
+        TRY(alloc_nbr(((mc + 15) / 16) * 16));
+        TRY(mlp_alloc(mlp, ns, sv.nbr_cap));
+      }
+      ppe_assembled = false;
+    }
+    if (nd > 0) {
+      TimeScope t2(tm(), "nnps_sort_permute_dem");
+      TRY(sort_species(nd, dx_.cur, dy_.cur, dz_.cur, ditype.cur, dsort, &dkeys, &dorder, dcs, dce));
+      std::vector<PermuteField> f;
+      auto add = [&](auto& fld, int bytes) { f.push_back(PermuteField{fld.cur, fld.spare, bytes}); };
+      for (auto* fl : {&dx_, &dy_, &dz_, &dux, &duy, &duz, &dwx, &dwy, &dwz, &dm, &dri, &drad, &dh, &drho, &dtemp,
+                       &dfdx, &dfdy, &dfdz, &dqsd})
+        add(*fl, 8);
+      for (auto* fl : {&dptype, &dbtype, &ditype, &didx}) add(*fl, 4);
+      for (int k = 0; k < kMaxHist; k++) {
+        f.push_back(PermuteField{hist_idx[0] + (size_t)k * nd, hist_idx[1] + (size_t)k * nd, 4});
+        for (int c = 0; c < 3; c++)
+          f.push_back(PermuteField{hist_ds[0] + ((size_t)k * 3 + c) * nd, hist_ds[1] + ((size_t)k * 3 + c) * nd, 8});
+      }
+      TRY(permute_fields(stream, nd, dorder, f.data(), (int)f.size()));
+      for (auto* fl : {&dx_, &dy_, &dz_, &dux, &duy, &duz, &dwx, &dwy, &dwz, &dm, &dri, &drad, &dh, &drho, &dtemp,
+                       &dfdx, &dfdy, &dfdz, &dqsd})
+        fl->swap();
+      for (auto* fl : {&dptype, &dbtype, &ditype, &didx}) fl->swap();
+      std::swap(hist_idx[0], hist_idx[1]);
+      std::swap(hist_ds[0], hist_ds[1]);
+      refresh_views();
+      for (double* a : {dv.fx, dv.fy, dv.fz, dv.tqx, dv.tqy, dv.tqz})
+        TRY(fill_zero_bytes(stream, a, (size_t)nd * 8));
+    } else {
+      TRY(fill_u32(stream, dcs, kEmpty, (size_t)d.num_cells));
+    }
+    nnps_done = true;
+    after_corrector = false;
+    return 0;
+  }
+
+  int need_nnps() const {
+    if (!nnps_done) {
+      set_error("stage called before sophia_nnps");
+      return SOPHIA_E_STATE;
+    }
+    return 0;
+  }
+
+  // --- stages ---------------------------------------------------------------
+  int prep() override {
+    TRY(need_nnps());
+    DevParams P = devparams();
+    CellView sc{scs, sce}, dc{dcs, dce};
+    bool on = sph_on();
+    if (p.dim != 3) return 0;
+    if (on) TRY(sph_mass_init<R>(stream, P, sv));
+    if (on && p.kgc_solve > 0) {
+      TimeScope t(tm(), "kgc");
+      TRY(sph_kgc<R>(stream, P, sv));
+    }
+    if (on) {
+      TimeScope t(tm(), "porosity_sph");
+      TRY(sph_porosity<R>(stream, P, sv, dv, dc));
+    }
+    {
+      TimeScope t(tm(), "porosity_dem");
+      TRY(dem_porosity<R>(stream, P, dv, sv, sc, dc));
+    }
+    if (on) {
+      TimeScope t(tm(), "prep");
+      TRY(sph_prep<R>(stream, P, sv));
+    }
+    return 0;
+  }
+
+  int interaction() override {
+    TRY(need_nnps());
+    if (!sph_on() || p.dim != 3) return 0;
+    TimeScope t(tm(), "advection");
+    return sph_advection<R>(stream, devparams(), sv);
+  }
+
+  int dem_interaction() override {
+    TRY(need_nnps());
+    if (p.dim != 3 || nd == 0) return 0;
+    CellView dc{dcs, dce};
+    {
+      TimeScope t(tm(), "dem_contact");
+      TRY(dem_contact(stream, devparams(), dv, dc));
+    }
+    std::swap(hist_idx[0], hist_idx[1]);
+    std::swap(hist_ds[0], hist_ds[1]);
+    refresh_views();
+    return 0;
+  }
+
+  int coupling() override {
+    TRY(need_nnps());
+    if (p.dim != 3) return 0;
+    DevParams P = devparams();
+    CellView sc{scs, sce}, dc{dcs, dce};
+    {
+      TimeScope t(tm(), "dem_coupling");
+      TRY(dem_coupling<R>(stream, P, dv, sv, sc));
+    }
+    if (sph_on()) {
+      TimeScope t(tm(), "sph_reaction");
+      TRY(sph_dem_reaction<R>(stream, P, sv, dv, dc));
+    }
+    return 0;
+  }
+
+  int time_stage(int phase) override {
+    TRY(need_nnps());
+    DevParams P = devparams();
+    if (phase == SOPHIA_PREDICT) {
+      if (p.time_type != 1) return 0;
+      TimeScope t(tm(), "predictor");
+      if (sph_on()) TRY(sph_predictor<R>(stream, P, sv));
+      TRY(dem_predictor(stream, P, dv));
+    } else {
+      TimeScope t(tm(), "corrector");
+      if (sph_on()) TRY(sph_corrector<R>(stream, P, sv));
+      TRY(dem_corrector(stream, P, dv));
+      after_corrector = true;
+    }
+    return 0;
+  }
+
+  int ppe(const sophia_ppe_opts* o, sophia_ppe_stats* st) override {
+    TRY(need_nnps());
+    sophia_ppe_stats local{};
+    if (!st) st = &local;
+    std::memset(st, 0, sizeof(*st));
+    if (!sph_on() || p.dim != 3 || ns == 0) return 0;
+    sophia_ppe_opts def{};
+    def.method = SOPHIA_PPE_JACOBI;
+    if (!o) o = &def;
+    if (o->method == SOPHIA_PPE_BICGSTAB) {
+      bool mixed_ws = ppe_w.mixed;
+      bool want_mixed = o->precision == SOPHIA_PPE_MIXED_IR;
+      if (mixed_ws != want_mixed) {
+        ppe_workspace_free(ppe_w);
+        TRY(ppe_workspace_alloc(ppe_w, ns, sv.nbr_cap, want_mixed));
+      }
+    }
+    DevParams P = devparams();
+    CellView sc{scs, sce};
+    HIP_TRY(hipEventRecord(ev0, stream));
+    if (count == 0) {
+      TimeScope t(tm(), "neumann");
+      TRY(sph_neumann<R>(stream, P, sv, sc));
+    }
+    {
+      TimeScope t(tm(), "ppe_assemble");
+      TRY(ppe_assemble<R>(stream, P, sv, ppe_w));
+    }
+    ppe_assembled = true;
+    if (o->method == SOPHIA_PPE_JACOBI) {
+      TRY(ppe_jacobi<R>(stream, sv, ppe_w));
+      st->iterations = 1;
+      st->converged = 1;
+    } else {
+      const double* guess = nullptr;
+      st->guess_used = o->guess == SOPHIA_GUESS_ZERO ? SOPHIA_GUESS_ZERO : SOPHIA_GUESS_PREVIOUS;
+      if (o->guess == SOPHIA_GUESS_LEARNED && mlp.ready) {
+        // learned guess with residual-based fallback to the previous pressure
+        TRY(mlp_predict_rows(stream, P, sv, mlp, ppe_w.row, mlp.guess));
+        double r_l = 0, r_p = 0;
+        TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, mlp.guess, &r_l));
+        TRY(mlp_gather_previous(stream, ns, ppe_w.row, sv.pres, ppe_w.x64));
+        TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, ppe_w.x64, &r_p));
+        st->r0_rel_learned = r_l;
+        if (r_l < r_p) {
+          guess = mlp.guess;
+          st->guess_used = SOPHIA_GUESS_LEARNED;
+        }
+      }
+      sophia_ppe_opts oo = *o;
+      if (oo.max_iter <= 0) oo.max_iter = 10000;
+      if (oo.rtol <= 0) oo.rtol = 1e-8;
+      TRY(ppe_solve(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, oo, *st, guess, tm()));
+    }
+    {
+      TimeScope t(tm(), "neumann");
+      TRY(sph_neumann<R>(stream, P, sv, sc));
+    }
+    {
+      TimeScope t(tm(), "pressure_force");
+      TRY(sph_pressure_force<R>(stream, P, sv));
+    }
+    HIP_TRY(hipEventRecord(ev1, stream));
+    HIP_TRY(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    st->ms = ms;
+    st->n_rows = n_rows;
+    return 0;
+  }
+
+  int xsph() override {
+    TRY(need_nnps());
+    if (!p.xsph_solve || !sph_on() || p.dim != 3) return 0;
+    TimeScope t(tm(), "xsph");
+    return sph_xsph<R>(stream, devparams(), sv);
+  }
+
+  // --- inspection -------------------------------------------------------------
+  template <typename T>
+  int get(std::vector<T>& h, const T* dptr, size_t n) const {
+    h.resize(n);
+    if (n == 0) return 0;
+    HIP_TRY(hipMemcpyAsync(h.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+  }
+
+  int get_nnps(int species, int32_t* keys, int32_t* order, uint32_t* cs, uint32_t* ce, int32_t* nc) const override {
+    bool S = species == SOPHIA_SPH;
+    int n = S ? ns : nd;
+    if (!nnps_done) {
+      set_error("get_nnps before sophia_nnps");
+      return SOPHIA_E_STATE;
+    }
+    std::vector<uint32_t> t;
+    if (keys && n) {
+      TRY(get(t, S ? skeys : dkeys, n));
+      std::memcpy(keys, t.data(), n * 4);
+    }
+    if (order && n) {
+      TRY(get(t, S ? sorder : dorder, n));
+      std::memcpy(order, t.data(), n * 4);
+    }
+    if (cs) {
+      TRY(get(t, S ? scs : dcs, d.num_cells));
+      std::memcpy(cs, t.data(), (size_t)d.num_cells * 4);
+    }
+    if (ce) {
+      TRY(get(t, S ? sce : dce, d.num_cells));
+      std::memcpy(ce, t.data(), (size_t)d.num_cells * 4);
+    }
+    if (nc && S && n) {
+      std::vector<int> c;
+      TRY(get(c, (const int*)sv.nbr_cnt, n));
+      std::memcpy(nc, c.data(), n * 4);
+    }
+    return 0;
+  }
+
+  int get_particles(int species, part1* p1, part2* p2, part3* p3) const override {
+    if (species == SOPHIA_SPH) {
+      int n = ns;
+      std::vector<double> X, Y, Z, H, PR, P1v, P2v, P3v, P4v;
+      std::vector<R> a[40];
+      std::vector<int> pt, bt, it;
+      TRY(get(X, sv.x, n)); TRY(get(Y, sv.y, n)); TRY(get(Z, sv.z, n)); TRY(get(H, sv.h, n));
+      TRY(get(PR, sv.pres, n));
+      TRY(get(P1v, sv.ppe1, n)); TRY(get(P2v, sv.ppe2, n)); TRY(get(P3v, sv.ppe3, n)); TRY(get(P4v, sv.ppe4, n));
+      TRY(get(pt, sv.ptype, n)); TRY(get(bt, sv.btype, n)); TRY(get(it, sv.itype, n));
+      std::vector<double> Mv, RHv, POv;
+      TRY(get(Mv, sv.m, n)); TRY(get(RHv, sv.rho, n)); TRY(get(POv, sv.por, n));
+      const R* src[] = {sv.ux, sv.uy, sv.uz, sv.ux, sv.ux, sv.temp, sv.flt_s, sv.dpor, sv.ux, sv.dpor_prev,
+                        sv.rho_ref, sv.rho0, sv.grx, sv.gry, sv.grz, sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx,
+                        sv.fpy, sv.fpz, sv.pgx, sv.pgy, sv.pgz, sv.dtemp, sv.ux0, sv.uy0, sv.uz0, sv.temp0,
+                        sv.xsx, sv.xsy, sv.xsz, sv.xst};
+      const int NF = sizeof(src) / sizeof(src[0]);
+      for (int k = 0; k < NF; k++) TRY(get(a[k], src[k], n));
+      std::vector<R> cm;
+      TRY(get(cm, sv.cm, (size_t)6 * n));
+      std::vector<int> rows;
+      if (ppe_assembled) TRY(get(rows, (const int*)ppe_w.row, n));
+      for (int i = 0; i < n; i++) {
+        auto A = [&](int k) { return (double)a[k][i]; };
+        if (p1) {
+          part1& q = p1[i];
+          std::memset(&q, 0, sizeof(q));
+          q.i_type = it[i]; q.buffer_type = bt[i]; q.p_type = pt[i];
+          q.x = X[i]; q.y = Y[i]; q.z = Z[i]; q.h = H[i]; q.pres = PR[i];
+          q.ux = A(0); q.uy = A(1); q.uz = A(2); q.m = Mv[i]; q.rho = RHv[i]; q.temp = A(5); q.flt_s = A(6);
+          q.dDEMpor = A(7); q.DEMpor = POv[i]; q.dDEMpor_prev = A(9);
+          q.grad_rhox = A(12); q.grad_rhoy = A(13); q.grad_rhoz = A(14);
+          q.pgf_x = A(22); q.pgf_y = A(23); q.pgf_z = A(24);
+          q.XSPH_ux = A(30); q.XSPH_uy = A(31); q.XSPH_uz = A(32); q.XSPH_temp = A(33);
+          double e = (pt[i] <= 1000 && p.eulerian_sph) ? 0.0 : 1.0;
+          q.elix = q.eliy = q.eliz = e;
+          bool row = !rows.empty() && rows[i];
+          q.PPE1 = row ? P1v[i] : 0; q.PPE2 = row ? P2v[i] : 0; q.PPE3 = row ? P3v[i] : 0; q.PPE4 = row ? P4v[i] : 0;
+        }
+        if (p2) {
+          part2& q = p2[i];
+          std::memset(&q, 0, sizeof(q));
+          q.rho_ref = A(10); q.rho0 = A(11);
+          q.ux0 = A(26); q.uy0 = A(27); q.uz0 = A(28); q.temp0 = A(29);
+          q.x0 = X[i]; q.y0 = Y[i]; q.z0 = Z[i];
+        }
+        if (p3) {
+          part3& q = p3[i];
+          std::memset(&q, 0, sizeof(q));
+          q.ftotalx = A(15); q.ftotaly = A(16); q.ftotalz = A(17); q.ftotal = A(18);
+          q.fpx = A(19); q.fpy = A(20); q.fpz = A(21); q.dtemp = A(25);
+          double c[6];
+          for (int k = 0; k < 6; k++) c[k] = (double)cm[(size_t)k * n + i];
+          q.Cm[0][0] = c[0]; q.Cm[0][1] = c[1]; q.Cm[0][2] = c[2];
+          q.Cm[1][0] = c[1]; q.Cm[1][1] = c[3]; q.Cm[1][2] = c[4];
+          q.Cm[2][0] = c[2]; q.Cm[2][1] = c[4]; q.Cm[2][2] = c[5];
+        }
+      }
+      return 0;
+    }
+    int n = nd;
+    const double* src[] = {dv.x, dv.y, dv.z, dv.ux, dv.uy, dv.uz, dv.wx, dv.wy, dv.wz, dv.m, dv.ri, dv.rad,
+                           dv.h, dv.rho, dv.temp, dv.fdx, dv.fdy, dv.fdz, dv.qsd, dv.flt_s, dv.flt_sd, dv.flt_sd2,
+                           dv.por, dv.fax, dv.fay, dv.faz, dv.fbx, dv.fby, dv.fbz, dv.fx, dv.fy, dv.fz,
+                           dv.tqx, dv.tqy, dv.tqz, dv.ux0, dv.uy0, dv.uz0, dv.test6};
+    const int NF = sizeof(src) / sizeof(src[0]);
+    std::vector<double> a[NF];
+    for (int k = 0; k < NF; k++) TRY(get(a[k], src[k], n));
+    std::vector<int> pt, bt, it, ix, hi;
+    std::vector<double> hd;
+    TRY(get(pt, dv.ptype, n)); TRY(get(bt, dv.btype, n)); TRY(get(it, dv.itype, n)); TRY(get(ix, dv.dem_idx, n));
+    TRY(get(hi, (const int*)hist_idx[0], (size_t)kMaxHist * n));
+    TRY(get(hd, (const double*)hist_ds[0], (size_t)kMaxHist * 3 * n));
+    for (int i = 0; i < n; i++) {
+      auto A = [&](int k) { return a[k][i]; };
+      if (p1) {
+        part1& q = p1[i];
+        std::memset(&q, 0, sizeof(q));
+        q.i_type = it[i]; q.buffer_type = bt[i]; q.p_type = pt[i]; q.dem_idx = ix[i];
+        q.x = A(0); q.y = A(1); q.z = A(2); q.ux = A(3); q.uy = A(4); q.uz = A(5);
+        q.wx = A(6); q.wy = A(7); q.wz = A(8); q.m = A(9); q.ri = A(10); q.rad = A(11); q.h = A(12);
+        q.rho = A(13); q.temp = A(14); q.Fdx_df = A(15); q.Fdy_df = A(16); q.Fdz_df = A(17); q.Q_sd = A(18);
+        q.flt_s = A(19); q.flt_sd = A(20); q.flt_sd_2 = A(21); q.DEMpor = A(22);
+        q.Fdx_da = A(23); q.Fdy_da = A(24); q.Fdz_da = A(25); q.Fdx_b = A(26); q.Fdy_b = A(27); q.Fdz_b = A(28);
+        q.test6 = A(38);
+        q.elix = q.eliy = q.eliz = 1.0;
+        for (int k = 0; k < kMaxHist; k++) {
+          q.ct_idx[k] = hi[(size_t)k * n + i];
+          for (int c = 0; c < 3; c++) q.del_s[k][c] = hd[((size_t)k * 3 + c) * n + i];
+        }
+      }
+      if (p2) {
+        part2& q = p2[i];
+        std::memset(&q, 0, sizeof(q));
+        q.ux0 = A(35); q.uy0 = A(36); q.uz0 = A(37);
+      }
+      if (p3) {
+        part3& q = p3[i];
+        std::memset(&q, 0, sizeof(q));
+        q.ftotalx = A(29); q.ftotaly = A(30); q.ftotalz = A(31);
+        q.torqx = A(32); q.torqy = A(33); q.torqz = A(34);
+      }
+    }
+    return 0;
+  }
+
+  int set_pressure(const double* pr, int n) override {
+    if (n != ns) {
+      set_error("set_pressure: n=%d but %d SPH particles", n, ns);
+      return SOPHIA_E_ARG;
+    }
+    HIP_TRY(hipMemcpyAsync(sv.pres, pr, (size_t)n * 8, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+  }
+
+  int get_ppe_system(double* diag, double* rhs, int32_t* row) const override {
+    if (!ppe_assembled) {
+      set_error("PPE not assembled yet");
+      return SOPHIA_E_STATE;
+    }
+    std::vector<double> t;
+    if (diag) { TRY(get(t, (const double*)ppe_w.diag, ns)); std::memcpy(diag, t.data(), (size_t)ns * 8); }
+    if (rhs) { TRY(get(t, (const double*)ppe_w.rhs, ns)); std::memcpy(rhs, t.data(), (size_t)ns * 8); }
+    if (row) {
+      std::vector<int> r;
+      TRY(get(r, (const int*)ppe_w.row, ns));
+      std::memcpy(row, r.data(), (size_t)ns * 4);
+    }
+    return 0;
+  }
+
+  int ppe_apply(const double* x, double* y, int n) override {
+    if (!ppe_assembled || n != ns) {
+      set_error("ppe_apply: operator not assembled or n mismatch");
+      return SOPHIA_E_STATE;
+    }
+    double *dx = nullptr, *dy = nullptr;
+    TRY(arena.alloc(&dx, n));
+    TRY(arena.alloc(&dy, n));
+    HIP_TRY(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, stream));
+    int rc = ppe_apply64(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, dx, dy, false, nullptr);
+    if (!rc) {
+      hipError_t e = hipMemcpyAsync(y, dy, (size_t)n * 8, hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) rc = hip_fail(e, "ppe_apply copy", __FILE__, __LINE__);
+    }
+    arena.drop(dx);
+    arena.drop(dy);
+    return rc;
+  }
+
+  // --- learned warm start -------------------------------------------------------
+  int mlp_predict(double* out, int n) override {
+    if (!ppe_assembled || n != ns) {
+      set_error("mlp_predict: PPE not assembled or n mismatch");
+      return SOPHIA_E_STATE;
+    }
+    TRY(mlp_predict_rows(stream, devparams(), sv, mlp, ppe_w.row, mlp.guess));
+    HIP_TRY(hipMemcpyAsync(out, mlp.guess, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+  }
+  int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) override {
+    if (!ppe_assembled || n != ns) {
+      set_error("mlp_train: PPE not assembled or n mismatch");
+      return SOPHIA_E_STATE;
+    }
+    return mlp_train_rows(stream, devparams(), sv, mlp, ppe_w.row, target, lr, loss, grad, n_params, step);
+  }
+  int mlp_set(const float* w, int n) override { return mlp_set_params(stream, mlp, w, n); }
+  int mlp_get(float* w, int n) const override { return mlp_get_params(stream, mlp, w, n); }
+  int mlp_reset() override { return mlp_reset_adam(stream, mlp); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// parameters and files
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* sophia_last_error(void) { return sophia::g_err.c_str(); }
+
+void sophia_params_default(sophia_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->solver_type = 1;
+  p->dim = 3;
+  p->kernel_type = SOPHIA_KNL_WENDLAND2;
+  p->rho_type = 1;
+  p->time_type = 1;
+  p->open_boundary = 0;
+  p->flag_z_index = 0;
+  p->freq_filt = 40;
+  p->freq_output = 10000;
+  p->fp_solve = 1;
+  p->fv_solve = 1;
+  p->fva_solve = 0;
+  p->fg_solve = 1;
+  p->con_solve = 0;
+  p->xsph_solve = 1;
+  p->kgc_solve = 1;
+  p->delSPH_solve = 2;
+  p->decouple_stride = 1;
+  p->min_iteration = 3;
+  p->max_iteration = 10;
+  p->kappa = 2.0;
+  p->p_ref = 0.0;
+  p->gamma = 1.0;
+  p->dt = 5.0e-7;
+  p->time = 0.0;
+  p->time_end = 5.0;
+  p->u_limit = 40.0;
+  p->c_xsph = 0.1;
+  p->soundspeed = 200.0;
+  p->rho0_eos = 1.784;
+  p->drho_th = 0.005;
+  p->dp_th = 10;
+  p->p_relaxation = 1;
+  // reference constants (Parameters.cuh, functions_PROP.cuh, functions_DEM_INTERACTION2.cuh)
+  p->rho_fluid = 1.784;
+  p->rho_water = 1000.0;
+  p->h_over_dx = 1.6;
+  p->mu_fluid = 2.0e-5;
+  p->mu_water = 1.0e-3;
+  p->k_fluid = 0.0518;
+  p->k_dem = 2.5;
+  p->cp_fluid = 1006.4;
+  p->cp_dem = 1068.0;
+  p->gravity = 9.8;
+  p->eos_b_factor = 1.186;
+  p->delta_sph = 0.1;
+  p->dem_shear_modulus = 8.3e8;
+  p->dem_poisson = 0.23;
+  p->dem_friction = 0.5;
+  p->dem_damping = std::sqrt(3.333333333) * 0.0535184;
+  p->dem_hertz_factor = 1.33333;
+  p->drag_switch = 0.65;
+  p->eulerian_sph = 1;
+  p->fix_xsph_position = 0;
+  p->dem_max_contacts = 16;
+  p->sph_max_neighbors = 0;
+}
+
+// read_solv_input (function_init.cuh:216-615): whitespace tokens; a key is
+// followed by a fixed number of tokens before its value
+int sophia_read_solv(const char* path, sophia_params* p) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    set_error("cannot open %s", path);
+    return SOPHIA_E_IO;
+  }
+  std::vector<std::string> tok;
+  char buf[1024];
+  while (std::fscanf(f, "%1023s", buf) == 1) tok.push_back(buf);
+  std::fclose(f);
+  auto at = [&](size_t i) -> std::string { return i < tok.size() ? tok[i] : std::string(); };
+  auto yes = [](const std::string& s) { return s == "YES" ? 1 : 0; };
+  for (size_t i = 0; i < tok.size(); i++) {
+    const std::string& k = tok[i];
+    if (k == "solver_type") {
+      std::string v = at(i + 2);
+      p->solver_type = v == "ISPH" ? 1 : (v == "DFSPH" ? 2 : 0);
+    } else if (k == "dimension(1/2/3):") {
+      p->dim = std::atoi(at(i + 1).c_str());
+    } else if (k == "kernel" && at(i + 1) == "type") {
+      std::string v = at(i + 3);
+      if (v == "Quartic") p->kernel_type = SOPHIA_KNL_QUARTIC;
+      if (v == "Gaussian") p->kernel_type = SOPHIA_KNL_GAUSSIAN;
+      if (v == "Quintic") p->kernel_type = SOPHIA_KNL_QUINTIC;
+      if (v == "Wendland2") p->kernel_type = SOPHIA_KNL_WENDLAND2;
+      if (v == "Wendland4") p->kernel_type = SOPHIA_KNL_WENDLAND4;
+      if (v == "Wendland6") p->kernel_type = SOPHIA_KNL_WENDLAND6;
+    } else if (k == "density" && at(i + 1) == "calculation") {
+      std::string v = at(i + 3);
+      if (v == "Direct") p->rho_type = 0;
+      if (v == "Continuity") p->rho_type = 1;
+    } else if (k == "time" && at(i + 1) == "integration") {
+      std::string v = at(i + 3);
+      if (v == "Euler") p->time_type = 0;
+      if (v == "Predictor_Corrector") p->time_type = 1;
+    } else if (k == "open" && at(i + 1) == "boundary") {
+      std::string v = at(i + 3);
+      if (v == "OPEN") p->open_boundary = 1;
+      if (v == "PERIODIC") p->open_boundary = 2;
+      if (v == "NO") p->open_boundary = 0;
+    } else if (k == "pressure-force") {
+      p->fp_solve = yes(at(i + 2));
+    } else if (k == "viscous-force") {
+      p->fv_solve = yes(at(i + 2));
+    } else if (k == "artificial-viscous-force") {
+      p->fva_solve = yes(at(i + 2));
+    } else if (k == "gravitational-force") {
+      p->fg_solve = yes(at(i + 2));
+    } else if (k == "Conduction(YES/NO):") {
+      p->con_solve = yes(at(i + 1));
+    } else if (k == "XSPH") {
+      p->xsph_solve = yes(at(i + 2));
+    } else if (k == "kernel-gradient-correction") {
+      std::string v = at(i + 2);
+      p->kgc_solve = v == "KGC" ? 1 : (v == "FPM" ? 2 : (v == "DFPM" ? 3 : (v == "KGF" ? 4 : 0)));
+    } else if (k == "delta-SPH") {
+      std::string v = at(i + 2);
+      p->delSPH_solve = v == "Molteni" ? 1 : (v == "Antuono" ? 2 : 0);
+    } else if (k == "reference-pressure") {
+      p->p_ref = std::atof(at(i + 2).c_str());
+    } else if (k == "sound-speed:") {
+      p->soundspeed = std::atof(at(i + 1).c_str());
+    } else if (k == "reference-density-eos:") {
+      p->rho0_eos = std::atof(at(i + 1).c_str());
+    } else if (k == "gamma") {
+      p->gamma = std::atof(at(i + 3).c_str());
+    } else if (k == "kappa:") {
+      p->kappa = std::atof(at(i + 1).c_str());
+    } else if (k == "XSPH-coefficient:") {
+      p->c_xsph = std::atof(at(i + 1).c_str());
+    } else if (k == "minimum-iteration:") {
+      p->min_iteration = std::atoi(at(i + 1).c_str());
+    } else if (k == "maximum-iteration:") {
+      p->max_iteration = std::atoi(at(i + 1).c_str());
+    } else if (k == "pressure-convergence") {
+      p->dp_th = std::atof(at(i + 2).c_str());
+    } else if (k == "density-convergence") {
+      p->drho_th = std::atof(at(i + 2).c_str());
+    } else if (k == "pressure-relaxation") {
+      p->p_relaxation = std::atof(at(i + 2).c_str());
+    } else if (k == "time-step") {
+      p->dt = std::atof(at(i + 2).c_str());
+    } else if (k == "start-time") {
+      p->time = std::atof(at(i + 2).c_str());
+    } else if (k == "end-time") {
+      p->time_end = std::atof(at(i + 2).c_str());
+    } else if (k == "z-indexing") {
+      p->flag_z_index = yes(at(i + 2));
+    } else if (k == "filtering" && at(i + 1) == "frequency:") {
+      p->freq_filt = std::atoi(at(i + 2).c_str());
+    } else if (k == "plot-output") {
+      p->freq_output = std::atoi(at(i + 2).c_str());
+    } else if (k == "velocity-limit") {
+      p->u_limit = std::atof(at(i + 2).c_str());
+    } else if (k == "Xmargin(-):") {
+      p->xmargin_m = std::atof(at(i + 1).c_str());
+    } else if (k == "Xmargin(+):") {
+      p->xmargin_p = std::atof(at(i + 1).c_str());
+    } else if (k == "Ymargin(-):") {
+      p->ymargin_m = std::atof(at(i + 1).c_str());
+    } else if (k == "Ymargin(+):") {
+      p->ymargin_p = std::atof(at(i + 1).c_str());
+    } else if (k == "Zmargin(-):") {
+      p->zmargin_m = std::atof(at(i + 1).c_str());
+    } else if (k == "Zmargin(+):") {
+      p->zmargin_p = std::atof(at(i + 1).c_str());
+    } else if (k == "decouple-stride:") {
+      p->decouple_stride = std::atoi(at(i + 1).c_str());
+    }
+  }
+  return 0;
+}
+
+// read_input (function_init.cuh:743-891): first line = tab-separated column
+// codes, then values row by row; i_type = 1 for every particle read
+int sophia_read_input(const char* path, part1* out, int32_t capacity, int32_t* n) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    set_error("cannot open %s", path);
+    return SOPHIA_E_IO;
+  }
+  char line[1 << 16];
+  if (!std::fgets(line, sizeof(line), f)) {
+    std::fclose(f);
+    set_error("%s: empty file", path);
+    return SOPHIA_E_IO;
+  }
+  std::vector<int> cols;
+  for (char* t = std::strtok(line, "\t\r\n "); t; t = std::strtok(nullptr, "\t\r\n ")) cols.push_back(std::atoi(t));
+  if (cols.empty()) {
+    std::fclose(f);
+    set_error("%s: no column codes", path);
+    return SOPHIA_E_IO;
+  }
+  int count = 0;
+  char tokb[256];
+  bool done = false;
+  while (!done) {
+    part1 q;
+    std::memset(&q, 0, sizeof(q));
+    for (size_t c = 0; c < cols.size(); c++) {
+      if (std::fscanf(f, "%255s", tokb) != 1) {
+        done = true;
+        break;
+      }
+      double v = std::atof(tokb);
+      int iv = std::atoi(tokb);
+      switch (cols[c]) {
+        case 1: q.x = v; break;
+        case 2: q.y = v; break;
+        case 3: q.z = v; break;
+        case 4: q.ux = v; break;
+        case 5: q.uy = v; break;
+        case 6: q.uz = v; break;
+        case 7: q.m = v; break;
+        case 8: q.p_type = iv; break;
+        case 9: q.h = v; break;
+        case 10: q.temp = v; break;
+        case 11: q.pres = v; break;
+        case 12: q.rho = v; break;
+        case 15: q.concn = v; break;
+        case 18: q.ct_boundary = iv; break;
+        case 26: q.flt_s = v; break;
+        case 27: q.rad = v; break;
+        case 28: q.ri = v; break;
+        case 29: q.dem_idx = iv; break;
+        case 30: q.wx = v; break;
+        case 31: q.wy = v; break;
+        case 32: q.wz = v; break;
+        case 33: q.temp1 = v; break;
+        case 34: q.temp2 = v; break;
+        case 35: q.temp3 = v; break;
+        case 36: q.vol_power = v; break;
+        default: break;  // codes the reader ignores (13, 14, 16, 17, 19-25)
+      }
+    }
+    if (done) break;
+    q.i_type = 1;
+    if (out) {
+      if (count >= capacity) {
+        std::fclose(f);
+        set_error("%s: more than %d particles", path, capacity);
+        return SOPHIA_E_CAPACITY;
+      }
+      out[count] = q;
+    }
+    count++;
+  }
+  std::fclose(f);
+  *n = count;
+  return 0;
+}
+
+int sophia_read_ptype(const char* path, int32_t* out, int32_t capacity, int32_t* n) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    set_error("cannot open %s", path);
+    return SOPHIA_E_IO;
+  }
+  int v, c = 0;
+  while (std::fscanf(f, "%d", &v) == 1) {
+    if (out) {
+      if (c >= capacity) {
+        std::fclose(f);
+        set_error("%s: more than %d entries", path, capacity);
+        return SOPHIA_E_CAPACITY;
+      }
+      out[c] = v;
+    }
+    c++;
+  }
+  std::fclose(f);
+  *n = c;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// context lifetime
+// ---------------------------------------------------------------------------
+int sophia_create_from_arrays(const sophia_params* p, const part1* particles, int32_t n, int32_t device,
+                              int32_t precision, sophia_ctx** out) {
+  if (!p || !particles || n <= 0 || !out) {
+    set_error("create: null argument or n <= 0");
+    return SOPHIA_E_ARG;
+  }
+  if (p->dim != 2 && p->dim != 3) {
+    set_error("create: dim must be 2 or 3");
+    return SOPHIA_E_ARG;
+  }
+  if (p->kappa <= 0 || p->dt <= 0) {
+    set_error("create: kappa and dt must be positive");
+    return SOPHIA_E_ARG;
+  }
+  for (int i = 0; i < n; i++)
+    if (!(particles[i].h > 0)) {
+      set_error("create: particle %d has h <= 0", i);
+      return SOPHIA_E_ARG;
+    }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    set_error("create: no HIP device available (%s)", hipGetErrorString(e));
+    return SOPHIA_E_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("create: device %d out of range (%d devices)", device, ndev);
+    return SOPHIA_E_ARG;
+  }
+  sophia_ctx* c = nullptr;
+  int rc;
+  if (precision == SOPHIA_PREC_FP64) {
+    auto* x = new Ctx<double>();
+    x->device = device;
+    x->precision = precision;
+    rc = x->init(*p, particles, n);
+    c = x;
+  } else if (precision == SOPHIA_PREC_MIXED) {
+    auto* x = new Ctx<float>();
+    x->device = device;
+    x->precision = precision;
+    rc = x->init(*p, particles, n);
+    c = x;
+  } else {
+    set_error("create: unknown precision %d", precision);
+    return SOPHIA_E_ARG;
+  }
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int sophia_create(const char* solv_path, const char* input_path, int32_t device, int32_t precision,
+                  sophia_ctx** out) {
+  sophia_params p;
+  sophia_params_default(&p);
+  int rc = sophia_read_solv(solv_path, &p);
+  if (rc) return rc;
+  int32_t n = 0;
+  rc = sophia_read_input(input_path, nullptr, 0, &n);
+  if (rc) return rc;
+  std::vector<part1> v((size_t)n);
+  rc = sophia_read_input(input_path, v.data(), n, &n);
+  if (rc) return rc;
+  return sophia_create_from_arrays(&p, v.data(), n, device, precision, out);
+}
+
+int sophia_destroy(sophia_ctx* ctx) {
+  delete ctx;
+  return 0;
+}
+
+#define CHECK_CTX(c)                  \
+  do {                                \
+    if (!(c)) {                       \
+      set_error("null context");      \
+      return SOPHIA_E_ARG;            \
+    }                                 \
+    hipError_t e_ = hipSetDevice((c)->device); \
+    if (e_ != hipSuccess) return hip_fail(e_, "hipSetDevice", __FILE__, __LINE__); \
+  } while (0)
+
+int sophia_get_domain(const sophia_ctx* c, sophia_domain* d) {
+  if (!c || !d) return SOPHIA_E_ARG;
+  *d = c->d;
+  return 0;
+}
+int sophia_get_params(const sophia_ctx* c, sophia_params* p) {
+  if (!c || !p) return SOPHIA_E_ARG;
+  *p = c->p;
+  return 0;
+}
+int sophia_get_clock(const sophia_ctx* c, int64_t* count, double* time) {
+  if (!c) return SOPHIA_E_ARG;
+  if (count) *count = c->count;
+  if (time) *time = c->time;
+  return 0;
+}
+int sophia_sync(sophia_ctx* c) {
+  CHECK_CTX(c);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+int sophia_count(const sophia_ctx* c, int32_t species, int32_t* n) {
+  if (!c || !n) return SOPHIA_E_ARG;
+  *n = species == SOPHIA_SPH ? c->d.n_sph : c->d.n_dem;
+  return 0;
+}
+
+int sophia_nnps(sophia_ctx* c) { CHECK_CTX(c); return c->nnps(); }
+int sophia_prep(sophia_ctx* c) { CHECK_CTX(c); return c->prep(); }
+int sophia_interaction(sophia_ctx* c) { CHECK_CTX(c); return c->interaction(); }
+int sophia_dem_interaction(sophia_ctx* c) { CHECK_CTX(c); return c->dem_interaction(); }
+int sophia_sph_dem_coupling(sophia_ctx* c) { CHECK_CTX(c); return c->coupling(); }
+int sophia_time(sophia_ctx* c, int32_t phase) {
+  CHECK_CTX(c);
+  if (phase != SOPHIA_PREDICT && phase != SOPHIA_CORRECT) {
+    set_error("time: phase must be SOPHIA_PREDICT or SOPHIA_CORRECT");
+    return SOPHIA_E_ARG;
+  }
+  return c->time_stage(phase);
+}
+int sophia_ppe(sophia_ctx* c, const sophia_ppe_opts* o, sophia_ppe_stats* st) { CHECK_CTX(c); return c->ppe(o, st); }
+int sophia_xsph(sophia_ctx* c) { CHECK_CTX(c); return c->xsph(); }
+
+int sophia_step(sophia_ctx* c, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
+  CHECK_CTX(c);
+  TRY(c->nnps());
+  TRY(c->prep());
+  TRY(c->interaction());
+  TRY(c->dem_interaction());
+  TRY(c->coupling());
+  TRY(c->time_stage(SOPHIA_PREDICT));
+  TRY(c->ppe(o, st));
+  TRY(c->time_stage(SOPHIA_CORRECT));
+  TRY(c->xsph());
+  c->time += c->p.dt;
+  c->count += 1;
+  return 0;
+}
+
+int sophia_run(sophia_ctx* c, int32_t n, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
+  for (int i = 0; i < n; i++) TRY(sophia_step(c, o, st));
+  return 0;
+}
+
+int sophia_get_nnps(const sophia_ctx* c, int32_t species, int32_t* keys, int32_t* order, uint32_t* cs,
+                    uint32_t* ce, int32_t* nc) {
+  if (!c) return SOPHIA_E_ARG;
+  return c->get_nnps(species, keys, order, cs, ce, nc);
+}
+int sophia_get_particles(const sophia_ctx* c, int32_t species, part1* p1, part2* p2, part3* p3) {
+  if (!c) return SOPHIA_E_ARG;
+  return c->get_particles(species, p1, p2, p3);
+}
+int sophia_set_pressure(sophia_ctx* c, const double* p, int32_t n) { CHECK_CTX(c); return c->set_pressure(p, n); }
+int sophia_get_ppe_system(const sophia_ctx* c, double* diag, double* rhs, int32_t* row) {
+  if (!c) return SOPHIA_E_ARG;
+  return c->get_ppe_system(diag, rhs, row);
+}
+int sophia_ppe_apply(sophia_ctx* c, const double* x, double* y, int32_t n) { CHECK_CTX(c); return c->ppe_apply(x, y, n); }
+
+int sophia_knl_eval(int32_t kt, int32_t dim, double h, const double* r, int32_t n, double* w, double* dw) {
+  if (!r || !w || !dw || n < 0) return SOPHIA_E_ARG;
+  if (n == 0) return 0;
+  double *dr = nullptr, *dwv = nullptr, *ddw = nullptr;
+  HIP_TRY(hipMalloc(&dr, (size_t)n * 8));
+  HIP_TRY(hipMalloc(&dwv, (size_t)n * 8));
+  HIP_TRY(hipMalloc(&ddw, (size_t)n * 8));
+  int rc = 0;
+  hipError_t e = hipMemcpy(dr, r, (size_t)n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    rc = knl_eval_device(nullptr, kt, dim, h, dr, n, dwv, ddw);
+    if (!rc) {
+      e = hipMemcpy(w, dwv, (size_t)n * 8, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(dw, ddw, (size_t)n * 8, hipMemcpyDeviceToHost);
+    }
+  }
+  if (e != hipSuccess) rc = hip_fail(e, "knl_eval copy", __FILE__, __LINE__);
+  (void)hipFree(dr);
+  (void)hipFree(dwv);
+  (void)hipFree(ddw);
+  return rc;
+}
+
+// learned warm start
+int sophia_mlp_param_count(int32_t* n) {
+  if (!n) return SOPHIA_E_ARG;
+  *n = mlp_param_count();
+  return 0;
+}
+int sophia_mlp_set_params(sophia_ctx* c, const float* w, int32_t n) { CHECK_CTX(c); return c->mlp_set(w, n); }
+int sophia_mlp_get_params(const sophia_ctx* c, float* w, int32_t n) {
+  if (!c) return SOPHIA_E_ARG;
+  return c->mlp_get(w, n);
+}
+int sophia_mlp_predict(sophia_ctx* c, double* out, int32_t n) { CHECK_CTX(c); return c->mlp_predict(out, n); }
+int sophia_mlp_train_step(sophia_ctx* c, const double* target, int32_t n, double lr, double* loss) {
+  CHECK_CTX(c);
+  return c->mlp_train(target, n, lr, loss, nullptr, 0, true);
+}
+int sophia_mlp_loss_grad(sophia_ctx* c, const double* target, int32_t n, double* loss, float* grad, int32_t np) {
+  CHECK_CTX(c);
+  return c->mlp_train(target, n, 0.0, loss, grad, np, false);
+}
+int sophia_mlp_reset_adam(sophia_ctx* c) { CHECK_CTX(c); return c->mlp_reset(); }
+
+int sophia_timers_enable(sophia_ctx* c, int32_t on) {
+  CHECK_CTX(c);
+  c->timers.reset();
+  c->timers.on = on != 0;
+  return 0;
+}
+
+int sophia_timers_report(sophia_ctx* c, char* buf, int32_t len) {
+  CHECK_CTX(c);
+  if (!buf || len <= 0) return SOPHIA_E_ARG;
+  c->timers.collect();
+  std::string out;
+  char line[256];
+  for (auto& kv : c->timers.acc) {
+    std::snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    out += line;
+  }
+  if ((int)out.size() + 1 > len) {
+    set_error("timers_report: buffer of %d bytes too small (%zu needed)", len, out.size() + 1);
+    return SOPHIA_E_CAPACITY;
+  }
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return 0;
+}
+
+}  // extern "C"

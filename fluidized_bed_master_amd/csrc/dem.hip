@@ -1,0 +1,424 @@
+// DEM and SPH-DEM coupling kernels on gfx950.
+//
+// DEM spheres are a few percent of the particles; their kernels are
+// latency bound, run in FP64 on every path, and walk the 27 neighbour cells
+// of the other species directly (no list).  The shear history is 16 slots
+// per sphere in slot-major SoA streams (slot k of 64 consecutive spheres is
+// contiguous), read from the start-of-step copy and written to a second
+// buffer, which is the reference's SP1 -> TP1 split
+// (ISPH_Calc_seperate.cuh:218).
+#include "common.hpp"
+
+namespace sophia {
+
+namespace {
+
+// KERNEL_clc_prep3D_coupling_dem (functions_PREP2.cuh:3367-3523): fluid
+// Shepard sums seen by each sphere and the sphere-phase porosity.
+template <typename R>
+__global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc, CellView dc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  if (d.itype[i] > 2) return;
+  const int pti = d.ptype[i];
+  const double h = d.h[i], A = knl_tmpA<double>(P.kernel_type, P.dim, h), Rs = P.kappa * h;
+  const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
+  double fltd = 0, fltd2 = 0, flt = 0, por = 0;
+  if (v.n > 0) {
+    for_cells(P, xi, yi, zi, sc, [&](int j) {
+      double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+      if (!(r < Rs)) return;
+      if (!(pti > 1000)) return;
+      int t = v.ptype[j];
+      if (t != 1 && t != 3) return;
+      double w = knl_w<double>(P.kernel_type, A, h, r);
+      double c = (double)v.m[j] / (double)v.rho[j] * w;
+      if (t == 1) {
+        fltd += c;
+        fltd2 += c;
+      } else {
+        flt += c;
+      }
+    });
+  }
+  for_cells(P, xi, yi, zi, dc, [&](int j) {
+    double r = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]) + 1e-20;
+    if (!(r < Rs)) return;
+    if (pti > 1000 && d.ptype[j] > 1000) {
+      double rj = d.rad[j];
+      por += knl_w<double>(P.kernel_type, A, h, r) * 4.18879 * rj * rj * rj;
+    }
+  });
+  if (pti > 1000) {
+    d.flt_s[i] = flt;
+    d.flt_sd[i] = fltd;
+    d.flt_sd2[i] = fltd2;
+    d.por[i] = 1.0 - (por / (flt + fltd + 1.0e-20));
+  }
+}
+
+// KERNEL_DEM_interaction3D_dem (functions_DEM_INTERACTION2.cuh:888-1766):
+// Hertz normal spring + damping, Mindlin tangential spring + damping with a
+// Coulomb cap, torque, gravity; shear displacement history keyed by dem_idx.
+__global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int n = d.n, S = P.dem_slots;
+  // the new history starts as a copy of the old one
+  for (int k = 0; k < S; k++) {
+    d.hist_idx_new[k * n + i] = d.hist_idx[k * n + i];
+    for (int c = 0; c < 3; c++) d.hist_ds_new[(k * 3 + c) * n + i] = d.hist_ds[(k * 3 + c) * n + i];
+  }
+  if (d.itype[i] != 1) return;
+  if (d.ptype[i] <= 1000) return;
+  const double xi = d.x[i], yi = d.y[i], zi = d.z[i], uxi = d.ux[i], uyi = d.uy[i], uzi = d.uz[i];
+  const double wxi = d.wx[i], wyi = d.wy[i], wzi = d.wz[i], radi = d.rad[i], mi = d.m[i], rii = d.ri[i];
+  const double G = P.dem_G, nu = P.dem_nu, E = G * (2 * (1 + nu));
+  double tx = 0, ty = 0, tz = 0, qx = 0, qy = 0, qz = 0;
+  for_cells(P, xi, yi, zi, dc, [&](int j) {
+    if (!(d.ptype[j] > 1000)) return;
+    const double radj = d.rad[j];
+    const int didx = d.dem_idx[j];
+    const double dist = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]);
+    const double ovlp = radi + radj - dist;
+    if (!(dist < 3.5 * radi && dist > 0.0)) return;
+    if (!(dist < 2.01 * radi)) return;
+    if (ovlp > 0) {
+      double dsx = 0, dsy = 0, dsz = 0;
+      int slot = -1;
+      for (int k = 0; k < S; k++)
+        if (d.hist_idx[k * n + i] == didx) {
+          dsx = d.hist_ds[(k * 3 + 0) * n + i];
+          dsy = d.hist_ds[(k * 3 + 1) * n + i];
+          dsz = d.hist_ds[(k * 3 + 2) * n + i];
+          slot = k;
+          break;
+        }
+      if (slot < 0)
+        for (int k = 0; k < S; k++)
+          if (d.hist_idx[k * n + i] < 1) {
+            slot = k;
+            break;
+          }
+      const double nx = (d.x[j] - xi) / dist, ny = (d.y[j] - yi) / dist, nz = (d.z[j] - zi) / dist;
+      const double ov = ovlp;
+      const double ai = radi - 0.5 * ov, aj = -(radj - 0.5 * ov);
+      const double rcxi = ai * nx, rcyi = ai * ny, rczi = ai * nz;
+      const double rcxj = aj * nx, rcyj = aj * ny, rczj = aj * nz;
+      const double wxj = d.wx[j], wyj = d.wy[j], wzj = d.wz[j];
+      const double urx = (d.ux[j] + (wyj * rczj - wzj * rcyj)) - (uxi + (wyi * rczi - wzi * rcyi));
+      const double ury = (d.uy[j] + (wzj * rcxj - wxj * rczj)) - (uyi + (wzi * rcxi - wxi * rczi));
+      const double urz = (d.uz[j] + (wxj * rcyj - wyj * rcxj)) - (uzi + (wxi * rcyi - wyi * rcxi));
+      const double un = nx * urx + ny * ury + nz * urz;
+      const double unx = un * nx, uny = un * ny, unz = un * nz;
+      const double usx = urx - unx, usy = ury - uny, usz = urz - unz;
+      dsx += usx * P.dt_dem;
+      dsy += usy * P.dt_dem;
+      dsz += usz * P.dt_dem;
+      const double usm = sqrt(usx * usx + usy * usy + usz * usz) + 1e-20;
+      const double sx = usx / usm, sy = usy / usm, sz = usz / usm;
+      const double ymij = 1 / (((1 - nu * nu) / E) + ((1 - nu * nu) / E));
+      const double Gij = 1 / (((2 - nu) / G) + ((2 - nu) / G));
+      const double radij = 1 / (1 / radi + 1 / radj);
+      const double mij = 1 / (1 / mi + 1 / d.m[j]);
+      const double Cspr = P.dem_hertz * ymij * sqrt(radij) * sqrt(ov) * ov / mi;
+      const double Csprs = 8.0 * Gij * sqrt(radij * ov) / mi;
+      const double Kn = 2 * ymij * sqrt(radij * ov);
+      const double Cd = P.dem_damping * sqrt(mij * Kn) / mi;
+      const double Cds = P.dem_damping * sqrt(mij * Csprs * mi) / mi;
+      tx += Cspr * (xi - d.x[j]) / dist;
+      ty += Cspr * (yi - d.y[j]) / dist;
+      tz += Cspr * (zi - d.z[j]) / dist;
+      tx += Cd * unx;
+      ty += Cd * uny;
+      tz += Cd * unz;
+      const double fssx = Csprs * dsx + Cds * usx, fssy = Csprs * dsy + Cds * usy, fssz = Csprs * dsz + Cds * usz;
+      const double mfss = sqrt(fssx * fssx + fssy * fssy + fssz * fssz);
+      const double ffx = P.dem_friction * Cspr * sx, ffy = P.dem_friction * Cspr * sy, ffz = P.dem_friction * Cspr * sz;
+      const double mfsf = sqrt(ffx * ffx + ffy * ffy + ffz * ffz);
+      const bool slip = !(mfss < mfsf);
+      const double fsx = slip ? (fssx >= 0 ? 1.0 : -1.0) * fabs(ffx) : fssx;
+      const double fsy = slip ? (fssy >= 0 ? 1.0 : -1.0) * fabs(ffy) : fssy;
+      const double fsz = slip ? (fssz >= 0 ? 1.0 : -1.0) * fabs(ffz) : fssz;
+      tx += fsx;
+      ty += fsy;
+      tz += fsz;
+      qx += mi * (rcyi * fsz - rczi * fsy);
+      qy += mi * (rczi * fsx - rcxi * fsz);
+      qz += mi * (rcxi * fsy - rcyi * fsx);
+      if (slot >= 0) {
+        d.hist_idx_new[slot * n + i] = didx;
+        d.hist_ds_new[(slot * 3 + 0) * n + i] = dsx;
+        d.hist_ds_new[(slot * 3 + 1) * n + i] = dsy;
+        d.hist_ds_new[(slot * 3 + 2) * n + i] = dsz;
+      }
+    } else {
+      for (int k = 0; k < S; k++)
+        if (d.hist_idx[k * n + i] == didx) {
+          d.hist_idx_new[k * n + i] = 0;
+          for (int c = 0; c < 3; c++) d.hist_ds_new[(k * 3 + c) * n + i] = 0.0;
+          break;
+        }
+    }
+  });
+  if (P.fg_solve) tz += -P.gravity;
+  d.fx[i] = tx;
+  d.fy[i] = ty;
+  d.fz[i] = tz;
+  d.tqx[i] = qx / rii;
+  d.tqy[i] = qy / rii;
+  d.tqz[i] = qz / rii;
+}
+
+// KERNEL_DEM_coupling3D_dem (functions_SPH_DEM_COUPLING2.cuh:627-1129):
+// kernel-averaged fluid velocity and porosity around each sphere, Di Felice
+// drag (fluid regime) and the pressure-gradient force.
+template <typename R>
+__global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  if (d.itype[i] != 1 || d.ptype[i] <= 1000) return;
+  const double xi = d.x[i], yi = d.y[i], zi = d.z[i], uxi = d.ux[i], uyi = d.uy[i], uzi = d.uz[i];
+  const double radi = d.rad[i], di = 2 * radi, voli = (4.0 / 3.0) * 3.14159 * radi * radi * radi;
+  const double mi = d.m[i], flt_air = d.flt_s[i], flt_fluid = d.flt_sd[i];
+  const double h = d.h[i], A = knl_tmpA<double>(P.kernel_type, P.dim, h), Rs = P.kappa * h;
+  double tx = 0, ty = 0, tz = 0, ufx = 0, ufy = 0, ufz = 0, uax = 0, uay = 0, uaz = 0, por = 0;
+  if (v.n > 0) {
+    for_cells(P, xi, yi, zi, sc, [&](int j) {
+      const int t = v.ptype[j];
+      const double porj = (double)v.por[j];
+      if (!(t <= 1000 && porj < 1.0)) return;
+      double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]);
+      if (!(r < Rs)) return;
+      double w = knl_w<double>(P.kernel_type, A, h, r);
+      double mj = v.m[j], rhoj = v.rho[j];
+      if (t == 1 && flt_fluid > 0.000001) {
+        ufx += (double)v.ux[j] * mj * w / rhoj;
+        ufy += (double)v.uy[j] * mj * w / rhoj;
+        ufz += (double)v.uz[j] * mj * w / rhoj;
+        por += porj * mj * w / rhoj;
+      }
+      if (t == 3 && flt_air > 0.000001) {
+        uax += (double)v.ux[j] * mj * w / rhoj;
+        uay += (double)v.uy[j] * mj * w / rhoj;
+        uaz += (double)v.uz[j] * mj * w / rhoj;
+        por += porj * mj * w / rhoj;
+      }
+      if (t != 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 3 && flt_fluid > 0.01) {
+        double c = (mj / rhoj) * w * voli / (mi * flt_fluid + 1.0e-15);
+        tx += (double)v.pgx[j] * c;
+        ty += (double)v.pgy[j] * c;
+        tz += (double)v.pgz[j] * c;
+      }
+    });
+  }
+  const double pori = por / (flt_fluid + 1.0e-15);
+  const double sf = flt_fluid + 1.0e-15, sa = flt_air + 1.0e-15, sm = flt_fluid + flt_air + 1.0e-15;
+  const double rfx = ufx / sf - uxi, rfy = ufy / sf - uyi, rfz = ufz / sf - uzi;
+  const double rax = uax / sa - uxi, ray = uay / sa - uyi, raz = uaz / sa - uzi;
+  const double rmx = (ufx + uax) / sm - uxi, rmy = (ufy + uay) / sm - uyi, rmz = (ufz + uaz) / sm - uzi;
+  const double f_fluid = flt_fluid / sm, f_air = flt_air / sm;
+  const double magf = sqrt(rfx * rfx + rfy * rfy + rfz * rfz);
+  const double maga = sqrt(rax * rax + ray * ray + raz * raz);
+  const double magm = sqrt(rmx * rmx + rmy * rmy + rmz * rmz);
+  d.test6[i] = magf;
+  d.por[i] = pori;
+  const double nu_f = P.mu_fluid / P.rho_fluid;
+  double dfx = 0, dfy = 0, dfz = 0, dax = 0, day = 0, daz = 0;
+  const bool any = (flt_fluid + flt_air) > 0.00001;
+  if (magf > 1.0e-8 && any && f_fluid > P.drag_switch) {
+    double Re = di * magf * pori / (nu_f + 1.0e-15);
+    double sq = 0.63 + 4.8 / sqrt(Re);
+    double Cd = sq * sq;
+    double lg = 1.5 - log10(Re);
+    double beta = 3.7 - 0.65 * exp(-lg * lg / 2.0);
+    double k = (1.0 / 8.0) * Cd * P.rho_fluid * 3.14159 * di * di;
+    double e = pow(pori, 2 - beta);
+    dfx = k * rfx * magf * e / mi;
+    dfy = k * rfy * magf * e / mi;
+    dfz = k * rfz * magf * e / mi;
+    d.fdx[i] = dfx;
+    d.fdy[i] = dfy;
+    d.fdz[i] = dfz;
+  }
+  if (maga > 1.0e-8 && any && f_air > P.drag_switch) {
+    double Re = di * pori * maga / (nu_f + 1.0e-15);
+    double sq = 0.63 + 4.8 / sqrt(Re);
+    double lg = 1.5 - log10(Re);
+    double beta = 2.65 * (1.0 + pori) - (5.3 - 3.5 * pori) * pori * pori * exp(-lg * lg / 2.0);
+    double k = (1.0 / 8.0) * sq * sq * P.rho_fluid * 3.14159 * di * di;
+    double e = pow(pori, -1 - beta);
+    dax = k * rax * maga * e / mi;
+    day = k * ray * maga * e / mi;
+    daz = k * raz * maga * e / mi;
+  }
+  if ((maga > 1.0e-8 || magf > 1.0e-8) && any && f_air <= P.drag_switch && f_fluid <= P.drag_switch) {
+    double rho_m = P.rho_fluid * f_air + P.rho_water * f_fluid;
+    double vis_m = P.mu_fluid * f_air + P.mu_water * f_fluid;
+    double nu_m = vis_m / (rho_m + 1.0e-15);
+    double Re = di * pori * magm / (nu_m + 1.0e-15);
+    double sq = 0.63 + 4.8 / sqrt(Re);
+    double lg = 1.5 - log10(Re);
+    double beta = 2.65 * (1.0 + pori) - (5.3 - 3.5 * pori) * pori * pori * exp(-lg * lg / 2.0);
+    double k = (1.0 / 8.0) * sq * sq * rho_m * 3.14159 * di * di;
+    double e = pow(pori, -1 - beta);
+    dax = k * rmx * magm * e / mi;
+    day = k * rmy * magm * e / mi;
+    daz = k * rmz * magm * e / mi;
+  }
+  d.fx[i] = d.fx[i] + dfx / pori + tx;
+  d.fy[i] = d.fy[i] + dfy / pori + ty;
+  d.fz[i] = d.fz[i] + dfz / pori + tz;
+  d.fax[i] = dax;
+  d.fay[i] = day;
+  d.faz[i] = daz;
+  d.fbx[i] = tx;
+  d.fby[i] = ty;
+  d.fbz[i] = tz;
+}
+
+// KERNEL_SPH_coupling3D_sph (functions_SPH_DEM_COUPLING2.cuh:1815-2019):
+// drag reaction distributed onto the fluid with the sphere's Shepard sum.
+template <typename R>
+__global__ void k_sph_reaction(DevParams P, SphView<R> v, DemView d, CellView dc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int t = v.ptype[i];
+  if (v.itype[i] != 1 || t > 1000) return;
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  const double hd = v.h[i], Rs = P.kappa * hd;
+  double tx = 0, ty = 0, tz = 0, tt = 0;
+  bool hit = false;
+  if (d.n > 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 0 && (t == 1 || t == 3)) {
+    const double A = knl_tmpA<double>(P.kernel_type, P.dim, hd);
+    const double rhoi = v.rho[i];
+    const double cpi = t > 1000 ? P.cp_dem : P.cp_fluid;
+    for_cells(P, xi, yi, zi, dc, [&](int j) {
+      if (!(d.ptype[j] > 1000)) return;
+      double r = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]);
+      if (!(r < Rs)) return;
+      double flt = d.flt_sd2[j] + d.flt_s[j];
+      if (!(flt > 0.0)) return;
+      double w = knl_w<double>(P.kernel_type, A, hd, r);
+      double c = w / (flt + 1.0e-20);
+      tx += -(d.m[j] / rhoi) * d.fdx[j] * c;
+      ty += -(d.m[j] / rhoi) * d.fdy[j] * c;
+      tz += -(d.m[j] / rhoi) * d.fdz[j] * c;
+      tt += -d.qsd[j] / (rhoi * cpi) * c;
+      hit = true;
+    });
+  }
+  if (!hit) {
+    // sum of nothing: ftotal and its magnitude are unchanged
+    R fx = v.fx[i], fy = v.fy[i], fz = v.fz[i];
+    v.fmag[i] = sqrt(fx * fx + fy * fy + fz * fz);
+    return;
+  }
+  R fx = (R)((double)v.fx[i] + tx), fy = (R)((double)v.fy[i] + ty), fz = (R)((double)v.fz[i] + tz);
+  v.fx[i] = fx;
+  v.fy[i] = fy;
+  v.fz[i] = fz;
+  v.fmag[i] = sqrt(fx * fx + fy * fy + fz * fz);
+  if (P.con_solve == 1) v.dtemp[i] = (R)((double)v.dtemp[i] + tt);
+}
+
+// KERNEL_clc_projection_dem (functions_TIME_ISPH2.cuh:214-308)
+__global__ void k_dem_predictor(DevParams P, DemView d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  if (d.itype[i] > 2) return;
+  const double ux0 = d.ux[i], uy0 = d.uy[i], uz0 = d.uz[i];
+  if (d.ptype[i] >= 0) {
+    const double bz = (d.btype[i] == 0);
+    d.ux[i] = ux0 + d.fx[i] * bz * P.dt_dem;
+    d.uy[i] = uy0 + d.fy[i] * bz * P.dt_dem;
+    d.uz[i] = uz0 + d.fz[i] * bz * P.dt_dem;
+  }
+  d.ux0[i] = ux0;
+  d.uy0[i] = uy0;
+  d.uz0[i] = uz0;
+}
+
+// KERNEL_time_update_projection_dem (functions_TIME_ISPH2.cuh:624-776):
+// spheres carry no pressure acceleration, so u = u*; x from the average.
+__global__ void k_dem_corrector(DevParams P, DemView d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  if (d.itype[i] > 2) return;
+  const int t = d.ptype[i];
+  double ux0, uy0, uz0;
+  if (t > 0) {
+    ux0 = d.ux[i]; uy0 = d.uy[i]; uz0 = d.uz[i];
+  } else {
+    ux0 = d.ux0[i]; uy0 = d.uy0[i]; uz0 = d.uz0[i];
+  }
+  double uxc = ux0, uyc = uy0, uzc = uz0;
+  if ((uxc * uxc + uyc * uyc + uzc * uzc) >= P.u_limit * P.u_limit) {
+    uxc = ux0; uyc = uy0; uzc = uz0;
+  }
+  const double el = t > 0 ? 1.0 : 0.0;
+  d.x[i] = d.x[i] + (d.ux0[i] + uxc) / 2.0 * P.dt_dem * el;
+  d.y[i] = d.y[i] + (d.uy0[i] + uyc) / 2.0 * P.dt_dem * el;
+  d.z[i] = d.z[i] + (d.uz0[i] + uzc) / 2.0 * P.dt_dem * el;
+  d.ux[i] = uxc;
+  d.uy[i] = uyc;
+  d.uz[i] = uzc;
+}
+
+}  // namespace
+
+template <typename R>
+int dem_porosity(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc,
+                 const CellView& dc) {
+  if (d.n <= 0) return 0;
+  k_porosity_dem<R><<<grid_for(d.n), kBlock, 0, s>>>(P, d, v, sc, dc);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int dem_contact(hipStream_t s, const DevParams& P, const DemView& d, const CellView& dc) {
+  if (d.n <= 0) return 0;
+  k_dem_contact<<<grid_for(d.n), kBlock, 0, s>>>(P, d, dc);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int dem_coupling(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc) {
+  if (d.n <= 0) return 0;
+  k_dem_coupling<R><<<grid_for(d.n), kBlock, 0, s>>>(P, d, v, sc);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int sph_dem_reaction(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dc) {
+  if (v.n <= 0) return 0;
+  k_sph_reaction<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v, d, dc);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d) {
+  if (d.n <= 0) return 0;
+  k_dem_predictor<<<grid_for(d.n), kBlock, 0, s>>>(P, d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d) {
+  if (d.n <= 0) return 0;
+  k_dem_corrector<<<grid_for(d.n), kBlock, 0, s>>>(P, d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+#define INST(R)                                                                                                    \
+  template int dem_porosity<R>(hipStream_t, const DevParams&, const DemView&, const SphView<R>&, const CellView&,  \
+                               const CellView&);                                                                   \
+  template int dem_coupling<R>(hipStream_t, const DevParams&, const DemView&, const SphView<R>&, const CellView&); \
+  template int sph_dem_reaction<R>(hipStream_t, const DevParams&, const SphView<R>&, const DemView&, const CellView&);
+INST(float)
+INST(double)
+
+}  // namespace sophia

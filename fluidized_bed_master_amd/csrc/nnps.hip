@@ -1,0 +1,294 @@
+// NNPS on gfx950: cell keys, stable LSD radix sort, cell ranges, SoA
+// permutation, and the SPH-SPH neighbour list.
+//
+// Reference: KERNEL_index_particle_to_cell_sph/_dem (functions_NNPS.cuh:
+// 762-821), cub::DeviceRadixSort::SortPairs (ISPH_Calc_seperate.cuh:82-85),
+// KERNEL_reorder_sph/_dem (functions_NNPS.cuh:670-730).
+//
+// The sort is a stable least-significant-digit radix sort over 8-bit digits
+// and only the bits the largest key needs (2-3 passes for 10^5-10^6 cells).
+// Each pass is three launches: per-tile digit histograms, one scan of the
+// digit-major histogram table, and a stable scatter in which every wave64
+// ranks its lanes by digit with eight ballots (lanes with equal digits get
+// consecutive slots in lane order) and the four waves of a workgroup are
+// ordered through an LDS prefix.  Stability makes the result equal to the
+// reference's SortPairs of (key, identity index).
+#include "common.hpp"
+
+namespace sophia {
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kTile = kSortThreads * kSortItems;  // 4096 keys per workgroup
+
+__global__ void k_keys(DevParams P, int n, const double* __restrict__ x, const double* __restrict__ y,
+                       const double* __restrict__ z, const int* __restrict__ itype, uint32_t* __restrict__ keys,
+                       uint32_t* __restrict__ vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t key;
+  if (itype[i] == 3) {
+    key = (uint32_t)(P.num_cells - 1);  // dummy particles go to the spare slot
+  } else {
+    int a, b, c;
+    cell_of(P, x[i], y[i], z[i], a, b, c);
+    key = (uint32_t)cell_key(P, a, b, c);
+  }
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_hist(int n, int shift, const uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ hist, int ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * kTile;
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    int i = base + r * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of m entries by one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ a, int m) {
+  __shared__ uint32_t part[1024];
+  int t = threadIdx.x;
+  int per = (m + 1023) / 1024;
+  int lo = t * per, hi = min(lo + per, m);
+  uint32_t s = 0;
+  for (int i = lo; i < hi; i++) s += a[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;  // exclusive prefix of this thread's chunk
+  for (int i = lo; i < hi; i++) {
+    uint32_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scatter(int n, int shift, const uint32_t* __restrict__ kin,
+                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                          uint32_t* __restrict__ vout, const uint32_t* __restrict__ hist,
+                                                          int ntiles) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[kSortThreads / 64][256];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  base[t] = hist[t * ntiles + blockIdx.x];
+  for (int w = 0; w < kSortThreads / 64; w++) wcnt[w][t] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int tile0 = blockIdx.x * kTile;
+  for (int r = 0; r < kSortItems; r++) {
+    int i = tile0 + r * kSortThreads + t;
+    bool valid = i < n;
+    uint32_t key = valid ? kin[i] : 0u;
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      bool bit = (d >> b) & 1u;
+      uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    uint32_t cnt = (uint32_t)__popcll(peers);
+    bool leader = valid && rank == 0;
+    if (leader) wcnt[wave][d] = cnt;
+    __syncthreads();
+    // exclusive prefix over waves for every digit, then advance the tile base
+    {
+      uint32_t run = base[t];
+      for (int w = 0; w < kSortThreads / 64; w++) {
+        uint32_t c = wcnt[w][t];
+        wcnt[w][t] = run;
+        run += c;
+      }
+      base[t] = run;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = wcnt[wave][d] + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    for (int w = 0; w < kSortThreads / 64; w++) wcnt[w][t] = 0;
+    __syncthreads();
+  }
+}
+
+__global__ void k_ranges(int n, const uint32_t* __restrict__ keys, uint32_t* __restrict__ start,
+                         uint32_t* __restrict__ end) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) start[k] = (uint32_t)i;
+  if (i == n - 1 || keys[i + 1] != k) end[k] = (uint32_t)(i + 1);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+constexpr int kMaxPermute = 48;
+struct PermuteArgs {
+  PermuteField f[kMaxPermute];
+  int nf;
+};
+
+__global__ void k_permute(int n, const uint32_t* __restrict__ order, PermuteArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t src = order[i];
+  for (int k = 0; k < a.nf; k++) {
+    if (a.f[k].bytes == 8)
+      ((uint64_t*)a.f[k].dst)[i] = ((const uint64_t*)a.f[k].src)[src];
+    else
+      ((uint32_t*)a.f[k].dst)[i] = ((const uint32_t*)a.f[k].src)[src];
+  }
+}
+
+// SPH-SPH neighbour list within kappa*h_i over the 27 cells, in the
+// reference's visiting order.  Entries that pass sqrt(r2) < R but fail
+// sqrt(r2)+1e-20 < R carry kNbrFlag (those stages that use the +1e-20
+// test skip them).
+template <typename R>
+__global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = 0;
+  if (i < v.n) {
+    const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+    const double Rs = P.kappa * v.h[i];
+    const int n = v.n, cap = v.nbr_cap;
+    for_cells(P, xi, yi, zi, cv, [&](int j) {
+      double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]);
+      if (r < Rs) {
+        uint32_t e = (uint32_t)j;
+        if (!(r + 1e-20 < Rs)) e |= kNbrFlag;
+        if (c < cap) v.nbr[(size_t)c * n + i] = e;
+        c++;
+      }
+    });
+    v.nbr_cnt[i] = c;
+  }
+  // one atomic per wave for the largest count (capacity check on the host)
+  for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxcnt, c);
+}
+
+__global__ void k_knl(int kt, int dim, double h, const double* r, int n, double* w, double* dw) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double A = knl_tmpA<double>(kt, dim, h);
+  w[i] = knl_w<double>(kt, A, h, r[i]);
+  dw[i] = knl_dw<double>(kt, A, h, r[i]);
+}
+
+}  // namespace
+
+int nnps_keys(hipStream_t s, const DevParams& P, int n, const double* x, const double* y, const double* z,
+              const int* itype, uint32_t* keys, uint32_t* vals) {
+  if (n <= 0) return 0;
+  k_keys<<<grid_for(n), kBlock, 0, s>>>(P, n, x, y, z, itype, keys, vals);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int radix_sort_pairs(hipStream_t s, int n, int nbits, SortScratch& sc, uint32_t** keys_out, uint32_t** vals_out) {
+  uint32_t *ka = sc.keys_a, *kb = sc.keys_b, *va = sc.vals_a, *vb = sc.vals_b;
+  if (n > 0) {
+    int ntiles = (n + kTile - 1) / kTile;
+    if (ntiles > sc.max_tiles) {
+      set_error("radix sort: %d tiles exceed scratch capacity %d", ntiles, sc.max_tiles);
+      return SOPHIA_E_CAPACITY;
+    }
+    for (int shift = 0; shift < nbits; shift += 8) {
+      k_hist<<<ntiles, kSortThreads, 0, s>>>(n, shift, ka, sc.hist, ntiles);
+      k_scan<<<1, 1024, 0, s>>>(sc.hist, 256 * ntiles);
+      k_scatter<<<ntiles, kSortThreads, 0, s>>>(n, shift, ka, va, kb, vb, sc.hist, ntiles);
+      HIP_TRY(hipGetLastError());
+      uint32_t* t = ka; ka = kb; kb = t;
+      t = va; va = vb; vb = t;
+    }
+  }
+  *keys_out = ka;
+  *vals_out = va;
+  return 0;
+}
+
+int cell_ranges(hipStream_t s, int n, int num_cells, const uint32_t* keys, uint32_t* start, uint32_t* end) {
+  int rc = fill_u32(s, start, kEmpty, (size_t)num_cells);
+  if (rc) return rc;
+  rc = fill_u32(s, end, 0u, (size_t)num_cells);
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  k_ranges<<<grid_for(n), kBlock, 0, s>>>(n, keys, start, end);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int permute_fields(hipStream_t s, int n, const uint32_t* order, const PermuteField* fields, int nf) {
+  if (n <= 0) return 0;
+  for (int off = 0; off < nf; off += kMaxPermute) {
+    PermuteArgs a;
+    a.nf = nf - off < kMaxPermute ? nf - off : kMaxPermute;
+    for (int k = 0; k < a.nf; k++) a.f[k] = fields[off + k];
+    k_permute<<<grid_for(n), kBlock, 0, s>>>(n, order, a);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
+int fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
+  if (n == 0) return 0;
+  size_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  k_fill_u32<<<(unsigned)blocks, kBlock, 0, s>>>(p, v, n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int fill_zero_bytes(hipStream_t s, void* p, size_t bytes) {
+  if (bytes % 4) {
+    set_error("fill_zero_bytes: size %zu not a multiple of 4", bytes);
+    return SOPHIA_E_ARG;
+  }
+  return fill_u32(s, (uint32_t*)p, 0u, bytes / 4);
+}
+
+template <typename R>
+int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt) {
+  if (v.n <= 0) return 0;
+  k_neighbors<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v, cv, d_maxcnt);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+template int build_sph_neighbors<float>(hipStream_t, const DevParams&, const SphView<float>&, const CellView&, int*);
+template int build_sph_neighbors<double>(hipStream_t, const DevParams&, const SphView<double>&, const CellView&, int*);
+
+int knl_eval_device(hipStream_t s, int kt, int dim, double h, const double* r, int n, double* w, double* dw) {
+  if (n <= 0) return 0;
+  k_knl<<<grid_for(n), kBlock, 0, s>>>(kt, dim, h, r, n, w, dw);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sophia

@@ -1,0 +1,527 @@
+// SPH stage kernels of the ISPH step on gfx950.
+//
+// One thread per particle.  Pair loops walk the neighbour list built at
+// NNPS (slot-major, so the k-th neighbour of 64 consecutive particles is one
+// coalesced 256 B read); the list keeps the reference's cell visiting order,
+// so sums are accumulated in the same order as the reference kernels.
+// Neighbour fields are gathered from SoA streams of cell-sorted particles;
+// spatial locality of the sort keeps those gathers in L2.
+//
+// Each kernel cites the reference kernel it computes.
+#include "common.hpp"
+
+namespace sophia {
+
+namespace {
+
+template <typename R, class F>
+__device__ inline void for_nbrs(const SphView<R>& v, int i, bool eps_rule, F&& f) {
+  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+  const size_t n = (size_t)v.n;
+  for (int k = 0; k < cnt; k++) {
+    uint32_t e = v.nbr[(size_t)k * n + i];
+    if (eps_rule && (e & kNbrFlag)) continue;
+    f((int)(e & ~kNbrFlag));
+  }
+}
+
+// KERNEL_clc_mass_init_sph (functions_MASS.cuh:24-43): rho0 <- rho
+template <typename R>
+__global__ void k_mass_init(DevParams P, SphView<R> v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.ptype[i] > 1000 || v.itype[i] > 2) return;
+  v.rho0[i] = v.rho[i];
+}
+
+// KERNEL_clc_correction_KGC_3D_sph (functions_PREP2.cuh:452-568)
+template <typename R>
+__global__ void k_kgc(DevParams P, SphView<R> v) {
+  typedef R T;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int n = v.n;
+  if (v.itype[i] > 2 || v.ptype[i] > 1000) {
+    for (int c = 0; c < 6; c++) v.cm[c * n + i] = 0;
+    return;
+  }
+  const double hd = v.h[i];
+  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rs = (T)(P.kappa * hd);
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  T xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
+  for_nbrs(v, i, false, [&](int j) {
+    if (v.ptype[j] > 1000) return;
+    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+    T r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(r > 0 && r < Rs)) return;
+    T dw = knl_dw<T>(P.kernel_type, A, h, r);
+    T mtd = (T)v.m[j] * dw;
+    T rtd = (T)1 / ((T)v.rho[j] * r);
+    xx += -mtd * dx * dx * rtd;
+    xy += -mtd * dy * dx * rtd;
+    yy += -mtd * dy * dy * rtd;
+    zx += -mtd * dx * dz * rtd;
+    yz += -mtd * dy * dz * rtd;
+    zz += -mtd * dz * dz * rtd;
+  });
+  T det = xx * (yy * zz - yz * yz);
+  det -= xy * (xy * zz - yz * zx);
+  det += zx * (xy * yz - yy * zx);
+  T c0, c1, c2, c3, c4, c5;
+  if (fabs((double)det) > 1e-4) {
+    T rr = (T)1 / det;
+    c0 = (yy * zz - yz * yz) * rr;
+    c1 = (zx * yz - xy * zz) * rr;
+    c2 = (xy * yz - zx * yy) * rr;
+    c3 = (xx * zz - zx * zx) * rr;
+    c4 = (zx * xy - xx * yz) * rr;
+    c5 = (xx * yy - xy * xy) * rr;
+  } else {
+    c0 = 1; c1 = 0; c2 = 0; c3 = 1; c4 = 0; c5 = 1;
+  }
+  v.cm[i] = c0; v.cm[n + i] = c1; v.cm[2 * n + i] = c2;
+  v.cm[3 * n + i] = c3; v.cm[4 * n + i] = c4; v.cm[5 * n + i] = c5;
+}
+
+// KERNEL_clc_prep3D_coupling_sph (functions_PREP2.cuh:3190-3364): porosity
+// eps_i = 1 - sum_a V_a W_ia / sum_{j fluid} V_j W_ij and its rate.
+template <typename R>
+__global__ void k_porosity_sph(DevParams P, SphView<R> v, DemView d, CellView dc) {
+  typedef R T;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.itype[i] > 2) return;
+  v.dpor_prev[i] = v.dpor[i];
+  const double hd = v.h[i];
+  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const double Rs = P.kappa * hd;
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  const int pti = v.ptype[i];
+  T flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
+  for_nbrs(v, i, true, [&](int j) {
+    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+    T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
+    T w = knl_w<T>(P.kernel_type, A, h, r);
+    int ptj = v.ptype[j];
+    if (pti <= 1000 && ptj <= 1000 && ptj == 1) flt += (T)v.m[j] / (T)v.rho[j] * w;
+  });
+  if (d.n > 0) {
+    for_cells(P, xi, yi, zi, dc, [&](int j) {
+      double rd = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]) + 1e-20;
+      if (!(rd < Rs)) return;
+      if (!(pti <= 1000 && d.ptype[j] > 1000)) return;
+      T dx = (T)(xi - d.x[j]), dy = (T)(yi - d.y[j]), dz = (T)(zi - d.z[j]);
+      T r = (T)rd;
+      T w = knl_w<T>(P.kernel_type, A, h, r);
+      T dw = knl_dw<T>(P.kernel_type, A, h, r);
+      T rj = (T)d.rad[j];
+      T Vj = (T)4.18879 * rj * rj * rj;
+      por += w * Vj;
+      dfx += dw * dx / r * (T)d.ux[j] * Vj;
+      dfy += dw * dy / r * (T)d.uy[j] * Vj;
+      dfz += dw * dz / r * (T)d.uz[j] * Vj;
+    });
+  }
+  if (pti <= 1000) {
+    v.por[i] = (T)1 - (por / (flt + (T)1e-20));
+    v.dpor[i] = (dfx + dfy + dfz) / (flt + (T)1e-20);
+  }
+}
+
+// KERNEL_clc_prep3D_prep_sph (functions_PREP2.cuh:1623-1679)
+template <typename R>
+__global__ void k_prep_prep(DevParams P, SphView<R> v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.itype[i] > 2 || v.ptype[i] > 1000) return;
+  int t = v.ptype[i];
+  double hv = v.h[i] / P.h_over_dx;
+  double rt = (t == -3 || t == 0 || t == 3 || t == 1) ? P.rho_fluid : P.rho_water;
+  if (t == 9 && v.z[i] < 0.05) rt = P.rho_fluid;
+  double m_ref = rt * hv * hv * hv * (double)v.por[i];
+  double V = P.dim == 2 ? hv * hv : hv * hv * hv;
+  v.rho_ref[i] = (R)(m_ref / V);
+  if (P.rho_type == 1) {
+    v.rho0[i] = (R)(m_ref / (hv * hv * hv));
+    v.rho[i] = m_ref / (hv * hv * hv);
+    v.m[i] = m_ref;
+  }
+}
+
+// KERNEL_clc_prep3D_sph (functions_PREP2.cuh:1985-2227): Shepard filter and
+// the Antuono density gradient.
+template <typename R>
+__global__ void k_prep(DevParams P, SphView<R> v) {
+  typedef R T;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.itype[i] > 2 || v.ptype[i] > 1000) return;
+  const int n = v.n;
+  const double hd = v.h[i];
+  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  const T rhoi = v.rho[i], rrefi = v.rho_ref[i];
+  const bool anto = P.delsph == 2;
+  T flt = 0, rx = 0, ry = 0, rz = 0;
+  for_nbrs(v, i, true, [&](int j) {
+    if (v.ptype[j] > 1000) return;
+    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+    T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
+    T mj = v.m[j], rhoj = v.rho[j];
+    if (P.filt_step) flt += mj / rhoj * knl_w<T>(P.kernel_type, A, h, r);
+    if (anto) {
+      T dw = knl_dw<T>(P.kernel_type, A, h, r);
+      T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      kgc_apply(P, v.cm, n, i, gx, gy, gz);
+      T c = -(rhoj / (T)v.rho_ref[j] - rhoi / rrefi) * (mj / rhoj);
+      rx += c * gx;
+      ry += c * gy;
+      rz += c * gz;
+    }
+  });
+  if (P.filt_step) v.flt_s[i] = flt;
+  if (anto) {
+    v.grx[i] = rx;
+    v.gry[i] = ry;
+    v.grz[i] = rz;
+  }
+}
+
+// KERNEL_advection_force3D_sph (functions_PPE2.cuh:1251-1607): viscous,
+// artificial viscosity, Eulerian advection, conduction, gravity and the
+// pressure-gradient force density used by the coupling.
+template <typename R>
+__global__ void k_advection(DevParams P, SphView<R> v) {
+  typedef R T;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int pti = v.ptype[i];
+  if (v.itype[i] > 2) return;
+  if (P.open_boundary > 0 && v.btype[i] > 0) return;
+  if (pti >= 1000 || pti <= 0) return;
+  const int n = v.n;
+  const double hd = v.h[i];
+  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i];
+  const double pid = v.pres[i];
+  const T mi = (T)v.m[i], rhoi = (T)v.rho[i], pori = (T)v.por[i], tempi = v.temp[i];
+  const T visi = (T)P.mu_fluid, kci = (T)(pti > 1000 ? P.k_dem : P.k_fluid);
+  const T cpi = (T)(pti > 1000 ? P.cp_dem : P.cp_fluid);
+  const T eli = P.eulerian ? (T)0 : (T)1;
+  const bool euler = eli < (T)1;
+  const double rhoid = v.rho[i], porid = v.por[i];
+  const double Ad = knl_tmpA<double>(P.kernel_type, P.dim, hd);
+  T tx = 0, ty = 0, tz = 0, Rc = 0, ex = 0, ey = 0, ez = 0, et = 0;
+  double px = 0, py = 0, pz = 0;
+  for_nbrs(v, i, true, [&](int j) {
+    if (!(v.ptype[j] < 1000)) return;
+    // geometry in FP64: the pressure-gradient term cancels a hydrostatic
+    // offset of order H/h times its result
+    const double dxd = xi - v.x[j], dyd = yi - v.y[j], dzd = zi - v.z[j];
+    const double rd = sqrt(dxd * dxd + dyd * dyd + dzd * dzd) + 1e-20;
+    const double dwd = knl_dw<double>(P.kernel_type, Ad, hd, rd);
+    double gxd = dwd * dxd / rd, gyd = dwd * dyd / rd, gzd = dwd * dzd / rd;
+    kgc_apply(P, v.cm, n, i, gxd, gyd, gzd);
+    if (P.fp_solve) {
+      const double Cp = -porid * v.m[j] * (pid + v.pres[j]) / (rhoid * v.rho[j]);
+      px += rhoid / porid * Cp * gxd;
+      py += rhoid / porid * Cp * gyd;
+      pz += rhoid / porid * Cp * gzd;
+    }
+    const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd, r = (T)rd, dw = (T)dwd;
+    const T gx = (T)gxd, gy = (T)gyd, gz = (T)gzd;
+    const T mj = (T)v.m[j], rhoj = (T)v.rho[j];
+    const T uxj = v.ux[j], uyj = v.uy[j], uzj = v.uz[j];
+    if (P.fv_solve) {
+      T visj = (T)P.mu_fluid;
+      T Cv = dx * gx + dy * gy + dz * gz;
+      Cv *= (visi * visj) / (visi + visj);
+      Cv *= 4 * pori * (mj / (rhoi * rhoj));
+      Cv /= r;
+      Cv /= r;
+      tx += Cv * (uxi - uxj);
+      ty += Cv * (uyi - uyj);
+      tz += Cv * (uzi - uzj);
+    }
+    if (P.fva_solve) {
+      T uij_xij = (uxi - uxj) * dx + (uyi - uyj) * dy + (uzi - uzj) * dz;
+      if (uij_xij < 0) {
+        T hij = (h + (T)v.h[j]) * (T)0.5;
+        T phi = hij * uij_xij;
+        phi /= (r * r + (T)0.01 * hij * hij);
+        T Pij = mi * phi * ((T)(-0.1 * P.soundspeed) + (T)0.02 * phi);
+        Pij /= (rhoi + rhoj);
+        Pij *= (T)0.5;
+        tx += -Pij * gx;
+        ty += -Pij * gy;
+        tz += -Pij * gz;
+      }
+    }
+    if (euler) {
+      T c = mj / rhoj;
+      ex += (uxi * (uxj - uxi) * gx + uyi * (uxj - uxi) * gy + uzi * (uxj - uxi) * gz) * c * (1 - eli);
+      ey += (uxi * (uyj - uyi) * gx + uyi * (uyj - uyi) * gy + uzi * (uyj - uyi) * gz) * c * (1 - eli);
+      ez += (uxi * (uzj - uzi) * gx + uyi * (uzj - uzi) * gy + uzi * (uzj - uzi) * gz) * c * (1 - eli);
+      if (P.con_solve) {
+        T tj = v.temp[j];
+        et += (uxi * (tj - tempi) * gx + uyi * (tj - tempi) * gy + uzi * (tj - tempi) * gz) * c * (1 - eli);
+      }
+    }
+    if (P.con_solve) {
+      T kcj = (T)(v.ptype[j] > 1000 ? P.k_dem : P.k_fluid);
+      T H = 4 * pori * mj * kcj * kci * (tempi - (T)v.temp[j]) * dw;
+      H /= (r + (T)1e-10) * rhoi * rhoj * (kci + kcj);
+      Rc += H;
+    }
+  });
+  if (P.fg_solve) tz += -(T)P.gravity;
+  v.fx[i] = tx - ex;
+  v.fy[i] = ty - ey;
+  v.fz[i] = tz - ez;
+  if (P.con_solve) v.dtemp[i] = Rc / cpi * (T)(pti != -1) - et;
+  v.pgx[i] = (R)px;
+  v.pgy[i] = (R)py;
+  v.pgz[i] = (R)pz;
+  v.fmag[i] = sqrt(tx * tx + ty * ty + tz * tz);
+}
+
+// KERNEL_Neumann_boundary3D_sph (functions_BC.cuh:139-274): wall particles
+// take a Shepard interpolation of fluid p, T and eps-weighted rho over the
+// 27 cells within 2*kappa*h, with W(2h) normalised by A(h).
+template <typename R>
+__global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
+  typedef R T;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.ptype[i] > 0) return;
+  const double hd = v.h[i];
+  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h), H2 = (T)(2.0 * hd);
+  const double R2 = 2.0 * (P.kappa * hd);
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  double tp = 0;
+  T fl = 0, tt = 0, tr = 0;
+  for_cells(P, xi, yi, zi, cv, [&](int j) {
+    double rd = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+    int ptj = v.ptype[j];
+    if (!(ptj < 1000)) return;
+    if (!(rd < R2)) return;
+    if (!(ptj >= 1)) return;
+    T w = knl_w<T>(P.kernel_type, A, H2, (T)rd);
+    T vj = (T)v.m[j] / (T)v.rho[j];
+    tp += (double)(vj * w) * v.pres[j];
+    fl += vj * w;
+    tt += vj * (T)v.temp[j] * w;
+    tr += (T)v.rho[j] / (T)v.por[j] * w * vj;
+  });
+  if (fl < (T)1e-6) {
+    v.rho[i] = v.rho_ref[i];
+    v.pres[i] = 0.0;
+  } else {
+    v.pres[i] = tp / (double)fl;
+    v.temp[i] = tt / fl;
+    v.rho[i] = (T)v.por[i] * tr / fl;
+  }
+}
+
+// KERNEL_pressureforce3D_sph (functions_PPE2.cuh:2154-2335)
+template <typename R>
+__global__ void k_pressure_force(DevParams P, SphView<R> v) {
+  typedef double T;  // FP64 pair sums: see the note on pressure-gradient cancellation
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int pti = v.ptype[i];
+  if (v.itype[i] > 2) return;
+  if (P.open_boundary > 0 && v.btype[i] > 0) return;
+  if (pti >= 1000 || pti <= 0) return;
+  const int n = v.n;
+  const double hd = v.h[i];
+  const T h = hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i], pid = v.pres[i];
+  const T rhoi = v.rho[i], pori = v.por[i];
+  T tx = 0, ty = 0, tz = 0;
+  if (P.fp_solve) {
+    for_nbrs(v, i, true, [&](int j) {
+      if (!(v.ptype[j] < 1000)) return;
+      T dx = xi - v.x[j], dy = yi - v.y[j], dz = zi - v.z[j];
+      T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
+      T dw = knl_dw<T>(P.kernel_type, A, h, r);
+      T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      kgc_apply(P, v.cm, n, i, gx, gy, gz);
+      T Cp = -pori * v.m[j] * (pid + v.pres[j]) / (rhoi * v.rho[j]);
+      tx += Cp * gx;
+      ty += Cp * gy;
+      tz += Cp * gz;
+    });
+  }
+  v.fpx[i] = (R)tx;
+  v.fpy[i] = (R)ty;
+  v.fpz[i] = (R)tz;
+}
+
+// KERNEL_clc_projection_sph (functions_TIME_ISPH2.cuh:104-212): predictor
+template <typename R>
+__global__ void k_predictor(DevParams P, SphView<R> v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.itype[i] > 2) return;
+  const int t = v.ptype[i];
+  const R ux0 = v.ux[i], uy0 = v.uy[i], uz0 = v.uz[i];
+  if (t >= 0) {
+    const R bz = (R)(v.btype[i] == 0), dt = (R)P.dt_sph;
+    v.ux[i] = ux0 + v.fx[i] * bz * dt;
+    v.uy[i] = uy0 + v.fy[i] * bz * dt;
+    v.uz[i] = uz0 + v.fz[i] * bz * dt;
+  }
+  v.ux0[i] = ux0;
+  v.uy0[i] = uy0;
+  v.uz0[i] = uz0;
+  if (P.first_step) v.rho_ref[i] = v.rho[i];
+  if (P.con_solve == 1 && t > 0) v.temp0[i] = v.temp[i];
+}
+
+// KERNEL_time_update_projection_sph (functions_TIME_ISPH2.cuh:467-621):
+// corrector with velocity clamp; positions integrated in FP64
+template <typename R>
+__global__ void k_corrector(DevParams P, SphView<R> v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  if (v.itype[i] > 2) return;
+  const int t = v.ptype[i];
+  const double dt = P.dt_sph;
+  R ux0, uy0, uz0, ax, ay, az;
+  if (t > 0) {
+    ux0 = v.ux[i]; uy0 = v.uy[i]; uz0 = v.uz[i];
+    R bz = (R)(v.btype[i] == 0);
+    ax = v.fpx[i] * bz; ay = v.fpy[i] * bz; az = v.fpz[i] * bz;
+  } else {
+    ux0 = v.ux0[i]; uy0 = v.uy0[i]; uz0 = v.uz0[i];
+    ax = ay = az = 0;
+  }
+  R uxc = ux0 + ax * (R)dt, uyc = uy0 + ay * (R)dt, uzc = uz0 + az * (R)dt;
+  if ((double)(uxc * uxc + uyc * uyc + uzc * uzc) >= P.u_limit * P.u_limit) {
+    uxc = ux0; uyc = uy0; uzc = uz0;
+  }
+  const double el = (t > 0 && !P.eulerian) ? 1.0 : 0.0;
+  if (!P.xsph_solve || t > 1000) {
+    v.x[i] = v.x[i] + ((double)v.ux0[i] + (double)uxc) / 2.0 * dt * el;
+    v.y[i] = v.y[i] + ((double)v.uy0[i] + (double)uyc) / 2.0 * dt * el;
+    v.z[i] = v.z[i] + ((double)v.uz0[i] + (double)uzc) / 2.0 * dt * el;
+  }
+  v.ux[i] = uxc;
+  v.uy[i] = uyc;
+  v.uz[i] = uzc;
+  if (P.con_solve == 1 && t > 0) v.temp[i] = v.temp0[i] + v.dtemp[i] * (R)dt * (R)(t != -1);
+}
+
+// KERNEL_xsph3D_sph + KERNEL_xsph3D_calc_sph (functions_TIME.cuh:1735-1850,
+// 1985-1997): Shepard-normalised XSPH velocity and temperature smoothing
+template <typename R>
+__global__ void k_xsph(DevParams P, SphView<R> v) {
+  typedef R T;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int pti = v.ptype[i];
+  if (v.itype[i] != 1 || pti > 1000 || pti == 0 || pti == 9) return;
+  const double hd = v.h[i];
+  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i], ti = v.temp[i], c = (T)P.c_xsph;
+  T tx = 0, ty = 0, tz = 0, tt = 0, fl = 0;
+  for_nbrs(v, i, false, [&](int j) {
+    int ptj = v.ptype[j];
+    if (!(ptj == 1 || ptj == 3)) return;
+    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+    T r = sqrt(dx * dx + dy * dy + dz * dz);
+    T w = knl_w<T>(P.kernel_type, A, h, r);
+    T vj = (T)v.m[j] / (T)v.rho[j];
+    tx += c * vj * (-uxi + (T)v.ux[j]) * w;
+    ty += c * vj * (-uyi + (T)v.uy[j]) * w;
+    tz += c * vj * (-uzi + (T)v.uz[j]) * w;
+    tt += (T)0.01 * c * vj * ((T)v.temp[j] - ti) * w;
+    fl += vj * w;
+  });
+  const T on = (T)(pti > 0);
+  v.xsx[i] = (uxi + tx / fl) * on;
+  v.xsy[i] = (uyi + ty / fl) * on;
+  v.xsz[i] = (uzi + tz / fl) * on;
+  v.xst[i] = (ti + tt / fl) * on;
+}
+
+template <typename R>
+__global__ void k_xsph_calc(DevParams P, SphView<R> v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int pti = v.ptype[i];
+  if (v.itype[i] != 1 || pti > 1000 || pti == 0 || pti == 9) return;
+  v.ux[i] = v.xsx[i];
+  v.uy[i] = v.xsy[i];
+  v.uz[i] = v.xsz[i];
+  v.temp[i] = v.xst[i];
+  if (P.fix_xsph && pti > 0 && !P.eulerian) {
+    // positions from the start-of-step position and the smoothed velocity
+    const double dt = P.dt_sph;
+    v.x[i] += (double)v.xsx[i] * dt;
+    v.y[i] += (double)v.xsy[i] * dt;
+    v.z[i] += (double)v.xsz[i] * dt;
+  }
+}
+
+}  // namespace
+
+#define LAUNCH(kern, ...)                                           \
+  do {                                                              \
+    if (v.n > 0) {                                                  \
+      kern<R><<<grid_for(v.n), kBlock, 0, s>>>(__VA_ARGS__);        \
+      HIP_TRY(hipGetLastError());                                   \
+    }                                                               \
+    return 0;                                                       \
+  } while (0)
+
+template <typename R> int sph_mass_init(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_mass_init, P, v); }
+template <typename R> int sph_kgc(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_kgc, P, v); }
+template <typename R>
+int sph_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dc) {
+  LAUNCH(k_porosity_sph, P, v, d, dc);
+}
+template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n > 0) {
+    k_prep_prep<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+    k_prep<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_advection, P, v); }
+template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv) {
+  LAUNCH(k_neumann, P, v, cv);
+}
+template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
+template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_predictor, P, v); }
+template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_corrector, P, v); }
+template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n > 0) {
+    k_xsph<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+    k_xsph_calc<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
+#define INST(R)                                                                                              \
+  template int sph_mass_init<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_kgc<R>(hipStream_t, const DevParams&, const SphView<R>&);                                \
+  template int sph_porosity<R>(hipStream_t, const DevParams&, const SphView<R>&, const DemView&, const CellView&); \
+  template int sph_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                               \
+  template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
+  template int sph_pressure_force<R>(hipStream_t, const DevParams&, const SphView<R>&);                     \
+  template int sph_predictor<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_corrector<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_xsph<R>(hipStream_t, const DevParams&, const SphView<R>&);
+INST(float)
+INST(double)
+
+}  // namespace sophia

@@ -1,0 +1,281 @@
+/*
+ * sophia-mi355x C ABI: the drop-in boundary for the SOPHIA SPH-DEM step.
+ *
+ * The reference drives one time step through ~33 raw kernel launches from
+ * SOPHIA_single_ISPH (ISPH_Calc_seperate.cuh:13-503) or SOPHIA_single
+ * (WCSPH_Calc.cuh:4-362), over AoS part1/part2/part3 buffers and the
+ * __constant__ parameter vectors k_vii/k_vif (SOPHIA_gpu.cu:43-48).  This
+ * header replaces that surface with an opaque context that owns all device
+ * memory and exposes the same stages by name:
+ *
+ *   NNPS               sophia_nnps                (ISPH_Calc_seperate.cuh:44-113)
+ *   KNL                sophia_knl_eval            (functions_KNL.cuh:2-173)
+ *   PREP               sophia_prep                (ISPH_Calc_seperate.cuh:123-178)
+ *   INTERACTION        sophia_interaction         (ISPH_Calc_seperate.cuh:203-206)
+ *   DEM_INTERACTION    sophia_dem_interaction     (ISPH_Calc_seperate.cuh:217-218)
+ *   SPH_DEM_COUPLING   sophia_sph_dem_coupling    (ISPH_Calc_seperate.cuh:232-249)
+ *   TIME               sophia_time                (ISPH_Calc_seperate.cuh:259-269, 391-396)
+ *   PPE                sophia_ppe                 (ISPH_Calc_seperate.cuh:305-371)
+ *   XSPH               sophia_xsph                (ISPH_Calc_seperate.cuh:448-466)
+ *   whole step         sophia_step
+ *
+ * Every call returns 0 on success or a negative SOPHIA_E* code; it never
+ * exits the process (the reference's CUDA_CHECK calls exit, ISPH.cuh:1-8).
+ * sophia_last_error() returns a thread-local message for the last failure.
+ * One context = one device + one HIP stream; calls on a context are
+ * stream-ordered and asynchronous unless documented otherwise; a context
+ * must not be used from two threads at once.  No pointer crosses the ABI
+ * except caller-owned host arrays.
+ */
+#ifndef SOPHIA_ABI_H
+#define SOPHIA_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "particles.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOPHIA_ABI_VERSION 1
+
+/* error codes */
+#define SOPHIA_OK 0
+#define SOPHIA_E_ARG -1       /* bad argument / size */
+#define SOPHIA_E_HIP -2       /* HIP runtime failure */
+#define SOPHIA_E_IO -3        /* file open / parse failure */
+#define SOPHIA_E_STATE -4     /* call out of order (e.g. stage before nnps) */
+#define SOPHIA_E_NOMEM -5     /* device allocation failed */
+#define SOPHIA_E_CAPACITY -6  /* neighbour/contact slot capacity exceeded */
+#define SOPHIA_E_NUMERIC -7   /* NaN/Inf detected or solver breakdown */
+
+/* species */
+#define SOPHIA_SPH 0
+#define SOPHIA_DEM 1
+
+/* precision of the device state */
+#define SOPHIA_PREC_FP64 0   /* every field and all arithmetic in FP64 */
+#define SOPHIA_PREC_MIXED 1  /* FP32 fields; FP64 positions, pressure and reductions */
+
+/* kernel ids (Parameters.cuh:38-44) */
+#define SOPHIA_KNL_GAUSSIAN 0
+#define SOPHIA_KNL_QUINTIC 1
+#define SOPHIA_KNL_QUARTIC 2
+#define SOPHIA_KNL_WENDLAND2 3
+#define SOPHIA_KNL_WENDLAND4 4
+#define SOPHIA_KNL_WENDLAND6 5
+
+/* particle types (Parameters.cuh:10-24) */
+#define SOPHIA_PT_BOUNDARY 0
+#define SOPHIA_PT_FLUID 1
+#define SOPHIA_PT_MOVING 9
+#define SOPHIA_PT_DEM 1001
+
+/*
+ * Run parameters.  The first block mirrors the solv.txt keys parsed by
+ * read_solv_input (function_init.cuh:216-615) into vii/vif
+ * (function_init.cuh:2-105).  The second block holds model constants that
+ * the reference hard-codes; sophia_params_default() sets them to the
+ * reference values so that parity runs reproduce it, and configs override
+ * them.
+ */
+typedef struct sophia_params {
+  /* --- solv.txt --- */
+  int32_t solver_type;        /* 0 WCSPH, 1 ISPH */
+  int32_t dim;                /* 2 or 3 */
+  int32_t kernel_type;        /* SOPHIA_KNL_* */
+  int32_t rho_type;           /* 0 Direct (mass sum), 1 Continuity */
+  int32_t time_type;          /* 0 Euler, 1 Predictor_Corrector */
+  int32_t open_boundary;      /* 0 NO, 1 OPEN, 2 PERIODIC */
+  int32_t flag_z_index;       /* 1: Morton cell keys */
+  int32_t freq_filt;          /* Shepard filter frequency */
+  int32_t freq_output;
+  int32_t fp_solve, fv_solve, fva_solve, fg_solve;
+  int32_t con_solve;          /* conduction */
+  int32_t xsph_solve;
+  int32_t kgc_solve;          /* 0 NO, 1 KGC */
+  int32_t delSPH_solve;       /* 0 NO, 1 Molteni, 2 Antuono */
+  int32_t decouple_stride;
+  int32_t min_iteration, max_iteration;
+  double kappa, p_ref, gamma, dt, time, time_end;
+  double xmargin_m, xmargin_p, ymargin_m, ymargin_p, zmargin_m, zmargin_p;
+  double u_limit, c_xsph, soundspeed, rho0_eos;
+  double drho_th, dp_th, p_relaxation;
+
+  /* --- model constants hard-coded in the reference --- */
+  double rho_fluid;      /* DENSITY_AIR (Parameters.cuh:53): m_ref and b_i scale, p_type 0/1/3/-3 */
+  double rho_water;      /* DENSITY_WATER (Parameters.cuh:52): p_type 9 and others */
+  double h_over_dx;      /* 1.6 in vol = (h/1.6)^dim (functions_MASS.cuh:33) */
+  double mu_fluid;       /* viscosity() -> VISCOSITY_AIR (functions_PROP.cuh:240-242) */
+  double mu_water;       /* VISCOSITY_WATER (mixture drag branch) */
+  double k_fluid;        /* conductivity() SPH value 0.0518 (functions_PROP.cuh:272) */
+  double k_dem;          /* conductivity() DEM value 2.5 */
+  double cp_fluid;       /* heat_capacity() SPH value 1006.4 */
+  double cp_dem;         /* heat_capacity() DEM value 1068 */
+  double gravity;        /* 9.8 (Parameters.cuh:81) */
+  double eos_b_factor;   /* 1.186 in B = c0^2*1.186/gamma (functions_PROP.cuh:22) */
+  double delta_sph;      /* delta = 0.1 (Parameters.cuh:93) */
+  double dem_shear_modulus;  /* G = 8.3e8 (functions_PROP.cuh:507) */
+  double dem_poisson;        /* nu = 0.23 (functions_PROP.cuh:492) */
+  double dem_friction;       /* Coulomb cap 0.5*F_n (functions_DEM_INTERACTION2.cuh:1528) */
+  double dem_damping;        /* sqrt(3.333333333)*0.0535184 (e = 0.9) */
+  double dem_hertz_factor;   /* 1.33333 (functions_DEM_INTERACTION2.cuh:1465) */
+  double drag_switch;        /* 0.65 fluid-fraction threshold (functions_SPH_DEM_COUPLING2.cuh:979) */
+  int32_t eulerian_sph;      /* 1: SPH particles fixed (elix=0, functions_ALE.cuh:35-39) */
+  int32_t fix_xsph_position; /* 0: reference (XSPH never writes x); 1: x from XSPH velocity */
+  int32_t dem_max_contacts;  /* history slots per sphere, <= 16 */
+  int32_t sph_max_neighbors; /* ELL width of the SPH neighbour list */
+  /* optional domain override (periodic boxes); used when domain_fixed != 0 */
+  int32_t domain_fixed;
+  int32_t periodic_mask;     /* bit0 x, bit1 y, bit2 z: periodic wrap in that axis */
+  double domain_min[3], domain_max[3];
+  double reserved[8];
+} sophia_params;
+
+/* Domain geometry after find_minmax (function_init.cuh:913-950) and the
+ * cell sizing of ISPH.cuh:93-113. */
+typedef struct sophia_domain {
+  double x_min, x_max, y_min, y_max, z_min, z_max;
+  double dcell;
+  int32_t NI, NJ, NK;
+  int32_t num_cells;      /* cells addressable by a key (+1 dummy slot) */
+  int32_t n_sph, n_dem;
+} sophia_domain;
+
+/* PPE solve selection */
+#define SOPHIA_PPE_JACOBI 0     /* reference: one Jacobi sweep (functions_PPE2.cuh:868-878) */
+#define SOPHIA_PPE_BICGSTAB 1   /* converged Krylov solve */
+
+#define SOPHIA_PPE_FP64 0       /* FP64 operator, FP64 vectors */
+#define SOPHIA_PPE_MIXED_IR 1   /* FP32 operator inside, FP64 outer residual + refinement */
+
+#define SOPHIA_GUESS_ZERO 0
+#define SOPHIA_GUESS_PREVIOUS 1
+#define SOPHIA_GUESS_LEARNED 2  /* learned guess; falls back if its residual is larger */
+
+typedef struct sophia_ppe_opts {
+  int32_t method;       /* SOPHIA_PPE_JACOBI / SOPHIA_PPE_BICGSTAB */
+  int32_t precision;    /* SOPHIA_PPE_FP64 / SOPHIA_PPE_MIXED_IR */
+  int32_t guess;        /* SOPHIA_GUESS_* */
+  int32_t max_iter;     /* total inner iterations cap */
+  int32_t max_outer;    /* refinement sweeps cap (mixed) */
+  int32_t zero_mean;    /* project out the constant null space (periodic, no walls) */
+  double rtol;          /* ||b - A p|| <= rtol * ||b|| */
+  double inner_rtol;    /* per refinement sweep (mixed) */
+} sophia_ppe_opts;
+
+typedef struct sophia_ppe_stats {
+  int32_t iterations;       /* inner Krylov iterations */
+  int32_t outer;            /* refinement sweeps */
+  int32_t converged;
+  int32_t guess_used;       /* SOPHIA_GUESS_* actually used (after fallback) */
+  double r0_rel;            /* ||b - A p0|| / ||b|| of the guess used */
+  double r0_rel_learned;    /* same for the learned guess (if tried) */
+  double r_rel;             /* final relative residual (FP64) */
+  double ms;                /* wall time of the PPE stage, device events */
+  int32_t n_rows;
+  int32_t reserved;
+} sophia_ppe_stats;
+
+typedef struct sophia_ctx sophia_ctx;
+
+/* --- parameters and files ------------------------------------------------ */
+void sophia_params_default(sophia_params* p);
+/* solv.txt parser (function_init.cuh:216-615); unknown keys are ignored */
+int sophia_read_solv(const char* path, sophia_params* p);
+/* input.txt: header row of column codes 1-36, tab separated
+ * (function_init.cuh:704-891).  With out == NULL only counts rows into *n. */
+int sophia_read_input(const char* path, part1* out, int32_t capacity, int32_t* n);
+/* p_type.txt: one integer per line (ISPH.cuh via gpu_count_boundary_numbers) */
+int sophia_read_ptype(const char* path, int32_t* out, int32_t capacity, int32_t* n);
+
+/* --- context ------------------------------------------------------------- */
+/* particles in reference order, SPH and DEM mixed; split by p_type <= 1000
+ * (read_sph_dem, function_init.cuh:893-911). */
+int sophia_create_from_arrays(const sophia_params* p, const part1* particles,
+                              int32_t n, int32_t device, int32_t precision,
+                              sophia_ctx** out);
+int sophia_create(const char* solv_path, const char* input_path, int32_t device,
+                  int32_t precision, sophia_ctx** out);
+int sophia_destroy(sophia_ctx* ctx);
+const char* sophia_last_error(void);
+int sophia_get_domain(const sophia_ctx* ctx, sophia_domain* d);
+int sophia_get_params(const sophia_ctx* ctx, sophia_params* p);
+int sophia_sync(sophia_ctx* ctx);
+/* step counter and simulated time (count, time in the reference vii/vif) */
+int sophia_get_clock(const sophia_ctx* ctx, int64_t* count, double* time);
+
+/* --- stages (ISPH-DEM order of SOPHIA_single_ISPH) ------------------------ */
+int sophia_nnps(sophia_ctx* ctx);
+int sophia_prep(sophia_ctx* ctx);
+int sophia_interaction(sophia_ctx* ctx);
+int sophia_dem_interaction(sophia_ctx* ctx);
+int sophia_sph_dem_coupling(sophia_ctx* ctx);
+#define SOPHIA_PREDICT 0
+#define SOPHIA_CORRECT 1
+int sophia_time(sophia_ctx* ctx, int32_t phase);
+int sophia_ppe(sophia_ctx* ctx, const sophia_ppe_opts* opts, sophia_ppe_stats* stats);
+int sophia_xsph(sophia_ctx* ctx);
+/* one full time step; advances count and time */
+int sophia_step(sophia_ctx* ctx, const sophia_ppe_opts* opts, sophia_ppe_stats* stats);
+/* n steps back to back (stats of the last step) */
+int sophia_run(sophia_ctx* ctx, int32_t n, const sophia_ppe_opts* opts, sophia_ppe_stats* stats);
+
+/* --- inspection / parity hooks (synchronous) ----------------------------- */
+/* sorted-order cell keys and the permutation applied this step (reference
+ * g_idx / p_idx), cell ranges over num_cells (g_str / g_end; empty =
+ * 0xffffffff) and, for SPH, per-particle neighbour counts within kappa*h_i.
+ * Any pointer may be NULL. */
+int sophia_get_nnps(const sophia_ctx* ctx, int32_t species, int32_t* keys,
+                    int32_t* order, uint32_t* cell_start, uint32_t* cell_end,
+                    int32_t* nbr_count);
+/* export the state in sorted order as reference records (any may be NULL) */
+int sophia_get_particles(const sophia_ctx* ctx, int32_t species, part1* p1,
+                         part2* p2, part3* p3);
+/* number of particles of a species */
+int sophia_count(const sophia_ctx* ctx, int32_t species, int32_t* n);
+/* evaluate W and dW/dr on the device (calc_kernel_wij / calc_kernel_dwij) */
+int sophia_knl_eval(int32_t kernel_type, int32_t dim, double h, const double* r,
+                    int32_t n, double* w, double* dw);
+/* overwrite the SPH pressure (sorted order) e.g. to pose a PPE problem */
+int sophia_set_pressure(sophia_ctx* ctx, const double* p, int32_t n);
+/* PPE system of the current step in sorted order: diag = sum_j a_ij,
+ * rhs b_i (PPE3 + wall part), row mask (1 = unknown).  Any may be NULL. */
+int sophia_get_ppe_system(const sophia_ctx* ctx, double* diag, double* rhs,
+                          int32_t* row_mask);
+/* y = A x with the assembled operator (FP64), sorted order, SPH count */
+int sophia_ppe_apply(sophia_ctx* ctx, const double* x, double* y, int32_t n);
+
+/* --- learned PPE warm start (message-passing MLP, BF16 MFMA) ------------- */
+#define SOPHIA_MLP_HIDDEN 64
+#define SOPHIA_MLP_EDGE_IN 6
+/* total parameter count of the network (weights + biases, FP32 master copy) */
+int sophia_mlp_param_count(int32_t* n);
+/* set / get the FP32 master parameters (BF16 copies are made on set) */
+int sophia_mlp_set_params(sophia_ctx* ctx, const float* params, int32_t n);
+int sophia_mlp_get_params(const sophia_ctx* ctx, float* params, int32_t n);
+/* predict pressure for the current step's PPE rows into out (sorted order) */
+int sophia_mlp_predict(sophia_ctx* ctx, double* out, int32_t n);
+/* one Adam step on the current step's features against target pressure
+ * (sorted order, n = SPH count); returns the loss before the step */
+int sophia_mlp_train_step(sophia_ctx* ctx, const double* target, int32_t n,
+                          double lr, double* loss);
+/* loss and gradient w.r.t. the FP32 parameters (no update) */
+int sophia_mlp_loss_grad(sophia_ctx* ctx, const double* target, int32_t n,
+                         double* loss, float* grad, int32_t n_params);
+/* node features the network sees (n x 4: x/h-free edge inputs are built
+ * on the fly): rho, div u*, pressure scale, row mask */
+int sophia_mlp_reset_adam(sophia_ctx* ctx);
+
+/* --- device timers (HIP events around stage kernels) ---------------------- */
+/* enable (1) or disable (0); enabling clears the accumulated totals */
+int sophia_timers_enable(sophia_ctx* ctx, int32_t on);
+/* "name launches total_ms" lines, NUL terminated */
+int sophia_timers_report(sophia_ctx* ctx, char* buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOPHIA_ABI_H */

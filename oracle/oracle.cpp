@@ -1,0 +1,1658 @@
+/*
+ * FP64 CPU oracle of the SOPHIA ISPH-DEM step (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain C++/OpenMP restatement of the reference kernels, run on the
+ * reference's own AoS records with the reference's buffer roles
+ * (P1 = carried state, SP1/SP2 = sorted work copies, P3 = per-step
+ * accumulators zeroed at NNPS, DEM history read from SP1 and written to P1).
+ * Each function cites the reference file:line it follows.  The oracle is the
+ * checker of the HIP library and the CPU baseline of the benchmark; nothing
+ * in the product links it.
+ *
+ * Built with -O3 -fopenmp -ffp-contract=off: no FMA contraction, so every
+ * expression is the rounded IEEE sequence written here.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   cell geometry / keys      closed form of find_minmax + floor((x-min)/span*NI),
+ *                             brute recomputation, Morton bit pattern vs a
+ *                             bit-by-bit interleave
+ *   sort / cell ranges        std::stable_sort contract (CUB SortPairs is
+ *                             stable); ranges recomputed from sorted keys
+ *   neighbour counts          O(N^2) brute force with the same FP64 test
+ *   kernel functions          closed forms, normalisation (quadrature of
+ *                             W over R^3 = 1), dW = numerical derivative of W
+ *   KGC                       Cm = I on a uniform lattice interior; corrected
+ *                             gradient of a linear field is exact
+ *   porosity                  one sphere in a uniform lattice: 1 - eps equals
+ *                             V_sphere * W / Shepard sum (closed form)
+ *   PPE operator              manufactured p = |x|^2 on a uniform lattice with
+ *                             eps = 1, KGC off: (A p)_i ~ -6 * 2 (interior)
+ *   BiCGStab                  dense numpy solve of the explicitly assembled
+ *                             tiny system
+ *   Hertz-Mindlin             two-sphere head-on: F = 1.33333 E* sqrt(R*) d^1.5,
+ *                             pair antisymmetry, tangential cap
+ *   Di Felice drag            single sphere at eps = 1: textbook formula
+ *   coupling reaction         momentum balance sum_i m_i da_i = -m_a F_a
+ *   predictor/corrector/XSPH  exact restatement, checked against hand-written
+ *                             per-particle formulas in the tests
+ *   parity unpinned           advection Eulerian term and Neumann wall
+ *                             interpolation beyond their closed-form special
+ *                             cases (uniform field reproduced exactly)
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xffffffffu;  // cu_memset (Parameters.cuh:1)
+constexpr double kPI = 3.141592653;       // PI (Parameters.cuh:63)
+constexpr double kMinDet = 1e-4;          // Min_det (Parameters.cuh:125)
+constexpr int kITypeCrt = 2;              // i_type_crt (Parameters.cuh:128)
+
+struct Species {
+  std::vector<part1> P1, SP1;
+  std::vector<part2> P2, SP2;
+  std::vector<part3> P3;
+  std::vector<int32_t> key_in, key, order;       // g_idx_in, g_idx, p_idx
+  std::vector<uint32_t> g_str, g_end;            // cell ranges
+  int n() const { return (int)P1.size(); }
+};
+
+struct PPERow {
+  int i;
+  std::vector<int> j;
+  std::vector<double> a;
+};
+
+}  // namespace
+
+struct orc_state {
+  sophia_params p;
+  sophia_domain d;
+  Species sph, dem;
+  int64_t count = 0;
+  double time = 0.0;
+  bool after_corrector = false;
+  // PPE system of the last assembly
+  std::vector<int32_t> row_mask;
+  std::vector<PPERow> rows;
+  std::vector<double> rhs;
+};
+
+static int g_threads = 0;
+
+/* ------------------------------------------------------------------------ */
+/* kernel functions: calc_tmpA, calc_kernel_wij, calc_kernel_dwij           */
+/* (functions_KNL.cuh:2-32, 47-109, 111-173)                                */
+/* ------------------------------------------------------------------------ */
+static double knl_tmpA(int kt, int dim, double tH) {
+  double tA = 0.0;
+  if (kt == SOPHIA_KNL_GAUSSIAN) {
+    if (dim == 2) tA = 1.0 / (kPI * std::pow(tH, 2));
+    else if (dim == 3) tA = 1.0 / (std::pow(kPI, 1.5) * std::pow(tH, 3));
+  } else if (kt == SOPHIA_KNL_QUINTIC) {
+    if (dim == 2) tA = 7.0 / (478.0 * kPI * std::pow(tH, 2));
+    else if (dim == 3) tA = 3.0 / (359.0 * kPI * std::pow(tH, 3));
+  } else if (kt == SOPHIA_KNL_QUARTIC) {
+    if (dim == 2) tA = 15.0 / (7.0 * kPI * std::pow(tH, 2));
+    else if (dim == 3) tA = 315.0 / (208.0 * kPI * std::pow(tH, 3));
+  } else if (kt == SOPHIA_KNL_WENDLAND2) {
+    if (dim == 2) tA = 2.228169203286535 / (4 * tH * tH);
+    else if (dim == 3) tA = 3.342253804929802 / (8 * tH * tH * tH);
+  } else if (kt == SOPHIA_KNL_WENDLAND4) {
+    if (dim == 2) tA = 2.864788975654116 / (4 * tH * tH);
+    else if (dim == 3) tA = 4.923856051905513 / (8 * tH * tH * tH);
+  } else if (kt == SOPHIA_KNL_WENDLAND6) {
+    if (dim == 2) tA = 3.546881588905096 / (4 * tH * tH);
+    else if (dim == 3) tA = 6.788953041263660 / (8 * tH * tH * tH);
+  }
+  return tA;
+}
+
+static inline double p5(double v) { return v * v * v * v * v; }
+static inline double p4(double v) { return v * v * v * v; }
+
+static double knl_w(int kt, double tA, double tH, double rr) {
+  double tR, w = 0.0;
+  switch (kt) {
+    case SOPHIA_KNL_GAUSSIAN:
+      tR = rr / tH;
+      w = tA * std::exp(-std::pow(tR, 2));
+      break;
+    case SOPHIA_KNL_QUINTIC:
+      tR = rr / tH;
+      if (tR < 1) w = tA * (std::pow(3.0 - tR, 5) - 6.0 * std::pow(2.0 - tR, 5) + 15.0 * std::pow(1.0 - tR, 5));
+      else if (1 <= tR && tR < 2) w = tA * (std::pow(3.0 - tR, 5) - 6.0 * std::pow(2.0 - tR, 5));
+      else if (2 <= tR && tR < 3) w = tA * (std::pow(3.0 - tR, 5));
+      break;
+    case SOPHIA_KNL_QUARTIC:
+      tR = rr / tH;
+      w = (tR < 2) * tA * (2.0 / 3.0 - 9.0 / 8.0 * std::pow(tR, 2) + 19.0 / 24.0 * std::pow(tR, 3) - 5.0 / 32.0 * std::pow(tR, 4));
+      break;
+    case SOPHIA_KNL_WENDLAND2: {
+      tR = rr / tH * 0.5;
+      double o = 1 - tR;
+      w = (tR < 1) * tA * o * o * o * o * (1 + 4 * tR);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND4: {
+      tR = rr / tH * 0.5;
+      double o = 1 - tR;
+      w = (tR < 1) * tA * o * o * o * o * o * o * (1 + 6 * tR + 11.666666666666666 * tR * tR);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND6: {
+      tR = rr / tH * 0.5;
+      double o = 1 - tR;
+      w = (tR < 1) * tA * o * o * o * o * o * o * o * o * (1 + 8 * tR + 25 * tR * tR + 32 * tR * tR * tR);
+      break;
+    }
+  }
+  return w;
+}
+
+static double knl_dw(int kt, double tA, double tH, double rr) {
+  double tR, dw = 0.0;
+  switch (kt) {
+    case SOPHIA_KNL_GAUSSIAN:
+      tR = rr / tH;
+      dw = (1.0 / tH) * tA * (-2.0) * tR * std::exp(-std::pow(tR, 2));
+      break;
+    case SOPHIA_KNL_QUINTIC:
+      tR = rr / tH;
+      if (tR < 1) dw = (1.0 / tH) * tA * (std::pow(3.0 - tR, 5) + 30.0 * std::pow(2.0 - tR, 4) - 75.0 * std::pow(1.0 - tR, 4));
+      else if (1 <= tR && tR < 2) dw = (1.0 / tH) * tA * (-5.0 * std::pow(3.0 - tR, 4) + 30.0 * std::pow(2.0 - tR, 4));
+      else if (2 <= tR && tR < 3) dw = (1.0 / tH) * tA * (-5.0 * std::pow(3.0 - tR, 4));
+      break;
+    case SOPHIA_KNL_QUARTIC:
+      tR = rr / tH;
+      dw = (tR < 2) * (1.0 / tH) * tA * (-9.0 / 8.0 * 2 * tR + 19.0 / 24.0 * 3.0 * std::pow(tR, 2) - 5.0 / 32.0 * 4.0 * std::pow(tR, 3));
+      break;
+    case SOPHIA_KNL_WENDLAND2: {
+      tR = rr / tH * 0.5;
+      double o = 1 - tR;
+      dw = (tR < 1) * (1 / (2 * tH)) * tA * (-20 * tR * o * o * o);
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND4: {
+      tR = rr / tH * 0.5;
+      double o = 1 - tR;
+      dw = (tR < 1) * (1 / (2 * tH)) * tA * (-18.666666666666668 * tR * o * o * o * o * o * (5 * tR + 1));
+      break;
+    }
+    case SOPHIA_KNL_WENDLAND6: {
+      tR = rr / tH * 0.5;
+      double o = 1 - tR;
+      dw = (tR < 1) * (1 / (2 * tH)) * tA * (-22 * tR * o * o * o * o * o * o * o * (16 * tR * tR + 7 * tR + 1));
+      break;
+    }
+  }
+  return dw;
+}
+
+/* ------------------------------------------------------------------------ */
+/* material properties with the reference constants as parameters           */
+/* (functions_PROP.cuh:228-514)                                             */
+/* ------------------------------------------------------------------------ */
+static double viscosity(const sophia_params& p) { return p.mu_fluid; }
+static double conductivity(const sophia_params& p, int ptype) {
+  return std::abs(ptype) > 1000 ? p.k_dem : p.k_fluid;
+}
+static double heat_capacity(const sophia_params& p, int ptype) {
+  return std::abs(ptype) > 1000 ? p.cp_dem : p.cp_fluid;
+}
+
+/* ------------------------------------------------------------------------ */
+/* cell geometry (function_init.cuh:913-950; ISPH.cuh:93-113;               */
+/* functions_NNPS.cuh:306-390, 776-790)                                     */
+/* ------------------------------------------------------------------------ */
+static uint64_t morton3(unsigned a, unsigned b, unsigned c) {
+  auto spread = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  return spread(a) | spread(b) << 1 | spread(c) << 2;
+}
+
+static uint64_t morton2(uint64_t x, uint64_t y) {
+  auto spread = [](uint64_t v) {
+    v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
+    v = (v | (v << 8)) & 0x00FF00FF00FF00FFULL;
+    v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+    v = (v | (v << 2)) & 0x3333333333333333ULL;
+    v = (v | (v << 1)) & 0x5555555555555555ULL;
+    return v;
+  };
+  return spread(x) | (spread(y) << 1);
+}
+
+static int axis_cell(double x, double mn, double mx, int N) {
+  if (mx == mn) return 0;
+  double f = std::floor((x - mn) / (mx - mn) * N);
+  double v = f < (N - 1) ? f : (double)(N - 1);
+  int c = (int)v;
+  if (c < 0) c = 0;
+  return c;
+}
+
+static int idx_cell(const orc_state& s, int ix, int iy, int iz) {
+  if (s.p.flag_z_index == 0) return ix + s.d.NI * iy + s.d.NI * s.d.NJ * iz;
+  if (s.p.dim == 2) return (int)morton2((uint64_t)ix, (uint64_t)iy);
+  return (int)morton3((unsigned)ix, (unsigned)iy, (unsigned)iz);
+}
+
+static void cell_of(const orc_state& s, double x, double y, double z, int& ic, int& jc, int& kc) {
+  ic = axis_cell(x, s.d.x_min, s.d.x_max, s.d.NI);
+  jc = axis_cell(y, s.d.y_min, s.d.y_max, s.d.NJ);
+  kc = axis_cell(z, s.d.z_min, s.d.z_max, s.d.NK);
+}
+
+/* Visit candidates of the 27 cells around (x,y,z) in reference order:
+ * z, then y, then x in {-1,0,1}, each cell's particles in sorted order. */
+template <class F>
+static void for_candidates(const orc_state& s, const Species& sp, double x, double y, double z, F&& f) {
+  int ic, jc, kc;
+  cell_of(s, x, y, z, ic, jc, kc);
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int a = ic + dx, b = jc + dy, c = kc + dz;
+        if (a < 0 || a > s.d.NI - 1 || b < 0 || b > s.d.NJ - 1 || c < 0 || c > s.d.NK - 1) continue;
+        int k = idx_cell(s, a, b, c);
+        if (sp.g_str[k] == kEmpty) continue;
+        uint32_t e = sp.g_end[k];
+        for (uint32_t j = sp.g_str[k]; j < e; j++) f((int)j);
+      }
+}
+
+static inline double dist_of(double dx, double dy, double dz) {
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* setup: find_minmax over all particles but the last (function_init.cuh:
+ * 916-924), margins applied in the reference's order, dcell = 1.1*kappa*h0
+ * with h0 the first particle's h (ISPH.cuh:93-108), NI = (int)(span/dcell)+1
+ * (ISPH.cuh:111-113), num_cells = NI*NJ*NK+1 (functions_NNPS.cuh:357-375). */
+static void setup_domain(orc_state& s, const part1* P, int n) {
+  sophia_domain& d = s.d;
+  const sophia_params& p = s.p;
+  if (p.domain_fixed) {
+    d.x_min = p.domain_min[0]; d.x_max = p.domain_max[0];
+    d.y_min = p.domain_min[1]; d.y_max = p.domain_max[1];
+    d.z_min = p.domain_min[2]; d.z_max = p.domain_max[2];
+  } else {
+    double mnx = P[0].x, mxx = P[0].x, mny = P[0].y, mxy = P[0].y, mnz = P[0].z, mxz = P[0].z;
+    for (int i = 0; i < n - 1; i++) {
+      if (P[i].x < mnx) mnx = P[i].x;
+      if (P[i].x > mxx) mxx = P[i].x;
+      if (P[i].y < mny) mny = P[i].y;
+      if (P[i].y > mxy) mxy = P[i].y;
+      if (P[i].z < mnz) mnz = P[i].z;
+      if (P[i].z > mxz) mxz = P[i].z;
+    }
+    mnx -= (mxx - mnx) * p.xmargin_m;
+    mxx += (mxx - mnx) * p.xmargin_p;
+    mny -= (mxy - mny) * p.ymargin_m;
+    mxy += (mxy - mny) * p.ymargin_p;
+    mnz -= (mxz - mnz) * p.zmargin_m;
+    mxz += (mxz - mnz) * p.zmargin_p;
+    d.x_min = mnx; d.x_max = mxx; d.y_min = mny; d.y_max = mxy; d.z_min = mnz; d.z_max = mxz;
+  }
+  double h0 = P[0].h;
+  d.dcell = 1.1 * p.kappa * h0;
+  d.NI = (int)((d.x_max - d.x_min) / d.dcell) + 1;
+  d.NJ = (int)((d.y_max - d.y_min) / d.dcell) + 1;
+  d.NK = (int)((d.z_max - d.z_min) / d.dcell) + 1;
+  if (p.flag_z_index == 0) {
+    d.num_cells = d.NI * d.NJ * d.NK + 1;
+  } else {
+    int m = std::max(std::max(d.NI, d.NJ), d.NK);
+    uint64_t last = p.dim == 2 ? morton2(m - 1, m - 1) : morton3(m - 1, m - 1, m - 1);
+    d.num_cells = (int)last + 2;  // every key plus one dummy slot
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+extern "C" {
+
+void orc_set_threads(int32_t n) {
+  g_threads = n;
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#endif
+}
+int32_t orc_get_threads(void) {
+#ifdef _OPENMP
+  return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+orc_state* orc_create(const sophia_params* p, const part1* particles, int32_t n) {
+  if (!p || !particles || n <= 0) return nullptr;
+  orc_state* s = new orc_state();
+  s->p = *p;
+  s->count = 0;
+  s->time = p->time;
+  setup_domain(*s, particles, n);
+  // read_sph_dem (function_init.cuh:893-911): SPH = p_type <= 1000
+  for (int i = 0; i < n; i++) {
+    part1 q = particles[i];
+    if (q.p_type <= 1000) s->sph.P1.push_back(q);
+    else s->dem.P1.push_back(q);
+  }
+  for (Species* sp : {&s->sph, &s->dem}) {
+    int m = sp->n();
+    sp->SP1 = sp->P1;
+    sp->P2.assign(m, part2{});
+    sp->SP2.assign(m, part2{});
+    sp->P3.assign(m, part3{});
+    sp->key_in.assign(m, 0);
+    sp->key.assign(m, 0);
+    sp->order.assign(m, 0);
+    sp->g_str.assign(s->d.num_cells + 1, kEmpty);
+    sp->g_end.assign(s->d.num_cells + 1, 0);
+  }
+  s->d.n_sph = s->sph.n();
+  s->d.n_dem = s->dem.n();
+  return s;
+}
+
+void orc_destroy(orc_state* s) { delete s; }
+void orc_get_domain(const orc_state* s, sophia_domain* d) { *d = s->d; }
+void orc_get_clock(const orc_state* s, int64_t* count, double* time) {
+  if (count) *count = s->count;
+  if (time) *time = s->time;
+}
+int orc_count(const orc_state* s, int32_t species) {
+  return species == SOPHIA_SPH ? s->sph.n() : s->dem.n();
+}
+
+/* ------------------------------------------------------------------------ */
+/* NNPS: KERNEL_set_alpha_sph/_dem (functions_ALE.cuh:26-69), cell keys     */
+/* (functions_NNPS.cuh:762-821), stable SortPairs (ISPH_Calc_seperate.cuh:  */
+/* 82-85), KERNEL_reorder (functions_NNPS.cuh:670-730), P3 reset and        */
+/* P1 <- SP1 (ISPH_Calc_seperate.cuh:102-113)                               */
+/* ------------------------------------------------------------------------ */
+static void nnps_species(orc_state& s, Species& sp, bool is_sph) {
+  int n = sp.n();
+  for (int i = 0; i < n; i++) {
+    part1& q = sp.P1[i];
+    if (q.i_type == 3) continue;
+    double e = (q.p_type <= 1000 && s.p.eulerian_sph) ? 0.0 : 1.0;
+    q.elix = q.eliy = q.eliz = e;
+  }
+  for (int i = 0; i < n; i++) {
+    const part1& q = sp.P1[i];
+    if (q.i_type == 3) {
+      sp.key_in[i] = s.d.num_cells - 1;
+    } else {
+      int a, b, c;
+      cell_of(s, q.x, q.y, q.z, a, b, c);
+      sp.key_in[i] = idx_cell(s, a, b, c);
+    }
+  }
+  std::vector<int32_t> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return sp.key_in[a] < sp.key_in[b]; });
+  std::fill(sp.g_str.begin(), sp.g_str.end(), kEmpty);
+  for (int i = 0; i < n; i++) {
+    sp.order[i] = idx[i];
+    sp.key[i] = sp.key_in[idx[i]];
+  }
+  for (int i = 0; i < n; i++) {
+    int h = sp.key[i];
+    if (i == 0 || h != sp.key[i - 1]) {
+      sp.g_str[h] = i;
+      if (i > 0) sp.g_end[sp.key[i - 1]] = i;
+    }
+    if (i == n - 1) sp.g_end[h] = i + 1;
+  }
+  for (int i = 0; i < n; i++) {
+    sp.SP1[i] = sp.P1[sp.order[i]];
+    sp.SP2[i] = sp.P2[sp.order[i]];
+  }
+  std::fill(sp.P3.begin(), sp.P3.end(), part3{});
+  sp.P1 = sp.SP1;
+}
+
+void orc_nnps(orc_state* s) {
+  int stride = std::max(1, s->p.decouple_stride);
+  bool sph_on = (s->count % stride) == 0;
+  if (sph_on) nnps_species(*s, s->sph, true);
+  nnps_species(*s, s->dem, false);
+  s->after_corrector = false;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PREP                                                                     */
+/* ------------------------------------------------------------------------ */
+static inline double vol_of(const sophia_params& p, double h) {
+  double hv = h / p.h_over_dx;
+  return hv * hv * hv;
+}
+
+/* KERNEL_clc_mass_init_sph (functions_MASS.cuh:24-43) */
+static void mass_init_sph(orc_state& s) {
+  Species& sp = s.sph;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    if (q.p_type > 1000 || q.i_type > kITypeCrt) continue;
+    q.vol0 = std::pow(q.h / s.p.h_over_dx, s.p.dim);
+    q.vol = q.vol0;
+    sp.SP2[i].rho0 = q.rho;
+    q.cond = conductivity(s.p, q.p_type);
+  }
+}
+
+/* KERNEL_clc_correction_KGC_3D_sph (functions_PREP2.cuh:452-568) */
+static void kgc_3d(orc_state& s) {
+  Species& sp = s.sph;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (qj.p_type > 1000) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz);
+      if (!(r > 0 && r < R)) return;
+      double dw = knl_dw(kt, A, h, r);
+      double mtd = qj.m * dw;
+      double rtd = 1.0 / (qj.rho * r);
+      double t;
+      t = -mtd * dx * dx; t *= rtd; xx += t;
+      t = -mtd * dy * dx; t *= rtd; xy += t;
+      t = -mtd * dy * dy; t *= rtd; yy += t;
+      t = -mtd * dx * dz; t *= rtd; zx += t;
+      t = -mtd * dy * dz; t *= rtd; yz += t;
+      t = -mtd * dz * dz; t *= rtd; zz += t;
+    });
+    double det = xx * (yy * zz - yz * yz);
+    det -= xy * (xy * zz - yz * zx);
+    det += zx * (xy * yz - yy * zx);
+    double (*C)[3] = sp.P3[i].Cm;
+    if (std::abs(det) > kMinDet) {
+      double r = 1.0 / det;
+      C[0][0] = (yy * zz - yz * yz) * r;
+      C[0][1] = (zx * yz - xy * zz) * r;
+      C[0][2] = (xy * yz - zx * yy) * r;
+      C[1][0] = (zx * yz - xy * zz) * r;
+      C[1][1] = (xx * zz - zx * zx) * r;
+      C[1][2] = (zx * xy - xx * yz) * r;
+      C[2][0] = (xy * yz - zx * yy) * r;
+      C[2][1] = (zx * xy - xx * yz) * r;
+      C[2][2] = (xx * yy - xy * xy) * r;
+    } else {
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) C[a][b] = (a == b);
+    }
+  }
+}
+
+/* apply_gradient_correction_3D (functions_KNL.cuh:763-783), KGC branch */
+static inline void kgc_apply(const orc_state& s, const double C[3][3], double& gx, double& gy, double& gz) {
+  if (s.p.kgc_solve != 1) return;
+  double x = C[0][0] * gx + C[0][1] * gy + C[0][2] * gz;
+  double y = C[1][0] * gx + C[1][1] * gy + C[1][2] * gz;
+  double z = C[2][0] * gx + C[2][1] * gy + C[2][2] * gz;
+  gx = x; gy = y; gz = z;
+}
+
+/* KERNEL_clc_prep3D_coupling_sph (functions_PREP2.cuh:3190-3364) */
+static void coupling_prep_sph(orc_state& s) {
+  Species& sp = s.sph;
+  Species& dm = s.dem;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt) continue;
+    qi.dDEMpor_prev = qi.dDEMpor;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z) + 1e-20;
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r);
+      if (qi.p_type <= 1000 && qj.p_type <= 1000) flt += qj.m / qj.rho * w * (qj.p_type == 1);
+    });
+    if (dm.n() > 0) {
+      for_candidates(s, dm, xi, yi, zi, [&](int j) {
+        const part1& qj = dm.SP1[j];
+        double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+        double r = dist_of(dx, dy, dz) + 1e-20;
+        if (!(r < R)) return;
+        double w = knl_w(kt, A, h, r);
+        double dw = knl_dw(kt, A, h, r);
+        double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+        if (qi.p_type <= 1000 && qj.p_type > 1000) {
+          double rj = qj.rad;
+          por += w * 4.18879 * rj * rj * rj;
+          dfx += gx * qj.ux * 4.18879 * rj * rj * rj;
+          dfy += gy * qj.uy * 4.18879 * rj * rj * rj;
+          dfz += gz * qj.uz * 4.18879 * rj * rj * rj;
+        }
+      });
+    }
+    if (qi.p_type <= 1000) {
+      qi.DEMpor = 1.0 - (por / (flt + 1e-20));
+      qi.dDEMpor = (dfx + dfy + dfz) / (flt + 1e-20);
+    }
+  }
+}
+
+/* KERNEL_clc_prep3D_coupling_dem (functions_PREP2.cuh:3367-3523) */
+static void coupling_prep_dem(orc_state& s) {
+  Species& sp = s.sph;
+  Species& dm = s.dem;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < dm.n(); i++) {
+    part1& qi = dm.SP1[i];
+    if (qi.i_type > kITypeCrt) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double fltd = 0, fltd2 = 0, flt = 0, por = 0;
+    if (sp.n() > 0) {
+      for_candidates(s, sp, xi, yi, zi, [&](int j) {
+        const part1& qj = sp.SP1[j];
+        double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z) + 1e-20;
+        if (!(r < R)) return;
+        double w = knl_w(kt, A, h, r);
+        if (qi.p_type > 1000) {
+          int t = qj.p_type;
+          if (t <= 1000 && t != 0 && t != SOPHIA_PT_MOVING && t == 1) {
+            fltd += qj.m / qj.rho * w;
+            fltd2 += qj.m / qj.rho * w;
+          }
+          if (t <= 1000 && t != 0 && t != SOPHIA_PT_MOVING && t == 3) flt += qj.m / qj.rho * w;
+        }
+      });
+    }
+    for_candidates(s, dm, xi, yi, zi, [&](int j) {
+      const part1& qj = dm.SP1[j];
+      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z) + 1e-20;
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r);
+      if (qi.p_type > 1000 && qj.p_type > 1000) por += w * 4.18879 * qj.rad * qj.rad * qj.rad;
+    });
+    if (qi.p_type > 1000) {
+      qi.flt_s = flt;
+      qi.flt_sd = fltd;
+      qi.flt_sd_2 = fltd2;
+      qi.DEMpor = 1.0 - (por / (flt + fltd + 1.0e-20));
+    }
+  }
+}
+
+/* KERNEL_clc_prep3D_prep_sph (functions_PREP2.cuh:1623-1679) */
+static void prep_prep_sph(orc_state& s) {
+  Species& sp = s.sph;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    if (q.i_type > kITypeCrt || q.p_type > 1000) continue;
+    int t = q.p_type;
+    double hv = q.h / s.p.h_over_dx;
+    double rt = (t == -3 || t == 0 || t == 3 || t == 1) ? s.p.rho_fluid : s.p.rho_water;
+    if (t == 9 && q.z < 0.05) rt = s.p.rho_fluid;
+    double m_ref = rt * hv * hv * hv * q.DEMpor;
+    if (s.p.dim == 2) sp.SP2[i].rho_ref = m_ref / (hv * hv);
+    if (s.p.dim == 3) sp.SP2[i].rho_ref = m_ref / (hv * hv * hv);
+    if (s.p.rho_type == 1) {
+      sp.SP2[i].rho0 = m_ref / (hv * hv * hv);
+      q.rho = m_ref / (hv * hv * hv);
+      q.m = m_ref;
+    }
+  }
+}
+
+/* KERNEL_clc_prep3D_sph (functions_PREP2.cuh:1985-2227): Shepard filter and
+ * Antuono delta-SPH density gradient (turbulence / surface terms off) */
+static void prep3d_sph(orc_state& s) {
+  Species& sp = s.sph;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+  int stride = std::max(1, s.p.decouple_stride);
+  bool filt = (s.count % ((int64_t)s.p.freq_filt * stride)) == 0;
+  std::vector<double> fltv(sp.n()), gx(sp.n()), gy(sp.n()), gz(sp.n());
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double rhoi = qi.rho, rrefi = sp.SP2[i].rho_ref;
+    double flt = 0, rx = 0, ry = 0, rz = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (qj.p_type > 1000) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz) + 1e-20;
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r), dw = knl_dw(kt, A, h, r);
+      double tx = dw * dx / r, ty = dw * dy / r, tz = dw * dz / r;
+      if (filt) flt += qj.m / qj.rho * w;
+      if (s.p.delSPH_solve == 2) {
+        kgc_apply(s, sp.P3[i].Cm, tx, ty, tz);
+        double c = -(qj.rho / sp.SP2[j].rho_ref - rhoi / rrefi) * (qj.m / qj.rho);
+        rx += c * tx;
+        ry += c * ty;
+        rz += c * tz;
+      }
+    });
+    fltv[i] = flt; gx[i] = rx; gy[i] = ry; gz[i] = rz;
+  }
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    if (q.i_type > kITypeCrt || q.p_type > 1000) continue;
+    if (filt) q.flt_s = fltv[i];
+    if (s.p.delSPH_solve == 2) {
+      q.grad_rhox = gx[i];
+      q.grad_rhoy = gy[i];
+      q.grad_rhoz = gz[i];
+      q.test1 = gy[i];
+      q.test5 = gz[i];
+    }
+  }
+}
+
+void orc_prep(orc_state* s) {
+  int stride = std::max(1, s->p.decouple_stride);
+  bool sph_on = (s->count % stride) == 0;
+  if (sph_on && s->p.dim == 3) mass_init_sph(*s);
+  if (sph_on && s->p.kgc_solve > 0 && s->p.dim == 3) kgc_3d(*s);
+  if (s->p.dim == 3) {
+    if (sph_on) coupling_prep_sph(*s);
+    coupling_prep_dem(*s);
+    if (sph_on) {
+      prep_prep_sph(*s);
+      prep3d_sph(*s);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* INTERACTION: KERNEL_advection_force3D_sph (functions_PPE2.cuh:1251-1607) */
+/* terms: pressure-gradient density for coupling, laminar viscous,          */
+/* artificial viscosity, Eulerian advection u.grad(u), conduction, gravity  */
+/* ------------------------------------------------------------------------ */
+void orc_interaction(orc_state* s) {
+  int stride = std::max(1, s->p.decouple_stride);
+  if ((s->count % stride) != 0 || s->p.dim != 3) return;
+  Species& sp = s->sph;
+  const sophia_params& p = s->p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt) continue;
+    if (p.open_boundary > 0 && qi.buffer_type > 0) continue;
+    if (qi.p_type >= 1000 || qi.p_type <= 0) continue;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
+    double hi = qi.h, pi = qi.pres, mi = qi.m, rhoi = qi.rho, pori = qi.DEMpor, tempi = qi.temp;
+    double A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
+    double kci = conductivity(p, qi.p_type), cpi = heat_capacity(p, qi.p_type);
+    double visi = viscosity(p) + sp.P3[i].vis_t;
+    double tx = 0, ty = 0, tz = 0, Rc = 0, px = 0, py = 0, pz = 0;
+    double ex = 0, ey = 0, ez = 0, et = 0;
+    for_candidates(*s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type < 1000)) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, hi, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      kgc_apply(*s, sp.P3[i].Cm, gx, gy, gz);
+      double mj = qj.m, rhoj = qj.rho, pj = qj.pres;
+      if (p.fp_solve) {
+        double Cp = -pori * mj * (pi + pj) / (rhoi * rhoj);
+        px += rhoi / pori * Cp * gx;
+        py += rhoi / pori * Cp * gy;
+        pz += rhoi / pori * Cp * gz;
+      }
+      if (p.fv_solve) {
+        double visj = viscosity(p) + sp.P3[j].vis_t;
+        double Cv = dx * gx + dy * gy + dz * gz;
+        Cv *= (visi * visj) / (visi + visj);
+        Cv *= 4 * pori * (mj / (rhoi * rhoj));
+        Cv /= r;
+        Cv /= r;
+        tx += Cv * (uxi - qj.ux);
+        ty += Cv * (uyi - qj.uy);
+        tz += Cv * (uzi - qj.uz);
+      }
+      if (p.fva_solve) {
+        double uij_xij = (uxi - qj.ux) * dx + (uyi - qj.uy) * dy + (uzi - qj.uz) * dz;
+        if (uij_xij < 0) {
+          double h_ij = (hi + qj.h) * 0.5;
+          double phi = h_ij * uij_xij;
+          phi /= (r * r + 0.01 * h_ij * h_ij);
+          double P = mi * phi * (-0.1 * p.soundspeed + 0.02 * phi);
+          P /= (rhoi + rhoj);
+          P *= 0.5;
+          tx += -P * gx;
+          ty += -P * gy;
+          tz += -P * gz;
+        }
+      }
+      if (qi.elix < 1.0 || qi.eliy < 1.0) {
+        double c = mj / rhoj;
+        ex += (uxi * (qj.ux - uxi) * gx + uyi * (qj.ux - uxi) * gy + uzi * (qj.ux - uxi) * gz) * c * (1 - qi.elix);
+        ey += (uxi * (qj.uy - uyi) * gx + uyi * (qj.uy - uyi) * gy + uzi * (qj.uy - uyi) * gz) * c * (1 - qi.eliy);
+        ez += (uxi * (qj.uz - uzi) * gx + uyi * (qj.uz - uzi) * gy + uzi * (qj.uz - uzi) * gz) * c * (1 - qi.eliz);
+        if (p.con_solve) {
+          double tj = qj.temp;
+          et += (uxi * (tj - tempi) * gx + uyi * (tj - tempi) * gy + uzi * (tj - tempi) * gz) * c * (1 - qi.elix);
+        }
+      }
+      if (p.con_solve) {
+        double kcj = conductivity(p, qj.p_type);
+        double H = 4.0 * pori * mj * kcj * kci * (tempi - qj.temp) * dw;
+        H /= (r + 1e-10) * rhoi * rhoj * (kci + kcj);
+        Rc += H;
+      }
+    });
+    if (p.fg_solve) tz += -p.gravity;
+    part3& o = sp.P3[i];
+    o.ftotalx = 1 * (tx - ex);
+    o.ftotaly = 1 * (ty - ey);
+    o.ftotalz = 1 * (tz - ez);
+    if (p.con_solve) o.dtemp = Rc / cpi * (qi.p_type != -1) - et;
+    o.fsx = o.fsy = o.fsz = 0;
+    qi.pgf_x = px;
+    qi.pgf_y = py;
+    qi.pgf_z = pz;
+    o.ftotal = std::sqrt(tx * tx + ty * ty + tz * tz);
+    if (p.con_solve) o.denthalpy = Rc;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* DEM_INTERACTION: KERNEL_DEM_interaction3D_dem                            */
+/* (functions_DEM_INTERACTION2.cuh:888-1766), Hertz-Mindlin with 16-slot    */
+/* shear history keyed by dem_idx; history read from SP1, written to P1     */
+/* ------------------------------------------------------------------------ */
+void orc_dem_interaction(orc_state* s) {
+  if (s->p.dim != 3) return;
+  Species& dm = s->dem;
+  const sophia_params& p = s->p;
+  const int slots = std::min(16, std::max(1, p.dem_max_contacts));
+  const double G = p.dem_shear_modulus, nu = p.dem_poisson;
+  const double E = G * (2 * (1 + nu));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < dm.n(); i++) {
+    const part1& qi = dm.SP1[i];
+    part1& ti = dm.P1[i];
+    if (qi.i_type != 1) continue;
+    if (qi.p_type <= 1000) continue;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
+    double wxi = qi.wx, wyi = qi.wy, wzi = qi.wz, radi = qi.rad, mi = qi.m, rii = qi.ri;
+    double tx = 0, ty = 0, tz = 0, qx = 0, qy = 0, qz = 0;
+    for_candidates(*s, dm, xi, yi, zi, [&](int j) {
+      const part1& qj = dm.SP1[j];
+      if (!(qj.p_type > 1000)) return;
+      double radj = qj.rad;
+      int didx = qj.dem_idx;
+      double dist = dist_of(xi - qj.x, yi - qj.y, zi - qj.z);
+      double ovlp = radi + radj - dist;
+      if (!(dist < 3.5 * radi && dist > 0.0)) return;
+      if (!(dist < 2.01 * radi)) return;
+      if (dist > 0 && ovlp > 0) {
+        double dsx = 0, dsy = 0, dsz = 0;
+        int slot = -1;
+        for (int k = 0; k < slots; k++)
+          if (qi.ct_idx[k] == didx) {
+            dsx = qi.del_s[k][0];
+            dsy = qi.del_s[k][1];
+            dsz = qi.del_s[k][2];
+            slot = k;
+            break;
+          }
+        if (slot < 0) {
+          for (int k = 0; k < slots; k++)
+            if (qi.ct_idx[k] < 1) {
+              ti.ct_idx[k] = didx;
+              slot = k;
+              break;
+            }
+        }
+        double nx = (qj.x - xi) / dist, ny = (qj.y - yi) / dist, nz = (qj.z - zi) / dist;
+        double ov = ovlp;
+        double rcxi = (radi - 0.5 * ov) * nx, rcyi = (radi - 0.5 * ov) * ny, rczi = (radi - 0.5 * ov) * nz;
+        double rcxj = -(radj - 0.5 * ov) * nx, rcyj = -(radj - 0.5 * ov) * ny, rczj = -(radj - 0.5 * ov) * nz;
+        double uroxi = wyi * rczi - wzi * rcyi, uroyi = wzi * rcxi - wxi * rczi, urozi = wxi * rcyi - wyi * rcxi;
+        double uroxj = qj.wy * rczj - qj.wz * rcyj, uroyj = qj.wz * rcxj - qj.wx * rczj, urozj = qj.wx * rcyj - qj.wy * rcxj;
+        double urx = (qj.ux + uroxj) - (uxi + uroxi);
+        double ury = (qj.uy + uroyj) - (uyi + uroyi);
+        double urz = (qj.uz + urozj) - (uzi + urozi);
+        double un = nx * urx + ny * ury + nz * urz;
+        double unx = un * nx, uny = un * ny, unz = un * nz;
+        double usx = urx - unx, usy = ury - uny, usz = urz - unz;
+        dsx += usx * p.dt;
+        dsy += usy * p.dt;
+        dsz += usz * p.dt;
+        double usm = std::sqrt(usx * usx + usy * usy + usz * usz) + 1e-20;
+        double sx = usx / usm, sy = usy / usm, sz = usz / usm;
+        double ymij = 1 / (((1 - nu * nu) / E) + ((1 - nu * nu) / E));
+        double Gij = 1 / (((2 - nu) / G) + ((2 - nu) / G));
+        double radij = 1 / (1 / radi + 1 / radj);
+        double mij = 1 / (1 / mi + 1 / qj.m);
+        double Cspr = p.dem_hertz_factor * ymij * std::sqrt(radij) * std::sqrt(ov) * ov / mi;
+        double Csprs = 8.0 * Gij * std::sqrt(radij * ov) / mi;
+        double Kn = 2 * ymij * std::sqrt(radij * ov);
+        double Kd = std::sqrt(mij * Kn);
+        double Cd = p.dem_damping * Kd / mi;
+        double Kds = std::sqrt(mij * Csprs * mi);
+        double Cds = p.dem_damping * Kds / mi;
+        tx += Cspr * (xi - qj.x) / dist;
+        ty += Cspr * (yi - qj.y) / dist;
+        tz += Cspr * (zi - qj.z) / dist;
+        tx += Cd * unx;
+        ty += Cd * uny;
+        tz += Cd * unz;
+        double fssx = Csprs * dsx + Cds * usx, fssy = Csprs * dsy + Cds * usy, fssz = Csprs * dsz + Cds * usz;
+        double sgx = 1.0 * (fssx >= 0) - 1.0 * (fssx < 0);
+        double sgy = 1.0 * (fssy >= 0) - 1.0 * (fssy < 0);
+        double sgz = 1.0 * (fssz >= 0) - 1.0 * (fssz < 0);
+        double mfss = std::sqrt(fssx * fssx + fssy * fssy + fssz * fssz);
+        double ffx = p.dem_friction * Cspr * sx, ffy = p.dem_friction * Cspr * sy, ffz = p.dem_friction * Cspr * sz;
+        double mfsf = std::sqrt(ffx * ffx + ffy * ffy + ffz * ffz);
+        double fsx = fssx * (mfss < mfsf) + sgx * std::abs(ffx) * (mfss >= mfsf);
+        double fsy = fssy * (mfss < mfsf) + sgy * std::abs(ffy) * (mfss >= mfsf);
+        double fsz = fssz * (mfss < mfsf) + sgz * std::abs(ffz) * (mfss >= mfsf);
+        tx += fsx;
+        ty += fsy;
+        tz += fsz;
+        qx += mi * (rcyi * fsz - rczi * fsy);
+        qy += mi * (rczi * fsx - rcxi * fsz);
+        qz += mi * (rcxi * fsy - rcyi * fsx);
+        if (slot >= 0) {
+          ti.ct_idx[slot] = didx;
+          ti.del_s[slot][0] = dsx;
+          ti.del_s[slot][1] = dsy;
+          ti.del_s[slot][2] = dsz;
+        }
+      } else {
+        for (int k = 0; k < slots; k++)
+          if (qi.ct_idx[k] == didx) {
+            ti.del_s[k][0] = ti.del_s[k][1] = ti.del_s[k][2] = 0.0;
+            ti.ct_idx[k] = 0;
+            break;
+          }
+      }
+    });
+    if (p.fg_solve) tz += -p.gravity;
+    part3& o = dm.P3[i];
+    o.ftotalx = tx;
+    o.ftotaly = ty;
+    o.ftotalz = tz;
+    o.torqx = qx / rii;
+    o.torqy = qy / rii;
+    o.torqz = qz / rii;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* SPH_DEM_COUPLING: KERNEL_DEM_coupling3D_dem (functions_SPH_DEM_COUPLING2 */
+/* .cuh:627-1129) then KERNEL_SPH_coupling3D_sph (:1815-2019)               */
+/* ------------------------------------------------------------------------ */
+static void dem_coupling(orc_state& s) {
+  Species& sp = s.sph;
+  Species& dm = s.dem;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < dm.n(); i++) {
+    part1& qi = dm.SP1[i];
+    if (qi.i_type != 1 || qi.p_type <= 1000) continue;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
+    double radi = qi.rad, di = 2 * radi, voli = (4.0 / 3.0) * 3.14159 * radi * radi * radi;
+    double mi = qi.m;
+    double flt_air = qi.flt_s, flt_fluid = qi.flt_sd;
+    part3& o = dm.P3[i];
+    double fx0 = o.ftotalx, fy0 = o.ftotaly, fz0 = o.ftotalz;
+    double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
+    double tx = 0, ty = 0, tz = 0, ufx = 0, ufy = 0, ufz = 0, uax = 0, uay = 0, uaz = 0, por = 0;
+    if (sp.n() > 0) {
+      for_candidates(s, sp, xi, yi, zi, [&](int j) {
+        const part1& qj = sp.SP1[j];
+        if (!(qj.p_type <= 1000 && qj.DEMpor < 1.0)) return;
+        double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z);
+        if (!(r < R)) return;
+        double w = knl_w(kt, A, hi, r);
+        double mj = qj.m, rhoj = qj.rho;
+        int t = qj.p_type;
+        if (t == 1 && flt_fluid > 0.000001) {
+          ufx += qj.ux * mj * w / rhoj;
+          ufy += qj.uy * mj * w / rhoj;
+          ufz += qj.uz * mj * w / rhoj;
+          por += qj.DEMpor * mj * w / rhoj;
+        }
+        if (t == 3 && flt_air > 0.000001) {
+          uax += qj.ux * mj * w / rhoj;
+          uay += qj.uy * mj * w / rhoj;
+          uaz += qj.uz * mj * w / rhoj;
+          por += qj.DEMpor * mj * w / rhoj;
+        }
+        if (t != 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 3) {
+          if (flt_fluid > 0.01) {
+            tx += qj.pgf_x * (mj / rhoj) * w * voli / (mi * flt_fluid + 1.0e-15);
+            ty += qj.pgf_y * (mj / rhoj) * w * voli / (mi * flt_fluid + 1.0e-15);
+            tz += qj.pgf_z * (mj / rhoj) * w * voli / (mi * flt_fluid + 1.0e-15);
+          }
+        }
+      });
+    }
+    double pori = por / (qi.flt_sd + 1.0e-15);
+    double vfx = ufx / (flt_fluid + 1.0e-15), vfy = ufy / (flt_fluid + 1.0e-15), vfz = ufz / (flt_fluid + 1.0e-15);
+    double vax = uax / (flt_air + 1.0e-15), vay = uay / (flt_air + 1.0e-15), vaz = uaz / (flt_air + 1.0e-15);
+    double vmx = (ufx + uax) / (flt_fluid + flt_air + 1.0e-15);
+    double vmy = (ufy + uay) / (flt_fluid + flt_air + 1.0e-15);
+    double vmz = (ufz + uaz) / (flt_fluid + flt_air + 1.0e-15);
+    double rfx = vfx - uxi, rfy = vfy - uyi, rfz = vfz - uzi;
+    double rax = vax - uxi, ray = vay - uyi, raz = vaz - uzi;
+    double rmx = vmx - uxi, rmy = vmy - uyi, rmz = vmz - uzi;
+    double f_fluid = flt_fluid / (flt_fluid + flt_air + 1.0e-15);
+    double f_air = flt_air / (flt_fluid + flt_air + 1.0e-15);
+    double magf = std::sqrt(rfx * rfx + rfy * rfy + rfz * rfz);
+    double maga = std::sqrt(rax * rax + ray * ray + raz * raz);
+    double magm = std::sqrt(rmx * rmx + rmy * rmy + rmz * rmz);
+    qi.test6 = magf;
+    double dfx = 0, dfy = 0, dfz = 0, dax = 0, day = 0, daz = 0;
+    qi.DEMpor = pori;
+    double nu_f = p.mu_fluid / p.rho_fluid;  // vis_kif = VISCOSITY_AIR/DENSITY_AIR
+    double rho_f = p.rho_fluid;
+    if (magf > 1.0e-8 && (flt_fluid + flt_air) > 0.00001 && f_fluid > p.drag_switch) {
+      double Re = di * magf * pori / (nu_f + 1.0e-15);
+      double Cd = (0.63 + 4.8 / std::sqrt(Re)) * (0.63 + 4.8 / std::sqrt(Re));
+      double beta = 3.7 - 0.65 * std::exp(-(1.5 - std::log10(Re)) * (1.5 - std::log10(Re)) / 2.0);
+      double k = (1.0 / 8.0) * Cd * rho_f * 3.14159 * di * di;
+      double e = std::pow(pori, 2 - beta);
+      dfx = k * rfx * magf * e / mi;
+      dfy = k * rfy * magf * e / mi;
+      dfz = k * rfz * magf * e / mi;
+      qi.Fdx_df = dfx;
+      qi.Fdy_df = dfy;
+      qi.Fdz_df = dfz;
+    }
+    if (maga > 1.0e-8 && (flt_fluid + flt_air) > 0.00001 && f_air > p.drag_switch) {
+      double Re = di * pori * maga / (nu_f + 1.0e-15);
+      double Cd = (0.63 + 4.8 / std::sqrt(Re)) * (0.63 + 4.8 / std::sqrt(Re));
+      double beta = 2.65 * (1.0 + pori) - (5.3 - 3.5 * pori) * pori * pori * std::exp(-(1.5 - std::log10(Re)) * (1.5 - std::log10(Re)) / 2.0);
+      double k = (1.0 / 8.0) * Cd * p.rho_fluid * 3.14159 * di * di;
+      double e = std::pow(pori, -1 - beta);
+      dax = k * rax * maga * e / mi;
+      day = k * ray * maga * e / mi;
+      daz = k * raz * maga * e / mi;
+    }
+    if ((maga > 1.0e-8 || magf > 1.0e-8) && (flt_fluid + flt_air) > 0.00001 && f_air <= p.drag_switch &&
+        f_fluid <= p.drag_switch) {
+      double rho_m = p.rho_fluid * f_air + p.rho_water * f_fluid;
+      double vis_m = p.mu_fluid * f_air + p.mu_water * f_fluid;
+      double nu_m = vis_m / (rho_m + 1.0e-15);
+      double Re = di * pori * magm / (nu_m + 1.0e-15);
+      double Cd = (0.63 + 4.8 / std::sqrt(Re)) * (0.63 + 4.8 / std::sqrt(Re));
+      double beta = 2.65 * (1.0 + pori) - (5.3 - 3.5 * pori) * pori * pori * std::exp(-(1.5 - std::log10(Re)) * (1.5 - std::log10(Re)) / 2.0);
+      double k = (1.0 / 8.0) * Cd * rho_m * 3.14159 * di * di;
+      double e = std::pow(pori, -1 - beta);
+      dax = k * rmx * magm * e / mi;
+      day = k * rmy * magm * e / mi;
+      daz = k * rmz * magm * e / mi;
+    }
+    o.ftotalx = fx0 + dfx / pori + tx;
+    o.ftotaly = fy0 + dfy / pori + ty;
+    o.ftotalz = fz0 + dfz / pori + tz;
+    qi.Fdx_da = dax;
+    qi.Fdy_da = day;
+    qi.Fdz_da = daz;
+    qi.Fdx_b = tx;
+    qi.Fdy_b = ty;
+    qi.Fdz_b = tz;
+  }
+}
+
+static void sph_coupling(orc_state& s) {
+  Species& sp = s.sph;
+  Species& dm = s.dem;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& qi = sp.SP1[i];
+    if (qi.i_type != 1 || qi.p_type > 1000) continue;
+    double xi = qi.x, yi = qi.y, zi = qi.z, rhoi = qi.rho;
+    int t = qi.p_type;
+    part3& o = sp.P3[i];
+    double fx0 = o.ftotalx, fy0 = o.ftotaly, fz0 = o.ftotalz, dti = o.dtemp;
+    double cpi = heat_capacity(p, t);
+    double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
+    double tx = 0, ty = 0, tz = 0, tt = 0;
+    if (dm.n() > 0) {
+      for_candidates(s, dm, xi, yi, zi, [&](int j) {
+        const part1& qj = dm.SP1[j];
+        if (!(qj.p_type > 1000)) return;
+        double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z);
+        if (!(r < R)) return;
+        if (t == SOPHIA_PT_MOVING || t == -3 || t == 0) return;
+        double w = knl_w(kt, A, hi, r);
+        double flt = qj.flt_sd_2 + qj.flt_s;
+        if ((t == 1 || t == 3) && flt > 0.0) {
+          tx += -(qj.m / rhoi) * qj.Fdx_df * w / (flt + 1.0e-20);
+          ty += -(qj.m / rhoi) * qj.Fdy_df * w / (flt + 1.0e-20);
+          tz += -(qj.m / rhoi) * qj.Fdz_df * w / (flt + 1.0e-20);
+          tt += -qj.Q_sd / (rhoi * cpi) * w / (flt + 1.0e-20);
+        }
+      });
+    }
+    o.ftotalx = fx0 + tx;
+    o.ftotaly = fy0 + ty;
+    o.ftotalz = fz0 + tz;
+    o.ftotal = std::sqrt((fx0 + tx) * (fx0 + tx) + (fy0 + ty) * (fy0 + ty) + (fz0 + tz) * (fz0 + tz));
+    qi.test4 = o.ftotal;
+    qi.test7 = tz;
+    if (p.con_solve == 1) o.dtemp = dti + tt;
+  }
+}
+
+void orc_sph_dem_coupling(orc_state* s) {
+  if (s->p.dim != 3) return;
+  dem_coupling(*s);
+  int stride = std::max(1, s->p.decouple_stride);
+  if ((s->count % stride) == 0) sph_coupling(*s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* TIME: KERNEL_clc_projection_sph/_dem (functions_TIME_ISPH2.cuh:104-308)  */
+/* and KERNEL_time_update_projection_sph/_dem (:467-776)                    */
+/* ------------------------------------------------------------------------ */
+static void predictor(orc_state& s, Species& sp, double dt) {
+  const sophia_params& p = s.p;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    part2& q2 = sp.SP2[i];
+    const part3& o = sp.P3[i];
+    if (q.i_type > kITypeCrt) continue;
+    int t = q.p_type;
+    double x0 = q.x, y0 = q.y, z0 = q.z, ux0 = q.ux, uy0 = q.uy, uz0 = q.uz;
+    double xp, yp, zp, uxp, uyp, uzp;
+    if (t >= 0) {
+      double bz = (q.buffer_type == 0);
+      uxp = ux0 + o.ftotalx * bz * dt;
+      uyp = uy0 + o.ftotaly * bz * dt;
+      uzp = uz0 + o.ftotalz * bz * dt;
+      xp = x0 + uxp * dt * (t > 0) * q.elix;
+      yp = y0 + uyp * dt * (t > 0) * q.eliy;
+      zp = z0 + uzp * dt * (t > 0) * q.eliz;
+    } else {
+      xp = x0; yp = y0; zp = z0;
+      uxp = q.ux; uyp = q.uy; uzp = q.uz;
+    }
+    q.x_star = xp; q.y_star = yp; q.z_star = zp;
+    q.ux = uxp; q.uy = uyp; q.uz = uzp;
+    q2.x0 = x0; q2.y0 = y0; q2.z0 = z0;
+    q2.ux0 = ux0; q2.uy0 = uy0; q2.uz0 = uz0;
+    if (s.time == 0) q2.rho_ref = q.rho;
+    if (p.con_solve == 1 && t > 0) q2.temp0 = q.temp;
+  }
+}
+
+static void corrector(orc_state& s, Species& sp, double dt, bool is_sph) {
+  const sophia_params& p = s.p;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& q = sp.SP1[i];
+    part1& T = sp.P1[i];
+    const part2& q2 = sp.SP2[i];
+    part2& T2 = sp.P2[i];
+    const part3& o = sp.P3[i];
+    if (q.i_type > kITypeCrt) continue;
+    int t = q.p_type;
+    double ux0, uy0, uz0, ax, ay, az;
+    if (t > 0) {
+      ux0 = q.ux; uy0 = q.uy; uz0 = q.uz;
+      double bz = (q.buffer_type == 0);
+      ax = o.fpx * bz + q.fbx / q.rho;
+      ay = o.fpy * bz + q.fby / q.rho;
+      az = o.fpz * bz + q.fbz / q.rho;
+    } else {
+      ux0 = q2.ux0; uy0 = q2.uy0; uz0 = q2.uz0;
+      ax = ay = az = 0.0;
+    }
+    double uxc = ux0 + ax * dt, uyc = uy0 + ay * dt, uzc = uz0 + az * dt;
+    if ((uxc * uxc + uyc * uyc + uzc * uzc) >= p.u_limit * p.u_limit) {
+      uxc = ux0; uyc = uy0; uzc = uz0;
+    }
+    double xc = q.x + (q2.ux0 + uxc) / 2.0 * dt * (t > 0) * q.elix;
+    double yc = q.y + (q2.uy0 + uyc) / 2.0 * dt * (t > 0) * q.eliy;
+    double zc = q.z + (q2.uz0 + uzc) / 2.0 * dt * (t > 0) * q.eliz;
+    if (!p.xsph_solve || t > 1000) {
+      T.x = xc; T.y = yc; T.z = zc;
+    }
+    T.ux = uxc; T.uy = uyc; T.uz = uzc;
+    if (p.con_solve == 1 && t > 0) {
+      T.temp = q2.temp0 + o.dtemp * dt * (t != -1);
+    } else {
+      T.enthalpy = q.enthalpy;
+      T.temp = q.temp;
+    }
+    T.pres = q.pres; T.flt_s = q.flt_s; T.m = q.m; T.h = q.h;
+    T.grad_rhox = q.grad_rhox; T.grad_rhoy = q.grad_rhoy; T.grad_rhoz = q.grad_rhoz;
+    T.k_turb = q.k_turb; T.e_turb = q.e_turb;
+    T.elix = q.elix; T.eliy = q.eliy; T.eliz = q.eliz;
+    T.vol = q.vol; T.vol0 = q.vol0;
+    T2.rho_ref = q2.rho_ref;
+    T.flt_sd = q.flt_sd; T.flt_sd_2 = q.flt_sd_2; T.Q_sd = q.Q_sd;
+    T.fcx = q.fcx; T.fcy = q.fcy; T.fcz = q.fcz;
+    T.Fdx_b = q.Fdx_b; T.Fdy_b = q.Fdy_b; T.Fdz_b = q.Fdz_b;
+    T.Fdx_da = q.Fdx_da; T.Fdy_da = q.Fdy_da; T.Fdz_da = q.Fdz_da;
+    T.Fdx_df = q.Fdx_df; T.Fdy_df = q.Fdy_df; T.Fdz_df = q.Fdz_df;
+    T.pgf_x = q.pgf_x; T.pgf_y = q.pgf_y; T.pgf_z = q.pgf_z;
+    T.DEMpor = q.DEMpor;
+    if (is_sph) {
+      T.dDEMpor = q.dDEMpor;
+      T.dDEMpor_prev = q.dDEMpor_prev;
+    }
+    T.DEMvf = q.DEMvf;
+    T.PPE1 = q.PPE1; T.PPE2 = q.PPE2;
+    T.test1 = q.test1; T.test2 = q.test2; T.test3 = q.test3; T.test4 = q.test4;
+    T.test5 = q.test5; T.test6 = q.test6; T.test7 = q.test7; T.test8 = q.test8;
+    T.rho = q.rho;
+  }
+}
+
+void orc_time(orc_state* s, int32_t phase) {
+  int stride = std::max(1, s->p.decouple_stride);
+  bool sph_on = (s->count % stride) == 0;
+  if (phase == SOPHIA_PREDICT) {
+    if (s->p.time_type != 1) return;
+    if (sph_on) predictor(*s, s->sph, stride * s->p.dt);
+    predictor(*s, s->dem, s->p.dt);
+  } else {
+    if (sph_on) corrector(*s, s->sph, stride * s->p.dt, true);
+    corrector(*s, s->dem, s->p.dt, false);
+    s->after_corrector = true;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PPE: KERNEL_Neumann_boundary3D_sph (functions_BC.cuh:139-274),           */
+/* KERNEL_PPE3D_sph (functions_PPE2.cuh:542-698), KERNEL_PPE3D_calc_sph     */
+/* (:868-878), KERNEL_pressureforce3D_sph (:2154-2335); Krylov solve new    */
+/* ------------------------------------------------------------------------ */
+static void neumann_walls(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+  int n = sp.n();
+  std::vector<double> np(n), nt(n), nr(n);
+  std::vector<char> act(n, 0), zero(n, 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.p_type > 0) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, 1.0 * h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double tp = 0, fl = 0, tt = 0, tr = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z) + 1e-20;
+      if (!(qj.p_type < 1000)) return;
+      if (!(r < 2.0 * R)) return;
+      double w = knl_w(kt, A, 2.0 * h, r);
+      double on = (qj.p_type >= 1);
+      double v = qj.m / qj.rho;
+      tp += v * qj.pres * w * on;
+      fl += v * w * on;
+      tt += v * qj.temp * w * on;
+      tr += qj.rho / qj.DEMpor * w * v * on;
+    });
+    act[i] = 1;
+    if (fl < 1e-6) {
+      zero[i] = 1;
+    } else {
+      np[i] = tp / fl;
+      nt[i] = tt / fl;
+      nr[i] = qi.DEMpor * tr / fl;
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    if (!act[i]) continue;
+    part1& q = sp.SP1[i];
+    if (zero[i]) {
+      q.rho = sp.SP2[i].rho_ref;
+      q.pres = 0.0;
+    } else {
+      q.pres = np[i];
+      q.temp = nt[i];
+      q.rho = nr[i];
+    }
+  }
+}
+
+static bool ppe_row(const orc_state& s, const part1& q) {
+  if (q.i_type > kITypeCrt) return false;
+  if (s.p.open_boundary > 0 && q.buffer_type > 0) return false;
+  if (q.p_type >= 1000 || q.p_type <= 0) return false;
+  return true;
+}
+
+static void ppe_assemble(orc_state& s, double tdt) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+  int n = sp.n();
+  s.row_mask.assign(n, 0);
+  s.rows.assign(n, PPERow{});
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    part1& qi = sp.SP1[i];
+    if (!ppe_row(s, qi)) continue;
+    s.row_mask[i] = 1;
+    PPERow& row = s.rows[i];
+    row.i = i;
+    double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
+    double rhoi = qi.rho, pori = qi.DEMpor;
+    double bx = 0, by = 0, bz = 0, Aij = 0, AP = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type < 1000)) return;
+      if (qj.i_type == 4) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, hi, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
+      double mj = qj.m, rhoj = qj.rho, porj = qj.DEMpor;
+      bx += mj / rhoj * (pori * uxi - porj * qj.ux) * gx;
+      by += mj / rhoj * (pori * uyi - porj * qj.uy) * gy;
+      bz += mj / rhoj * (pori * uzi - porj * qj.uz) * gz;
+      double c = -(dx * gx + dy * gy + dz * gz) / r / r;
+      double a = 1 * mj * ((pori * rhoi + porj * rhoj) / (rhoj * rhoi)) * c;
+      Aij += a;
+      AP += 1 * mj * ((pori * rhoi + porj * rhoj) / (rhoj * rhoi)) * c * qj.pres;
+      row.j.push_back(j);
+      row.a.push_back(a);
+    });
+    double bi = p.rho_fluid * (bx + by + bz + 2 * qi.dDEMpor - qi.dDEMpor_prev) / tdt;
+    qi.PPE3 = bi;
+    qi.PPE4 = AP;
+    qi.PPE1 = rhoi * (bx + by + bz) / tdt;
+    qi.PPE2 = Aij;
+  }
+  // Krylov right-hand side: b_i plus the lagged non-row columns
+  s.rhs.assign(n, 0.0);
+  for (int i = 0; i < n; i++) {
+    if (!s.row_mask[i]) continue;
+    double r = sp.SP1[i].PPE3;
+    const PPERow& row = s.rows[i];
+    for (size_t k = 0; k < row.j.size(); k++)
+      if (!s.row_mask[row.j[k]]) r += row.a[k] * sp.SP1[row.j[k]].pres;
+    s.rhs[i] = r;
+  }
+}
+
+/* y = A x over rows: (A x)_i = d_i x_i - sum_{j in rows} a_ij x_j */
+static void ppe_matvec(const orc_state& s, const std::vector<double>& x, std::vector<double>& y) {
+  int n = (int)x.size();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    if (!s.row_mask[i]) {
+      y[i] = 0.0;
+      continue;
+    }
+    const PPERow& row = s.rows[i];
+    double acc = s.sph.SP1[i].PPE2 * x[i];
+    for (size_t k = 0; k < row.j.size(); k++)
+      if (s.row_mask[row.j[k]]) acc -= row.a[k] * x[row.j[k]];
+    y[i] = acc;
+  }
+}
+
+static double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+  double acc = 0.0;
+  for (size_t i = 0; i < a.size(); i++) acc += a[i] * b[i];
+  return acc;
+}
+
+static void project_mean(const orc_state& s, std::vector<double>& v) {
+  double sum = 0;
+  int cnt = 0;
+  for (size_t i = 0; i < v.size(); i++)
+    if (s.row_mask[i]) { sum += v[i]; cnt++; }
+  if (!cnt) return;
+  double m = sum / cnt;
+  for (size_t i = 0; i < v.size(); i++)
+    if (s.row_mask[i]) v[i] -= m;
+}
+
+/* BiCGStab (van der Vorst 1992) with right Jacobi preconditioning on the
+ * fluid rows; walls and other non-rows enter through the right-hand side */
+static int ppe_bicgstab(orc_state& s, const sophia_ppe_opts& o, sophia_ppe_stats& st) {
+  Species& sp = s.sph;
+  int n = sp.n();
+  std::vector<double> x(n, 0.0), r(n), rh(n), v(n, 0.0), pv(n, 0.0), ph(n), sv(n), sh(n), t(n), Ax(n);
+  for (int i = 0; i < n; i++)
+    if (s.row_mask[i]) x[i] = (o.guess == SOPHIA_GUESS_ZERO) ? 0.0 : sp.SP1[i].pres;
+  std::vector<double> b = s.rhs;
+  if (o.zero_mean) project_mean(s, b);
+  double bn = std::sqrt(dotv(b, b));
+  if (bn == 0) bn = 1;
+  ppe_matvec(s, x, Ax);
+  for (int i = 0; i < n; i++) r[i] = s.row_mask[i] ? b[i] - Ax[i] : 0.0;
+  rh = r;
+  st.r0_rel = std::sqrt(dotv(r, r)) / bn;
+  st.guess_used = o.guess == SOPHIA_GUESS_ZERO ? SOPHIA_GUESS_ZERO : SOPHIA_GUESS_PREVIOUS;
+  double rho = 1, alpha = 1, omega = 1;
+  int it = 0;
+  double rn = st.r0_rel * bn;
+  st.converged = rn <= o.rtol * bn;
+  auto dinv = [&](int i) { return s.row_mask[i] && sp.SP1[i].PPE2 != 0 ? 1.0 / sp.SP1[i].PPE2 : 0.0; };
+  while (!st.converged && it < o.max_iter) {
+    double rho1 = dotv(rh, r);
+    if (rho1 == 0) break;
+    double beta = (rho1 / rho) * (alpha / omega);
+    for (int i = 0; i < n; i++) pv[i] = r[i] + beta * (pv[i] - omega * v[i]);
+    for (int i = 0; i < n; i++) ph[i] = pv[i] * dinv(i);
+    ppe_matvec(s, ph, v);
+    alpha = rho1 / dotv(rh, v);
+    for (int i = 0; i < n; i++) sv[i] = r[i] - alpha * v[i];
+    it++;
+    double sn = std::sqrt(dotv(sv, sv));
+    if (sn <= o.rtol * bn) {
+      for (int i = 0; i < n; i++) x[i] += alpha * ph[i];
+      r = sv;
+      rn = sn;
+      st.converged = 1;
+      break;
+    }
+    for (int i = 0; i < n; i++) sh[i] = sv[i] * dinv(i);
+    ppe_matvec(s, sh, t);
+    double tt = dotv(t, t);
+    omega = tt > 0 ? dotv(t, sv) / tt : 0.0;
+    for (int i = 0; i < n; i++) x[i] += alpha * ph[i] + omega * sh[i];
+    for (int i = 0; i < n; i++) r[i] = sv[i] - omega * t[i];
+    rho = rho1;
+    rn = std::sqrt(dotv(r, r));
+    if (rn <= o.rtol * bn) st.converged = 1;
+    if (omega == 0) break;
+  }
+  if (o.zero_mean) project_mean(s, x);
+  // true residual
+  ppe_matvec(s, x, Ax);
+  double tr = 0;
+  for (int i = 0; i < n; i++)
+    if (s.row_mask[i]) tr += (b[i] - Ax[i]) * (b[i] - Ax[i]);
+  st.r_rel = std::sqrt(tr) / bn;
+  st.iterations = it;
+  st.outer = 1;
+  for (int i = 0; i < n; i++)
+    if (s.row_mask[i]) sp.SP1[i].pres = x[i];
+  return 0;
+}
+
+static void pressure_force(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (!ppe_row(s, qi)) continue;
+    double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
+    double xi = qi.x, yi = qi.y, zi = qi.z, pi = qi.pres, rhoi = qi.rho, pori = qi.DEMpor;
+    double tx = 0, ty = 0, tz = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type < 1000)) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, hi, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
+      if (p.fp_solve) {
+        double Cp = -pori * 1 * qj.m * (pi + qj.pres) / (rhoi * qj.rho);
+        tx += Cp * gx;
+        ty += Cp * gy;
+        tz += Cp * gz;
+      }
+    });
+    part3& o = sp.P3[i];
+    o.fpx = tx;
+    o.fpy = ty;
+    o.fpz = tz;
+  }
+}
+
+int orc_ppe(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
+  sophia_ppe_stats local{};
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof(*st));
+  int stride = std::max(1, s->p.decouple_stride);
+  if ((s->count % stride) != 0 || s->p.dim != 3) return 0;
+  sophia_ppe_opts def{};
+  def.method = SOPHIA_PPE_JACOBI;
+  if (!o) o = &def;
+  if (s->count == 0) neumann_walls(*s);
+  ppe_assemble(*s, stride * s->p.dt);
+  int nrows = 0;
+  for (int v : s->row_mask) nrows += v;
+  st->n_rows = nrows;
+  if (o->method == SOPHIA_PPE_JACOBI) {
+    Species& sp = s->sph;
+    for (int i = 0; i < sp.n(); i++)
+      if (s->row_mask[i]) sp.SP1[i].pres = (sp.SP1[i].PPE3 + sp.SP1[i].PPE4) / sp.SP1[i].PPE2;
+    st->iterations = 1;
+    st->converged = 1;
+  } else {
+    ppe_bicgstab(*s, *o, *st);
+  }
+  neumann_walls(*s);
+  pressure_force(*s);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* XSPH: KERNEL_xsph3D_sph + KERNEL_xsph3D_calc_sph (functions_TIME.cuh:    */
+/* 1735-1850, 1985-1997), on the carried state P1                           */
+/* ------------------------------------------------------------------------ */
+void orc_xsph(orc_state* s) {
+  int stride = std::max(1, s->p.decouple_stride);
+  if (!s->p.xsph_solve || (s->count % stride) != 0 || s->p.dim != 3) return;
+  Species& sp = s->sph;
+  const sophia_params& p = s->p;
+  const int kt = p.kernel_type, dim = p.dim;
+  double dt = stride * p.dt;
+  int n = sp.n();
+  std::vector<double> vx(n), vy(n), vz(n), vt(n);
+  std::vector<char> act(n, 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& qi = sp.P1[i];
+    if (qi.i_type != 1 || qi.p_type > 1000 || qi.p_type == 0 || qi.p_type == 9) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz, ti = qi.temp;
+    double tx = 0, ty = 0, tz = 0, tt = 0, fl = 0;
+    for_candidates(*s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.P1[j];
+      if (!(qj.p_type == 1 || qj.p_type == 3)) return;
+      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z);
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r);
+      double v = qj.m / qj.rho;
+      tx += p.c_xsph * v * (-uxi + qj.ux) * w;
+      ty += p.c_xsph * v * (-uyi + qj.uy) * w;
+      tz += p.c_xsph * v * (-uzi + qj.uz) * w;
+      tt += 0.01 * p.c_xsph * v * (qj.temp - ti) * w;
+      fl += v * w;
+    });
+    double on = (qi.p_type > 0);
+    vx[i] = (uxi + tx / fl) * on;
+    vy[i] = (uyi + ty / fl) * on;
+    vz[i] = (uzi + tz / fl) * on;
+    vt[i] = (ti + tt / fl) * on;
+    act[i] = 1;
+  }
+  for (int i = 0; i < n; i++) {
+    if (!act[i]) continue;
+    part1& q = sp.P1[i];
+    q.XSPH_ux = vx[i]; q.XSPH_uy = vy[i]; q.XSPH_uz = vz[i]; q.XSPH_temp = vt[i];
+    q.ux = vx[i]; q.uy = vy[i]; q.uz = vz[i]; q.temp = vt[i];
+    if (p.fix_xsph_position && q.p_type > 0 && !p.eulerian_sph) {
+      const part2& q2 = sp.SP2[i];
+      q.x = q2.x0 + vx[i] * dt;
+      q.y = q2.y0 + vy[i] * dt;
+      q.z = q2.z0 + vz[i] * dt;
+    }
+  }
+}
+
+/* one ISPH-DEM time step in the order of SOPHIA_single_ISPH
+ * (ISPH_Calc_seperate.cuh:13-503); open-boundary and DEM wall kernels are
+ * outside this build's scope */
+int orc_step(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
+  orc_nnps(s);
+  orc_prep(s);
+  orc_interaction(s);
+  orc_dem_interaction(s);
+  orc_sph_dem_coupling(s);
+  orc_time(s, SOPHIA_PREDICT);
+  orc_ppe(s, o, st);
+  orc_time(s, SOPHIA_CORRECT);
+  orc_xsph(s);
+  s->time += s->p.dt;
+  s->count += 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+void orc_get_nnps(const orc_state* s, int32_t species, int32_t* keys, int32_t* order,
+                  uint32_t* cell_start, uint32_t* cell_end, int32_t* nbr_count) {
+  const Species& sp = species == SOPHIA_SPH ? s->sph : s->dem;
+  int n = sp.n();
+  if (keys) std::copy(sp.key.begin(), sp.key.end(), keys);
+  if (order) std::copy(sp.order.begin(), sp.order.end(), order);
+  if (cell_start) std::copy(sp.g_str.begin(), sp.g_str.begin() + s->d.num_cells, cell_start);
+  if (cell_end) std::copy(sp.g_end.begin(), sp.g_end.begin() + s->d.num_cells, cell_end);
+  if (nbr_count) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+      const part1& qi = sp.SP1[i];
+      double R = s->p.kappa * qi.h;
+      int c = 0;
+      for_candidates(*s, sp, qi.x, qi.y, qi.z, [&](int j) {
+        const part1& qj = sp.SP1[j];
+        double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z) + 1e-20;
+        if (r < R) c++;
+      });
+      nbr_count[i] = c;
+    }
+  }
+}
+
+void orc_get_particles(const orc_state* s, int32_t species, part1* p1, part2* p2, part3* p3) {
+  const Species& sp = species == SOPHIA_SPH ? s->sph : s->dem;
+  int n = sp.n();
+  if (p1) {
+    const std::vector<part1>& src = s->after_corrector ? sp.P1 : sp.SP1;
+    std::copy(src.begin(), src.end(), p1);
+    if (species == SOPHIA_DEM && !s->after_corrector) {
+      for (int i = 0; i < n; i++) {
+        std::memcpy(p1[i].ct_idx, sp.P1[i].ct_idx, sizeof(p1[i].ct_idx));
+        std::memcpy(p1[i].del_s, sp.P1[i].del_s, sizeof(p1[i].del_s));
+      }
+    }
+  }
+  if (p2) std::copy(sp.SP2.begin(), sp.SP2.end(), p2);
+  if (p3) std::copy(sp.P3.begin(), sp.P3.end(), p3);
+}
+
+void orc_set_pressure(orc_state* s, const double* p, int32_t n) {
+  Species& sp = s->sph;
+  for (int i = 0; i < n && i < sp.n(); i++) {
+    sp.SP1[i].pres = p[i];
+    sp.P1[i].pres = p[i];
+  }
+}
+
+void orc_get_ppe_system(const orc_state* s, double* diag, double* rhs, int32_t* row_mask) {
+  const Species& sp = s->sph;
+  int n = sp.n();
+  for (int i = 0; i < n; i++) {
+    bool on = i < (int)s->row_mask.size() && s->row_mask[i];
+    if (diag) diag[i] = on ? sp.SP1[i].PPE2 : 0.0;
+    if (rhs) rhs[i] = on ? s->rhs[i] : 0.0;
+    if (row_mask) row_mask[i] = on;
+  }
+}
+
+void orc_ppe_apply(const orc_state* s, const double* x, double* y) {
+  int n = s->sph.n();
+  std::vector<double> xv(x, x + n), yv(n);
+  ppe_matvec(*s, xv, yv);
+  std::copy(yv.begin(), yv.end(), y);
+}
+
+double orc_knl_tmpA(int32_t kt, int32_t dim, double h) { return knl_tmpA(kt, dim, h); }
+
+void orc_knl_eval(int32_t kt, int32_t dim, double h, const double* r, int32_t n, double* w, double* dw) {
+  double A = knl_tmpA(kt, dim, h);
+  for (int i = 0; i < n; i++) {
+    if (w) w[i] = knl_w(kt, A, h, r[i]);
+    if (dw) dw[i] = knl_dw(kt, A, h, r[i]);
+  }
+}
+
+int32_t orc_cell_key(const orc_state* s, double x, double y, double z) {
+  int a, b, c;
+  cell_of(*s, x, y, z, a, b, c);
+  return idx_cell(*s, a, b, c);
+}
+
+void orc_brute_neighbor_counts(const orc_state* s, int32_t* counts) {
+  const Species& sp = s->sph;
+  int n = sp.n();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& qi = sp.SP1[i];
+    double R = s->p.kappa * qi.h;
+    int c = 0;
+    for (int j = 0; j < n; j++) {
+      const part1& qj = sp.SP1[j];
+      double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z) + 1e-20;
+      if (r < R) c++;
+    }
+    counts[i] = c;
+  }
+}
+
+}  // extern "C"

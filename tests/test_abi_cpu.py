@@ -1,0 +1,118 @@
+"""The C-ABI boundary on the CPU: the library loads without a GPU, exports
+every symbol include/sophia/abi.h declares, keeps the reference's record
+layout, parses the reference's input formats, and fails loudly (an error
+code, not a fallback) when no GPU is present."""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fluidized_bed_master_amd import abi
+from fluidized_bed_master_amd.sophia import Sophia, read_input, read_ptype, read_solv
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load_library()
+    declared = abi.header_symbols()
+    assert len(declared) >= 35
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    # and the Python binding covers all of them
+    assert set(declared) <= set(abi.SIGNATURES), set(declared) - set(abi.SIGNATURES)
+
+
+def test_record_layout_matches_reference():
+    # class_Cuda_Particle_Array2.cuh: part1 1272 B, part2 176 B, part3 496 B
+    assert abi.PART1.itemsize == 1272
+    assert abi.PART2.itemsize == 176
+    assert abi.PART3.itemsize == 496
+    # a few field offsets that follow from the reference's field order
+    assert abi.PART1.fields["x"][1] == 24          # 5 ints, padded to 8
+    assert abi.PART1.fields["m"][1] == 24 + 12 * 8  # after x..wz
+    assert abi.PART1.fields["ct_idx"][1] == abi.PART1.fields["Q_f"][1] + 8
+    assert abi.PART3.fields["Cm"][1] == 29 * 8
+
+
+def test_params_default_are_reference_constants():
+    p = abi.default_params()
+    assert p.rho_fluid == 1.784 and p.rho_water == 1000.0    # Parameters.cuh:52-53
+    assert p.h_over_dx == 1.6                                 # functions_MASS.cuh:33
+    assert p.mu_fluid == 2.0e-5                               # VISCOSITY_AIR
+    assert p.gravity == 9.8
+    assert p.dem_shear_modulus == 8.3e8 and p.dem_poisson == 0.23
+    assert p.dem_friction == 0.5
+    assert abs(p.dem_damping - np.sqrt(3.333333333) * 0.0535184) < 1e-15
+    assert p.dem_hertz_factor == 1.33333
+    assert p.eulerian_sph == 1
+
+
+def test_read_solv_reference_file():
+    p = read_solv(str(GOLD / "reference_solv.txt"))
+    assert p.solver_type == 1 and p.dim == 3
+    assert p.kernel_type == abi.KNL_WENDLAND6
+    assert p.rho_type == 1 and p.time_type == 1
+    assert p.open_boundary == 1
+    assert p.fp_solve == 1 and p.fv_solve == 1 and p.fva_solve == 0 and p.fg_solve == 1
+    assert p.con_solve == 0 and p.xsph_solve == 1 and p.kgc_solve == 1 and p.delSPH_solve == 2
+    assert p.soundspeed == 200.0 and p.rho0_eos == 1.784 and p.gamma == 1.0 and p.kappa == 2.0
+    assert p.c_xsph == 0.1 and p.dt == 5.0e-7 and p.time == 0.0 and p.time_end == 5.0
+    assert p.freq_filt == 40 and p.freq_output == 10000 and p.u_limit == 40.0
+    assert p.min_iteration == 3 and p.max_iteration == 10
+    assert p.decouple_stride == 1
+    assert p.xmargin_m == 0.0 and p.zmargin_p == 0.0
+
+
+def test_read_input_reference_format(tmp_path):
+    # header of column codes (function_init.cuh:704-739), tab separated
+    f = tmp_path / "input.txt"
+    rows = [
+        (0.001, 0.002, 0.003, 0.1, 0.0, -0.2, 1.0e-9, 1, 0.0013, 293.0, 5.0, 1000.0, 0.0, 0, 0),
+        (0.004, 0.005, 0.006, 0.0, 0.0, 0.0, 2.0e-8, 1001, 0.0013, 300.0, 0.0, 2500.0, 0.001, 7, 1),
+    ]
+    with open(f, "w") as fh:
+        fh.write("\t".join(str(c) for c in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 27, 29, 18)) + "\n")
+        for r in rows:
+            fh.write("\t".join(repr(v) for v in r) + "\n")
+    a = read_input(str(f))
+    assert len(a) == 2
+    assert a["x"][0] == 0.001 and a["uz"][0] == -0.2 and a["p_type"][1] == 1001
+    assert a["rad"][1] == 0.001 and a["dem_idx"][1] == 7 and a["ct_boundary"][1] == 1
+    assert (a["i_type"] == 1).all()
+
+
+def test_read_ptype_reference_counts(tmp_path):
+    d = json.load(open(GOLD / "reference_ptype_rle.json"))
+    vals = np.concatenate([np.full(n, v, np.int32) for v, n in d["runs"]])
+    f = tmp_path / "p_type.txt"
+    np.savetxt(f, vals, fmt="%d")
+    out = read_ptype(str(f))
+    assert len(out) == d["n"] == 122574
+    u, c = np.unique(out, return_counts=True)
+    assert dict(zip(map(str, u), map(int, c))) == d["counts"]
+
+
+def test_no_gpu_is_an_error_not_a_fallback():
+    """Without a device the product path raises (SOPHIA_E_HIP); with one it runs."""
+    from fluidized_bed_master_amd import configs
+    p, parts = configs.small_isph_dem()
+    try:
+        g = Sophia(p, parts)
+    except abi.SophiaError as e:
+        assert e.rc == -2, e
+        assert "no HIP device" in str(e)
+        return
+    g.close()
+    pytest.skip("a GPU is present: the GPU suite covers this path")
+
+
+def test_create_rejects_bad_arguments():
+    from fluidized_bed_master_amd import configs
+    p, parts = configs.small_isph_dem()
+    p.dim = 4
+    with pytest.raises(abi.SophiaError) as e:
+        Sophia(p, parts)
+    assert e.value.rc == -1

@@ -1,0 +1,144 @@
+"""HIP library vs the FP64 oracle, stage by stage, on seeded columns.
+
+Bar: cell keys, sorted order, cell ranges and neighbour counts bit-exact;
+fields within 1e-9 (FP64 path) and 1e-5 (mixed FP32 path) norm-wise
+relative error max|gpu - oracle| / max|oracle| per field.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fluidized_bed_master_amd import abi, configs
+from tests.helpers import (DEM_STAGE_FIELDS, DEM_STAGE_P3, SPH_STAGE_FIELDS, SPH_STAGE_P3, compare_fields, rel_err)
+
+pytestmark = pytest.mark.gpu
+
+TOL = {abi.PREC_FP64: 1e-9, abi.PREC_MIXED: 1e-5}
+
+
+def _report(name, data):
+    path = os.environ.get("SOPHIA_REPORT")
+    if not path:
+        return
+    try:
+        old = json.load(open(path)) if os.path.exists(path) else {}
+    except Exception:
+        old = {}
+    old[name] = data
+    with open(path, "w") as f:
+        json.dump(old, f, indent=1, sort_keys=True)
+
+
+def _pair(prec, **kw):
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.small_isph_dem(**kw)
+    return Sophia(p, parts, precision=prec), Oracle(p, parts), p
+
+
+def _check_nnps(g, o):
+    for sp in (abi.SPH, abi.DEM):
+        a = g.get_nnps(sp)
+        b = o.get_nnps(sp, counts=(sp == abi.SPH))
+        assert np.array_equal(a["keys"], b["keys"]), f"species {sp}: keys differ"
+        assert np.array_equal(a["order"], b["order"]), f"species {sp}: order differs"
+        assert np.array_equal(a["cell_start"], b["cell_start"]), f"species {sp}: cell starts differ"
+        nonempty = b["cell_start"] != 0xFFFFFFFF
+        assert np.array_equal(a["cell_end"][nonempty], b["cell_end"][nonempty]), f"species {sp}: cell ends differ"
+        if sp == abi.SPH:
+            assert np.array_equal(a["nbr_count"], b["nbr_count"]), "neighbour counts differ"
+
+
+def _cmp(g, o, stage, tol, rep):
+    for sp, F1, F3 in ((abi.SPH, SPH_STAGE_FIELDS, SPH_STAGE_P3), (abi.DEM, DEM_STAGE_FIELDS, DEM_STAGE_P3)):
+        a1, a2, a3 = g.get_particles(sp)
+        b1, b2, b3 = o.get_particles(sp)
+        if stage in F1:
+            compare_fields(a1, b1, F1[stage], tol, f"{stage}/{'sph' if sp == 0 else 'dem'}", report=rep)
+        if stage in F3:
+            compare_fields(a3, b3, F3[stage], tol, f"{stage}/{'sph' if sp == 0 else 'dem'}/p3", report=rep)
+
+
+@pytest.mark.parametrize("n_pre", [0, 1], ids=["step0", "step1"])
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_stages_match_oracle(prec, n_pre):
+    """Every stage of one step; after n_pre full steps the second step also
+    exercises the carried state (shear-history lookups, eps rate history)."""
+    g, o, p = _pair(prec)
+    tol = TOL[prec] * (10 if n_pre else 1)
+    rep = {}
+    opts = abi.ppe_opts(method=abi.PPE_JACOBI)
+    for _ in range(n_pre):
+        g.step(opts)
+        o.step(opts)
+    g.nnps(); o.nnps()
+    _check_nnps(g, o)
+    g.prep(); o.prep()
+    _cmp(g, o, "prep", tol, rep)
+    g.interaction(); o.interaction()
+    _cmp(g, o, "interaction", tol, rep)
+    g.dem_interaction(); o.dem_interaction()
+    _cmp(g, o, "dem_interaction", tol, rep)
+    g.sph_dem_coupling(); o.sph_dem_coupling()
+    _cmp(g, o, "coupling", tol, rep)
+    g.predict(); o.predict()
+    _cmp(g, o, "predict", tol, rep)
+    g.ppe(opts); o.ppe(opts)
+    _cmp(g, o, "ppe", tol, rep)
+    g.correct(); o.correct()
+    _cmp(g, o, "correct", tol, rep)
+    g.xsph(); o.xsph()
+    _cmp(g, o, "xsph", tol, rep)
+    _report(f"stages_{'fp64' if prec == 0 else 'mixed'}_pre{n_pre}", rep)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_full_steps_jacobi(prec):
+    g, o, p = _pair(prec, seed=11)
+    opts = abi.ppe_opts(method=abi.PPE_JACOBI)
+    for _ in range(3):
+        g.step(opts)
+        o.step(opts)
+    rep = {}
+    tol = TOL[prec] * 10  # three steps of accumulated rounding
+    a1, _, a3 = g.get_particles(abi.SPH)
+    b1, _, b3 = o.get_particles(abi.SPH)
+    compare_fields(a1, b1, ["ux", "uy", "uz", "pres", "rho", "DEMpor"], tol, "3 steps/sph", report=rep)
+    a1, _, a3 = g.get_particles(abi.DEM)
+    b1, _, b3 = o.get_particles(abi.DEM)
+    compare_fields(a1, b1, ["x", "y", "z", "ux", "uy", "uz"], tol, "3 steps/dem", report=rep)
+    _report(f"steps_{'fp64' if prec == 0 else 'mixed'}", rep)
+
+
+@pytest.mark.parametrize("prec,ppe_prec", [(abi.PREC_FP64, abi.PPE_FP64), (abi.PREC_MIXED, abi.PPE_MIXED_IR)],
+                         ids=["fp64", "mixed_ir"])
+def test_bicgstab_matches_oracle(prec, ppe_prec):
+    g, o, p = _pair(prec, seed=3)
+    go = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=ppe_prec, rtol=1e-8)
+    oo = abi.ppe_opts(method=abi.PPE_BICGSTAB, rtol=1e-12)
+    for s in (g, o):
+        s.nnps(); s.prep(); s.interaction(); s.dem_interaction(); s.sph_dem_coupling(); s.predict()
+    sg = g.ppe(go)
+    so = o.ppe(oo)
+    assert sg["converged"], sg
+    assert sg["r_rel"] <= 1e-8 * 1.0001, sg
+    a1, _, _ = g.get_particles(abi.SPH)
+    b1, _, _ = o.get_particles(abi.SPH)
+    rows = (b1["p_type"] >= 1) & (b1["p_type"] < 1000)
+    err = rel_err(a1["pres"][rows], b1["pres"][rows])
+    _report(f"bicgstab_{'fp64' if prec == 0 else 'mixed'}", {"p_rel_err": err, "gpu": sg, "oracle": so})
+    assert err <= 1e-4, err
+
+
+def test_knl_eval_matches_oracle():
+    from fluidized_bed_master_amd.sophia import knl_eval
+    from oracle import oracle
+    r = np.linspace(0, 3.2e-3, 257)
+    for kt in range(6):
+        for dim in (2, 3):
+            w, dw = knl_eval(kt, dim, 1.3e-3, r)
+            w0, dw0 = oracle.knl_eval(kt, dim, 1.3e-3, r)
+            assert rel_err(w, w0) < 1e-13, (kt, dim)
+            assert rel_err(dw, dw0) < 1e-13, (kt, dim)
