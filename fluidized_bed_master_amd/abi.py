@@ -109,6 +109,7 @@ class PpeOpts(C.Structure):
     _fields_ = [
         ("method", C.c_int32), ("precision", C.c_int32), ("guess", C.c_int32), ("max_iter", C.c_int32),
         ("max_outer", C.c_int32), ("zero_mean", C.c_int32), ("rtol", C.c_double), ("inner_rtol", C.c_double),
+        ("train", C.c_int32), ("reserved_i", C.c_int32), ("lr", C.c_double),
     ]
 
 
@@ -116,7 +117,7 @@ class PpeStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("guess_used", C.c_int32),
         ("r0_rel", C.c_double), ("r0_rel_learned", C.c_double), ("r_rel", C.c_double), ("ms", C.c_double),
-        ("n_rows", C.c_int32), ("reserved", C.c_int32),
+        ("n_rows", C.c_int32), ("reserved", C.c_int32), ("loss", C.c_double),
     ]
 
     def as_dict(self):
@@ -124,8 +125,8 @@ class PpeStats(C.Structure):
 
 
 def ppe_opts(method=PPE_JACOBI, precision=PPE_FP64, guess=GUESS_PREVIOUS, rtol=1e-8, max_iter=10000,
-             max_outer=20, inner_rtol=1e-4, zero_mean=0) -> PpeOpts:
-    return PpeOpts(method, precision, guess, max_iter, max_outer, zero_mean, rtol, inner_rtol)
+             max_outer=20, inner_rtol=1e-4, zero_mean=0, train=0, lr=1e-3) -> PpeOpts:
+    return PpeOpts(method, precision, guess, max_iter, max_outer, zero_mean, rtol, inner_rtol, train, 0, lr)
 
 
 _P = C.POINTER
@@ -170,6 +171,7 @@ SIGNATURES = {
     "sophia_mlp_train_step": (C.c_int, [_ctx, _VP, C.c_int32, C.c_double, _P(C.c_double)]),
     "sophia_mlp_loss_grad": (C.c_int, [_ctx, _VP, C.c_int32, _P(C.c_double), _VP, C.c_int32]),
     "sophia_mlp_reset_adam": (C.c_int, [_ctx]),
+    "sophia_mlp_features": (C.c_int, [_ctx, _VP, C.c_int32, _P(C.c_double)]),
     "sophia_timers_enable": (C.c_int, [_ctx, C.c_int32]),
     "sophia_timers_report": (C.c_int, [_ctx, C.c_char_p, C.c_int32]),
 }

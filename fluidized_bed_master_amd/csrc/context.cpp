@@ -126,6 +126,7 @@ struct sophia_ctx {
   virtual int mlp_set(const float* w, int n) = 0;
   virtual int mlp_get(float* w, int n) const = 0;
   virtual int mlp_reset() = 0;
+  virtual int mlp_feat(float* nf, int n, double* scale) = 0;
   Timers timers;
   Timers* tm() { return timers.on ? &timers : nullptr; }
 };
@@ -476,7 +477,10 @@ struct Ctx : public sophia_ctx {
       for (auto* fl : {&sptype, &sbtype, &sitype}) fl->swap();
       refresh_views();
       // P3 reset (ISPH_Calc_seperate.cuh:102-105)
-      for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp})
+      // pgf lives in P1 in the reference and is rewritten each step for every
+      // fluid particle; the others (walls, skipped types) keep their input
+      // value, which is zero, and the DEM drag reads it for walls
+      for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz})
         TRY(fill_zero_bytes(stream, a, (size_t)ns * sizeof(R)));
       TRY(fill_zero_bytes(stream, sv.cm, (size_t)6 * ns * sizeof(R)));
       // neighbour list with capacity check
@@ -657,13 +661,19 @@ struct Ctx : public sophia_ctx {
     } else {
       const double* guess = nullptr;
       st->guess_used = o->guess == SOPHIA_GUESS_ZERO ? SOPHIA_GUESS_ZERO : SOPHIA_GUESS_PREVIOUS;
+      const bool want_mlp = (o->guess == SOPHIA_GUESS_LEARNED || o->train) && mlp.ready;
+      double scale = 0;
+      if (want_mlp) {
+        TRY(mlp_prepare(&scale));
+        HIP_TRY(hipMemcpyAsync(mlp.pprev, sv.pres, (size_t)ns * 8, hipMemcpyDeviceToDevice, stream));
+      }
       if (o->guess == SOPHIA_GUESS_LEARNED && mlp.ready) {
         // learned guess with residual-based fallback to the previous pressure
-        TRY(mlp_predict_rows(stream, P, sv, mlp, ppe_w.row, mlp.guess));
+        TimeScope t(tm(), "mlp_forward");
+        TRY(mlp_forward<R>(stream, P, sv, mlp, ppe_w.row, sv.pres, scale, mlp.guess));
         double r_l = 0, r_p = 0;
         TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, mlp.guess, &r_l));
-        TRY(mlp_gather_previous(stream, ns, ppe_w.row, sv.pres, ppe_w.x64));
-        TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, ppe_w.x64, &r_p));
+        TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, &r_p));
         st->r0_rel_learned = r_l;
         if (r_l < r_p) {
           guess = mlp.guess;
@@ -674,6 +684,14 @@ struct Ctx : public sophia_ctx {
       if (oo.max_iter <= 0) oo.max_iter = 10000;
       if (oo.rtol <= 0) oo.rtol = 1e-8;
       TRY(ppe_solve(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, oo, *st, guess, tm()));
+      if (o->train && mlp.ready) {
+        // one Adam step on (features before the solve, converged pressure)
+        TimeScope t(tm(), "mlp_train");
+        double loss = 0;
+        TRY(mlp_backward<R>(stream, P, sv, mlp, ppe_w.row, mlp.pprev, sv.pres, scale, n_rows, true,
+                            o->lr > 0 ? o->lr : 1e-3, &loss));
+        st->loss = loss;
+      }
     }
     {
       TimeScope t(tm(), "neumann");
@@ -894,22 +912,60 @@ struct Ctx : public sophia_ctx {
   }
 
   // --- learned warm start -------------------------------------------------------
-  int mlp_predict(double* out, int n) override {
+  // residual of the current pressure and the network's node inputs
+  int mlp_prepare(double* scale) {
+    TRY(ppe_apply64(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, mlp.r0, true, nullptr));
+    return mlp_features<R>(stream, devparams(), sv, mlp, ppe_w.row, ppe_w.diag, scale);
+  }
+  int mlp_check(int n) const {
     if (!ppe_assembled || n != ns) {
-      set_error("mlp_predict: PPE not assembled or n mismatch");
+      set_error("learned warm start: PPE not assembled this step or n=%d != %d", n, ns);
       return SOPHIA_E_STATE;
     }
-    TRY(mlp_predict_rows(stream, devparams(), sv, mlp, ppe_w.row, mlp.guess));
+    if (!mlp.ready) {
+      set_error("learned warm start: parameters not set");
+      return SOPHIA_E_STATE;
+    }
+    return 0;
+  }
+  int mlp_predict(double* out, int n) override {
+    TRY(mlp_check(n));
+    double scale = 0;
+    TRY(mlp_prepare(&scale));
+    TRY(mlp_forward<R>(stream, devparams(), sv, mlp, ppe_w.row, sv.pres, scale, mlp.guess));
     HIP_TRY(hipMemcpyAsync(out, mlp.guess, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return 0;
   }
   int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) override {
+    TRY(mlp_check(n));
+    if (grad && n_params != kNParam) {
+      set_error("mlp_loss_grad: buffer of %d, network has %d", n_params, kNParam);
+      return SOPHIA_E_ARG;
+    }
+    double scale = 0;
+    TRY(mlp_prepare(&scale));
+    HIP_TRY(hipMemcpyAsync(mlp.target, target, (size_t)n * 8, hipMemcpyHostToDevice, stream));
+    TRY(mlp_backward<R>(stream, devparams(), sv, mlp, ppe_w.row, sv.pres, mlp.target, scale, n_rows, step, lr, loss));
+    if (grad) {
+      HIP_TRY(hipMemcpyAsync(grad, mlp.grad, (size_t)kNParam * 4, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    return 0;
+  }
+  int mlp_feat(float* nf, int n, double* scale) override {
     if (!ppe_assembled || n != ns) {
-      set_error("mlp_train: PPE not assembled or n mismatch");
+      set_error("mlp_features: PPE not assembled this step or n mismatch");
       return SOPHIA_E_STATE;
     }
-    return mlp_train_rows(stream, devparams(), sv, mlp, ppe_w.row, target, lr, loss, grad, n_params, step);
+    double sc = 0;
+    TRY(mlp_prepare(&sc));
+    if (scale) *scale = sc;
+    if (nf) {
+      HIP_TRY(hipMemcpyAsync(nf, mlp.nf, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    return 0;
   }
   int mlp_set(const float* w, int n) override { return mlp_set_params(stream, mlp, w, n); }
   int mlp_get(float* w, int n) const override { return mlp_get_params(stream, mlp, w, n); }
@@ -1415,6 +1471,10 @@ int sophia_mlp_loss_grad(sophia_ctx* c, const double* target, int32_t n, double*
   return c->mlp_train(target, n, 0.0, loss, grad, np, false);
 }
 int sophia_mlp_reset_adam(sophia_ctx* c) { CHECK_CTX(c); return c->mlp_reset(); }
+int sophia_mlp_features(sophia_ctx* c, float* nf, int32_t n, double* scale) {
+  CHECK_CTX(c);
+  return c->mlp_feat(nf, n, scale);
+}
 
 int sophia_timers_enable(sophia_ctx* c, int32_t on) {
   CHECK_CTX(c);

@@ -1,57 +1,616 @@
-// Learned PPE warm start (placeholder until the MFMA kernels land).
+// Learned PPE warm start on gfx950 MFMA.
+//
+// Network: edge MLP e_ij -> 64 -> 64 (ReLU), sum over the neighbour list
+// scaled by 1/64, node MLP 64 -> 64 (ReLU) -> 1.  One wave64 owns one
+// particle at a time (grid-stride over particles) and walks its neighbour
+// list in tiles of 16 edges:
+//   layer 1  D[16 x 64] = E[16 x 32] W1^T[32 x 64]   4 x v_mfma_f32_16x16x32_bf16
+//            (the 6 edge inputs padded to K = 32 with zeros)
+//   layer 2  D[16 x 64] = H1[16 x 64] W2^T[64 x 64]  8 x v_mfma_f32_16x16x32_bf16
+// MFMA operand layout (16x16x32): lane l holds row (l mod 16) of A and
+// column (l mod 16) of B, K elements 8*(l/16) .. 8*(l/16)+7; the FP32
+// result holds rows 4*(l/16) .. +3 of column (l mod 16).  The layer-1
+// result is re-laid out into the layer-2 A operand through a 16 x 64 BF16
+// LDS tile.  The node MLP (64 x 64 per particle, 1/80 of the edge work)
+// runs on the VALU with the same BF16 weights.
+//
+// Backward recomputes the forward per particle (two passes over its edges)
+// and accumulates weight gradients with v_mfma_f32_16x16x16_bf16, whose A
+// and B operands are exactly the FP32 result layout above (K = the 16 edges
+// of a tile), so dW2 += dZ2^T H1 and dW1 += dZ1^T E need no re-layout.
+// dH1 = dZ2 W2 goes through a second LDS tile.  Each persistent block
+// writes its partial gradient; a second kernel sums the partials in a fixed
+// order (deterministic), then Adam updates the FP32 master copy and the
+// BF16 copy is refreshed.
 #include "learned.hpp"
 
 namespace sophia {
 
 namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kFwdBlocks = 4096;
+
+__device__ inline f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma16(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+struct Net {
+  const __bf16* wbf;
+  const float* master;  // biases are read in FP32
+};
+
+// B operands of layer 1 (B[k][n] = W1[n][k], k < 6) and layer 2
+// (B[k][n] = W2[n][k]) for this lane
+__device__ inline void load_fwd_operands(const Net& net, int lane, bf8 (&B1)[4], bf8 (&B2)[4][2]) {
+  const int kq = 8 * (lane >> 4), nl = lane & 15;
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+    const int nn = nl + 16 * nb;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = kq + j;
+      B1[nb][j] = k < kEin ? net.wbf[OFF_W1 + nn * kEin + k] : (__bf16)0.0f;
+      B2[nb][0][j] = net.wbf[OFF_W2 + nn * kH + k];
+      B2[nb][1][j] = net.wbf[OFF_W2 + nn * kH + 32 + k];
+    }
+  }
+}
+
+template <typename R>
+struct EdgeTile {
+  bf8 a1;            // layer-1 A operand (lanes 0-15 carry the 6 inputs)
+  uint32_t valid;    // bit e: edge e of the tile exists
+};
+
+// inputs of edge (i, j): (x_i - x_j)/h_i, |x_i - x_j|/h_i, rho_j/rho_f, s_j
+template <typename R>
+__device__ inline EdgeTile<R> edge_tile(const SphView<R>& v, const float2* nf, int i, int c, int cnt, int lane,
+                                        float ein[kEin]) {
+  EdgeTile<R> t;
+  for (int q = 0; q < 8; q++) t.a1[q] = (__bf16)0.0f;
+  for (int q = 0; q < kEin; q++) ein[q] = 0.f;
+  bool ok = false;
+  if (lane < 16) {
+    const int e = c * 16 + lane;
+    if (e < cnt) {
+      const uint32_t ent = v.nbr[(size_t)e * v.n + i];
+      if (!(ent & kNbrFlag)) {
+        const int j = (int)ent;
+        const double ih = 1.0 / v.h[i];
+        const double dx = (v.x[i] - v.x[j]) * ih, dy = (v.y[i] - v.y[j]) * ih, dz = (v.z[i] - v.z[j]) * ih;
+        const float2 f = nf[j];
+        ein[0] = (float)dx;
+        ein[1] = (float)dy;
+        ein[2] = (float)dz;
+        ein[3] = (float)sqrt(dx * dx + dy * dy + dz * dz);
+        ein[4] = f.x;
+        ein[5] = f.y;
+        ok = true;
+      }
+    }
+    for (int q = 0; q < kEin; q++) t.a1[q] = (__bf16)ein[q];
+  }
+  t.valid = (uint32_t)(__ballot(ok) & 0xffffull);
+  return t;
+}
+
+// layer 1 + ReLU for one tile; z1 in result layout, H (LDS) gets BF16 h1
+__device__ inline void layer1(const Net& net, const bf8 (&B1)[4], bf8 a1, int lane, f4 (&h1)[4], __bf16 (*H)[kH + 8]) {
+  const int nl = lane & 15, er = 4 * (lane >> 4);
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+    f4 z = {0.f, 0.f, 0.f, 0.f};
+    z = mfma32(a1, B1[nb], z);
+    const float b = net.master[OFF_B1 + nl + 16 * nb];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float hv = fmaxf(z[r] + b, 0.f);
+      __bf16 hb = (__bf16)hv;
+      h1[nb][r] = (float)hb;
+      H[er + r][nl + 16 * nb] = hb;
+    }
+  }
+}
+
+// layer 2 pre-activation for one tile (A operand read back from H)
+__device__ inline void layer2(const bf8 (&B2)[4][2], int lane, __bf16 (*H)[kH + 8], f4 (&z2)[4]) {
+  bf8 a0, a1;
+  const int e = lane & 15, kq = 8 * (lane >> 4);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    a0[j] = H[e][kq + j];
+    a1[j] = H[e][32 + kq + j];
+  }
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+    f4 z = {0.f, 0.f, 0.f, 0.f};
+    z = mfma32(a0, B2[nb][0], z);
+    z = mfma32(a1, B2[nb][1], z);
+    z2[nb] = z;
+  }
+}
+
+// node MLP forward: z3 = W3 m + b3 (lane l = output l), y = W4 relu(z3) + b4
+__device__ inline float node_forward(const Net& net, const float* Ms, const __bf16* W3s, int lane, float& z3) {
+  float z = net.master[OFF_B3 + lane];
+  for (int k = 0; k < kH; k++) z += (float)W3s[lane * kH + k] * Ms[k];
+  z3 = z;
+  float g = fmaxf(z, 0.f);
+  float y = (float)net.wbf[OFF_W4 + lane] * g;
+  for (int off = 32; off > 0; off >>= 1) y += __shfl_xor(y, off);
+  return y + net.master[OFF_B4];
+}
+
+// Forward over the edges of particle i: returns the aggregated message
+// (ready in Ms, LDS) -- shared by the forward and backward kernels.
+template <typename R>
+__device__ inline void aggregate(const Net& net, const SphView<R>& v, const float2* nf, int i, int lane,
+                                 const bf8 (&B1)[4], const bf8 (&B2)[4][2], __bf16 (*H)[kH + 8], float* Ms) {
+  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+  const int nl = lane & 15, er = 4 * (lane >> 4);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  float ein[kEin];
+  for (int c = 0; c * 16 < cnt; c++) {
+    EdgeTile<R> t = edge_tile(v, nf, i, c, cnt, lane, ein);
+    f4 h1[4];
+    layer1(net, B1, t.a1, lane, h1, H);
+    __syncthreads();
+    f4 z2[4];
+    layer2(B2, lane, H, z2);
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const float b = net.master[OFF_B2 + nl + 16 * nb];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if ((t.valid >> (er + r)) & 1u) acc[nb] += fmaxf(z2[nb][r] + b, 0.f);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+    acc[nb] += __shfl_xor(acc[nb], 16);
+    acc[nb] += __shfl_xor(acc[nb], 32);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) Ms[lane + 16 * nb] = acc[nb] * (1.0f / 64.0f);
+  }
+  __syncthreads();
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_mlp_forward(Net net, SphView<R> v, const float2* __restrict__ nf,
+                                                     const int* __restrict__ row, const double* __restrict__ pprev,
+                                                     double scale, double* __restrict__ guess) {
+  __shared__ __bf16 H[16][kH + 8];
+  __shared__ float Ms[kH];
+  __shared__ __bf16 W3s[kH * kH];
+  const int lane = threadIdx.x;
+  for (int q = lane; q < kH * kH; q += 64) W3s[q] = net.wbf[OFF_W3 + q];
+  bf8 B1[4], B2[4][2];
+  load_fwd_operands(net, lane, B1, B2);
+  __syncthreads();
+  for (int i = blockIdx.x; i < v.n; i += gridDim.x) {
+    if (!row[i]) {
+      if (lane == 0) guess[i] = pprev[i];
+      continue;
+    }
+    aggregate(net, v, nf, i, lane, B1, B2, H, Ms);
+    float z3;
+    float y = node_forward(net, Ms, W3s, lane, z3);
+    if (lane == 0) guess[i] = pprev[i] + scale * (double)y;
+    __syncthreads();
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_mlp_backward(Net net, SphView<R> v, const float2* __restrict__ nf,
+                                                      const int* __restrict__ row, const double* __restrict__ pprev,
+                                                      const double* __restrict__ target, double inv_scale,
+                                                      float inv_rows, float* __restrict__ partials) {
+  __shared__ __bf16 H[16][kH + 8];
+  __shared__ __bf16 DZ[16][kH + 8];
+  __shared__ __bf16 EIN[16][16];
+  __shared__ float Ms[kH];
+  __shared__ float DG[kH];
+  __shared__ float DM[kH];
+  __shared__ __bf16 W3s[kH * kH];
+  const int lane = threadIdx.x;
+  const int nl = lane & 15, er = 4 * (lane >> 4), kq = 8 * (lane >> 4);
+  for (int q = lane; q < kH * kH; q += 64) W3s[q] = net.wbf[OFF_W3 + q];
+  bf8 B1[4], B2[4][2];
+  load_fwd_operands(net, lane, B1, B2);
+  // B operand of dH1 = dZ2 W2: B[k = n][col = kout] = W2[n][kout]
+  bf8 Bt[4][2];
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      Bt[kb][0][j] = net.wbf[OFF_W2 + (kq + j) * kH + nl + 16 * kb];
+      Bt[kb][1][j] = net.wbf[OFF_W2 + (32 + kq + j) * kH + nl + 16 * kb];
+    }
+  const float w4 = (float)net.wbf[OFF_W4 + lane];
+  __syncthreads();
+
+  f4 gW2[16], gW1[4];
+#pragma unroll
+  for (int q = 0; q < 16; q++) gW2[q] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; q++) gW1[q] = (f4){0.f, 0.f, 0.f, 0.f};
+  float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0};
+  float gW3[kH];
+#pragma unroll
+  for (int k = 0; k < kH; k++) gW3[k] = 0.f;
+  float gb3 = 0.f, gW4 = 0.f, gb4 = 0.f, loss = 0.f;
+
+  for (int i = blockIdx.x; i < v.n; i += gridDim.x) {
+    if (!row[i]) continue;
+    // pass 1: forward
+    aggregate(net, v, nf, i, lane, B1, B2, H, Ms);
+    float z3;
+    const float y = node_forward(net, Ms, W3s, lane, z3);
+    const float t = (float)((target[i] - pprev[i]) * inv_scale);
+    const float err = y - t;
+    const float dy = 2.f * err * inv_rows;
+    if (lane == 0) {
+      loss += err * err * inv_rows;
+      gb4 += dy;
+    }
+    const float g = fmaxf(z3, 0.f);
+    gW4 += dy * g;
+    const float dg = z3 > 0.f ? dy * w4 : 0.f;
+    gb3 += dg;
+#pragma unroll
+    for (int k = 0; k < kH; k++) gW3[k] += dg * Ms[k];
+    DG[lane] = dg;
+    __syncthreads();
+    {
+      float dm = 0.f;
+      for (int l = 0; l < kH; l++) dm += (float)W3s[l * kH + lane] * DG[l];
+      DM[lane] = dm * (1.0f / 64.0f);
+    }
+    __syncthreads();
+    // pass 2: edge backward
+    const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+    float ein[kEin];
+    for (int c = 0; c * 16 < cnt; c++) {
+      EdgeTile<R> tt = edge_tile(v, nf, i, c, cnt, lane, ein);
+      if (lane < 16)
+        for (int q = 0; q < 16; q++) EIN[lane][q] = q < kEin ? (__bf16)ein[q] : (__bf16)0.0f;
+      f4 h1[4];
+      layer1(net, B1, tt.a1, lane, h1, H);
+      __syncthreads();
+      f4 z2[4];
+      layer2(B2, lane, H, z2);
+      // dZ2 in result layout; edges outside the list contribute nothing
+      f4 dz2[4];
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
+        const float b = net.master[OFF_B2 + nl + 16 * nb];
+        const float dmv = DM[nl + 16 * nb];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const bool on = ((tt.valid >> (er + r)) & 1u) && (z2[nb][r] + b > 0.f);
+          const float d = on ? dmv : 0.f;
+          dz2[nb][r] = d;
+          gb2[nb] += d;
+          DZ[er + r][nl + 16 * nb] = (__bf16)d;
+        }
+      }
+      // dW2 tiles: A = dZ2 (rows n, K = edges), B = H1 (K = edges, cols k)
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
+        bf4 a;
+        for (int r = 0; r < 4; r++) a[r] = (__bf16)dz2[nb][r];
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+          bf4 b;
+          for (int r = 0; r < 4; r++) b[r] = (__bf16)h1[kb][r];
+          gW2[nb * 4 + kb] = mfma16(a, b, gW2[nb * 4 + kb]);
+        }
+      }
+      __syncthreads();
+      // dH1 = dZ2 W2 (A from the DZ tile), then dZ1 through the ReLU
+      bf8 ad0, ad1;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        ad0[j] = DZ[nl][kq + j];
+        ad1[j] = DZ[nl][32 + kq + j];
+      }
+      bf4 be;
+      for (int r = 0; r < 4; r++) be[r] = EIN[er + r][nl];
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) {
+        f4 dh = {0.f, 0.f, 0.f, 0.f};
+        dh = mfma32(ad0, Bt[kb][0], dh);
+        dh = mfma32(ad1, Bt[kb][1], dh);
+        bf4 a;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const bool on = ((tt.valid >> (er + r)) & 1u) && h1[kb][r] > 0.f;
+          const float d = on ? dh[r] : 0.f;
+          gb1[kb] += d;
+          a[r] = (__bf16)d;
+        }
+        gW1[kb] = mfma16(a, be, gW1[kb]);
+      }
+      __syncthreads();
+    }
+  }
+  // partial gradients of this block
+  float* P = partials + (size_t)blockIdx.x * (kNParam + 1);
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const int n = 16 * nb + er + jj;
+      if (nl < kEin) P[OFF_W1 + n * kEin + nl] = gW1[nb][jj];
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) P[OFF_W2 + n * kH + 16 * kb + nl] = gW2[nb * 4 + kb][jj];
+    }
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+    float a = gb1[nb], b = gb2[nb];
+    a += __shfl_xor(a, 16);
+    a += __shfl_xor(a, 32);
+    b += __shfl_xor(b, 16);
+    b += __shfl_xor(b, 32);
+    if (lane < 16) {
+      P[OFF_B1 + lane + 16 * nb] = a;
+      P[OFF_B2 + lane + 16 * nb] = b;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kH; k++) P[OFF_W3 + lane * kH + k] = gW3[k];
+  P[OFF_B3 + lane] = gb3;
+  P[OFF_W4 + lane] = gW4;
+  if (lane == 0) {
+    P[OFF_B4] = gb4;
+    P[kNParam] = loss;
+  }
+}
+
+__global__ void k_reduce_partials(const float* __restrict__ partials, int nblocks, float* __restrict__ grad,
+                                  double* __restrict__ loss) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kNParam) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * (kNParam + 1) + p];
+  if (p < kNParam) grad[p] = (float)s;
+  else *loss = s;
+}
+
+// one Adam step; skipped as a whole when the batch loss is not finite, so a
+// single bad batch cannot poison the weights
+__global__ void k_adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1t, float b2t,
+                       const double* loss) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !isfinite(*loss)) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float gm = b1 * m[p] + (1.f - b1) * g[p];
+  float gv = b2 * v[p] + (1.f - b2) * g[p] * g[p];
+  m[p] = gm;
+  v[p] = gv;
+  float mh = gm / (1.f - b1t), vh = gv / (1.f - b2t);
+  w[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+__global__ void k_to_bf16(const float* w, __bf16* o, int n) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) o[p] = (__bf16)w[p];
+}
+
+// sums over rows of r0^2 and diag, row count -> scal[0..2]
+__global__ __launch_bounds__(1024) void k_feature_sums(int n, const int* row, const double* r0, const double* diag,
+                                                       double* scal) {
+  __shared__ double sh[3][16];
+  double a = 0, b = 0, c = 0;
+  for (int i = threadIdx.x; i < n; i += 1024)
+    if (row[i]) {
+      a += r0[i] * r0[i];
+      b += diag[i];
+      c += 1;
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+    c += __shfl_xor(c, off);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = a;
+    sh[1][w] = b;
+    sh[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int q = 0; q < 16; q++) {
+      s0 += sh[0][q];
+      s1 += sh[1][q];
+      s2 += sh[2][q];
+    }
+    scal[0] = s0;
+    scal[1] = s1;
+    scal[2] = s2;
+  }
+}
+
+template <typename R>
+__global__ void k_node_features(int n, const int* row, const double* r0, const double* rho, double rho_f,
+                                double inv_rms, float2* nf) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  nf[i] = make_float2((float)(rho[i] / rho_f), row[i] ? (float)(r0[i] * inv_rms) : 0.f);
+}
+
 __global__ void k_prev(int n, const int* row, const double* pres, double* x) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = row[i] ? pres[i] : 0.0;
 }
+
 }  // namespace
 
-int mlp_param_count() { return 0; }
+int mlp_param_count() { return kNParam; }
+
 int mlp_alloc(Mlp& m, int n, int cap) {
+  size_t N = (size_t)(n > 0 ? n : 1);
+  if (!m.master) {
+    HIP_TRY(hipMalloc(&m.master, kNParam * 4));
+    HIP_TRY(hipMalloc(&m.grad, kNParam * 4));
+    HIP_TRY(hipMalloc(&m.adam_m, kNParam * 4));
+    HIP_TRY(hipMalloc(&m.adam_v, kNParam * 4));
+    HIP_TRY(hipMalloc(&m.wbf, kNParam * 2));
+    HIP_TRY(hipMalloc(&m.partials, (size_t)kMlpBwdBlocks * (kNParam + 1) * 4));
+    HIP_TRY(hipMalloc(&m.scal, 8 * 8));
+    HIP_TRY(hipHostMalloc((void**)&m.host_scal, 8 * 8, hipHostMallocDefault));
+  }
+  if (m.n != n || !m.nf) {
+    if (m.nf) (void)hipFree(m.nf);
+    if (m.r0) (void)hipFree(m.r0);
+    if (m.guess) (void)hipFree(m.guess);
+    if (m.target) (void)hipFree(m.target);
+    if (m.pprev) (void)hipFree(m.pprev);
+    HIP_TRY(hipMalloc(&m.pprev, N * 8));
+    HIP_TRY(hipMalloc(&m.nf, N * sizeof(float2)));
+    HIP_TRY(hipMalloc(&m.r0, N * 8));
+    HIP_TRY(hipMalloc(&m.guess, N * 8));
+    HIP_TRY(hipMalloc(&m.target, N * 8));
+  }
   m.n = n;
   m.cap = cap;
-  if (!m.guess) HIP_TRY(hipMalloc(&m.guess, (size_t)(n > 0 ? n : 1) * 8));
   return 0;
 }
+
 void mlp_free(Mlp& m) {
-  if (m.guess) (void)hipFree(m.guess);
+  for (void* p : {(void*)m.master, (void*)m.grad, (void*)m.adam_m, (void*)m.adam_v, (void*)m.wbf, (void*)m.partials,
+                  (void*)m.scal, (void*)m.nf, (void*)m.r0, (void*)m.guess, (void*)m.target, (void*)m.pprev})
+    if (p) (void)hipFree(p);
+  if (m.host_scal) (void)hipHostFree(m.host_scal);
   m = Mlp{};
 }
-int mlp_set_params(hipStream_t, Mlp&, const float*, int) {
-  set_error("learned warm start not built");
-  return SOPHIA_E_STATE;
+
+int mlp_set_params(hipStream_t s, Mlp& m, const float* host, int n) {
+  if (n != kNParam) {
+    set_error("mlp_set_params: %d parameters given, network has %d", n, kNParam);
+    return SOPHIA_E_ARG;
+  }
+  HIP_TRY(hipMemcpyAsync(m.master, host, kNParam * 4, hipMemcpyHostToDevice, s));
+  k_to_bf16<<<grid_for(kNParam), kBlock, 0, s>>>(m.master, m.wbf, kNParam);
+  HIP_TRY(hipGetLastError());
+  int rc = mlp_reset_adam(s, m);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  m.ready = true;
+  return 0;
 }
-int mlp_get_params(hipStream_t, const Mlp&, float*, int) {
-  set_error("learned warm start not built");
-  return SOPHIA_E_STATE;
+
+int mlp_get_params(hipStream_t s, const Mlp& m, float* host, int n) {
+  if (n != kNParam) {
+    set_error("mlp_get_params: buffer of %d, network has %d", n, kNParam);
+    return SOPHIA_E_ARG;
+  }
+  if (!m.ready) {
+    set_error("mlp_get_params: parameters not set");
+    return SOPHIA_E_STATE;
+  }
+  HIP_TRY(hipMemcpyAsync(host, m.master, kNParam * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
 }
-int mlp_reset_adam(hipStream_t, Mlp&) { return 0; }
+
+int mlp_reset_adam(hipStream_t s, Mlp& m) {
+  if (!m.adam_m) return 0;
+  int rc = fill_zero_bytes(s, m.adam_m, kNParam * 4);
+  if (!rc) rc = fill_zero_bytes(s, m.adam_v, kNParam * 4);
+  m.adam_t = 0;
+  return rc;
+}
+
 template <typename R>
-int mlp_predict_rows(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, double*) {
-  set_error("learned warm start not built");
-  return SOPHIA_E_STATE;
+int mlp_features(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, const int* row, const double* diag,
+                 double* scale) {
+  if (v.n <= 0) {
+    *scale = 1;
+    return 0;
+  }
+  k_feature_sums<<<1, 1024, 0, s>>>(v.n, row, m.r0, diag, m.scal);
+  HIP_TRY(hipMemcpyAsync(m.host_scal, m.scal, 3 * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const double rows = m.host_scal[2] > 0 ? m.host_scal[2] : 1;
+  double rms = sqrt(m.host_scal[0] / rows);
+  const double dmean = m.host_scal[1] / rows;
+  if (!(rms > 0)) rms = 1e-300;
+  *scale = dmean > 0 ? rms / dmean : 0.0;
+  k_node_features<R><<<grid_for(v.n), kBlock, 0, s>>>(v.n, row, m.r0, v.rho, P.rho_fluid, 1.0 / rms, m.nf);
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
+
 template <typename R>
-int mlp_train_rows(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*, double, double*,
-                   float*, int, bool) {
-  set_error("learned warm start not built");
-  return SOPHIA_E_STATE;
+int mlp_forward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, const int* row, const double* pprev,
+                double scale, double* guess) {
+  if (!m.ready) {
+    set_error("learned warm start: parameters not set");
+    return SOPHIA_E_STATE;
+  }
+  if (v.n <= 0) return 0;
+  Net net{m.wbf, m.master};
+  int g = v.n < kFwdBlocks ? v.n : kFwdBlocks;
+  k_mlp_forward<R><<<g, 64, 0, s>>>(net, v, m.nf, row, pprev, scale, guess);
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
+
+template <typename R>
+int mlp_backward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, const int* row, const double* pprev,
+                 const double* target, double scale, int n_rows, bool step, double lr, double* loss) {
+  if (!m.ready) {
+    set_error("learned warm start: parameters not set");
+    return SOPHIA_E_STATE;
+  }
+  if (v.n <= 0 || n_rows <= 0) return 0;
+  Net net{m.wbf, m.master};
+  const int g = kMlpBwdBlocks;
+  const double inv_scale = scale != 0 ? 1.0 / scale : 0.0;
+  k_mlp_backward<R><<<g, 64, 0, s>>>(net, v, m.nf, row, pprev, target, inv_scale, 1.0f / (float)n_rows,
+                                     m.partials);
+  k_reduce_partials<<<grid_for(kNParam + 1), kBlock, 0, s>>>(m.partials, g, m.grad, m.scal + 4);
+  HIP_TRY(hipGetLastError());
+  if (step) {
+    m.adam_t += 1;
+    const float b1t = (float)pow(0.9, m.adam_t), b2t = (float)pow(0.999, m.adam_t);
+    k_adam<<<grid_for(kNParam), kBlock, 0, s>>>(m.master, m.grad, m.adam_m, m.adam_v, kNParam, (float)lr, b1t, b2t,
+                                                 m.scal + 4);
+    k_to_bf16<<<grid_for(kNParam), kBlock, 0, s>>>(m.master, m.wbf, kNParam);
+    HIP_TRY(hipGetLastError());
+  }
+  if (loss) {
+    HIP_TRY(hipMemcpyAsync(m.host_scal + 4, m.scal + 4, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *loss = m.host_scal[4];
+  }
+  return 0;
+}
+
 int mlp_gather_previous(hipStream_t s, int n, const int* row, const double* pres, double* x) {
   if (n <= 0) return 0;
   k_prev<<<grid_for(n), kBlock, 0, s>>>(n, row, pres, x);
   HIP_TRY(hipGetLastError());
   return 0;
 }
-template int mlp_predict_rows<float>(hipStream_t, const DevParams&, const SphView<float>&, Mlp&, const int*, double*);
-template int mlp_predict_rows<double>(hipStream_t, const DevParams&, const SphView<double>&, Mlp&, const int*, double*);
-template int mlp_train_rows<float>(hipStream_t, const DevParams&, const SphView<float>&, Mlp&, const int*,
-                                   const double*, double, double*, float*, int, bool);
-template int mlp_train_rows<double>(hipStream_t, const DevParams&, const SphView<double>&, Mlp&, const int*,
-                                    const double*, double, double*, float*, int, bool);
+
+#define INST(R)                                                                                                    \
+  template int mlp_features<R>(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*, \
+                               double*);                                                                          \
+  template int mlp_forward<R>(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*,  \
+                              double, double*);                                                                   \
+  template int mlp_backward<R>(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*, \
+                               const double*, double, int, bool, double, double*);
+INST(float)
+INST(double)
 
 }  // namespace sophia

@@ -163,6 +163,10 @@ typedef struct sophia_ppe_opts {
   int32_t zero_mean;    /* project out the constant null space (periodic, no walls) */
   double rtol;          /* ||b - A p|| <= rtol * ||b|| */
   double inner_rtol;    /* per refinement sweep (mixed) */
+  int32_t train;        /* 1: after the solve, one Adam step of the warm-start
+                           network on (features before the solve, solution) */
+  int32_t reserved_i;
+  double lr;            /* Adam learning rate when train != 0 */
 } sophia_ppe_opts;
 
 typedef struct sophia_ppe_stats {
@@ -176,6 +180,7 @@ typedef struct sophia_ppe_stats {
   double ms;                /* wall time of the PPE stage, device events */
   int32_t n_rows;
   int32_t reserved;
+  double loss;              /* training loss of the in-solve Adam step */
 } sophia_ppe_stats;
 
 typedef struct sophia_ctx sophia_ctx;
@@ -248,25 +253,33 @@ int sophia_get_ppe_system(const sophia_ctx* ctx, double* diag, double* rhs,
 int sophia_ppe_apply(sophia_ctx* ctx, const double* x, double* y, int32_t n);
 
 /* --- learned PPE warm start (message-passing MLP, BF16 MFMA) ------------- */
+/* The network predicts the pressure change of the PPE rows from the
+ * residual of the previous pressure: p0 = p_prev + s * y, where y is the
+ * network output, s = rms(r)/mean(diag A), r = b - A p_prev.  Edge inputs
+ * over the SPH neighbour list: (x_i - x_j)/h_i (3), |x_i - x_j|/h_i,
+ * rho_j/rho_fluid, r_j/rms(r).  Edge MLP 6 -> 64 -> 64 (ReLU, BF16 weights
+ * on MFMA, FP32 accumulation), sum aggregation scaled by 1/64, node MLP
+ * 64 -> 64 -> 1.  Parameters are an FP32 master vector (layout in
+ * learned.hpp); the kernels use BF16 copies of the weight matrices.  All
+ * calls below need an assembled PPE system of the current step (they run
+ * after sophia_ppe, or inside it through the opts). */
 #define SOPHIA_MLP_HIDDEN 64
 #define SOPHIA_MLP_EDGE_IN 6
-/* total parameter count of the network (weights + biases, FP32 master copy) */
 int sophia_mlp_param_count(int32_t* n);
-/* set / get the FP32 master parameters (BF16 copies are made on set) */
 int sophia_mlp_set_params(sophia_ctx* ctx, const float* params, int32_t n);
 int sophia_mlp_get_params(const sophia_ctx* ctx, float* params, int32_t n);
-/* predict pressure for the current step's PPE rows into out (sorted order) */
+/* prediction p0 for the current PPE rows with the current pressure as
+ * p_prev (sorted order, SPH count; non-rows return p_prev) */
 int sophia_mlp_predict(sophia_ctx* ctx, double* out, int32_t n);
-/* one Adam step on the current step's features against target pressure
- * (sorted order, n = SPH count); returns the loss before the step */
-int sophia_mlp_train_step(sophia_ctx* ctx, const double* target, int32_t n,
-                          double lr, double* loss);
-/* loss and gradient w.r.t. the FP32 parameters (no update) */
-int sophia_mlp_loss_grad(sophia_ctx* ctx, const double* target, int32_t n,
-                         double* loss, float* grad, int32_t n_params);
-/* node features the network sees (n x 4: x/h-free edge inputs are built
- * on the fly): rho, div u*, pressure scale, row mask */
+/* one Adam step against target pressure (sorted order); loss before step */
+int sophia_mlp_train_step(sophia_ctx* ctx, const double* target, int32_t n, double lr, double* loss);
+/* loss and gradient w.r.t. the FP32 parameters, no update */
+int sophia_mlp_loss_grad(sophia_ctx* ctx, const double* target, int32_t n, double* loss, float* grad,
+                         int32_t n_params);
 int sophia_mlp_reset_adam(sophia_ctx* ctx);
+/* node inputs the network sees (n x 2: rho_j/rho_fluid, r_j/rms(r)) and the
+ * output scale s */
+int sophia_mlp_features(sophia_ctx* ctx, float* node_feat, int32_t n, double* scale);
 
 /* --- device timers (HIP events around stage kernels) ---------------------- */
 /* enable (1) or disable (0); enabling clears the accumulated totals */
