@@ -85,7 +85,8 @@ class Params(C.Structure):
         ("drag_switch", C.c_double),
         ("eulerian_sph", C.c_int32), ("fix_xsph_position", C.c_int32), ("dem_max_contacts", C.c_int32),
         ("sph_max_neighbors", C.c_int32), ("domain_fixed", C.c_int32), ("periodic_mask", C.c_int32),
-        ("domain_min", C.c_double * 3), ("domain_max", C.c_double * 3), ("reserved", C.c_double * 8),
+        ("domain_min", C.c_double * 3), ("domain_max", C.c_double * 3),
+        ("pgrad_form", C.c_int32), ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
     ]
 
     def as_dict(self):
@@ -117,7 +118,7 @@ class PpeStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("guess_used", C.c_int32),
         ("r0_rel", C.c_double), ("r0_rel_learned", C.c_double), ("r_rel", C.c_double), ("ms", C.c_double),
-        ("n_rows", C.c_int32), ("reserved", C.c_int32), ("loss", C.c_double),
+        ("n_rows", C.c_int32), ("nnz", C.c_int32), ("loss", C.c_double),
     ]
 
     def as_dict(self):
@@ -125,7 +126,7 @@ class PpeStats(C.Structure):
 
 
 def ppe_opts(method=PPE_JACOBI, precision=PPE_FP64, guess=GUESS_PREVIOUS, rtol=1e-8, max_iter=10000,
-             max_outer=20, inner_rtol=1e-4, zero_mean=0, train=0, lr=1e-3) -> PpeOpts:
+             max_outer=20, inner_rtol=0.1, zero_mean=0, train=0, lr=1e-3) -> PpeOpts:
     return PpeOpts(method, precision, guess, max_iter, max_outer, zero_mean, rtol, inner_rtol, train, 0, lr)
 
 
@@ -164,6 +165,7 @@ SIGNATURES = {
     "sophia_set_pressure": (C.c_int, [_ctx, _VP, C.c_int32]),
     "sophia_get_ppe_system": (C.c_int, [_ctx, _VP, _VP, _VP]),
     "sophia_ppe_apply": (C.c_int, [_ctx, _VP, _VP, C.c_int32]),
+    "sophia_ppe_layout": (C.c_int, [_ctx, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sophia_mlp_param_count": (C.c_int, [_P(C.c_int32)]),
     "sophia_mlp_set_params": (C.c_int, [_ctx, _VP, C.c_int32]),
     "sophia_mlp_get_params": (C.c_int, [_ctx, _VP, C.c_int32]),

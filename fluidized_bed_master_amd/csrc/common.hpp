@@ -36,12 +36,18 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     hipError_t e__ = (call);                                             \
     if (e__ != hipSuccess) return hip_fail(e__, #call, __FILE__, __LINE__); \
   } while (0)
+// propagate a non-zero status code
+#define TRY(x)             \
+  do {                     \
+    int rc__ = (x);        \
+    if (rc__) return rc__; \
+  } while (0)
 
 // Parameters passed by value to every kernel.
 struct DevParams {
   int dim, kernel_type, kgc, delsph, rho_type;
   int fp_solve, fv_solve, fva_solve, fg_solve, con_solve, xsph_solve;
-  int open_boundary, z_index, eulerian, fix_xsph;
+  int open_boundary, z_index, eulerian, fix_xsph, pgrad_form;
   int NI, NJ, NK, num_cells;
   int dem_slots;
   int filt_step;          // 1 when the Shepard filter is due this step

@@ -68,11 +68,6 @@ struct Arena {
   }
 };
 
-#define TRY(x)            \
-  do {                    \
-    int rc__ = (x);       \
-    if (rc__) return rc__; \
-  } while (0)
 
 // persistent field with its permutation spare
 template <typename T>
@@ -121,6 +116,7 @@ struct sophia_ctx {
   virtual int set_pressure(const double* p, int n) = 0;
   virtual int get_ppe_system(double* diag, double* rhs, int32_t* row) const = 0;
   virtual int ppe_apply(const double* x, double* y, int n) = 0;
+  virtual int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const = 0;
   virtual int mlp_predict(double* out, int n) = 0;
   virtual int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) = 0;
   virtual int mlp_set(const float* w, int n) = 0;
@@ -156,6 +152,7 @@ struct Ctx : public sophia_ctx {
   DemView dv{};
   // cells and sort
   uint32_t *scs = nullptr, *sce = nullptr, *dcs = nullptr, *dce = nullptr;
+  uint32_t* scoff = nullptr;  // SPH first particle of each cell, dense (num_cells + 1)
   SortScratch ssort{}, dsort{};
   uint32_t *skeys = nullptr, *sorder = nullptr, *dkeys = nullptr, *dorder = nullptr;
   int* maxcnt = nullptr;
@@ -191,6 +188,7 @@ struct Ctx : public sophia_ctx {
     P.z_index = p.flag_z_index;
     P.eulerian = p.eulerian_sph;
     P.fix_xsph = p.fix_xsph_position;
+    P.pgrad_form = p.pgrad_form;
     P.NI = d.NI;
     P.NJ = d.NJ;
     P.NK = d.NK;
@@ -312,6 +310,7 @@ struct Ctx : public sophia_ctx {
     size_t NC = (size_t)d.num_cells;
     TRY(arena.alloc(&scs, NC));
     TRY(arena.alloc(&sce, NC));
+    TRY(arena.alloc(&scoff, NC + 1));
     TRY(arena.alloc(&dcs, NC));
     TRY(arena.alloc(&dce, NC));
     for (auto pr : {std::make_pair(&ssort, ns), std::make_pair(&dsort, nd)}) {
@@ -472,17 +471,22 @@ struct Ctx : public sophia_ctx {
       add(sdpor, rb);
       add(sptype, 4); add(sbtype, 4); add(sitype, 4);
       TRY(permute_fields(stream, ns, sorder, f.data(), (int)f.size()));
+      TRY(cell_offsets(stream, ns, skeys, d.num_cells, scoff));
       for (auto* fl : {&sx, &sy, &sz, &sh, &spres, &sm, &srho}) fl->swap();
       for (auto* fl : {&sux, &suy, &suz, &stemp, &sflt, &sdpor}) fl->swap();
       for (auto* fl : {&sptype, &sbtype, &sitype}) fl->swap();
       refresh_views();
       // P3 reset (ISPH_Calc_seperate.cuh:102-105)
-      // pgf lives in P1 in the reference and is rewritten each step for every
-      // fluid particle; the others (walls, skipped types) keep their input
-      // value, which is zero, and the DEM drag reads it for walls
-      for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz})
+      // pgf and the XSPH velocities live in P1 in the reference and are
+      // rewritten each step for the particles their kernels compute; the
+      // others (walls, skipped types) keep their input value, which is zero
+      // (the DEM drag reads pgf of walls)
+      for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz, sv.xsx,
+                   sv.xsy, sv.xsz, sv.xst})
         TRY(fill_zero_bytes(stream, a, (size_t)ns * sizeof(R)));
       TRY(fill_zero_bytes(stream, sv.cm, (size_t)6 * ns * sizeof(R)));
+      // PPE1..4 (P1) are written for PPE rows only; the rest keep zero
+      for (double* a : {sv.ppe1, sv.ppe2, sv.ppe3, sv.ppe4}) TRY(fill_zero_bytes(stream, a, (size_t)ns * 8));
       // neighbour list with capacity check
       for (int attempt = 0; attempt < 2; attempt++) {
         TRY(fill_u32(stream, (uint32_t*)maxcnt, 0u, 1));
@@ -651,7 +655,7 @@ struct Ctx : public sophia_ctx {
     }
     {
       TimeScope t(tm(), "ppe_assemble");
-      TRY(ppe_assemble<R>(stream, P, sv, ppe_w));
+      TRY(ppe_assemble<R>(stream, P, sv, ppe_w, skeys, scoff));
     }
     ppe_assembled = true;
     if (o->method == SOPHIA_PPE_JACOBI) {
@@ -672,8 +676,8 @@ struct Ctx : public sophia_ctx {
         TimeScope t(tm(), "mlp_forward");
         TRY(mlp_forward<R>(stream, P, sv, mlp, ppe_w.row, sv.pres, scale, mlp.guess));
         double r_l = 0, r_p = 0;
-        TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, mlp.guess, &r_l));
-        TRY(ppe_residual_of(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, &r_p));
+        TRY(ppe_residual_of(stream, ppe_w, mlp.guess, &r_l));
+        TRY(ppe_residual_of(stream, ppe_w, sv.pres, &r_p));
         st->r0_rel_learned = r_l;
         if (r_l < r_p) {
           guess = mlp.guess;
@@ -683,7 +687,7 @@ struct Ctx : public sophia_ctx {
       sophia_ppe_opts oo = *o;
       if (oo.max_iter <= 0) oo.max_iter = 10000;
       if (oo.rtol <= 0) oo.rtol = 1e-8;
-      TRY(ppe_solve(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, oo, *st, guess, tm()));
+      TRY(ppe_solve(stream, ppe_w, sv.pres, oo, *st, guess, tm()));
       if (o->train && mlp.ready) {
         // one Adam step on (features before the solve, converged pressure)
         TimeScope t(tm(), "mlp_train");
@@ -707,6 +711,7 @@ struct Ctx : public sophia_ctx {
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     st->ms = ms;
     st->n_rows = n_rows;
+    st->nnz = (int32_t)std::min<int64_t>(ppe_w.nnz, INT32_MAX);
     return 0;
   }
 
@@ -891,6 +896,16 @@ struct Ctx : public sophia_ctx {
     return 0;
   }
 
+  int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const override {
+    if (!ppe_assembled) {
+      set_error("PPE not assembled yet");
+      return SOPHIA_E_STATE;
+    }
+    if (nnz) *nnz = ppe_w.nnz;
+    if (ib) *ib = ppe_w.fmt == kFmtRun8 ? 1 : 4;
+    if (cb) *cb = ppe_w.mixed ? 4 : 8;
+    return 0;
+  }
   int ppe_apply(const double* x, double* y, int n) override {
     if (!ppe_assembled || n != ns) {
       set_error("ppe_apply: operator not assembled or n mismatch");
@@ -900,7 +915,7 @@ struct Ctx : public sophia_ctx {
     TRY(arena.alloc(&dx, n));
     TRY(arena.alloc(&dy, n));
     HIP_TRY(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, stream));
-    int rc = ppe_apply64(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, dx, dy, false, nullptr);
+    int rc = ppe_apply64(stream, ppe_w, dx, dy, false, nullptr);
     if (!rc) {
       hipError_t e = hipMemcpyAsync(y, dy, (size_t)n * 8, hipMemcpyDeviceToHost, stream);
       if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -914,7 +929,7 @@ struct Ctx : public sophia_ctx {
   // --- learned warm start -------------------------------------------------------
   // residual of the current pressure and the network's node inputs
   int mlp_prepare(double* scale) {
-    TRY(ppe_apply64(stream, ppe_w, sv.nbr, sv.nbr_cnt, sv.nbr_cap, sv.pres, mlp.r0, true, nullptr));
+    TRY(ppe_apply64(stream, ppe_w, sv.pres, mlp.r0, true, nullptr));
     return mlp_features<R>(stream, devparams(), sv, mlp, ppe_w.row, ppe_w.diag, scale);
   }
   int mlp_check(int n) const {
@@ -1426,6 +1441,10 @@ int sophia_get_ppe_system(const sophia_ctx* c, double* diag, double* rhs, int32_
   return c->get_ppe_system(diag, rhs, row);
 }
 int sophia_ppe_apply(sophia_ctx* c, const double* x, double* y, int32_t n) { CHECK_CTX(c); return c->ppe_apply(x, y, n); }
+int sophia_ppe_layout(const sophia_ctx* c, int64_t* nnz, int32_t* ib, int32_t* cb) {
+  if (!c) return SOPHIA_E_ARG;
+  return c->ppe_layout(nnz, ib, cb);
+}
 
 int sophia_knl_eval(int32_t kt, int32_t dim, double h, const double* r, int32_t n, double* w, double* dw) {
   if (!r || !w || !dw || n < 0) return SOPHIA_E_ARG;

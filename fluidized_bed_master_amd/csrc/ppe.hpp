@@ -1,4 +1,4 @@
-// PPE workspace and entry points (see ppe.hip).
+// PPE operator storage, workspace and entry points (ppe.hip).
 #pragma once
 
 #include "common.hpp"
@@ -6,41 +6,78 @@
 
 namespace sophia {
 
-struct PpeView {
-  int n, cap;
-  const uint32_t* nbr;
-  const int* cnt;
-  int* row;
-  double *diag, *dinv, *rhs;
-  double* coef64;
-  float *coef_hi, *coef_lo;
+// Column encodings of the compact operator.
+//   kFmtRun8 : row-major cell keys.  The 27-cell stencil of a particle is 9
+//              runs of 3 x-adjacent cells, each a contiguous range of the
+//              cell-sorted particle order; a column is stored as a u8 offset
+//              from its run's first particle, and a u64 per row holds the 9
+//              run lengths (7 bits each).  Needs every run <= 256 particles.
+//   kFmtIdx32: plain u32 column indices (Morton keys or overfull runs).
+enum { kFmtRun8 = 0, kFmtIdx32 = 1 };
+
+// Compact operator: per row only the columns that are PPE rows themselves
+// (wall/buffer columns live in the right-hand side), slot-quad major:
+// entry k of row i sits at ((k / 4) * n + i) * 4 + k % 4, so a wave reads
+// 64 consecutive 16 B (FP32) or 32 B (FP64) quads per four slots.
+struct PpeOp {
+  int n = 0, nq = 0, fmt = kFmtRun8;
+  int NI = 1, NJ = 1;
+  const uint32_t* keys = nullptr;      // sorted cell key per particle
+  const uint32_t* cell_off = nullptr;  // first particle of each cell (dense, num_cells + 1)
+  const int* row = nullptr;
+  const int* ccnt = nullptr;           // compact entries per row
+  const uint64_t* runc = nullptr;      // kFmtRun8 run lengths
+  const uint32_t* idx = nullptr;       // kFmtRun8: u32 [q][i] (4 offsets); kFmtIdx32: u32 [q][i][4]
+  const float* hi = nullptr;           // mixed: fl32(a)            [q][i][4]
+  const float* lo = nullptr;           // mixed: fl32(a - hi)       [q][i][4]
+  const double* c64 = nullptr;         // FP64 path                 [q][i][4]
+  const double* diag = nullptr;
+  const double* dinv = nullptr;
+  const double* rhs = nullptr;
 };
 
 struct PpeWork {
-  int n = 0, nb = 0, cap = 0;
+  int n = 0, cap4 = 0, nbg = 0;
+  int64_t nnz = 0;  // stored entries of the last assembly
   bool mixed = false;
+  int fmt = kFmtRun8;
+  int NI = 1, NJ = 1;
+  const uint32_t* keys = nullptr;
+  const uint32_t* cell_off = nullptr;
   int* row = nullptr;
+  int* ccnt = nullptr;
+  uint64_t* runc = nullptr;
+  uint32_t* idx = nullptr;
+  float *hi = nullptr, *lo = nullptr;
+  double* c64 = nullptr;
   double *diag = nullptr, *dinv = nullptr, *rhs = nullptr, *x64 = nullptr, *r64 = nullptr;
-  double *partials = nullptr, *sc = nullptr, *host_sc = nullptr;
+  double* partials = nullptr;
+  void* state = nullptr;       // device solver state
+  void* host_state = nullptr;  // pinned mirror
+  unsigned* flags = nullptr;   // assembly overflow flags
+  unsigned* host_flags = nullptr;
   void* vec[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  void* d32 = nullptr;
-  double* coef64 = nullptr;
-  float *coef_hi = nullptr, *coef_lo = nullptr;
+  int vec_bytes = 0;
 };
 
 int ppe_workspace_alloc(PpeWork& w, int n, int cap, bool mixed);
 void ppe_workspace_free(PpeWork& w);
-PpeView ppe_view(const PpeWork& w, const uint32_t* nbr, const int* cnt, int cap);
+PpeOp ppe_op(const PpeWork& w);
+
+// dense first-particle-of-cell table from sorted keys (num_cells + 1 entries)
+int cell_offsets(hipStream_t s, int n, const uint32_t* keys, int num_cells, uint32_t* off);
+
 template <typename R>
-int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork& w);
+int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork& w, const uint32_t* keys,
+                 const uint32_t* cell_off);
 template <typename R>
 int ppe_jacobi(hipStream_t s, const SphView<R>& v, PpeWork& w);
-int ppe_apply64(hipStream_t s, PpeWork& w, const uint32_t* nbr, const int* cnt, int cap, const double* x, double* y,
-                bool residual, double* norm2);
-int ppe_solve(hipStream_t s, PpeWork& w, const uint32_t* nbr, const int* cnt, int cap, double* pres,
-              const sophia_ppe_opts& o, sophia_ppe_stats& st, const double* guess_or_null, Timers* tm);
-int ppe_residual_of(hipStream_t s, PpeWork& w, const uint32_t* nbr, const int* cnt, int cap, const double* x,
-                    double* rel);
+// y = A x with the full-precision operator (FP64, or hi + lo); residual form
+// y = rhs - A x when residual; ||y||^2 over rows into *norm2 (synchronises)
+int ppe_apply64(hipStream_t s, PpeWork& w, const double* x, double* y, bool residual, double* norm2);
+int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o, sophia_ppe_stats& st,
+              const double* guess_or_null, Timers* tm);
+int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel);
 int ppe_rhs_norm(hipStream_t s, PpeWork& w, double* bn);
 
 }  // namespace sophia

@@ -224,7 +224,8 @@ __global__ void k_advection(DevParams P, SphView<R> v) {
     double gxd = dwd * dxd / rd, gyd = dwd * dyd / rd, gzd = dwd * dzd / rd;
     kgc_apply(P, v.cm, n, i, gxd, gyd, gzd);
     if (P.fp_solve) {
-      const double Cp = -porid * v.m[j] * (pid + v.pres[j]) / (rhoid * v.rho[j]);
+      const double pp = P.pgrad_form ? v.pres[j] - pid : pid + v.pres[j];
+      const double Cp = -porid * v.m[j] * pp / (rhoid * v.rho[j]);
       px += rhoid / porid * Cp * gxd;
       py += rhoid / porid * Cp * gyd;
       pz += rhoid / porid * Cp * gzd;
@@ -348,7 +349,8 @@ __global__ void k_pressure_force(DevParams P, SphView<R> v) {
       T dw = knl_dw<T>(P.kernel_type, A, h, r);
       T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
       kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      T Cp = -pori * v.m[j] * (pid + v.pres[j]) / (rhoi * v.rho[j]);
+      const T pp = P.pgrad_form ? v.pres[j] - pid : pid + v.pres[j];
+      T Cp = -pori * v.m[j] * pp / (rhoi * v.rho[j]);
       tx += Cp * gx;
       ty += Cp * gy;
       tz += Cp * gz;

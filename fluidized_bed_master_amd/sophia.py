@@ -170,6 +170,13 @@ class Sophia:
         check(self.lib.sophia_ppe_apply(self._ctx, ptr(x), ptr(y), len(x)), "sophia_ppe_apply")
         return y
 
+    def ppe_layout(self) -> dict:
+        nnz = C.c_int64()
+        ib = C.c_int32()
+        cb = C.c_int32()
+        check(self.lib.sophia_ppe_layout(self._ctx, C.byref(nnz), C.byref(ib), C.byref(cb)), "sophia_ppe_layout")
+        return {"nnz": nnz.value, "index_bytes": ib.value, "coef_bytes": cb.value}
+
     # --- learned warm start ----------------------------------------------------
     @staticmethod
     def mlp_param_count() -> int:

@@ -130,7 +130,14 @@ typedef struct sophia_params {
   int32_t domain_fixed;
   int32_t periodic_mask;     /* bit0 x, bit1 y, bit2 z: periodic wrap in that axis */
   double domain_min[3], domain_max[3];
-  double reserved[8];
+  /* pressure-gradient pair term of the pressure force and of pgf:
+   * 0: sum_j m_j (p_i + p_j)/(rho_i rho_j) grad~W_ij (reference,
+   *    functions_PPE2.cuh:2303-2315, :1445-1450);
+   * 1: sum_j m_j (p_j - p_i)/(rho_i rho_j) grad~W_ij (zero for a constant
+   *    pressure; keeps a converged PPE solve stable, see the design notes) */
+  int32_t pgrad_form;
+  int32_t reserved_i;
+  double reserved[7];
 } sophia_params;
 
 /* Domain geometry after find_minmax (function_init.cuh:913-950) and the
@@ -148,7 +155,8 @@ typedef struct sophia_domain {
 #define SOPHIA_PPE_BICGSTAB 1   /* converged Krylov solve */
 
 #define SOPHIA_PPE_FP64 0       /* FP64 operator, FP64 vectors */
-#define SOPHIA_PPE_MIXED_IR 1   /* FP32 operator inside, FP64 outer residual + refinement */
+#define SOPHIA_PPE_MIXED_IR 1   /* FP32 vectors and operator, folded into an FP64 solution
+                                   with FP64 residuals (reliable updates) */
 
 #define SOPHIA_GUESS_ZERO 0
 #define SOPHIA_GUESS_PREVIOUS 1
@@ -159,10 +167,12 @@ typedef struct sophia_ppe_opts {
   int32_t precision;    /* SOPHIA_PPE_FP64 / SOPHIA_PPE_MIXED_IR */
   int32_t guess;        /* SOPHIA_GUESS_* */
   int32_t max_iter;     /* total inner iterations cap */
-  int32_t max_outer;    /* refinement sweeps cap (mixed) */
+  int32_t max_outer;    /* restarts after a Krylov breakdown */
   int32_t zero_mean;    /* project out the constant null space (periodic, no walls) */
   double rtol;          /* ||b - A p|| <= rtol * ||b|| */
-  double inner_rtol;    /* per refinement sweep (mixed) */
+  double inner_rtol;    /* mixed: fold the FP32 iterate into the FP64 solution and
+                           recompute the FP64 residual each time the FP32 residual
+                           has dropped by this factor (0 -> 0.1) */
   int32_t train;        /* 1: after the solve, one Adam step of the warm-start
                            network on (features before the solve, solution) */
   int32_t reserved_i;
@@ -171,7 +181,7 @@ typedef struct sophia_ppe_opts {
 
 typedef struct sophia_ppe_stats {
   int32_t iterations;       /* inner Krylov iterations */
-  int32_t outer;            /* refinement sweeps */
+  int32_t outer;            /* FP64 residual recomputations (folds) */
   int32_t converged;
   int32_t guess_used;       /* SOPHIA_GUESS_* actually used (after fallback) */
   double r0_rel;            /* ||b - A p0|| / ||b|| of the guess used */
@@ -179,7 +189,7 @@ typedef struct sophia_ppe_stats {
   double r_rel;             /* final relative residual (FP64) */
   double ms;                /* wall time of the PPE stage, device events */
   int32_t n_rows;
-  int32_t reserved;
+  int32_t nnz;              /* stored operator entries (row columns) */
   double loss;              /* training loss of the in-solve Adam step */
 } sophia_ppe_stats;
 
@@ -251,6 +261,10 @@ int sophia_get_ppe_system(const sophia_ctx* ctx, double* diag, double* rhs,
                           int32_t* row_mask);
 /* y = A x with the assembled operator (FP64), sorted order, SPH count */
 int sophia_ppe_apply(sophia_ctx* ctx, const double* x, double* y, int32_t n);
+/* storage of the assembled operator: stored entries, bytes per column index
+ * (1: u8 offset inside a cell run, 4: u32 index) and per coefficient as read
+ * by the inner iterations (4: FP32, 8: FP64).  Any may be NULL. */
+int sophia_ppe_layout(const sophia_ctx* ctx, int64_t* nnz, int32_t* index_bytes, int32_t* coef_bytes);
 
 /* --- learned PPE warm start (message-passing MLP, BF16 MFMA) ------------- */
 /* The network predicts the pressure change of the PPE rows from the

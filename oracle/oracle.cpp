@@ -731,7 +731,7 @@ void orc_interaction(orc_state* s) {
       kgc_apply(*s, sp.P3[i].Cm, gx, gy, gz);
       double mj = qj.m, rhoj = qj.rho, pj = qj.pres;
       if (p.fp_solve) {
-        double Cp = -pori * mj * (pi + pj) / (rhoi * rhoj);
+        double Cp = -pori * mj * (p.pgrad_form ? pj - pi : pi + pj) / (rhoi * rhoj);
         px += rhoi / pori * Cp * gx;
         py += rhoi / pori * Cp * gy;
         pz += rhoi / pori * Cp * gz;
@@ -1440,7 +1440,7 @@ static void pressure_force(orc_state& s) {
       double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
       kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
       if (p.fp_solve) {
-        double Cp = -pori * 1 * qj.m * (pi + qj.pres) / (rhoi * qj.rho);
+        double Cp = -pori * 1 * qj.m * (p.pgrad_form ? qj.pres - pi : pi + qj.pres) / (rhoi * qj.rho);
         tx += Cp * gx;
         ty += Cp * gy;
         tz += Cp * gz;

@@ -132,6 +132,76 @@ def test_bicgstab_matches_oracle(prec, ppe_prec):
     assert err <= 1e-4, err
 
 
+@pytest.mark.parametrize("prec,ppe_prec", [(abi.PREC_FP64, abi.PPE_FP64), (abi.PREC_MIXED, abi.PPE_MIXED_IR)],
+                         ids=["fp64", "mixed_ir"])
+def test_converged_steps_difference_form(prec, ppe_prec):
+    """Three full steps with a converged PPE and the difference-form pressure
+    gradient (the stable pairing for a converged solve): pressure-gradient
+    fields after the first interaction, then pressure and velocity after
+    three steps against the oracle solved to rtol 1e-12."""
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.small_isph_dem(seed=9)
+    p.pgrad_form = 1
+    g, o = Sophia(p, parts, precision=prec), Oracle(p, parts)
+    go = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=ppe_prec, rtol=1e-10)
+    oo = abi.ppe_opts(method=abi.PPE_BICGSTAB, rtol=1e-12)
+    rep = {}
+    for s in (g, o):
+        s.nnps(); s.prep(); s.interaction()
+    _cmp(g, o, "interaction", TOL[prec], rep)
+    for s, opts in ((g, go), (o, oo)):
+        s.dem_interaction(); s.sph_dem_coupling(); s.predict(); s.ppe(opts)
+    _cmp(g, o, "ppe", 1e-4, rep)
+    for s in (g, o):
+        s.correct(); s.xsph()
+    for _ in range(2):
+        sg = g.step(go)
+        o.step(oo)
+        assert sg["converged"], sg
+    a1, _, _ = g.get_particles(abi.SPH)
+    b1, _, _ = o.get_particles(abi.SPH)
+    rows = (b1["p_type"] >= 1) & (b1["p_type"] < 1000)
+    errs = {f: rel_err(a1[f][rows], b1[f][rows]) for f in ("pres", "ux", "uy", "uz")}
+    rep.update({f"3 steps:{k}": v for k, v in errs.items()})
+    _report(f"converged_diff_{'fp64' if prec == 0 else 'mixed'}", rep)
+    for f, e in errs.items():
+        assert e <= 1e-4, (f, e)
+
+
+@pytest.mark.parametrize("z_index", [0, 1], ids=["rowmajor", "morton"])
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_operator_matches_oracle(prec, z_index):
+    """The compact PPE operator (u8 run offsets for row-major keys, u32
+    indices for Morton keys) applied to a vector equals the oracle's
+    operator; diagonal and right-hand side match too."""
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.small_isph_dem(seed=7)
+    p.flag_z_index = z_index
+    g, o = Sophia(p, parts, precision=prec), Oracle(p, parts)
+    for s in (g, o):
+        s.nnps(); s.prep(); s.interaction(); s.dem_interaction(); s.sph_dem_coupling(); s.predict()
+        s.ppe(abi.ppe_opts(method=abi.PPE_JACOBI))
+    lay = g.ppe_layout()
+    assert lay["index_bytes"] == (4 if z_index else 1), lay
+    dg, bg, rg = g.get_ppe_system()
+    do, bo, ro = o.get_ppe_system()
+    assert np.array_equal(rg, ro)
+    tol = TOL[prec]
+    assert rel_err(dg, do) <= tol
+    assert rel_err(bg, bo) <= tol
+    rng = np.random.default_rng(3)
+    x = rng.normal(0, 1, len(rg)) * ro
+    yg = g.ppe_apply(x)
+    yo = o.ppe_apply(x)
+    err = rel_err(yg, yo)
+    _report(f"operator_{'fp64' if prec == 0 else 'mixed'}_{'morton' if z_index else 'rowmajor'}",
+            {"apply_rel_err": err, "nnz": lay["nnz"]})
+    assert err <= tol, err
+    assert lay["nnz"] > 0
+
+
 def test_knl_eval_matches_oracle():
     from fluidized_bed_master_amd.sophia import knl_eval
     from oracle import oracle

@@ -402,3 +402,42 @@ def test_xsph_keeps_uniform_velocity():
     fl = s1["p_type"] == 1
     assert np.max(np.abs(s1["ux"][fl] - 0.2)) < 1e-14
     assert np.max(np.abs(s1["uz"][fl] - 0.05)) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# pressure-gradient form (pgrad_form = 1): sum_j V_j (p_j - p_i) grad~W_ij
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("field", ["constant", "linear"])
+def test_pgrad_difference_form_exact(field):
+    """With the difference form the pressure-gradient pair sum is zero for a
+    constant pressure and, with KGC, exactly -grad p for a linear one (the
+    KGC matrix inverts the same V_j-weighted moment sum); the reference's
+    symmetric form is not."""
+    p, parts = configs.small_isph_dem(n=12, seed=4, jitter=0.1)
+    p.pgrad_form = 1
+    o = Oracle(p, parts)
+    _wide_cells(o, p, parts["h"][0])
+    o.nnps()
+    o.prep()
+    s1, _, _ = o.get_particles(abi.SPH)
+    X = np.stack([s1["x"], s1["y"], s1["z"]], 1)
+    a = np.array([0.0, 0.0, 0.0]) if field == "constant" else np.array([310.0, -170.0, 2200.0])
+    o.set_pressure(X @ a + 1.0e5)
+    o.interaction()
+    s1, _, _ = o.get_particles(abi.SPH)
+    rows = (s1["p_type"] >= 1) & (s1["p_type"] < 1000)
+    pgf = np.stack([s1["pgf_x"], s1["pgf_y"], s1["pgf_z"]], 1)[rows]
+    if field == "constant":
+        assert np.max(np.abs(pgf)) < 1e-6 * 1.0e5, np.max(np.abs(pgf))
+    else:
+        assert np.max(np.abs(pgf + a)) < 1e-6 * np.max(np.abs(a)), np.max(np.abs(pgf + a))
+    # the symmetric form does not cancel the 1e5 offset
+    p.pgrad_form = 0
+    o2 = Oracle(p, parts)
+    o2.nnps()
+    o2.prep()
+    o2.set_pressure(X @ a + 1.0e5)
+    o2.interaction()
+    t1, _, _ = o2.get_particles(abi.SPH)
+    pgf0 = np.stack([t1["pgf_x"], t1["pgf_y"], t1["pgf_z"]], 1)[rows]
+    assert np.max(np.abs(pgf0 + a)) > 1.0
