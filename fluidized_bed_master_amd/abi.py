@@ -26,7 +26,7 @@ KNL_GAUSSIAN, KNL_QUINTIC, KNL_QUARTIC, KNL_WENDLAND2, KNL_WENDLAND4, KNL_WENDLA
 PT_BOUNDARY, PT_FLUID, PT_MOVING, PT_DEM = 0, 1, 9, 1001
 PPE_JACOBI, PPE_BICGSTAB = 0, 1
 PPE_FP64, PPE_MIXED_IR = 0, 1
-GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 PREDICT, CORRECT = 0, 1
 ERRORS = {
     -1: "SOPHIA_E_ARG", -2: "SOPHIA_E_HIP", -3: "SOPHIA_E_IO", -4: "SOPHIA_E_STATE",
@@ -165,6 +165,7 @@ SIGNATURES = {
     "sophia_set_pressure": (C.c_int, [_ctx, _VP, C.c_int32]),
     "sophia_get_ppe_system": (C.c_int, [_ctx, _VP, _VP, _VP]),
     "sophia_ppe_apply": (C.c_int, [_ctx, _VP, _VP, C.c_int32]),
+    "sophia_monitors": (C.c_int, [_ctx, C.c_int32, _VP]),
     "sophia_ppe_layout": (C.c_int, [_ctx, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sophia_mlp_param_count": (C.c_int, [_P(C.c_int32)]),
     "sophia_mlp_set_params": (C.c_int, [_ctx, _VP, C.c_int32]),

@@ -371,6 +371,10 @@ template <typename R> int sph_dem_reaction(hipStream_t s, const DevParams& P, co
 int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
 
+// max |u|, rho, ftotal of one species (scratch: 3 * 1024 + 3 doubles)
+template <typename U>
+int monitor_max(hipStream_t s, int n, const int* ptype, const int* itype, const U* ux, const U* uy, const U* uz,
+                const double* rho, const U* ft, double* scratch, double* out);
 int knl_eval_device(hipStream_t s, int kt, int dim, double h, const double* r, int n, double* w, double* dw);
 
 }  // namespace sophia

@@ -117,6 +117,7 @@ struct sophia_ctx {
   virtual int get_ppe_system(double* diag, double* rhs, int32_t* row) const = 0;
   virtual int ppe_apply(const double* x, double* y, int n) = 0;
   virtual int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const = 0;
+  virtual int monitors(int species, double* out) = 0;
   virtual int mlp_predict(double* out, int n) = 0;
   virtual int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) = 0;
   virtual int mlp_set(const float* w, int n) = 0;
@@ -141,6 +142,10 @@ struct Ctx : public sophia_ctx {
   int ns = 0, nd = 0;
   // SPH persistent
   PField<double> sx, sy, sz, sh, spres, sm, srho;
+  PField<double> sph1, sph2;  // pressure of the two solves before the last one
+  int p_hist = 0;             // valid history levels (0..2)
+  double* pbase = nullptr;    // extrapolated PPE guess
+  double* mon_scratch = nullptr;
   PField<R> sux, suy, suz, stemp, sflt, sdpor;
   PField<int> sptype, sbtype, sitype;
   // DEM persistent
@@ -280,7 +285,9 @@ struct Ctx : public sophia_ctx {
     d.n_dem = nd;
     size_t NS = ns, ND = nd;
     // SPH persistent
-    for (auto* f : {&sx, &sy, &sz, &sh, &spres, &sm, &srho}) TRY(f->alloc(arena, NS));
+    for (auto* f : {&sx, &sy, &sz, &sh, &spres, &sm, &srho, &sph1, &sph2}) TRY(f->alloc(arena, NS));
+    TRY(arena.alloc(&pbase, NS));
+    TRY(arena.alloc(&mon_scratch, (size_t)3 * 1024 + 3));
     for (auto* f : {&sux, &suy, &suz, &stemp, &sflt, &sdpor}) TRY(f->alloc(arena, NS));
     for (auto* f : {&sptype, &sbtype, &sitype}) TRY(f->alloc(arena, NS));
     // SPH scratch
@@ -466,6 +473,8 @@ struct Ctx : public sophia_ctx {
       std::vector<PermuteField> f;
       auto add = [&](auto& fld, int bytes) { f.push_back(PermuteField{fld.cur, fld.spare, bytes}); };
       add(sx, 8); add(sy, 8); add(sz, 8); add(sh, 8); add(spres, 8); add(sm, 8); add(srho, 8);
+      if (p_hist > 0) add(sph1, 8);
+      if (p_hist > 1) add(sph2, 8);
       const int rb = (int)sizeof(R);
       add(sux, rb); add(suy, rb); add(suz, rb); add(stemp, rb); add(sflt, rb);
       add(sdpor, rb);
@@ -473,6 +482,8 @@ struct Ctx : public sophia_ctx {
       TRY(permute_fields(stream, ns, sorder, f.data(), (int)f.size()));
       TRY(cell_offsets(stream, ns, skeys, d.num_cells, scoff));
       for (auto* fl : {&sx, &sy, &sz, &sh, &spres, &sm, &srho}) fl->swap();
+      if (p_hist > 0) sph1.swap();
+      if (p_hist > 1) sph2.swap();
       for (auto* fl : {&sux, &suy, &suz, &stemp, &sflt, &sdpor}) fl->swap();
       for (auto* fl : {&sptype, &sbtype, &sitype}) fl->swap();
       refresh_views();
@@ -659,31 +670,42 @@ struct Ctx : public sophia_ctx {
     }
     ppe_assembled = true;
     if (o->method == SOPHIA_PPE_JACOBI) {
+      TRY(rotate_pressure_history());
       TRY(ppe_jacobi<R>(stream, sv, ppe_w));
       st->iterations = 1;
       st->converged = 1;
     } else {
       const double* guess = nullptr;
       st->guess_used = o->guess == SOPHIA_GUESS_ZERO ? SOPHIA_GUESS_ZERO : SOPHIA_GUESS_PREVIOUS;
-      const bool want_mlp = (o->guess == SOPHIA_GUESS_LEARNED || o->train) && mlp.ready;
+      const bool learned = o->guess == SOPHIA_GUESS_LEARNED && mlp.ready;
+      const bool want_mlp = (learned || o->train) && mlp.ready;
+      // the extrapolated guess, also the base of the learned correction
+      if (o->guess == SOPHIA_GUESS_EXTRAPOLATE || want_mlp) {
+        TRY(ppe_extrapolate(stream, ppe_w, sv.pres, sph1.cur, sph2.cur, p_hist, pbase));
+        if (o->guess == SOPHIA_GUESS_EXTRAPOLATE || learned) {
+          guess = pbase;
+          st->guess_used = SOPHIA_GUESS_EXTRAPOLATE;
+        }
+      }
       double scale = 0;
       if (want_mlp) {
-        TRY(mlp_prepare(&scale));
-        HIP_TRY(hipMemcpyAsync(mlp.pprev, sv.pres, (size_t)ns * 8, hipMemcpyDeviceToDevice, stream));
+        TRY(mlp_prepare(pbase, &scale));
+        HIP_TRY(hipMemcpyAsync(mlp.pprev, pbase, (size_t)ns * 8, hipMemcpyDeviceToDevice, stream));
       }
-      if (o->guess == SOPHIA_GUESS_LEARNED && mlp.ready) {
-        // learned guess with residual-based fallback to the previous pressure
+      if (learned) {
+        // learned correction with residual-based fallback to the extrapolation
         TimeScope t(tm(), "mlp_forward");
-        TRY(mlp_forward<R>(stream, P, sv, mlp, ppe_w.row, sv.pres, scale, mlp.guess));
-        double r_l = 0, r_p = 0;
+        TRY(mlp_forward<R>(stream, P, sv, mlp, ppe_w.row, pbase, scale, mlp.guess));
+        double r_l = 0, r_b = 0;
         TRY(ppe_residual_of(stream, ppe_w, mlp.guess, &r_l));
-        TRY(ppe_residual_of(stream, ppe_w, sv.pres, &r_p));
+        TRY(ppe_residual_of(stream, ppe_w, pbase, &r_b));
         st->r0_rel_learned = r_l;
-        if (r_l < r_p) {
+        if (r_l < r_b) {
           guess = mlp.guess;
           st->guess_used = SOPHIA_GUESS_LEARNED;
         }
       }
+      TRY(rotate_pressure_history());
       sophia_ppe_opts oo = *o;
       if (oo.max_iter <= 0) oo.max_iter = 10000;
       if (oo.rtol <= 0) oo.rtol = 1e-8;
@@ -896,6 +918,18 @@ struct Ctx : public sophia_ctx {
     return 0;
   }
 
+  int monitors(int species, double* out) override {
+    if (species == SOPHIA_SPH) {
+      if (!sph_on() || ns == 0) {
+        out[0] = out[1] = out[2] = 0.0;
+        return 0;
+      }
+      return monitor_max<R>(stream, ns, sv.ptype, sv.itype, sv.ux, sv.uy, sv.uz, sv.rho, sv.fmag, mon_scratch, out);
+    }
+    // the ISPH-DEM path never writes P3.ftotal of spheres: that monitor is 0
+    return monitor_max<double>(stream, nd, dv.ptype, dv.itype, dv.ux, dv.uy, dv.uz, dv.rho, nullptr, mon_scratch,
+                               out);
+  }
   int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const override {
     if (!ppe_assembled) {
       set_error("PPE not assembled yet");
@@ -926,10 +960,18 @@ struct Ctx : public sophia_ctx {
     return rc;
   }
 
+  // the pressure before this solve becomes history level 1, level 1 level 2
+  int rotate_pressure_history() {
+    std::swap(sph1.cur, sph2.cur);
+    HIP_TRY(hipMemcpyAsync(sph1.cur, sv.pres, (size_t)ns * 8, hipMemcpyDeviceToDevice, stream));
+    if (p_hist < 2) p_hist++;
+    return 0;
+  }
+
   // --- learned warm start -------------------------------------------------------
-  // residual of the current pressure and the network's node inputs
-  int mlp_prepare(double* scale) {
-    TRY(ppe_apply64(stream, ppe_w, sv.pres, mlp.r0, true, nullptr));
+  // residual of the base guess and the network's node inputs
+  int mlp_prepare(const double* base, double* scale) {
+    TRY(ppe_apply64(stream, ppe_w, base, mlp.r0, true, nullptr));
     return mlp_features<R>(stream, devparams(), sv, mlp, ppe_w.row, ppe_w.diag, scale);
   }
   int mlp_check(int n) const {
@@ -946,7 +988,7 @@ struct Ctx : public sophia_ctx {
   int mlp_predict(double* out, int n) override {
     TRY(mlp_check(n));
     double scale = 0;
-    TRY(mlp_prepare(&scale));
+    TRY(mlp_prepare(sv.pres, &scale));
     TRY(mlp_forward<R>(stream, devparams(), sv, mlp, ppe_w.row, sv.pres, scale, mlp.guess));
     HIP_TRY(hipMemcpyAsync(out, mlp.guess, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
@@ -959,7 +1001,7 @@ struct Ctx : public sophia_ctx {
       return SOPHIA_E_ARG;
     }
     double scale = 0;
-    TRY(mlp_prepare(&scale));
+    TRY(mlp_prepare(sv.pres, &scale));
     HIP_TRY(hipMemcpyAsync(mlp.target, target, (size_t)n * 8, hipMemcpyHostToDevice, stream));
     TRY(mlp_backward<R>(stream, devparams(), sv, mlp, ppe_w.row, sv.pres, mlp.target, scale, n_rows, step, lr, loss));
     if (grad) {
@@ -974,7 +1016,7 @@ struct Ctx : public sophia_ctx {
       return SOPHIA_E_STATE;
     }
     double sc = 0;
-    TRY(mlp_prepare(&sc));
+    TRY(mlp_prepare(sv.pres, &sc));
     if (scale) *scale = sc;
     if (nf) {
       HIP_TRY(hipMemcpyAsync(nf, mlp.nf, (size_t)n * sizeof(float2), hipMemcpyDeviceToHost, stream));
@@ -1441,6 +1483,11 @@ int sophia_get_ppe_system(const sophia_ctx* c, double* diag, double* rhs, int32_
   return c->get_ppe_system(diag, rhs, row);
 }
 int sophia_ppe_apply(sophia_ctx* c, const double* x, double* y, int32_t n) { CHECK_CTX(c); return c->ppe_apply(x, y, n); }
+int sophia_monitors(sophia_ctx* c, int32_t species, double* out) {
+  CHECK_CTX(c);
+  if (!out || (species != SOPHIA_SPH && species != SOPHIA_DEM)) return SOPHIA_E_ARG;
+  return c->monitors(species, out);
+}
 int sophia_ppe_layout(const sophia_ctx* c, int64_t* nnz, int32_t* ib, int32_t* cb) {
   if (!c) return SOPHIA_E_ARG;
   return c->ppe_layout(nnz, ib, cb);

@@ -110,15 +110,16 @@ __device__ void block_partials(const double (&v)[NV], double* partials) {
   }
 }
 
-// fixed-order sum of G partials per value, one block of kRedThreads
+// fixed-order sum of G partials per value, one block of kFinThreads
+constexpr int kFinThreads = 1024;
 template <int NV>
 __device__ void sum_partials(const double* partials, int G, double (&tot)[NV]) {
-  __shared__ double red[NV][kRedThreads / 64];
+  __shared__ double red[NV][kFinThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double s = 0;
-    for (int b = threadIdx.x; b < G; b += kRedThreads) s += partials[q * G + b];
+    for (int b = threadIdx.x; b < G; b += kFinThreads) s += partials[q * G + b];
     s = wave_sum(s);
     if (lane == 0) red[q][wave] = s;
   }
@@ -126,7 +127,7 @@ __device__ void sum_partials(const double* partials, int G, double (&tot)[NV]) {
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double s = 0;
-    for (int w = 0; w < kRedThreads / 64; w++) s += red[q][w];
+    for (int w = 0; w < kFinThreads / 64; w++) s += red[q][w];
     tot[q] = s;
   }
 }
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(kRedThreads) void k_bi_true(PpeOp o, const double* 
 
 // One block: sums the G partials of the preceding kernel and updates the
 // state.  Skips exactly when the producer skipped.
-__global__ __launch_bounds__(kRedThreads) void k_fin(const double* partials, int G, int op, BiState* st,
+__global__ __launch_bounds__(kFinThreads) void k_fin(const double* partials, int G, int op, BiState* st,
                                                      unsigned* count_out) {
   if ((op == kFinAlpha || op == kFinOmega || op == kFinRho) && !iter_live(st)) return;
   if (op == kFinTrue && (st->done || !st->update)) return;
@@ -734,13 +735,26 @@ __global__ void k_scatter_rows(int n, const int* row, const double* x, double* p
   if (row[i]) pres[i] = x[i];
 }
 
+// polynomial extrapolation of the pressure history on rows
+__global__ void k_extrapolate(int n, const int* row, const double* p, const double* h1, const double* h2, int depth,
+                              double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = p[i];
+  if (row[i]) {
+    if (depth >= 2) v = 3.0 * p[i] - 3.0 * h1[i] + h2[i];
+    else if (depth == 1) v = 2.0 * p[i] - h1[i];
+  }
+  out[i] = v;
+}
+
 __global__ void k_zero_u32(unsigned* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
 
 void launch_fin(hipStream_t s, const PpeWork& w, int op, unsigned* count_out = nullptr) {
-  k_fin<<<1, kRedThreads, 0, s>>>(w.partials, w.nbg, op, (BiState*)w.state, count_out);
+  k_fin<<<1, kFinThreads, 0, s>>>(w.partials, w.nbg, op, (BiState*)w.state, count_out);
 }
 
 // host-side dispatch on (format, coefficient mode)
@@ -925,6 +939,14 @@ int ppe_rhs_norm(hipStream_t s, PpeWork& w, double* bn) {
   BiState h;
   TRY(read_state(s, w, &h));
   *bn = sqrt(h.norm);
+  return 0;
+}
+
+int ppe_extrapolate(hipStream_t s, const PpeWork& w, const double* p, const double* h1, const double* h2, int depth,
+                    double* out) {
+  if (w.n <= 0) return 0;
+  k_extrapolate<<<grid_for(w.n), kBlock, 0, s>>>(w.n, w.row, p, h1, h2, depth, out);
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 

@@ -78,6 +78,9 @@ int ppe_apply64(hipStream_t s, PpeWork& w, const double* x, double* y, bool resi
 int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o, sophia_ppe_stats& st,
               const double* guess_or_null, Timers* tm);
 int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel);
+// out = 3 p - 3 h1 + h2 (depth 2), 2 p - h1 (depth 1) or p on rows; p elsewhere
+int ppe_extrapolate(hipStream_t s, const PpeWork& w, const double* p, const double* h1, const double* h2, int depth,
+                    double* out);
 int ppe_rhs_norm(hipStream_t s, PpeWork& w, double* bn);
 
 }  // namespace sophia

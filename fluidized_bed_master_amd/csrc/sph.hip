@@ -14,14 +14,52 @@ namespace sophia {
 
 namespace {
 
+// Fields of one neighbour, gathered before any of them is used.
+template <typename R>
+struct Nb {
+  double x, y, z, m, rho, pres, h;
+  R ux, uy, uz, temp, rref;
+  int pt;
+};
+template <typename R>
+__device__ inline Nb<R> load_nb(const SphView<R>& v, int j) {
+  Nb<R> q;
+  q.x = v.x[j]; q.y = v.y[j]; q.z = v.z[j];
+  q.m = v.m[j]; q.rho = v.rho[j]; q.pres = v.pres[j]; q.h = v.h[j];
+  q.ux = v.ux[j]; q.uy = v.uy[j]; q.uz = v.uz[j]; q.temp = v.temp[j]; q.rref = v.rho_ref[j];
+  q.pt = v.ptype[j];
+  return q;
+}
+
+// The list in batches of four, in list order, in two phases: the fields of
+// all four neighbours are gathered (straight-line code, so every load of the
+// batch is in flight together), then f(q, on) runs per neighbour; the next
+// batch's indices are loaded before this batch is consumed.  Loads of fields
+// a body never reads are dropped by the compiler.  Slots past the count
+// repeat the last entry with on = false; bodies add `on ? term : 0` (a
+// select: exact, and NaN-safe for the unused slots).
 template <typename R, class F>
-__device__ inline void for_nbrs(const SphView<R>& v, int i, bool eps_rule, F&& f) {
+__device__ inline void for_nbrs4(const SphView<R>& v, int i, bool eps_rule, F&& f) {
   const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+  if (cnt <= 0) return;
   const size_t n = (size_t)v.n;
-  for (int k = 0; k < cnt; k++) {
-    uint32_t e = v.nbr[(size_t)k * n + i];
-    if (eps_rule && (e & kNbrFlag)) continue;
-    f((int)(e & ~kNbrFlag));
+  uint32_t en[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(u, cnt - 1) * n + i];
+  for (int k = 0; k < cnt; k += 4) {
+    uint32_t e[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) e[u] = en[u];
+    Nb<R> q[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) q[u] = load_nb(v, (int)(e[u] & ~kNbrFlag));
+#pragma unroll
+    for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(k + 4 + u, cnt - 1) * n + i];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const bool on = (k + u < cnt) && !(eps_rule && (e[u] & kNbrFlag));
+      f(q[u], on);
+    }
   }
 }
 
@@ -36,7 +74,7 @@ __global__ void k_mass_init(DevParams P, SphView<R> v) {
 
 // KERNEL_clc_correction_KGC_3D_sph (functions_PREP2.cuh:452-568)
 template <typename R>
-__global__ void k_kgc(DevParams P, SphView<R> v) {
+__global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
   typedef R T;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
@@ -49,20 +87,19 @@ __global__ void k_kgc(DevParams P, SphView<R> v) {
   const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rs = (T)(P.kappa * hd);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   T xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
-  for_nbrs(v, i, false, [&](int j) {
-    if (v.ptype[j] > 1000) return;
-    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+  for_nbrs4(v, i, false, [&](const Nb<R>& q, bool on) {
+    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
     T r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(r > 0 && r < Rs)) return;
+    const bool use = on && q.pt <= 1000 && r > 0 && r < Rs;
     T dw = knl_dw<T>(P.kernel_type, A, h, r);
-    T mtd = (T)v.m[j] * dw;
-    T rtd = (T)1 / ((T)v.rho[j] * r);
-    xx += -mtd * dx * dx * rtd;
-    xy += -mtd * dy * dx * rtd;
-    yy += -mtd * dy * dy * rtd;
-    zx += -mtd * dx * dz * rtd;
-    yz += -mtd * dy * dz * rtd;
-    zz += -mtd * dz * dz * rtd;
+    T mtd = (T)q.m * dw;
+    T rtd = (T)1 / ((T)q.rho * r);
+    xx += use ? -mtd * dx * dx * rtd : (T)0;
+    xy += use ? -mtd * dy * dx * rtd : (T)0;
+    yy += use ? -mtd * dy * dy * rtd : (T)0;
+    zx += use ? -mtd * dx * dz * rtd : (T)0;
+    yz += use ? -mtd * dy * dz * rtd : (T)0;
+    zz += use ? -mtd * dz * dz * rtd : (T)0;
   });
   T det = xx * (yy * zz - yz * yz);
   det -= xy * (xy * zz - yz * zx);
@@ -86,7 +123,7 @@ __global__ void k_kgc(DevParams P, SphView<R> v) {
 // KERNEL_clc_prep3D_coupling_sph (functions_PREP2.cuh:3190-3364): porosity
 // eps_i = 1 - sum_a V_a W_ia / sum_{j fluid} V_j W_ij and its rate.
 template <typename R>
-__global__ void k_porosity_sph(DevParams P, SphView<R> v, DemView d, CellView dc) {
+__global__ __launch_bounds__(kBlock) void k_porosity_sph(DevParams P, SphView<R> v, DemView d, CellView dc) {
   typedef R T;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
@@ -98,12 +135,13 @@ __global__ void k_porosity_sph(DevParams P, SphView<R> v, DemView d, CellView dc
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const int pti = v.ptype[i];
   T flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
-  for_nbrs(v, i, true, [&](int j) {
-    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
     T w = knl_w<T>(P.kernel_type, A, h, r);
-    int ptj = v.ptype[j];
-    if (pti <= 1000 && ptj <= 1000 && ptj == 1) flt += (T)v.m[j] / (T)v.rho[j] * w;
+    int ptj = q.pt;
+    const bool use = on && pti <= 1000 && ptj <= 1000 && ptj == 1;
+    flt += use ? (T)q.m / (T)q.rho * w : (T)0;
   });
   if (d.n > 0) {
     for_cells(P, xi, yi, zi, dc, [&](int j) {
@@ -151,7 +189,7 @@ __global__ void k_prep_prep(DevParams P, SphView<R> v) {
 // KERNEL_clc_prep3D_sph (functions_PREP2.cuh:1985-2227): Shepard filter and
 // the Antuono density gradient.
 template <typename R>
-__global__ void k_prep(DevParams P, SphView<R> v) {
+__global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   typedef R T;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
@@ -163,20 +201,20 @@ __global__ void k_prep(DevParams P, SphView<R> v) {
   const T rhoi = v.rho[i], rrefi = v.rho_ref[i];
   const bool anto = P.delsph == 2;
   T flt = 0, rx = 0, ry = 0, rz = 0;
-  for_nbrs(v, i, true, [&](int j) {
-    if (v.ptype[j] > 1000) return;
-    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+    const bool use = on && q.pt <= 1000;
+    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
-    T mj = v.m[j], rhoj = v.rho[j];
-    if (P.filt_step) flt += mj / rhoj * knl_w<T>(P.kernel_type, A, h, r);
+    T mj = q.m, rhoj = q.rho;
+    if (P.filt_step) flt += use ? mj / rhoj * knl_w<T>(P.kernel_type, A, h, r) : (T)0;
     if (anto) {
       T dw = knl_dw<T>(P.kernel_type, A, h, r);
       T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
       kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      T c = -(rhoj / (T)v.rho_ref[j] - rhoi / rrefi) * (mj / rhoj);
-      rx += c * gx;
-      ry += c * gy;
-      rz += c * gz;
+      T c = -(rhoj / (T)q.rref - rhoi / rrefi) * (mj / rhoj);
+      rx += use ? c * gx : (T)0;
+      ry += use ? c * gy : (T)0;
+      rz += use ? c * gz : (T)0;
     }
   });
   if (P.filt_step) v.flt_s[i] = flt;
@@ -191,7 +229,7 @@ __global__ void k_prep(DevParams P, SphView<R> v) {
 // artificial viscosity, Eulerian advection, conduction, gravity and the
 // pressure-gradient force density used by the coupling.
 template <typename R>
-__global__ void k_advection(DevParams P, SphView<R> v) {
+__global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v) {
   typedef R T;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
@@ -214,26 +252,26 @@ __global__ void k_advection(DevParams P, SphView<R> v) {
   const double Ad = knl_tmpA<double>(P.kernel_type, P.dim, hd);
   T tx = 0, ty = 0, tz = 0, Rc = 0, ex = 0, ey = 0, ez = 0, et = 0;
   double px = 0, py = 0, pz = 0;
-  for_nbrs(v, i, true, [&](int j) {
-    if (!(v.ptype[j] < 1000)) return;
+  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+    const bool use = on && q.pt < 1000;
     // geometry in FP64: the pressure-gradient term cancels a hydrostatic
     // offset of order H/h times its result
-    const double dxd = xi - v.x[j], dyd = yi - v.y[j], dzd = zi - v.z[j];
+    const double dxd = xi - q.x, dyd = yi - q.y, dzd = zi - q.z;
     const double rd = sqrt(dxd * dxd + dyd * dyd + dzd * dzd) + 1e-20;
     const double dwd = knl_dw<double>(P.kernel_type, Ad, hd, rd);
     double gxd = dwd * dxd / rd, gyd = dwd * dyd / rd, gzd = dwd * dzd / rd;
     kgc_apply(P, v.cm, n, i, gxd, gyd, gzd);
     if (P.fp_solve) {
-      const double pp = P.pgrad_form ? v.pres[j] - pid : pid + v.pres[j];
-      const double Cp = -porid * v.m[j] * pp / (rhoid * v.rho[j]);
-      px += rhoid / porid * Cp * gxd;
-      py += rhoid / porid * Cp * gyd;
-      pz += rhoid / porid * Cp * gzd;
+      const double pp = P.pgrad_form ? q.pres - pid : pid + q.pres;
+      const double Cp = -porid * q.m * pp / (rhoid * q.rho);
+      px += use ? rhoid / porid * Cp * gxd : 0.0;
+      py += use ? rhoid / porid * Cp * gyd : 0.0;
+      pz += use ? rhoid / porid * Cp * gzd : 0.0;
     }
     const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd, r = (T)rd, dw = (T)dwd;
     const T gx = (T)gxd, gy = (T)gyd, gz = (T)gzd;
-    const T mj = (T)v.m[j], rhoj = (T)v.rho[j];
-    const T uxj = v.ux[j], uyj = v.uy[j], uzj = v.uz[j];
+    const T mj = (T)q.m, rhoj = (T)q.rho;
+    const T uxj = q.ux, uyj = q.uy, uzj = q.uz;
     if (P.fv_solve) {
       T visj = (T)P.mu_fluid;
       T Cv = dx * gx + dy * gy + dz * gz;
@@ -241,39 +279,39 @@ __global__ void k_advection(DevParams P, SphView<R> v) {
       Cv *= 4 * pori * (mj / (rhoi * rhoj));
       Cv /= r;
       Cv /= r;
-      tx += Cv * (uxi - uxj);
-      ty += Cv * (uyi - uyj);
-      tz += Cv * (uzi - uzj);
+      tx += use ? Cv * (uxi - uxj) : (T)0;
+      ty += use ? Cv * (uyi - uyj) : (T)0;
+      tz += use ? Cv * (uzi - uzj) : (T)0;
     }
     if (P.fva_solve) {
       T uij_xij = (uxi - uxj) * dx + (uyi - uyj) * dy + (uzi - uzj) * dz;
-      if (uij_xij < 0) {
-        T hij = (h + (T)v.h[j]) * (T)0.5;
-        T phi = hij * uij_xij;
-        phi /= (r * r + (T)0.01 * hij * hij);
-        T Pij = mi * phi * ((T)(-0.1 * P.soundspeed) + (T)0.02 * phi);
-        Pij /= (rhoi + rhoj);
-        Pij *= (T)0.5;
-        tx += -Pij * gx;
-        ty += -Pij * gy;
-        tz += -Pij * gz;
-      }
+      T hij = (h + (T)q.h) * (T)0.5;
+      T phi = hij * uij_xij;
+      phi /= (r * r + (T)0.01 * hij * hij);
+      T Pij = mi * phi * ((T)(-0.1 * P.soundspeed) + (T)0.02 * phi);
+      Pij /= (rhoi + rhoj);
+      Pij *= (T)0.5;
+      const bool av = use && uij_xij < 0;
+      tx += av ? -Pij * gx : (T)0;
+      ty += av ? -Pij * gy : (T)0;
+      tz += av ? -Pij * gz : (T)0;
     }
     if (euler) {
       T c = mj / rhoj;
-      ex += (uxi * (uxj - uxi) * gx + uyi * (uxj - uxi) * gy + uzi * (uxj - uxi) * gz) * c * (1 - eli);
-      ey += (uxi * (uyj - uyi) * gx + uyi * (uyj - uyi) * gy + uzi * (uyj - uyi) * gz) * c * (1 - eli);
-      ez += (uxi * (uzj - uzi) * gx + uyi * (uzj - uzi) * gy + uzi * (uzj - uzi) * gz) * c * (1 - eli);
+      ex += use ? (uxi * (uxj - uxi) * gx + uyi * (uxj - uxi) * gy + uzi * (uxj - uxi) * gz) * c * (1 - eli) : (T)0;
+      ey += use ? (uxi * (uyj - uyi) * gx + uyi * (uyj - uyi) * gy + uzi * (uyj - uyi) * gz) * c * (1 - eli) : (T)0;
+      ez += use ? (uxi * (uzj - uzi) * gx + uyi * (uzj - uzi) * gy + uzi * (uzj - uzi) * gz) * c * (1 - eli) : (T)0;
       if (P.con_solve) {
-        T tj = v.temp[j];
-        et += (uxi * (tj - tempi) * gx + uyi * (tj - tempi) * gy + uzi * (tj - tempi) * gz) * c * (1 - eli);
+        T tj = q.temp;
+        et += use ? (uxi * (tj - tempi) * gx + uyi * (tj - tempi) * gy + uzi * (tj - tempi) * gz) * c * (1 - eli)
+                  : (T)0;
       }
     }
     if (P.con_solve) {
-      T kcj = (T)(v.ptype[j] > 1000 ? P.k_dem : P.k_fluid);
-      T H = 4 * pori * mj * kcj * kci * (tempi - (T)v.temp[j]) * dw;
+      T kcj = (T)(q.pt > 1000 ? P.k_dem : P.k_fluid);
+      T H = 4 * pori * mj * kcj * kci * (tempi - (T)q.temp) * dw;
       H /= (r + (T)1e-10) * rhoi * rhoj * (kci + kcj);
-      Rc += H;
+      Rc += use ? H : (T)0;
     }
   });
   if (P.fg_solve) tz += -(T)P.gravity;
@@ -327,7 +365,7 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
 
 // KERNEL_pressureforce3D_sph (functions_PPE2.cuh:2154-2335)
 template <typename R>
-__global__ void k_pressure_force(DevParams P, SphView<R> v) {
+__global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<R> v) {
   typedef double T;  // FP64 pair sums: see the note on pressure-gradient cancellation
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
@@ -342,18 +380,18 @@ __global__ void k_pressure_force(DevParams P, SphView<R> v) {
   const T rhoi = v.rho[i], pori = v.por[i];
   T tx = 0, ty = 0, tz = 0;
   if (P.fp_solve) {
-    for_nbrs(v, i, true, [&](int j) {
-      if (!(v.ptype[j] < 1000)) return;
-      T dx = xi - v.x[j], dy = yi - v.y[j], dz = zi - v.z[j];
+    for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+      const bool use = on && q.pt < 1000;
+      T dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
       T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
       T dw = knl_dw<T>(P.kernel_type, A, h, r);
       T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
       kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      const T pp = P.pgrad_form ? v.pres[j] - pid : pid + v.pres[j];
-      T Cp = -pori * v.m[j] * pp / (rhoi * v.rho[j]);
-      tx += Cp * gx;
-      ty += Cp * gy;
-      tz += Cp * gz;
+      const T pp = P.pgrad_form ? q.pres - pid : pid + q.pres;
+      T Cp = -pori * q.m * pp / (rhoi * q.rho);
+      tx += use ? Cp * gx : (T)0;
+      ty += use ? Cp * gy : (T)0;
+      tz += use ? Cp * gz : (T)0;
     });
   }
   v.fpx[i] = (R)tx;
@@ -419,7 +457,7 @@ __global__ void k_corrector(DevParams P, SphView<R> v) {
 // KERNEL_xsph3D_sph + KERNEL_xsph3D_calc_sph (functions_TIME.cuh:1735-1850,
 // 1985-1997): Shepard-normalised XSPH velocity and temperature smoothing
 template <typename R>
-__global__ void k_xsph(DevParams P, SphView<R> v) {
+__global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
   typedef R T;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
@@ -430,18 +468,18 @@ __global__ void k_xsph(DevParams P, SphView<R> v) {
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i], ti = v.temp[i], c = (T)P.c_xsph;
   T tx = 0, ty = 0, tz = 0, tt = 0, fl = 0;
-  for_nbrs(v, i, false, [&](int j) {
-    int ptj = v.ptype[j];
-    if (!(ptj == 1 || ptj == 3)) return;
-    T dx = (T)(xi - v.x[j]), dy = (T)(yi - v.y[j]), dz = (T)(zi - v.z[j]);
+  for_nbrs4(v, i, false, [&](const Nb<R>& q, bool on) {
+    int ptj = q.pt;
+    const bool use = on && (ptj == 1 || ptj == 3);
+    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
     T r = sqrt(dx * dx + dy * dy + dz * dz);
     T w = knl_w<T>(P.kernel_type, A, h, r);
-    T vj = (T)v.m[j] / (T)v.rho[j];
-    tx += c * vj * (-uxi + (T)v.ux[j]) * w;
-    ty += c * vj * (-uyi + (T)v.uy[j]) * w;
-    tz += c * vj * (-uzi + (T)v.uz[j]) * w;
-    tt += (T)0.01 * c * vj * ((T)v.temp[j] - ti) * w;
-    fl += vj * w;
+    T vj = (T)q.m / (T)q.rho;
+    tx += use ? c * vj * (-uxi + (T)q.ux) * w : (T)0;
+    ty += use ? c * vj * (-uyi + (T)q.uy) * w : (T)0;
+    tz += use ? c * vj * (-uzi + (T)q.uz) * w : (T)0;
+    tt += use ? (T)0.01 * c * vj * ((T)q.temp - ti) * w : (T)0;
+    fl += use ? vj * w : (T)0;
   });
   const T on = (T)(pti > 0);
   v.xsx[i] = (uxi + tx / fl) * on;
@@ -470,6 +508,63 @@ __global__ void k_xsph_calc(DevParams P, SphView<R> v) {
 }
 
 }  // namespace
+
+// kernel_copy_max_sph/_dem + cub::DeviceReduce::Max (functions_OUTPUT.cuh:35-65,
+// ISPH_Calc_seperate.cuh:928-969): max |u|, rho and ftotal over one species;
+// walls (p_type 0) and i_type >= i_type_crt (2) count as 0.  Max is order
+// independent, so the two-stage reduction is exact.
+constexpr int kMonBlocks = 1024;
+template <typename U>
+__global__ __launch_bounds__(256) void k_monitor(int n, const int* ptype, const int* itype, const U* ux, const U* uy,
+                                                 const U* uz, const double* rho, const U* ft, double* partials) {
+  __shared__ double red[3][4];
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (ptype[i] == 0 || itype[i] >= 2) continue;
+    const double a = (double)ux[i], b = (double)uy[i], c = (double)uz[i];
+    m[0] = fmax(m[0], sqrt(a * a + b * b + c * c));
+    m[1] = fmax(m[1], rho[i]);
+    if (ft) m[2] = fmax(m[2], (double)ft[i]);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < 3; q++) {
+    double v = m[q];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if (lane == 0) red[q][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    partials[q * gridDim.x + blockIdx.x] = fmax(fmax(red[q][0], red[q][1]), fmax(red[q][2], red[q][3]));
+  }
+}
+
+__global__ void k_monitor_fin(const double* partials, int G, double* out) {
+  const int q = threadIdx.x;
+  if (q >= 3) return;
+  double v = 0.0;
+  for (int b = 0; b < G; b++) v = fmax(v, partials[q * G + b]);
+  out[q] = v;
+}
+
+template <typename U>
+int monitor_max(hipStream_t s, int n, const int* ptype, const int* itype, const U* ux, const U* uy, const U* uz,
+                const double* rho, const U* ft, double* scratch, double* out) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (n <= 0) return 0;
+  int G = grid_for(n);
+  if (G > kMonBlocks) G = kMonBlocks;
+  k_monitor<U><<<G, 256, 0, s>>>(n, ptype, itype, ux, uy, uz, rho, ft, scratch);
+  k_monitor_fin<<<1, 64, 0, s>>>(scratch, G, scratch + 3 * kMonBlocks);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, scratch + 3 * kMonBlocks, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+template int monitor_max<float>(hipStream_t, int, const int*, const int*, const float*, const float*, const float*,
+                                const double*, const float*, double*, double*);
+template int monitor_max<double>(hipStream_t, int, const int*, const int*, const double*, const double*,
+                                 const double*, const double*, const double*, double*, double*);
 
 #define LAUNCH(kern, ...)                                           \
   do {                                                              \

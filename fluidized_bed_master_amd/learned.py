@@ -76,11 +76,12 @@ def train(sim, snapshots: int = 400, lr: float = 1e-3, precision: int = abi.PPE_
 
 def evaluate(make_sim, params: np.ndarray, steps: int = 20, warm: int = 0,
              precision: int = abi.PPE_MIXED_IR, rtol: float = 1e-8):
-    """Run the same state with zero, previous-step and learned guesses in
+    """Run the same state with zero, previous-step, extrapolated and learned guesses in
     lockstep; returns per-guess lists of PPE iterations and initial
     residuals, plus the learned path's fallback count."""
     res = {}
-    for name, guess in (("zero", abi.GUESS_ZERO), ("previous", abi.GUESS_PREVIOUS), ("learned", abi.GUESS_LEARNED)):
+    for name, guess in (("zero", abi.GUESS_ZERO), ("previous", abi.GUESS_PREVIOUS),
+                        ("extrapolate", abi.GUESS_EXTRAPOLATE), ("learned", abi.GUESS_LEARNED)):
         sim = make_sim()
         if guess == abi.GUESS_LEARNED:
             sim.mlp_set_params(params)

@@ -170,6 +170,12 @@ class Sophia:
         check(self.lib.sophia_ppe_apply(self._ctx, ptr(x), ptr(y), len(x)), "sophia_ppe_apply")
         return y
 
+    def monitors(self, species: int = SPH) -> np.ndarray:
+        """max |u|, max rho, max ftotal of one species."""
+        out = np.zeros(3)
+        check(self.lib.sophia_monitors(self._ctx, species, ptr(out)), "sophia_monitors")
+        return out
+
     def ppe_layout(self) -> dict:
         nnz = C.c_int64()
         ib = C.c_int32()

@@ -160,7 +160,10 @@ typedef struct sophia_domain {
 
 #define SOPHIA_GUESS_ZERO 0
 #define SOPHIA_GUESS_PREVIOUS 1
-#define SOPHIA_GUESS_LEARNED 2  /* learned guess; falls back if its residual is larger */
+#define SOPHIA_GUESS_LEARNED 2  /* extrapolated guess plus a learned correction; falls back
+                                   to the extrapolated guess if its residual is larger */
+#define SOPHIA_GUESS_EXTRAPOLATE 3 /* 3 p^n - 3 p^(n-1) + p^(n-2) from the last three solutions
+                                      (2 p^n - p^(n-1) with two, p^n with one) */
 
 typedef struct sophia_ppe_opts {
   int32_t method;       /* SOPHIA_PPE_JACOBI / SOPHIA_PPE_BICGSTAB */
@@ -248,6 +251,11 @@ int sophia_get_nnps(const sophia_ctx* ctx, int32_t species, int32_t* keys,
 /* export the state in sorted order as reference records (any may be NULL) */
 int sophia_get_particles(const sophia_ctx* ctx, int32_t species, part1* p1,
                          part2* p2, part3* p3);
+/* run monitors of one species (kernel_copy_max_sph/_dem + cub::DeviceReduce::Max,
+ * functions_OUTPUT.cuh:35-65, ISPH_Calc_seperate.cuh:928-969): out[0] = max |u|,
+ * out[1] = max rho, out[2] = max P3.ftotal, over particles with p_type != 0 and
+ * i_type < 2 (others count as 0) */
+int sophia_monitors(sophia_ctx* ctx, int32_t species, double out[3]);
 /* number of particles of a species */
 int sophia_count(const sophia_ctx* ctx, int32_t species, int32_t* n);
 /* evaluate W and dW/dr on the device (calc_kernel_wij / calc_kernel_dwij) */

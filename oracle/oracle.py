@@ -166,6 +166,19 @@ class Oracle:
         self.lib.orc_get_particles(self._s, species, ptr(p1), ptr(p2), ptr(p3))
         return p1, p2, p3
 
+    def monitors(self, species: int = SPH) -> np.ndarray:
+        """kernel_copy_max_sph/_dem + DeviceReduce::Max (functions_OUTPUT.cuh:35-65):
+        max |u|, rho, P3.ftotal over p_type != 0 and i_type < i_type_crt (2);
+        excluded particles count as 0."""
+        p1, _, p3 = self.get_particles(species)
+        keep = (p1["p_type"] != 0) & (p1["i_type"] < 2)
+        u = np.sqrt(p1["ux"] ** 2 + p1["uy"] ** 2 + p1["uz"] ** 2)
+        out = np.zeros(3)
+        for k, a in enumerate((u, p1["rho"], p3["ftotal"])):
+            v = np.where(keep, a, 0.0)
+            out[k] = max(0.0, float(v.max())) if len(v) else 0.0
+        return out
+
     def set_pressure(self, p):
         p = np.ascontiguousarray(p, np.float64)
         self.lib.orc_set_pressure(self._s, ptr(p), len(p))

@@ -441,3 +441,26 @@ def test_pgrad_difference_form_exact(field):
     t1, _, _ = o2.get_particles(abi.SPH)
     pgf0 = np.stack([t1["pgf_x"], t1["pgf_y"], t1["pgf_z"]], 1)[rows]
     assert np.max(np.abs(pgf0 + a)) > 1.0
+
+
+# ---------------------------------------------------------------------------
+# run monitors (functions_OUTPUT.cuh:35-65): walls and i_type >= 2 excluded
+# ---------------------------------------------------------------------------
+def test_monitors_exclusion_rule():
+    p, parts = configs.small_isph_dem(n=8, nz=8, seed=3)
+    parts = parts.copy()
+    wall = np.flatnonzero(parts["p_type"] == 0)[0]
+    fluid = np.flatnonzero(parts["p_type"] == 1)
+    parts["ux"][wall] = 1.0e3           # a wall: never counted
+    parts["i_type"][fluid[0]] = 2       # i_type_crt: never counted
+    parts["rho"][fluid[0]] = 5.0e4
+    parts["uz"][fluid[1]] = 7.5         # the fastest counted particle
+    o = Oracle(p, parts)
+    o.nnps()
+    mon = o.monitors(abi.SPH)
+    assert mon[0] == pytest.approx(np.sqrt(parts["ux"][fluid[1]] ** 2 + parts["uy"][fluid[1]] ** 2 + 7.5 ** 2))
+    sph = parts["p_type"] <= 1000
+    counted = sph & (parts["p_type"] != 0) & (parts["i_type"] < 2)
+    assert not counted[fluid[0]] and not counted[wall]
+    assert mon[1] == pytest.approx(parts["rho"][counted].max())
+    assert mon[1] < 5.0e4
