@@ -103,7 +103,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--weights", default=str(ROOT / "fluidized_bed_master_amd" / "weights" / "ppe_mlp_c2.npy"))
+    ap.add_argument("--weights", default=str(ROOT / "fluidized_bed_master_amd" / "weights" / "ppe_mlp.npy"))
     a = ap.parse_args(argv)
 
     if a.gpus != 1:

@@ -166,6 +166,7 @@ SIGNATURES = {
     "sophia_get_ppe_system": (C.c_int, [_ctx, _VP, _VP, _VP]),
     "sophia_ppe_apply": (C.c_int, [_ctx, _VP, _VP, C.c_int32]),
     "sophia_monitors": (C.c_int, [_ctx, C.c_int32, _VP]),
+    "sophia_wcsph_stage": (C.c_int, [_ctx, C.c_int32]),
     "sophia_ppe_layout": (C.c_int, [_ctx, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sophia_mlp_param_count": (C.c_int, [_P(C.c_int32)]),
     "sophia_mlp_set_params": (C.c_int, [_ctx, _VP, C.c_int32]),

@@ -186,6 +186,42 @@ def c5(seed: int = 5):
                            seed=seed, dt=2.0e-6)
 
 
+def wcsph_column(nx: int = 10, ny: int = 10, nz: int = 20, dx: float = 1.0e-3, *, seed: int = 1,
+                 jitter: float = 0.02, u_noise: float = 0.0, dt: float | None = None, con: int = 0,
+                 delsph: int = 2, kgc: int = 1):
+    """Weakly compressible fluid column in a walled tank (no spheres): Tait
+    EOS with gamma = 7, continuity density, delta-SPH, predictor-corrector,
+    Wendland C2 h = 1.3 dx.  The reference EOS uses B = 1.186 c0^2 / gamma
+    with no rho0 factor (functions_PROP.cuh:21-26), so c0 is set to
+    c * sqrt(rho0 / 1.186) with c = 10 sqrt(2 g H): B = rho0 c^2 / gamma."""
+    p, parts = isph_dem_column(nx, ny, nz, dx, seed=seed, jitter=jitter, u_noise=u_noise, dt=2.0e-6)
+    H = nz * dx
+    p.solver_type = 0
+    p.time_type = 1
+    p.rho_type = 1
+    p.delSPH_solve = delsph
+    p.kgc_solve = kgc
+    p.soundspeed = 10.0 * math.sqrt(2.0 * 9.8 * H) * math.sqrt(p.rho_fluid / 1.186)
+    p.gamma = 7.0
+    p.con_solve = con
+    p.dt = dt if dt is not None else 0.2 * 1.3 * dx / p.soundspeed
+    if con:
+        parts = parts.copy()
+        parts["temp"] = 293.0 + 50.0 * (parts["z"] < 0.3 * H)
+    return p, parts
+
+
+def c4_train(seed: int = 4):
+    """Warm-start training column: the C5 sphere size and packing (d = 2 mm,
+    phi = 0.35) in a 0.06 x 0.06 box filled to 0.06 (216,000 fluid) at
+    dx = 1 mm, dt = 2e-6.  (C2's 3 mm spheres are wider than the kernel
+    support at h = 1.3 mm and drive the ISPH porosity estimate below zero.)"""
+    dx = 1.0e-3
+    s = 2.0e-3 * (math.pi / (6.0 * 0.35)) ** (1.0 / 3.0)
+    layers = math.ceil(0.3 * 0.06 / s)
+    return isph_dem_column(60, 60, 60, dx, d_dem=2.0e-3, dem_height=layers * s + 1e-9, seed=seed, dt=2.0e-6)
+
+
 def c2_isph(seed: int = 2):
     """C2's particle set (50 x 50 x 150 fluid at dx = 1 mm, 8000 spheres
     d = 3 mm, rho = 2500) run through the ISPH step, dt = 5e-7."""

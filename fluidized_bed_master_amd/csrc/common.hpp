@@ -72,7 +72,10 @@ struct SphView {
   int *ptype, *btype, *itype;
   // scratch
   double* por;  // porosity eps (FP64: pressure-gradient sums cancel against it)
-  R *dpor_prev, *rho_ref, *rho0;
+  R *dpor_prev;
+  double *rho_ref, *rho0;  // reference / start-of-step density (P2; the Tait EOS amplifies their rounding)
+  double *vol, *vol0;      // WCSPH particle volume and its initial value (P1.vol, P1.vol0)
+  double *x0, *y0, *z0;    // WCSPH start-of-step position (P2.x0)
   R *cm;              // 6 * n, symmetric KGC inverse (xx, xy, xz, yy, yz, zz)
   R *grx, *gry, *grz; // delta-SPH density gradient
   R *fx, *fy, *fz, *fmag;   // ftotal (P3)
@@ -176,6 +179,31 @@ __device__ inline void for_cells(const DevParams& P, double x, double y, double 
       int b = jc + dy;
       if (b < 0 || b > P.NJ - 1) continue;
       for (int dx = -1; dx <= 1; dx++) {
+        int a = ic + dx;
+        if (a < 0 || a > P.NI - 1) continue;
+        int k = cell_key(P, a, b, c);
+        uint32_t s = cv.start[k];
+        if (s == kEmpty) continue;
+        uint32_t e = cv.end[k];
+        for (uint32_t j = s; j < e; j++) f((int)j);
+      }
+    }
+  }
+}
+
+// the (2 rng + 1)^3 cells around (x,y,z), same visiting order as for_cells
+template <class F>
+__device__ inline void for_cells_r(const DevParams& P, double x, double y, double z, const CellView& cv, int rng,
+                                   F&& f) {
+  int ic, jc, kc;
+  cell_of(P, x, y, z, ic, jc, kc);
+  for (int dz = -rng; dz <= rng; dz++) {
+    int c = kc + dz;
+    if (c < 0 || c > P.NK - 1) continue;
+    for (int dy = -rng; dy <= rng; dy++) {
+      int b = jc + dy;
+      if (b < 0 || b > P.NJ - 1) continue;
+      for (int dx = -rng; dx <= rng; dx++) {
         int a = ic + dx;
         if (a < 0 || a > P.NI - 1) continue;
         int k = cell_key(P, a, b, c);
@@ -361,6 +389,7 @@ template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, 
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int sph_xsph_only(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_set_alpha_noop(hipStream_t s, const DevParams& P, const SphView<R>& v);
 
 // DEM stages
@@ -371,6 +400,14 @@ template <typename R> int sph_dem_reaction(hipStream_t s, const DevParams& P, co
 int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
 
+// WCSPH stages (wcsph.hip)
+template <typename R> int wc_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc_reset_p3(hipStream_t s, const SphView<R>& v);
+template <typename R> int wc_volume_mass(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int count);
+template <typename R> int wc_continuity(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc_eos(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc_interaction(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v);
 // max |u|, rho, ftotal of one species (scratch: 3 * 1024 + 3 doubles)
 template <typename U>
 int monitor_max(hipStream_t s, int n, const int* ptype, const int* itype, const U* ux, const U* uy, const U* uz,

@@ -55,6 +55,10 @@ struct Arena {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    // every array starts at zero: fields a stage leaves untouched for some
+    // particles read as the reference's zero-initialised records
+    e = hipMemset(q, 0, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMemset", __FILE__, __LINE__);
     ptrs.push_back(q);
     *p = (T*)q;
     return 0;
@@ -118,6 +122,7 @@ struct sophia_ctx {
   virtual int ppe_apply(const double* x, double* y, int n) = 0;
   virtual int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const = 0;
   virtual int monitors(int species, double* out) = 0;
+  virtual int wcsph_stage(int stage) = 0;
   virtual int mlp_predict(double* out, int n) = 0;
   virtual int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) = 0;
   virtual int mlp_set(const float* w, int n) = 0;
@@ -191,7 +196,8 @@ struct Ctx : public sophia_ctx {
     P.xsph_solve = p.xsph_solve;
     P.open_boundary = p.open_boundary;
     P.z_index = p.flag_z_index;
-    P.eulerian = p.eulerian_sph;
+    // WCSPH: KERNEL_set_alpha (functions_ALE.cuh:2-20) makes SPH Eulerian
+    P.eulerian = p.solver_type == 0 ? 1 : p.eulerian_sph;
     P.fix_xsph = p.fix_xsph_position;
     P.pgrad_form = p.pgrad_form;
     P.NI = d.NI;
@@ -293,7 +299,8 @@ struct Ctx : public sophia_ctx {
     // SPH scratch
     SphView<R>& v = sv;
     TRY(arena.alloc(&v.por, NS));
-    for (R** f : {&v.dpor_prev, &v.rho_ref, &v.rho0, &v.grx, &v.gry, &v.grz, &v.fx, &v.fy, &v.fz, &v.fmag,
+    for (double** f : {&v.rho_ref, &v.rho0, &v.vol, &v.vol0, &v.x0, &v.y0, &v.z0}) TRY(arena.alloc(f, NS));
+    for (R** f : {&v.dpor_prev, &v.grx, &v.gry, &v.grz, &v.fx, &v.fy, &v.fz, &v.fmag,
                   &v.fpx, &v.fpy, &v.fpz, &v.pgx, &v.pgy, &v.pgz, &v.dtemp, &v.temp0, &v.ux0, &v.uy0, &v.uz0,
                   &v.drho, &v.xsx, &v.xsy, &v.xsz, &v.xst})
       TRY(arena.alloc(f, NS));
@@ -799,9 +806,13 @@ struct Ctx : public sophia_ctx {
       std::vector<double> Mv, RHv, POv;
       TRY(get(Mv, sv.m, n)); TRY(get(RHv, sv.rho, n)); TRY(get(POv, sv.por, n));
       const R* src[] = {sv.ux, sv.uy, sv.uz, sv.ux, sv.ux, sv.temp, sv.flt_s, sv.dpor, sv.ux, sv.dpor_prev,
-                        sv.rho_ref, sv.rho0, sv.grx, sv.gry, sv.grz, sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx,
+                        sv.ux, sv.ux, sv.grx, sv.gry, sv.grz, sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx,
                         sv.fpy, sv.fpz, sv.pgx, sv.pgy, sv.pgz, sv.dtemp, sv.ux0, sv.uy0, sv.uz0, sv.temp0,
-                        sv.xsx, sv.xsy, sv.xsz, sv.xst};
+                        sv.xsx, sv.xsy, sv.xsz, sv.xst, sv.drho};
+      std::vector<double> RRv, R0v, VOLv, VOL0v, X0v, Y0v, Z0v;
+      TRY(get(RRv, sv.rho_ref, n)); TRY(get(R0v, sv.rho0, n));
+      TRY(get(VOLv, sv.vol, n)); TRY(get(VOL0v, sv.vol0, n));
+      TRY(get(X0v, sv.x0, n)); TRY(get(Y0v, sv.y0, n)); TRY(get(Z0v, sv.z0, n));
       const int NF = sizeof(src) / sizeof(src[0]);
       for (int k = 0; k < NF; k++) TRY(get(a[k], src[k], n));
       std::vector<R> cm;
@@ -820,23 +831,29 @@ struct Ctx : public sophia_ctx {
           q.grad_rhox = A(12); q.grad_rhoy = A(13); q.grad_rhoz = A(14);
           q.pgf_x = A(22); q.pgf_y = A(23); q.pgf_z = A(24);
           q.XSPH_ux = A(30); q.XSPH_uy = A(31); q.XSPH_uz = A(32); q.XSPH_temp = A(33);
-          double e = (pt[i] <= 1000 && p.eulerian_sph) ? 0.0 : 1.0;
+          // WCSPH: KERNEL_set_alpha makes every SPH particle Eulerian
+          double e = (pt[i] <= 1000 && (p.eulerian_sph || p.solver_type == 0)) ? 0.0 : 1.0;
           q.elix = q.eliy = q.eliz = e;
+          q.vol = VOLv[i]; q.vol0 = VOL0v[i];
           bool row = !rows.empty() && rows[i];
           q.PPE1 = row ? P1v[i] : 0; q.PPE2 = row ? P2v[i] : 0; q.PPE3 = row ? P3v[i] : 0; q.PPE4 = row ? P4v[i] : 0;
         }
         if (p2) {
           part2& q = p2[i];
           std::memset(&q, 0, sizeof(q));
-          q.rho_ref = A(10); q.rho0 = A(11);
+          q.rho_ref = RRv[i]; q.rho0 = R0v[i];
           q.ux0 = A(26); q.uy0 = A(27); q.uz0 = A(28); q.temp0 = A(29);
-          q.x0 = X[i]; q.y0 = Y[i]; q.z0 = Z[i];
+          if (p.solver_type == 0) {
+            q.x0 = X0v[i]; q.y0 = Y0v[i]; q.z0 = Z0v[i];
+          } else {
+            q.x0 = X[i]; q.y0 = Y[i]; q.z0 = Z[i];
+          }
         }
         if (p3) {
           part3& q = p3[i];
           std::memset(&q, 0, sizeof(q));
           q.ftotalx = A(15); q.ftotaly = A(16); q.ftotalz = A(17); q.ftotal = A(18);
-          q.fpx = A(19); q.fpy = A(20); q.fpz = A(21); q.dtemp = A(25);
+          q.fpx = A(19); q.fpy = A(20); q.fpz = A(21); q.dtemp = A(25); q.drho = A(34);
           double c[6];
           for (int k = 0; k < 6; k++) c[k] = (double)cm[(size_t)k * n + i];
           q.Cm[0][0] = c[0]; q.Cm[0][1] = c[1]; q.Cm[0][2] = c[2];
@@ -914,6 +931,88 @@ struct Ctx : public sophia_ctx {
       std::vector<int> r;
       TRY(get(r, (const int*)ppe_w.row, ns));
       std::memcpy(row, r.data(), (size_t)ns * 4);
+    }
+    return 0;
+  }
+
+  // WCSPH stages in SOPHIA_single's order (WCSPH_Calc.cuh:4-340); see abi.h
+  int wcsph_stage(int stage) override {
+    if (p.solver_type != 0) {
+      set_error("WCSPH stage on a context with solver_type %d", p.solver_type);
+      return SOPHIA_E_STATE;
+    }
+    if (nd > 0 || p.dim != 3 || p.rho_type != 1) {
+      set_error("WCSPH: supported set is 3D fluid only with continuity density (spheres %d, dim %d, rho_type %d)",
+                nd, p.dim, p.rho_type);
+      return SOPHIA_E_ARG;
+    }
+    if (ns == 0) return 0;
+    DevParams P = devparams();
+    CellView sc{scs, sce};
+    switch (stage) {
+      case 0:
+        if (count == 0) {
+          // NNPS once (Eulerian); P3 starts at zero, so the predictor may
+          // follow the sort: it is per particle
+          TRY(nnps());
+          if (p.time_type == 1) TRY(wc_predictor<R>(stream, P, sv));
+        } else {
+          if (p.time_type == 1) TRY(wc_predictor<R>(stream, P, sv));
+          TRY(wc_reset_p3<R>(stream, sv));
+        }
+        after_corrector = false;
+        break;
+      case 1:
+        TRY(wc_volume_mass<R>(stream, P, sv, sc, (int)std::min<int64_t>(count, INT32_MAX)));
+        break;
+      case 2:
+        if (p.kgc_solve > 0 || p.delSPH_solve == 2) {
+          TimeScope t(tm(), "kgc");
+          TRY(sph_kgc<R>(stream, P, sv));
+        }
+        {
+          TimeScope t(tm(), "porosity_sph");
+          CellView dc{dcs, dce};
+          TRY(sph_porosity<R>(stream, P, sv, dv, dc));
+        }
+        {
+          TimeScope t(tm(), "prep");
+          TRY(sph_prep<R>(stream, P, sv));
+        }
+        break;
+      case 3: {
+        TimeScope t(tm(), "continuity");
+        TRY(wc_continuity<R>(stream, P, sv));
+        break;
+      }
+      case 4:
+        TRY(wc_eos<R>(stream, P, sv));
+        break;
+      case 5: {
+        TimeScope t(tm(), "neumann");
+        TRY(sph_neumann<R>(stream, P, sv, sc));
+        break;
+      }
+      case 6: {
+        TimeScope t(tm(), "wc_interaction");
+        TRY(wc_interaction<R>(stream, P, sv));
+        break;
+      }
+      case 7:
+        TRY(wc_corrector<R>(stream, P, sv));
+        after_corrector = true;
+        break;
+      case 8:
+        if (p.xsph_solve) {
+          TimeScope t(tm(), "xsph");
+          TRY(sph_xsph_only<R>(stream, P, sv));
+        }
+        time += p.dt;
+        count += 1;
+        break;
+      default:
+        set_error("WCSPH stage %d out of range 0..8", stage);
+        return SOPHIA_E_ARG;
     }
     return 0;
   }
@@ -1447,8 +1546,18 @@ int sophia_time(sophia_ctx* c, int32_t phase) {
 int sophia_ppe(sophia_ctx* c, const sophia_ppe_opts* o, sophia_ppe_stats* st) { CHECK_CTX(c); return c->ppe(o, st); }
 int sophia_xsph(sophia_ctx* c) { CHECK_CTX(c); return c->xsph(); }
 
+int sophia_wcsph_stage(sophia_ctx* c, int32_t stage) {
+  CHECK_CTX(c);
+  return c->wcsph_stage(stage);
+}
+
 int sophia_step(sophia_ctx* c, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
   CHECK_CTX(c);
+  if (c->p.solver_type == 0) {
+    if (st) std::memset(st, 0, sizeof(*st));
+    for (int k = 0; k <= 8; k++) TRY(c->wcsph_stage(k));
+    return 0;
+  }
   TRY(c->nnps());
   TRY(c->prep());
   TRY(c->interaction());

@@ -9,59 +9,11 @@
 //
 // Each kernel cites the reference kernel it computes.
 #include "common.hpp"
+#include "nbr.hpp"
 
 namespace sophia {
 
 namespace {
-
-// Fields of one neighbour, gathered before any of them is used.
-template <typename R>
-struct Nb {
-  double x, y, z, m, rho, pres, h;
-  R ux, uy, uz, temp, rref;
-  int pt;
-};
-template <typename R>
-__device__ inline Nb<R> load_nb(const SphView<R>& v, int j) {
-  Nb<R> q;
-  q.x = v.x[j]; q.y = v.y[j]; q.z = v.z[j];
-  q.m = v.m[j]; q.rho = v.rho[j]; q.pres = v.pres[j]; q.h = v.h[j];
-  q.ux = v.ux[j]; q.uy = v.uy[j]; q.uz = v.uz[j]; q.temp = v.temp[j]; q.rref = v.rho_ref[j];
-  q.pt = v.ptype[j];
-  return q;
-}
-
-// The list in batches of four, in list order, in two phases: the fields of
-// all four neighbours are gathered (straight-line code, so every load of the
-// batch is in flight together), then f(q, on) runs per neighbour; the next
-// batch's indices are loaded before this batch is consumed.  Loads of fields
-// a body never reads are dropped by the compiler.  Slots past the count
-// repeat the last entry with on = false; bodies add `on ? term : 0` (a
-// select: exact, and NaN-safe for the unused slots).
-template <typename R, class F>
-__device__ inline void for_nbrs4(const SphView<R>& v, int i, bool eps_rule, F&& f) {
-  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
-  if (cnt <= 0) return;
-  const size_t n = (size_t)v.n;
-  uint32_t en[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(u, cnt - 1) * n + i];
-  for (int k = 0; k < cnt; k += 4) {
-    uint32_t e[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) e[u] = en[u];
-    Nb<R> q[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) q[u] = load_nb(v, (int)(e[u] & ~kNbrFlag));
-#pragma unroll
-    for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(k + 4 + u, cnt - 1) * n + i];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const bool on = (k + u < cnt) && !(eps_rule && (e[u] & kNbrFlag));
-      f(q[u], on);
-    }
-  }
-}
 
 // KERNEL_clc_mass_init_sph (functions_MASS.cuh:24-43): rho0 <- rho
 template <typename R>
@@ -178,9 +130,9 @@ __global__ void k_prep_prep(DevParams P, SphView<R> v) {
   if (t == 9 && v.z[i] < 0.05) rt = P.rho_fluid;
   double m_ref = rt * hv * hv * hv * (double)v.por[i];
   double V = P.dim == 2 ? hv * hv : hv * hv * hv;
-  v.rho_ref[i] = (R)(m_ref / V);
+  v.rho_ref[i] = m_ref / V;
   if (P.rho_type == 1) {
-    v.rho0[i] = (R)(m_ref / (hv * hv * hv));
+    v.rho0[i] = m_ref / (hv * hv * hv);
     v.rho[i] = m_ref / (hv * hv * hv);
     v.m[i] = m_ref;
   }
@@ -597,6 +549,13 @@ template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const S
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_predictor, P, v); }
 template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_corrector, P, v); }
+template <typename R> int sph_xsph_only(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n > 0) {
+    k_xsph<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
 template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
     k_xsph<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
@@ -617,7 +576,8 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_pressure_force<R>(hipStream_t, const DevParams&, const SphView<R>&);                     \
   template int sph_predictor<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_corrector<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
-  template int sph_xsph<R>(hipStream_t, const DevParams&, const SphView<R>&);
+  template int sph_xsph<R>(hipStream_t, const DevParams&, const SphView<R>&);                                \
+  template int sph_xsph_only<R>(hipStream_t, const DevParams&, const SphView<R>&);
 INST(float)
 INST(double)
 

@@ -170,6 +170,10 @@ class Sophia:
         check(self.lib.sophia_ppe_apply(self._ctx, ptr(x), ptr(y), len(x)), "sophia_ppe_apply")
         return y
 
+    def wcsph_stage(self, stage: int):
+        """One stage (0-8) of the WCSPH step (solver_type 0)."""
+        check(self.lib.sophia_wcsph_stage(self._ctx, stage), "sophia_wcsph_stage")
+
     def monitors(self, species: int = SPH) -> np.ndarray:
         """max |u|, max rho, max ftotal of one species."""
         out = np.zeros(3)

@@ -1,11 +1,11 @@
-"""Train the learned PPE warm start on C2's particle set and report the
-iteration counts of every guess.
+"""Train the learned PPE warm start and report the iteration counts of every
+guess.
 
-    python -m fluidized_bed_master_amd.train_ppe_mlp [--snapshots 400] [--out weights/ppe_mlp_c2.npy]
+    python -m fluidized_bed_master_amd.train_ppe_mlp [--snapshots 400] [--out weights/ppe_mlp.npy]
 
-C4 workload: C2's particles (50 x 50 x 150 fluid at dx = 1 mm, 8000 spheres
-d = 3 mm) through the ISPH step with a converged PPE (difference-form
-pressure gradient).  Every training snapshot is one time step whose mixed
+Workload: a 60 x 60 x 60 column (216,000 fluid at dx = 1 mm) with spheres of
+the benchmark's size and packing (d = 2 mm, phi = 0.35), through the ISPH
+step with a converged PPE (difference-form pressure gradient).  Every training snapshot is one time step whose mixed
 BiCGStab solve to rtol 1e-8 is followed by one Adam step of the network on
 (features before the solve, converged pressure), all on the device.  The
 evaluation then runs the same initial state with the zero, previous-step,
@@ -30,11 +30,11 @@ def main(argv=None) -> int:
     ap.add_argument("--eval-steps", type=int, default=20)
     ap.add_argument("--eval-warm", type=int, default=20)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--out", default=str(learned.WEIGHTS_DIR / "ppe_mlp_c2.npy"))
+    ap.add_argument("--out", default=str(learned.WEIGHTS_DIR / "ppe_mlp.npy"))
     a = ap.parse_args(argv)
 
     from .sophia import Sophia
-    params, parts = configs.c2_isph()
+    params, parts = configs.c4_train()
     params.pgrad_form = 1
 
     def make():
@@ -50,7 +50,7 @@ def main(argv=None) -> int:
     ev = learned.evaluate(make, w, steps=a.eval_steps, warm=a.eval_warm)
     losses = np.array([h["loss"] for h in hist])
     rep = {
-        "workload": "C4: C2 particles (375,000 fluid + walls + 8000 spheres), ISPH, converged mixed PPE rtol 1e-8",
+        "workload": f"60x60x60 column ({len(parts)} particles, 2 mm spheres phi 0.35), ISPH, converged mixed PPE rtol 1e-8",
         "snapshots": a.snapshots,
         "train_seconds": secs,
         "loss_first20": float(losses[:20].mean()),

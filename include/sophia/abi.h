@@ -251,6 +251,18 @@ int sophia_get_nnps(const sophia_ctx* ctx, int32_t species, int32_t* keys,
 /* export the state in sorted order as reference records (any may be NULL) */
 int sophia_get_particles(const sophia_ctx* ctx, int32_t species, part1* p1,
                          part2* p2, part3* p3);
+/* One stage of the WCSPH step (solver_type 0; SOPHIA_single, WCSPH_Calc.cuh:
+ * 4-340), fluid only, 3D, continuity density: 0 set_alpha + predictor
+ * (KERNEL_clc_predictor, functions_TIME.cuh:77-292) + NNPS at step 0 / P3
+ * reset later, 1 KERNEL_clc_volume_update3D + KERNEL_clc_mass_update
+ * (functions_MASS.cuh:148-222, 45-60), 2 KGC + KERNEL_clc_prep3D_coupling +
+ * KERNEL_clc_prep3D, 3 KERNEL_clc_continuity_norm3D (functions_MASS.cuh:
+ * 771-914), 4 KERNEL_EOS (functions_PROP.cuh:7-28), 5
+ * KERNEL_Neumann_boundary3D (functions_BC.cuh:2-137), 6 KERNEL_interaction3D
+ * (functions_INTERACTION.cuh:413-832), 7 KERNEL_time_update_single
+ * (functions_TIME.cuh:676-877), 8 KERNEL_xsph3D + clock.  sophia_step runs
+ * stages 0-8 when solver_type is 0. */
+int sophia_wcsph_stage(sophia_ctx* ctx, int32_t stage);
 /* run monitors of one species (kernel_copy_max_sph/_dem + cub::DeviceReduce::Max,
  * functions_OUTPUT.cuh:35-65, ISPH_Calc_seperate.cuh:928-969): out[0] = max |u|,
  * out[1] = max rho, out[2] = max P3.ftotal, over particles with p_type != 0 and

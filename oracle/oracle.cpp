@@ -280,6 +280,24 @@ static void for_candidates(const orc_state& s, const Species& sp, double x, doub
       }
 }
 
+/* the (2 rng + 1)^3 cells around (x,y,z), same order (KERNEL_clc_volume_update3D
+ * scans +-3 cells, functions_MASS.cuh:190-192) */
+template <class F>
+static void for_candidates_r(const orc_state& s, const Species& sp, double x, double y, double z, int rng, F&& f) {
+  int ic, jc, kc;
+  cell_of(s, x, y, z, ic, jc, kc);
+  for (int dz = -rng; dz <= rng; dz++)
+    for (int dy = -rng; dy <= rng; dy++)
+      for (int dx = -rng; dx <= rng; dx++) {
+        int a = ic + dx, b = jc + dy, c = kc + dz;
+        if (a < 0 || a > s.d.NI - 1 || b < 0 || b > s.d.NJ - 1 || c < 0 || c > s.d.NK - 1) continue;
+        int k = idx_cell(s, a, b, c);
+        if (sp.g_str[k] == kEmpty) continue;
+        uint32_t e = sp.g_end[k];
+        for (uint32_t j = sp.g_str[k]; j < e; j++) f((int)j);
+      }
+}
+
 static inline double dist_of(double dx, double dy, double dz) {
   return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
@@ -1485,9 +1503,18 @@ int orc_ppe(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
 /* XSPH: KERNEL_xsph3D_sph + KERNEL_xsph3D_calc_sph (functions_TIME.cuh:    */
 /* 1735-1850, 1985-1997), on the carried state P1                           */
 /* ------------------------------------------------------------------------ */
+static void xsph_sph(orc_state* s, bool apply);
 void orc_xsph(orc_state* s) {
   int stride = std::max(1, s->p.decouple_stride);
   if (!s->p.xsph_solve || (s->count % stride) != 0 || s->p.dim != 3) return;
+  xsph_sph(s, true);
+}
+
+/* KERNEL_xsph3D_sph computes XSPH_u/XSPH_temp; with apply the calc kernel
+ * (KERNEL_xsph3D_calc_sph) copies them into u/temp.  The WCSPH driver calls
+ * only KERNEL_xsph3D (WCSPH_Calc.cuh:313-314), which is the same text. */
+static void xsph_sph(orc_state* s, bool apply) {
+  int stride = std::max(1, s->p.decouple_stride);
   Species& sp = s->sph;
   const sophia_params& p = s->p;
   const int kt = p.kernel_type, dim = p.dim;
@@ -1526,6 +1553,7 @@ void orc_xsph(orc_state* s) {
     if (!act[i]) continue;
     part1& q = sp.P1[i];
     q.XSPH_ux = vx[i]; q.XSPH_uy = vy[i]; q.XSPH_uz = vz[i]; q.XSPH_temp = vt[i];
+    if (!apply) continue;
     q.ux = vx[i]; q.uy = vy[i]; q.uz = vz[i]; q.temp = vt[i];
     if (p.fix_xsph_position && q.p_type > 0 && !p.eulerian_sph) {
       const part2& q2 = sp.SP2[i];
@@ -1536,10 +1564,412 @@ void orc_xsph(orc_state* s) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* WCSPH step (SOPHIA_single, WCSPH_Calc.cuh:4-340): fluid only, 3D.        */
+/* KERNEL_set_alpha makes every SPH particle Eulerian (functions_ALE.cuh:   */
+/* 2-20), so NNPS runs at count 0 only (WCSPH_Calc.cuh:34); later steps     */
+/* copy P1 -> SP1, P2 -> SP2.  P3 is reset after the predictor, which       */
+/* therefore uses the previous step's ftotal, drho and dtemp.               */
+/* ------------------------------------------------------------------------ */
+constexpr int kMoving = 9;  // MOVING (Parameters.cuh:12)
+
+/* KERNEL_set_alpha (functions_ALE.cuh:2-20) */
+static void wc_set_alpha(Species& sp, std::vector<part1>& P) {
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = P[i];
+    if (q.i_type == 3) continue;
+    double e = q.p_type <= 1000 ? 0.0 : 1.0;
+    q.elix = q.eliy = q.eliz = e;
+  }
+}
+
+/* KERNEL_clc_predictor (functions_TIME.cuh:77-292) on P1/P2: half step of
+ * x (Lagrangian only), u, rho (continuity) and T with the previous forces */
+static void wc_predictor(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const double dt = p.dt;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.P1[i];
+    part2& q2 = sp.P2[i];
+    const part3& o = sp.P3[i];
+    if (q.i_type == 3) continue;
+    const int t = q.p_type;
+    const double bz = (q.buffer_type == 0);
+    if (t != kMoving) {
+      double x0 = q.x, y0 = q.y, z0 = q.z;
+      double ux0 = q.ux, uy0 = q.uy, uz0 = q.uz;
+      double xp, yp, zp, uxp, uyp, uzp;
+      if (t >= 0) {
+        xp = x0 + ux0 * (dt * 0.5) * (t > 0) * q.elix;
+        yp = y0 + uy0 * (dt * 0.5) * (t > 0) * q.eliy;
+        zp = z0 + uz0 * (dt * 0.5) * (t > 0) * q.eliz;
+        uxp = ux0 + o.ftotalx * bz * (dt * 0.5);
+        uyp = uy0 + o.ftotaly * bz * (dt * 0.5);
+        uzp = uz0 + o.ftotalz * bz * (dt * 0.5);
+      } else {
+        xp = x0; yp = y0; zp = z0;
+        uxp = q.ux; uyp = q.uy; uzp = q.uz;
+      }
+      q.x = xp; q.y = yp; q.z = zp;
+      q.ux = uxp; q.uy = uyp; q.uz = uzp;
+      q2.x0 = x0; q2.y0 = y0; q2.z0 = z0;
+      q2.ux0 = ux0; q2.uy0 = uy0; q2.uz0 = uz0;
+    }
+    if (s.time == 0) q2.rho_ref = q.rho;
+    if (p.rho_type == 1 && t > 0) {
+      double trho = q.rho;
+      if (s.time < 1.0e-10) q.vol0 = q.m / trho;
+      q2.rho0 = trho;
+      q.rho = trho + o.drho * (dt * 0.5);
+    } else {
+      double trho = q.rho;
+      double hv = q.h / p.h_over_dx;
+      if (s.time < 1.0e-10) q.vol0 = hv * hv * hv;
+      q2.rho0 = trho;
+    }
+    if (p.con_solve == 1) {
+      q2.temp0 = q.temp;
+      q.temp = q.temp + o.dtemp * (dt * 0.5);
+    }
+  }
+}
+
+/* KERNEL_clc_volume_update3D (functions_MASS.cuh:148-222): Shepard-corrected
+ * volume over the +-3 cells; Eulerian particles only at count 0 */
+static void wc_volume_update(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+  std::vector<double> vol(sp.n());
+  std::vector<char> act(sp.n(), 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt) continue;
+    if (qi.elix < 1e-8 && qi.eliy < 1e-8 && s.count > 0) continue;
+    if (qi.p_type >= 1000) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double tR = 0, filt = 0;
+    for_candidates_r(s, sp, qi.x, qi.y, qi.z, 3, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type < 1000) || qj.i_type == 4) return;
+      double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z) + 1e-20;
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r);
+      tR += w;
+      filt += qj.m / qj.rho / qj.vol0 * w;
+    });
+    vol[i] = qi.vol0 * 1.0 / tR * filt;
+    act[i] = 1;
+  }
+  for (int i = 0; i < sp.n(); i++)
+    if (act[i]) sp.SP1[i].vol = vol[i];
+}
+
+/* KERNEL_clc_mass_update (functions_MASS.cuh:45-60): m = rho vol, Eulerian */
+static void wc_mass_update(orc_state& s) {
+  Species& sp = s.sph;
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    if (q.p_type > 1000 || q.i_type > kITypeCrt) continue;
+    if (q.elix == 1.0 && q.eliy == 1.0) continue;
+    q.m = q.rho * q.vol;
+  }
+}
+
+/* KERNEL_clc_continuity_norm3D (functions_MASS.cuh:771-914) -> P3.drho */
+static void wc_continuity(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    if (p.open_boundary > 0 && qi.buffer_type > 0) continue;
+    if (qi.p_type <= 0) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
+    double rhoi = qi.rho, pori = qi.DEMpor;
+    double gxi = qi.grad_rhox, gyi = qi.grad_rhoy, gzi = qi.grad_rhoz;
+    double tx = 0, ty = 0, tz = 0, tR = 0, px = 0, py = 0, pz = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type <= 1000)) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, h, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      if (p.kgc_solve > 0) kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
+      double mj = qj.m, porj = qj.DEMpor, rhoj = qj.rho;
+      double tmprho = mj * (rhoi / rhoj);
+      tx += (uxi - qj.ux) * tmprho * gx;
+      ty += (uyi - qj.uy) * tmprho * gy;
+      tz += (uzi - qj.uz) * tmprho * gz;
+      px += uxi * mj / rhoj * (rhoi / pori - rhoj / porj) * gx;
+      py += uyi * mj / rhoj * (rhoi / pori - rhoj / porj) * gy;
+      pz += uzi * mj / rhoj * (rhoi / pori - rhoj / porj) * gz;
+      if (p.delSPH_solve > 0) {
+        double rri = sp.SP2[i].rho_ref, rrj = sp.SP2[j].rho_ref;
+        double phi = (gxi + qj.grad_rhox) * (qj.x - xi);
+        phi += (gyi + qj.grad_rhoy) * (qj.y - yi);
+        phi += (gzi + qj.grad_rhoz) * (qj.z - zi);
+        phi = (-0.5 * phi) * (p.delSPH_solve == 2) + (rhoj / rrj - rhoi / rri);
+        double tr = -2 * p.delta_sph * h * rri * p.soundspeed * (mj / rhoj);
+        tr *= phi * dw / r;
+        tR += tr;
+      }
+    });
+    sp.P3[i].drho = (tx + ty + tz) + pori * (px + py + pz) * (1 - qi.elix) + tR * (p.delSPH_solve > 0);
+  }
+}
+
+/* KERNEL_EOS (functions_PROP.cuh:7-28): Tait, B = 1.186 c0^2 / gamma */
+static void wc_eos(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const double B = p.soundspeed * p.soundspeed * (1.186) / p.gamma;
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    if (q.i_type > kITypeCrt || q.p_type > 1000 || q.buffer_type != 0) continue;
+    q.pres = B * (std::pow(q.rho / sp.SP2[i].rho_ref, p.gamma) - 1.0);
+  }
+}
+
+/* KERNEL_interaction3D (functions_INTERACTION.cuh:413-832), i_type 1:
+ * pressure, laminar and artificial viscosity, gravity, conduction and the
+ * Eulerian advection term (interface, repulsive, surface-tension,
+ * Boussinesq and concentration terms off) */
+static void wc_interaction(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& qi = sp.SP1[i];
+    if (qi.i_type != 1 || qi.p_type > 1000) continue;
+    const int pti = qi.p_type;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
+    double pi = qi.pres, mi = qi.m, rhoi = qi.rho, pori = qi.DEMpor, tempi = qi.temp;
+    double eta = 0.001 * h, kci = 0, cpi = 1;
+    if (p.con_solve) {
+      kci = conductivity(p, pti);
+      cpi = heat_capacity(p, pti);
+    }
+    double visi = viscosity(p) + sp.P3[i].vis_t;
+    double tx = 0, ty = 0, tz = 0, Rc = 0, pgx = 0, pgy = 0, pgz = 0;
+    double ex = 0, ey = 0, ez = 0, et = 0;
+    const bool euler = qi.elix < 1.0 || qi.eliy < 1.0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type <= 1000)) return;
+      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double r = dist_of(dx, dy, dz) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, h, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      if (p.kgc_solve > 0) kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
+      const int ptj = qj.p_type;
+      double uxj = qj.ux, uyj = qj.uy, uzj = qj.uz, mj = qj.m, tempj = qj.temp, rhoj = qj.rho;
+      double pj = qj.pres, hj = qj.h;
+      if (p.fp_solve) {
+        double pp = p.pgrad_form ? pj - pi : pi + pj;
+        double Cp = -pori * (mj) * pp / (rhoi * rhoj);
+        tx += Cp * gx;
+        ty += Cp * gy;
+        tz += Cp * gz;
+        pgx += rhoi / pori * Cp * gx;
+        pgy += rhoi / pori * Cp * gy;
+        pgz += rhoi / pori * Cp * gz;
+      }
+      if (p.fv_solve) {
+        double visj = viscosity(p) + sp.P3[j].vis_t;
+        double Cv = dx * gx + dy * gy + dz * gz;
+        Cv *= (visi * visj) / (visi + visj);
+        Cv *= 4 * pori * (mj / (rhoi * rhoj));
+        Cv /= r;
+        Cv /= r;
+        if (pti <= 0) Cv = 0.0;
+        tx += Cv * (uxi - uxj);
+        ty += Cv * (uyi - uyj);
+        tz += Cv * (uzi - uzj);
+      }
+      if (p.fva_solve) {
+        double uij_xij = (uxi - uxj) * dx + (uyi - uyj) * dy + (uzi - uzj) * dz;
+        if (uij_xij < 0) {
+          double hij = (h + hj) * 0.5;
+          double phi = hij * uij_xij;
+          phi /= (r * r + 0.01 * hij * hij);
+          double Pij = mi * phi * (-0.1 * p.soundspeed + 0.02 * phi);
+          Pij /= (rhoi + rhoj);
+          Pij *= 0.5;
+          tx += -(Pij)*gx;
+          ty += -(Pij)*gy;
+          tz += -(Pij)*gz;
+        }
+      }
+      if (p.con_solve) {
+        double kcj = conductivity(p, ptj);
+        double H = 4.0 * mj * kcj * kci * (tempi - tempj) * dw;
+        H /= cpi * (r + eta * eta) * rhoi * rhoj * (kci + kcj);
+        Rc += H;
+      }
+      if (euler) {
+        ex += (uxi * (uxj - uxi) * gx + uyi * (uxj - uxi) * gy + uzi * (uxj - uxi) * gz) * mj / rhoj * (1 - qi.elix);
+        ey += (uxi * (uyj - uyi) * gx + uyi * (uyj - uyi) * gy + uzi * (uyj - uyi) * gz) * mj / rhoj * (1 - qi.eliy);
+        ez += (uxi * (uzj - uzi) * gx + uyi * (uzj - uzi) * gy + uzi * (uzj - uzi) * gz) * mj / rhoj * (1 - qi.eliz);
+        if (p.con_solve)
+          et += (uxi * (tempj - tempi) * gx + uyi * (tempj - tempi) * gy + uzi * (tempj - tempi) * gz) * mj / rhoj *
+                (1 - qi.elix);
+      }
+    });
+    if (p.fg_solve) tz += -p.gravity;
+    part3& o = sp.P3[i];
+    o.fsx = o.fsy = o.fsz = 0;
+    o.ftotalx = tx - ex;
+    o.ftotaly = ty - ey;
+    o.ftotalz = tz - ez;
+    qi.pgf_x = pgx;
+    qi.pgf_y = pgy;
+    qi.pgf_z = pgz;
+    o.ftotal = std::sqrt(tx * tx + ty * ty + tz * tz);
+    if (p.con_solve) o.dtemp = Rc * (pti != -3) - et;
+  }
+}
+
+/* KERNEL_time_update_single (functions_TIME.cuh:676-877): SP1 -> P1.  The
+ * temperature branch reads `k_con_solve==1 * p_type_i>0`, which C parses as
+ * k_con_solve == (p_type_i > 0): with conduction off, walls take
+ * temp0 + dtemp dt and fluid keeps its temperature. */
+static void wc_corrector(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const double dt = p.dt;
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& q = sp.SP1[i];
+    const part2& q2 = sp.SP2[i];
+    const part3& o = sp.P3[i];
+    part1& T = sp.P1[i];
+    if (q.i_type == 3) continue;
+    const int t = q.p_type;
+    const double bz = (q.buffer_type == 0);
+    T = q;  // every other field is copied from SP1
+    if (t == kMoving) {
+      T.x = q2.x0 + q.ux * dt;
+      T.y = q2.y0 + q.uy * dt;
+      T.z = q2.z0 + q.uz * dt;
+    } else {
+      double ux0 = q2.ux0, uy0 = q2.uy0, uz0 = q2.uz0;
+      double ax = 0, ay = 0, az = 0;
+      if (t > 0) {
+        ax = o.ftotalx * bz;
+        ay = o.ftotaly * bz;
+        az = o.ftotalz * bz;
+      }
+      double uxc = ux0 + ax * dt, uyc = uy0 + ay * dt, uzc = uz0 + az * dt;
+      if (uxc * uxc + uyc * uyc + uzc * uzc >= p.u_limit * p.u_limit) {
+        uxc = ux0;
+        uyc = uy0;
+        uzc = uz0;
+      }
+      double xc = q2.x0 + uxc * dt * (t > 0) * q.elix;
+      double yc = q2.y0 + uyc * dt * (t > 0) * q.eliy;
+      double zc = q2.z0 + uzc * dt * (t > 0) * q.eliz;
+      if (!p.xsph_solve) {
+        T.x = xc;
+        T.y = yc;
+        T.z = zc;
+      } else {
+        T.x = sp.P1[i].x;
+        T.y = sp.P1[i].y;
+        T.z = sp.P1[i].z;
+      }
+      T.ux = uxc;
+      T.uy = uyc;
+      T.uz = uzc;
+    }
+    if (p.rho_type == 1 && t > 0) T.rho = q2.rho0 + o.drho * dt;
+    else T.rho = q.rho;
+    if (p.con_solve == (t > 0 ? 1 : 0)) T.temp = q2.temp0 + o.dtemp * dt;
+    else T.temp = q.temp;
+    sp.P2[i].rho_ref = q2.rho_ref;
+  }
+}
+
+static int wc_check(const orc_state& s) {
+  if (s.dem.n() > 0 || s.p.dim != 3 || s.p.rho_type != 1) return -1;
+  return 0;
+}
+
+/* the WCSPH stages in driver order; stage 0 also does the NNPS (count 0) or
+ * the P1 -> SP1 copy (WCSPH_Calc.cuh:34-69) */
+int orc_wcsph_stage(orc_state* s, int32_t stage) {
+  if (wc_check(*s)) return -1;
+  Species& sp = s->sph;
+  switch (stage) {
+    case 0:
+      wc_set_alpha(sp, sp.P1);
+      if (s->p.time_type == 1) wc_predictor(*s);
+      if (s->count == 0) {
+        nnps_species(*s, sp, true);
+        wc_set_alpha(sp, sp.SP1);
+        wc_set_alpha(sp, sp.P1);
+      } else {
+        std::fill(sp.P3.begin(), sp.P3.end(), part3{});
+        sp.SP1 = sp.P1;
+        sp.SP2 = sp.P2;
+      }
+      s->after_corrector = false;
+      break;
+    case 1:
+      wc_volume_update(*s);
+      wc_mass_update(*s);
+      break;
+    case 2:
+      if (s->p.kgc_solve > 0 || s->p.delSPH_solve == 2) kgc_3d(*s);
+      coupling_prep_sph(*s);
+      prep3d_sph(*s);
+      break;
+    case 3:
+      wc_continuity(*s);
+      break;
+    case 4:
+      wc_eos(*s);
+      break;
+    case 5:
+      neumann_walls(*s);
+      break;
+    case 6:
+      wc_interaction(*s);
+      break;
+    case 7:
+      wc_corrector(*s);
+      s->after_corrector = true;
+      break;
+    case 8:
+      if (s->p.xsph_solve) xsph_sph(s, false);
+      s->time += s->p.dt;
+      s->count += 1;
+      break;
+    default:
+      return -1;
+  }
+  return 0;
+}
+
 /* one ISPH-DEM time step in the order of SOPHIA_single_ISPH
  * (ISPH_Calc_seperate.cuh:13-503); open-boundary and DEM wall kernels are
  * outside this build's scope */
 int orc_step(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
+  if (s->p.solver_type == 0) {
+    for (int k = 0; k <= 8; k++)
+      if (orc_wcsph_stage(s, k)) return -1;
+    return 0;
+  }
   orc_nnps(s);
   orc_prep(s);
   orc_interaction(s);

@@ -37,6 +37,12 @@ void orc_time(orc_state* s, int32_t phase);
 int orc_ppe(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st);
 void orc_xsph(orc_state* s);
 int orc_step(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st);
+/* WCSPH (solver_type 0) stages of SOPHIA_single (WCSPH_Calc.cuh:4-340):
+ * 0 set_alpha + predictor + NNPS (count 0) or P1 -> SP1 copy, 1 volume and
+ * mass update, 2 KGC + porosity + prep, 3 continuity, 4 EOS, 5 wall
+ * Neumann, 6 interaction, 7 corrector, 8 XSPH + clock.  -1 when the state
+ * is outside the supported set (DEM present, 2D, mass summation). */
+int orc_wcsph_stage(orc_state* s, int32_t stage);
 
 /* inspection */
 void orc_get_nnps(const orc_state* s, int32_t species, int32_t* keys, int32_t* order,

@@ -37,6 +37,7 @@ _SIG = {
     "orc_step": (C.c_int, [_VP, _P(PpeOpts), _P(PpeStats)]),
     "orc_get_nnps": (None, [_VP, C.c_int32, _VP, _VP, _VP, _VP, _VP]),
     "orc_get_particles": (None, [_VP, C.c_int32, _VP, _VP, _VP]),
+    "orc_wcsph_stage": (C.c_int, [_VP, C.c_int32]),
     "orc_set_pressure": (None, [_VP, _VP, C.c_int32]),
     "orc_get_ppe_system": (None, [_VP, _VP, _VP, _VP]),
     "orc_ppe_apply": (None, [_VP, _VP, _VP]),
@@ -144,8 +145,14 @@ class Oracle:
 
     def step(self, opts: PpeOpts | None = None) -> dict:
         st = PpeStats()
-        self.lib.orc_step(self._s, C.byref(opts) if opts is not None else None, C.byref(st))
+        rc = self.lib.orc_step(self._s, C.byref(opts) if opts is not None else None, C.byref(st))
+        if rc:
+            raise RuntimeError("oracle step: configuration outside the supported set")
         return st.as_dict()
+
+    def wcsph_stage(self, stage: int):
+        if self.lib.orc_wcsph_stage(self._s, stage):
+            raise RuntimeError("oracle WCSPH: configuration outside the supported set")
 
     def get_nnps(self, species: int = SPH, counts: bool = True) -> dict:
         n = self.count(species)

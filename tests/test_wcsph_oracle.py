@@ -1,0 +1,126 @@
+"""CPU pins of the oracle's WCSPH step (SOPHIA_single, WCSPH_Calc.cuh:4-340).
+
+Each check is fixed by mathematics rather than by the formula text:
+continuity of a uniform flow is zero, continuity of a linear velocity field
+is -rho div u (KGC makes SPH gradients exact for linear fields), a column
+under gravity alone accelerates by exactly g dt per step through the
+predictor-corrector pair, the Tait EOS vanishes at the reference density,
+and the Shepard-corrected volume is the initial volume when m / rho equals
+it everywhere.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fluidized_bed_master_amd import abi, configs
+from oracle.oracle import Oracle
+
+
+def _interior(p1, dx, n, nz, margin=3.0):
+    lo, hi = margin * dx, (n - margin) * dx
+    return (p1["p_type"] == 1) & (p1["x"] > lo) & (p1["x"] < hi) & (p1["y"] > lo) & (p1["y"] < hi) & \
+        (p1["z"] > lo) & (p1["z"] < (nz - margin) * dx)
+
+
+def _run_stages(o, upto):
+    for k in range(upto + 1):
+        o.wcsph_stage(k)
+
+
+def test_uniform_flow_has_zero_density_rate():
+    n, nz, dx = 10, 12, 1.0e-3
+    p, parts = configs.wcsph_column(n, n, nz, dx, jitter=0.0)
+    parts = parts.copy()
+    parts["ux"] = 0.3
+    parts["uz"] = -0.1
+    parts["pres"] = 0.0
+    o = Oracle(p, parts)
+    _run_stages(o, 3)
+    p1, _, p3 = o.get_particles(abi.SPH)
+    m = _interior(p1, dx, n, nz)
+    assert m.sum() > 20
+    scale = 1000.0 * 0.3 / (1.3 * dx)
+    assert np.max(np.abs(p3["drho"][m])) < 1e-9 * scale
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.1])
+def test_linear_velocity_gives_minus_rho_divergence(jitter):
+    n, nz, dx = 10, 12, 1.0e-3
+    p, parts = configs.wcsph_column(n, n, nz, dx, jitter=jitter, delsph=0, kgc=1)
+    parts = parts.copy()
+    a, b, c = 3.0, -1.0, 0.5
+    parts["ux"] = a * (parts["x"] - 0.005)
+    parts["uy"] = b * (parts["y"] - 0.005)
+    parts["uz"] = c * (parts["z"] - 0.006)
+    parts["pres"] = 0.0
+    o = Oracle(p, parts)
+    _run_stages(o, 3)
+    p1, _, p3 = o.get_particles(abi.SPH)
+    m = _interior(p1, dx, n, nz)
+    want = -p1["rho"][m] * (a + b + c)
+    err = np.max(np.abs(p3["drho"][m] - want)) / np.max(np.abs(want))
+    assert err < 1e-8, err
+
+
+def test_gravity_only_midpoint_kinematics():
+    n, nz, dx = 10, 12, 1.0e-3
+    p, parts = configs.wcsph_column(n, n, nz, dx, jitter=0.0, delsph=0)
+    p.fp_solve = 0
+    p.fv_solve = 0
+    p.fva_solve = 0
+    p.xsph_solve = 0
+    parts = parts.copy()
+    parts["pres"] = 0.0
+    o = Oracle(p, parts)
+    steps = 3
+    for _ in range(steps):
+        o.step()
+    p1, _, _ = o.get_particles(abi.SPH)
+    m = _interior(p1, dx, n, nz, margin=4.0)
+    assert m.sum() > 0
+    want = -p.gravity * p.dt * steps
+    assert np.allclose(p1["uz"][m], want, rtol=1e-12, atol=0), (p1["uz"][m].min(), p1["uz"][m].max(), want)
+    assert np.allclose(p1["ux"][m], 0.0, atol=1e-15)
+    walls = p1["p_type"] == 0
+    assert np.all(p1["uz"][walls] == 0.0)
+
+
+def test_eos_zero_at_reference_and_monotone():
+    n, nz, dx = 8, 8, 1.0e-3
+    p, parts = configs.wcsph_column(n, n, nz, dx, jitter=0.0, delsph=0)
+    parts = parts.copy()
+    parts["pres"] = 0.0
+    o = Oracle(p, parts)
+    _run_stages(o, 4)
+    p1, p2, _ = o.get_particles(abi.SPH)
+    fl = p1["p_type"] == 1
+    # at step 0 rho_ref is the start-of-step density; the predictor moved rho by
+    # drho dt / 2 = 0 (no previous rate), so every pressure is zero
+    assert np.all(p2["rho_ref"][fl] == 1000.0)
+    assert np.max(np.abs(p1["pres"][fl])) == 0.0
+    # Tait: compression raises p, B = 1.186 c0^2 / gamma
+    B = 1.186 * p.soundspeed ** 2 / p.gamma
+    r = np.array([0.99, 1.0, 1.01])
+    pr = B * (r ** p.gamma - 1.0)
+    assert pr[0] < 0 < pr[2] and pr[1] == 0.0
+
+
+def test_volume_update_keeps_initial_volume():
+    n, nz, dx = 8, 10, 1.0e-3
+    p, parts = configs.wcsph_column(n, n, nz, dx, jitter=0.1)
+    o = Oracle(p, parts)
+    _run_stages(o, 1)
+    p1, _, _ = o.get_particles(abi.SPH)
+    sph = p1["p_type"] < 1000
+    assert np.allclose(p1["vol"][sph], p1["vol0"][sph], rtol=1e-13)
+    assert np.allclose(p1["vol0"][sph], dx ** 3, rtol=1e-12)
+    assert np.allclose(p1["m"][sph], p1["rho"][sph] * p1["vol"][sph], rtol=1e-15)
+
+
+def test_unsupported_set_is_refused():
+    p, parts = configs.small_isph_dem(n=6, nz=6)
+    p.solver_type = 0
+    o = Oracle(p, parts)
+    with pytest.raises(RuntimeError):
+        o.step()
