@@ -104,6 +104,13 @@ def main(argv=None) -> int:
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--weights", default=str(ROOT / "fluidized_bed_master_amd" / "weights" / "ppe_mlp.npy"))
+    ap.add_argument("--precond", default="two_level", choices=["two_level", "jacobi"],
+                    help="preconditioner of the mixed-precision solve")
+    ap.add_argument("--inner-rtol", type=float, default=0.01,
+                    help="mixed solve: fold into the FP64 solution each time the FP32 residual drops by this factor")
+    ap.add_argument("--legs", default="fp64,fp64_pc,prev",
+                    help="comparison legs to run: fp64 (plain FP64 path), fp64_pc (FP64, same preconditioner), "
+                         "prev (mixed, previous-step guess)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -127,7 +134,12 @@ def main(argv=None) -> int:
     if Path(a.weights).exists():
         weights = np.load(a.weights).astype(np.float32)
     guess = abi.GUESS_LEARNED if weights is not None else abi.GUESS_EXTRAPOLATE
-    mixed_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=guess, rtol=1e-8)
+    precond = abi.PRECOND_TWO_LEVEL if a.precond == "two_level" else abi.PRECOND_JACOBI
+    mixed_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=guess, rtol=1e-8,
+                              precond=precond, inner_rtol=a.inner_rtol)
+    legs = set(a.legs.split(",")) if not a.no_fp64 else set()
+    # the plain FP64 path: the reference's Jacobi-preconditioned system, FP64
+    # operator and vectors, previous-step guess
     fp64_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8)
 
     dt, st, rep, layout, n_sph, n_dem = run_gpu(params, parts, abi.PREC_MIXED, mixed_opts, a.steps, a.warmup,
@@ -157,18 +169,29 @@ def main(argv=None) -> int:
                         "index_bytes": layout["index_bytes"]}
 
     fp64 = None
-    if not a.no_fp64:
+    if "fp64" in legs:
         dt64, st64, _, _, _, _ = run_gpu(params, parts, abi.PREC_FP64, fp64_opts, a.steps, a.warmup)
         fp64 = {"value": n_total * a.steps / dt64, "ms_per_step": dt64 / a.steps * 1e3,
                 "ppe_ms": float(np.mean([s["ms"] for s in st64])),
                 "ppe_iters": float(np.mean([s["iterations"] for s in st64])),
                 "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in st64]))}
 
+    # the FP64 path with the same preconditioner: what precision alone buys
+    fp64_pc = None
+    if "fp64_pc" in legs and precond != abi.PRECOND_JACOBI:
+        pc_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8,
+                               precond=precond)
+        dtq, stq, _, _, _, _ = run_gpu(params, parts, abi.PREC_FP64, pc_opts, a.steps, a.warmup)
+        fp64_pc = {"value": n_total * a.steps / dtq, "ms_per_step": dtq / a.steps * 1e3,
+                   "ppe_ms": float(np.mean([s["ms"] for s in stq])),
+                   "ppe_iters": float(np.mean([s["iterations"] for s in stq])),
+                   "precond": a.precond, "guess": "previous"}
+
     # the same mixed path with the plain previous-step guess: what the warm start buys
     prev = None
-    if not a.no_fp64:
+    if "prev" in legs:
         prev_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=abi.GUESS_PREVIOUS,
-                                 rtol=1e-8)
+                                 rtol=1e-8, precond=precond, inner_rtol=a.inner_rtol)
         dtp, stp, _, _, _, _ = run_gpu(params, parts, abi.PREC_MIXED, prev_opts, a.steps, a.warmup)
         prev = {"ms_per_step": dtp / a.steps * 1e3, "ppe_ms": float(np.mean([s["ms"] for s in stp])),
                 "ppe_iters": float(np.mean([s["iterations"] for s in stp])),
@@ -197,6 +220,8 @@ def main(argv=None) -> int:
                         f"Wendland C2, dt={params.dt:g}",
             "particles": n_total,
             "ppe": "BiCGStab rtol 1e-8, FP32 iterations with FP64 reliable updates, "
+                   + ("two-level (Jacobi + 3x3x3-cell aggregate coarse correction) preconditioner, "
+                      if precond == abi.PRECOND_TWO_LEVEL else "Jacobi preconditioner, ")
                    + ("learned correction of the extrapolated guess" if weights is not None
                       else "quadratic extrapolation of the last three pressures"),
             "pressure_gradient": "difference form sum_j V_j (p_j - p_i) grad~W_ij",
@@ -205,11 +230,17 @@ def main(argv=None) -> int:
         "ppe_ms_per_step": ppe_ms,
         "ppe_iters_per_step": iters,
         "ppe_folds_per_step": float(np.mean([s["outer"] for s in st])),
+        "ppe_inner_rtol": a.inner_rtol,
         "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in st])),
         "ppe_guess_learned_frac": float(np.mean([s["guess_used"] == abi.GUESS_LEARNED for s in st])),
+        "ppe_precond_used": sorted({int(s["precond_used"]) for s in st}),
+        "ppe_n_coarse": int(st[-1]["n_coarse"]),
+        "ppe_coarse_res": float(np.max([s["coarse_res"] for s in st])),
         "fp64_path": fp64,
+        "fp64_same_precond": fp64_pc,
         "mixed_previous_guess": prev,
         "speedup_vs_fp64_path": (value / fp64["value"]) if fp64 else None,
+        "speedup_vs_fp64_same_precond": (value / fp64_pc["value"]) if fp64_pc else None,
         "kernel_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in sorted(rep.items())},
         "roofline": roofline,
         "cpu_baseline": cpu,

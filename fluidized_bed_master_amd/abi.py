@@ -27,6 +27,7 @@ PT_BOUNDARY, PT_FLUID, PT_MOVING, PT_DEM = 0, 1, 9, 1001
 PPE_JACOBI, PPE_BICGSTAB = 0, 1
 PPE_FP64, PPE_MIXED_IR = 0, 1
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
+PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
 PREDICT, CORRECT = 0, 1
 ERRORS = {
     -1: "SOPHIA_E_ARG", -2: "SOPHIA_E_HIP", -3: "SOPHIA_E_IO", -4: "SOPHIA_E_STATE",
@@ -110,7 +111,7 @@ class PpeOpts(C.Structure):
     _fields_ = [
         ("method", C.c_int32), ("precision", C.c_int32), ("guess", C.c_int32), ("max_iter", C.c_int32),
         ("max_outer", C.c_int32), ("zero_mean", C.c_int32), ("rtol", C.c_double), ("inner_rtol", C.c_double),
-        ("train", C.c_int32), ("reserved_i", C.c_int32), ("lr", C.c_double),
+        ("train", C.c_int32), ("precond", C.c_int32), ("lr", C.c_double),
     ]
 
 
@@ -119,6 +120,7 @@ class PpeStats(C.Structure):
         ("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32), ("guess_used", C.c_int32),
         ("r0_rel", C.c_double), ("r0_rel_learned", C.c_double), ("r_rel", C.c_double), ("ms", C.c_double),
         ("n_rows", C.c_int32), ("nnz", C.c_int32), ("loss", C.c_double),
+        ("precond_used", C.c_int32), ("n_coarse", C.c_int32), ("coarse_res", C.c_double),
     ]
 
     def as_dict(self):
@@ -126,8 +128,8 @@ class PpeStats(C.Structure):
 
 
 def ppe_opts(method=PPE_JACOBI, precision=PPE_FP64, guess=GUESS_PREVIOUS, rtol=1e-8, max_iter=10000,
-             max_outer=20, inner_rtol=0.1, zero_mean=0, train=0, lr=1e-3) -> PpeOpts:
-    return PpeOpts(method, precision, guess, max_iter, max_outer, zero_mean, rtol, inner_rtol, train, 0, lr)
+             max_outer=20, inner_rtol=0.01, zero_mean=0, train=0, lr=1e-3, precond=PRECOND_JACOBI) -> PpeOpts:
+    return PpeOpts(method, precision, guess, max_iter, max_outer, zero_mean, rtol, inner_rtol, train, precond, lr)
 
 
 _P = C.POINTER

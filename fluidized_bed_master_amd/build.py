@@ -24,9 +24,10 @@ BUILD = PKG / "_build"
 LIB = PKG / "libsophia.so"
 ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
 
-SOURCES = ["nnps.hip", "sph.hip", "dem.hip", "wcsph.hip", "ppe.hip", "learned.hip", "context.cpp"]
+SOURCES = ["nnps.hip", "sph.hip", "dem.hip", "wcsph.hip", "ppe.hip", "coarse.hip", "learned.hip", "context.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+ROCM_LIB = str(Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib")
 
 COMMON = [
     "-O3",
@@ -54,7 +55,8 @@ def build_library(jobs: int = 8) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(jobs, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           f"-L{ROCM_LIB}", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -1,0 +1,52 @@
+// Two-level additive preconditioner of the PPE Krylov solve (coarse.hip).
+//
+//   M^-1 = D^-1 + Z E^-1 Z^T,   E = Z^T A Z,
+//
+// Z the indicator matrix of aggregates of kAgg^3 cells (a particle belongs to
+// the aggregate of its cell).  D^-1 is the Jacobi part the solver already
+// applies; the coarse part restricts a vector to aggregate sums, multiplies by
+// a dense approximate inverse X ~ E^-1 and adds the result back to every row of
+// the aggregate.  X is kept from step to step and refreshed by Newton-Schulz
+// steps X <- X (2 I - E X) (two GEMMs) whenever ||I - E X||_F / sqrt(na) of the
+// new E exceeds kCoarseTol; a cold start begins from X = diag(E)^-1.
+#pragma once
+
+#include "ppe.hpp"
+
+namespace sophia {
+
+constexpr int kAgg = 3;              // cells per aggregate edge
+constexpr double kCoarseTol = 0.02;  // accepted ||I - E X||_F / sqrt(na)
+constexpr int kCoarseMaxNa = 16384;  // dense E and X (4 x na^2 x 4 B) above this are refused
+
+struct CoarseWork {
+  int NI = 0, NJ = 0, NK = 0;
+  int AX = 0, AY = 0, AZ = 0, na = 0;  // aggregates per axis, total
+  int ld = 0;            // leading dimension of the dense matrices (na rounded up to 64)
+  float* E = nullptr;    // Z^T A Z, row-major na x na
+  float* X = nullptr;    // approximate inverse, row-major
+  float* X2 = nullptr;   // Newton-Schulz product
+  float* U = nullptr;    // 2 I - E X
+  double* c = nullptr;   // restricted vector (na)
+  double* y = nullptr;   // coarse correction (na)
+  double* partials = nullptr;
+  double* slots = nullptr; // 8 aggregate-slot sums per particle (coarse build)
+  int slots_n = 0;
+  double* host = nullptr;  // pinned copy of the partials
+  void* blas = nullptr;    // rocblas_handle bound to the solver stream
+  bool valid = false;      // X is an accepted inverse of some earlier E
+  double res = 0;          // ||I - E X||_F / sqrt(na) of X against the current E
+  int ns_steps = 0;        // Newton-Schulz steps of the last setup
+};
+
+// Build E from the assembled operator (row-major cell keys) and bring X to
+// ||I - E X|| <= kCoarseTol.  Returns 0 with *usable = false when X cannot be
+// brought there (the solve then runs with Jacobi alone).
+int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, Timers* tm);
+// out += Z X Z^T v over rows (skipped when *done or *brk)
+template <typename V>
+int coarse_apply(hipStream_t s, const CoarseWork& c, const PpeWork& w, const V* v, V* out, const int* done,
+                 const int* brk);
+void coarse_free(CoarseWork& c);
+
+}  // namespace sophia

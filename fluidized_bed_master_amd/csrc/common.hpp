@@ -383,6 +383,9 @@ template <typename R> int sph_mass_init(hipStream_t s, const DevParams& P, const
 template <typename R> int sph_kgc(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dcells);
 template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// KERNEL_clc_prep3D_sph alone, without the ISPH mass/density reset (the WCSPH
+// driver integrates density by continuity)
+template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv);
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v);

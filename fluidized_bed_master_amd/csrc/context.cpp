@@ -977,7 +977,7 @@ struct Ctx : public sophia_ctx {
         }
         {
           TimeScope t(tm(), "prep");
-          TRY(sph_prep<R>(stream, P, sv));
+          TRY(sph_prep_sums<R>(stream, P, sv));
         }
         break;
       case 3: {

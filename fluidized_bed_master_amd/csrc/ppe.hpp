@@ -14,6 +14,8 @@ namespace sophia {
 //              run lengths (7 bits each).  Needs every run <= 256 particles.
 //   kFmtIdx32: plain u32 column indices (Morton keys or overfull runs).
 enum { kFmtRun8 = 0, kFmtIdx32 = 1 };
+constexpr int kRunBits = 7;  // kFmtRun8: bits per run length in the row's u64
+constexpr int kRunMax = (1 << kRunBits) - 1;
 
 // Compact operator: per row only the columns that are PPE rows themselves
 // (wall/buffer columns live in the right-hand side), slot-quad major:
@@ -21,7 +23,7 @@ enum { kFmtRun8 = 0, kFmtIdx32 = 1 };
 // 64 consecutive 16 B (FP32) or 32 B (FP64) quads per four slots.
 struct PpeOp {
   int n = 0, nq = 0, fmt = kFmtRun8;
-  int NI = 1, NJ = 1;
+  int NI = 1, NJ = 1, ncell = 1;
   const uint32_t* keys = nullptr;      // sorted cell key per particle
   const uint32_t* cell_off = nullptr;  // first particle of each cell (dense, num_cells + 1)
   const int* row = nullptr;
@@ -41,7 +43,8 @@ struct PpeWork {
   int64_t nnz = 0;  // stored entries of the last assembly
   bool mixed = false;
   int fmt = kFmtRun8;
-  int NI = 1, NJ = 1;
+  int NI = 1, NJ = 1, ncell = 1;
+  bool rowmajor = true;  // cell keys are row-major (not Morton)
   const uint32_t* keys = nullptr;
   const uint32_t* cell_off = nullptr;
   int* row = nullptr;
@@ -58,6 +61,7 @@ struct PpeWork {
   unsigned* host_flags = nullptr;
   void* vec[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int vec_bytes = 0;
+  struct CoarseWork* coarse = nullptr;  // two-level preconditioner, created on first use
 };
 
 int ppe_workspace_alloc(PpeWork& w, int n, int cap, bool mixed);

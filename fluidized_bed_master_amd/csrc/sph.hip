@@ -150,7 +150,9 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   const double hd = v.h[i];
   const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
-  const T rhoi = v.rho[i], rrefi = v.rho_ref[i];
+  // rho / rho_ref differs from 1 by ~drho dt / rho: the difference of two
+  // such ratios cancels to a few significant digits, so it is taken in FP64
+  const double ratio_i = v.rho[i] / v.rho_ref[i];
   const bool anto = P.delsph == 2;
   T flt = 0, rx = 0, ry = 0, rz = 0;
   for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
       T dw = knl_dw<T>(P.kernel_type, A, h, r);
       T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
       kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      T c = -(rhoj / (T)q.rref - rhoi / rrefi) * (mj / rhoj);
+      T c = (T)(-(q.rho / q.rref - ratio_i)) * (mj / rhoj);
       rx += use ? c * gx : (T)0;
       ry += use ? c * gy : (T)0;
       rz += use ? c * gz : (T)0;
@@ -290,8 +292,11 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
   const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h), H2 = (T)(2.0 * hd);
   const double R2 = 2.0 * (P.kappa * hd);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
-  double tp = 0;
-  T fl = 0, tt = 0, tr = 0;
+  // weights in R, sums in FP64: the wall density is a weighted mean of
+  // values within ~1e-4 of each other, and the delta-SPH term differences
+  // density ratios at that level
+  double tp = 0, fl = 0, tr = 0;
+  T tt = 0;
   for_cells(P, xi, yi, zi, cv, [&](int j) {
     double rd = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
     int ptj = v.ptype[j];
@@ -300,18 +305,19 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
     if (!(ptj >= 1)) return;
     T w = knl_w<T>(P.kernel_type, A, H2, (T)rd);
     T vj = (T)v.m[j] / (T)v.rho[j];
-    tp += (double)(vj * w) * v.pres[j];
-    fl += vj * w;
-    tt += vj * (T)v.temp[j] * w;
-    tr += (T)v.rho[j] / (T)v.por[j] * w * vj;
+    const T wv = vj * w;
+    tp += (double)wv * v.pres[j];
+    fl += (double)wv;
+    tt += wv * (T)v.temp[j];
+    tr += v.rho[j] / v.por[j] * (double)wv;
   });
-  if (fl < (T)1e-6) {
+  if (fl < 1e-6) {
     v.rho[i] = v.rho_ref[i];
     v.pres[i] = 0.0;
   } else {
-    v.pres[i] = tp / (double)fl;
-    v.temp[i] = tt / fl;
-    v.rho[i] = (T)v.por[i] * tr / fl;
+    v.pres[i] = tp / fl;
+    v.temp[i] = tt / (T)fl;
+    v.rho[i] = v.por[i] * tr / fl;
   }
 }
 
@@ -542,6 +548,7 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
   }
   return 0;
 }
+template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_prep, P, v); }
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_advection, P, v); }
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv) {
   LAUNCH(k_neumann, P, v, cv);
@@ -571,6 +578,7 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_kgc<R>(hipStream_t, const DevParams&, const SphView<R>&);                                \
   template int sph_porosity<R>(hipStream_t, const DevParams&, const SphView<R>&, const DemView&, const CellView&); \
   template int sph_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                               \
+  template int sph_prep_sums<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
   template int sph_pressure_force<R>(hipStream_t, const DevParams&, const SphView<R>&);                     \

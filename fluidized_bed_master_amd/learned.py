@@ -54,14 +54,14 @@ def init_params(seed: int = 4, out_scale: float = 0.1) -> np.ndarray:
 
 
 def train(sim, snapshots: int = 400, lr: float = 1e-3, precision: int = abi.PPE_MIXED_IR, rtol: float = 1e-8,
-          params: np.ndarray | None = None, log_every: int = 0):
+          params: np.ndarray | None = None, log_every: int = 0, precond: int = abi.PRECOND_TWO_LEVEL):
     """Advance `sim` (a Sophia context) `snapshots` steps; each PPE solve is
     followed by one Adam step of the warm-start network on that solve."""
     if params is None:
         params = init_params()
     sim.mlp_set_params(params)
     opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=precision, guess=abi.GUESS_LEARNED, rtol=rtol,
-                        train=1, lr=lr)
+                        train=1, lr=lr, precond=precond)
     hist = []
     t0 = time.perf_counter()
     for k in range(snapshots):
@@ -75,7 +75,7 @@ def train(sim, snapshots: int = 400, lr: float = 1e-3, precision: int = abi.PPE_
 
 
 def evaluate(make_sim, params: np.ndarray, steps: int = 20, warm: int = 0,
-             precision: int = abi.PPE_MIXED_IR, rtol: float = 1e-8):
+             precision: int = abi.PPE_MIXED_IR, rtol: float = 1e-8, precond: int = abi.PRECOND_TWO_LEVEL):
     """Run the same state with zero, previous-step, extrapolated and learned guesses in
     lockstep; returns per-guess lists of PPE iterations and initial
     residuals, plus the learned path's fallback count."""
@@ -85,8 +85,9 @@ def evaluate(make_sim, params: np.ndarray, steps: int = 20, warm: int = 0,
         sim = make_sim()
         if guess == abi.GUESS_LEARNED:
             sim.mlp_set_params(params)
-        opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=precision, guess=guess, rtol=rtol)
-        warm_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=precision, guess=abi.GUESS_PREVIOUS, rtol=rtol)
+        opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=precision, guess=guess, rtol=rtol, precond=precond)
+        warm_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=precision, guess=abi.GUESS_PREVIOUS, rtol=rtol,
+                                 precond=precond)
         for _ in range(warm):
             sim.step(warm_opts)
         its, r0, fb, ms = [], [], 0, []

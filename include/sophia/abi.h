@@ -165,6 +165,11 @@ typedef struct sophia_domain {
 #define SOPHIA_GUESS_EXTRAPOLATE 3 /* 3 p^n - 3 p^(n-1) + p^(n-2) from the last three solutions
                                       (2 p^n - p^(n-1) with two, p^n with one) */
 
+#define SOPHIA_PRECOND_JACOBI 0    /* D^-1 (the reference's diagonal) */
+#define SOPHIA_PRECOND_TWO_LEVEL 1 /* D^-1 + Z E^-1 Z^T: additive coarse correction over
+                                      aggregates of 3x3x3 cells, E = Z^T A Z (row-major
+                                      cell keys; Jacobi otherwise) */
+
 typedef struct sophia_ppe_opts {
   int32_t method;       /* SOPHIA_PPE_JACOBI / SOPHIA_PPE_BICGSTAB */
   int32_t precision;    /* SOPHIA_PPE_FP64 / SOPHIA_PPE_MIXED_IR */
@@ -175,10 +180,10 @@ typedef struct sophia_ppe_opts {
   double rtol;          /* ||b - A p|| <= rtol * ||b|| */
   double inner_rtol;    /* mixed: fold the FP32 iterate into the FP64 solution and
                            recompute the FP64 residual each time the FP32 residual
-                           has dropped by this factor (0 -> 0.1) */
+                           has dropped by this factor (0 -> 0.01) */
   int32_t train;        /* 1: after the solve, one Adam step of the warm-start
                            network on (features before the solve, solution) */
-  int32_t reserved_i;
+  int32_t precond;      /* SOPHIA_PRECOND_* (BiCGStab) */
   double lr;            /* Adam learning rate when train != 0 */
 } sophia_ppe_opts;
 
@@ -194,6 +199,9 @@ typedef struct sophia_ppe_stats {
   int32_t n_rows;
   int32_t nnz;              /* stored operator entries (row columns) */
   double loss;              /* training loss of the in-solve Adam step */
+  int32_t precond_used;     /* SOPHIA_PRECOND_* actually used */
+  int32_t n_coarse;         /* coarse unknowns (two-level) */
+  double coarse_res;        /* ||I - E X||_F / sqrt(n_coarse) of the coarse inverse X used */
 } sophia_ppe_stats;
 
 typedef struct sophia_ctx sophia_ctx;
