@@ -66,6 +66,28 @@ def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp
     return dt, st_timed, rep, layout, n_sph, n_dem
 
 
+PMC_FILE = ROOT / "profiles" / "ppe_matvec_fp32_c5_1m_pmc.txt"
+
+
+def _pmc_traffic():
+    """HBM bytes per launch of the inner matvec on c5_1m from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (KiB per dispatch, each scaled
+    by the factor the same counter reads on a copy kernel of known size);
+    None when the summary is absent."""
+    if not PMC_FILE.exists():
+        return None
+    kv = {}
+    for line in PMC_FILE.read_text().splitlines():
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            kv[k.strip()] = float(v)
+    try:
+        return round(kv["fetch_kib_per_dispatch"] * 1024 * kv["fetch_factor"]
+                     + kv["write_kib_per_dispatch"] * 1024 * kv["write_factor"])
+    except KeyError:
+        return None
+
+
 def cpu_baseline(target_seconds: float = 15.0):
     """The FP64 C++/OpenMP oracle (same formulas, same BiCGStab) timed on a
     bounded sample of the same workload kind: a 40 x 40 x 40 ISPH-DEM column."""
@@ -162,9 +184,11 @@ def main(argv=None) -> int:
             ab = _matvec_bytes(layout, n_sph, fp32=True)
             avg_s = tot / real / 1e3
             ach = ab / avg_s / 1e9
-            roofline = {"bound": "hbm", "kernel": "k_bi_mv (inner FP32 matvec)", "achieved": round(ach, 1),
+            roofline = {"bound": "hbm", "kernel": "k_bi_mv_staged (inner FP32 matvec)", "achieved": round(ach, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": None, "launches": real, "launches_incl_early_exit": cnt,
+                        "traffic": _pmc_traffic() if a.config == "c5_1m" else None,
+                        "traffic_source": str(PMC_FILE.relative_to(ROOT)) if a.config == "c5_1m" else None,
+                        "launches": real, "launches_incl_early_exit": cnt,
                         "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": ab, "nnz": layout["nnz"],
                         "index_bytes": layout["index_bytes"]}
 

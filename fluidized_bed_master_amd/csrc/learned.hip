@@ -3,7 +3,9 @@
 // Network: edge MLP e_ij -> 64 -> 64 (ReLU), sum over the neighbour list
 // scaled by 1/64, node MLP 64 -> 64 (ReLU) -> 1.  One wave64 owns one
 // particle at a time (grid-stride over particles) and walks its neighbour
-// list in tiles of 16 edges:
+// list in tiles of 16 edges.  The forward kernel computes each layer
+// transposed (weights as the A operand, see k_mlp_forward); the backward
+// kernel uses the layout below:
 //   layer 1  D[16 x 64] = E[16 x 32] W1^T[32 x 64]   4 x v_mfma_f32_16x16x32_bf16
 //            (the 6 edge inputs padded to K = 32 with zeros)
 //   layer 2  D[16 x 64] = H1[16 x 64] W2^T[64 x 64]  8 x v_mfma_f32_16x16x32_bf16
@@ -181,24 +183,150 @@ __device__ inline void aggregate(const Net& net, const SphView<R>& v, const floa
   __syncthreads();
 }
 
+// Forward kernel, weights as the A operand.  Each layer is computed
+// transposed, Z^T[hidden x 16 edges] = W[hidden x K] E^T[K x 16 edges], so a
+// layer's MFMA result (lane l: hidden units 16 nb + 4 (l / 16) + r of edge
+// l mod 16) is already the next layer's B operand once the hidden dimension
+// is enumerated in that order: K-step s of layer 2 takes, in lane l,
+// hidden-1 units 16 (2 s + j / 4) + 4 (l / 16) + j mod 4 (j = 0..7), and the
+// W2 A operand is loaded in the same order.  No LDS transpose between the
+// layers.  Edge inputs are gathered 64 at a time (one edge per lane) into an
+// LDS tile; the messages are summed per lane over the tiles and then over
+// the 16 lanes of each group.  Biases are added in FP32 after each MFMA, as
+// in the backward kernel's recomputation.
+struct FwdOperands {
+  bf8 A1[4];      // W1 rows 16 nb + (l mod 16), inputs 0..5 (lanes 0-15)
+  bf8 A2[4][2];   // W2 rows 16 mb + (l mod 16), hidden-1 units in the layer-1 result order
+  float b1[4][4]; // b1 of hidden-1 unit 16 nb + 4 (l / 16) + r
+  float b2[4][4]; // b2 of hidden-2 unit 16 mb + 4 (l / 16) + r
+};
+
+__device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
+  const int m = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      w.A1[nb][j] = (g == 0 && j < kEin) ? net.wbf[OFF_W1 + (16 * nb + m) * kEin + j] : (__bf16)0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      w.b1[nb][r] = net.master[OFF_B1 + 16 * nb + 4 * g + r];
+      w.b2[nb][r] = net.master[OFF_B2 + 16 * nb + 4 * g + r];
+    }
+  }
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (int sk = 0; sk < 2; sk++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int u = 16 * (2 * sk + j / 4) + 4 * g + (j & 3);
+        w.A2[mb][sk][j] = net.wbf[OFF_W2 + (16 * mb + m) * kH + u];
+      }
+}
+
+template <typename R>
+__device__ inline void aggregate_fwd(const SphView<R>& v, const float2* nf, int i, int lane, const FwdOperands& w,
+                                     __bf16 (*Et)[8], float* Ms) {
+  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+  const int m = lane & 15;
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i], ih = 1.0 / v.h[i];
+  float acc[4][4] = {};
+  for (int c0 = 0; c0 < cnt; c0 += 64) {
+    const int e = c0 + lane;
+    float ein[kEin] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool ok = false;
+    if (e < cnt) {
+      const uint32_t ent = v.nbr[(size_t)e * v.n + i];
+      if (!(ent & kNbrFlag)) {
+        const int j = (int)ent;
+        const double dx = (xi - v.x[j]) * ih, dy = (yi - v.y[j]) * ih, dz = (zi - v.z[j]) * ih;
+        const float2 f = nf[j];
+        ein[0] = (float)dx;
+        ein[1] = (float)dy;
+        ein[2] = (float)dz;
+        ein[3] = (float)sqrt(dx * dx + dy * dy + dz * dz);
+        ein[4] = f.x;
+        ein[5] = f.y;
+        ok = true;
+      }
+    }
+    const uint64_t valid = __ballot(ok);
+    __syncthreads();  // the previous chunk's tiles are done with Et
+#pragma unroll
+    for (int q = 0; q < kEin; q++) Et[lane][q] = (__bf16)ein[q];
+    Et[lane][6] = (__bf16)0.0f;
+    Et[lane][7] = (__bf16)0.0f;
+    __syncthreads();
+    for (int t = 0; t < 4 && c0 + 16 * t < cnt; t++) {
+      bf8 b;
+#pragma unroll
+      for (int q = 0; q < 8; q++) b[q] = lane < 16 ? Et[16 * t + m][q] : (__bf16)0.0f;
+      // layer 1: 16 hidden x 16 edges per block nb
+      float h1[4][4];
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
+        f4 z = {0.f, 0.f, 0.f, 0.f};
+        z = mfma32(w.A1[nb], b, z);
+#pragma unroll
+        for (int r = 0; r < 4; r++) h1[nb][r] = fmaxf(z[r] + w.b1[nb][r], 0.f);
+      }
+      bf8 B2[2];
+#pragma unroll
+      for (int sk = 0; sk < 2; sk++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) B2[sk][j] = (__bf16)h1[2 * sk + j / 4][j & 3];
+      // layer 2 and the masked sum over this tile's edges (edge = lane mod 16)
+      const bool on = (valid >> (16 * t + m)) & 1ull;
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++) {
+        f4 z = {0.f, 0.f, 0.f, 0.f};
+        z = mfma32(w.A2[mb][0], B2[0], z);
+        z = mfma32(w.A2[mb][1], B2[1], z);
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[mb][r] += on ? fmaxf(z[r] + w.b2[mb][r], 0.f) : 0.f;
+      }
+    }
+  }
+  // sum over the 16 edge lanes of each group: hidden-2 unit 16 mb + 4 (l / 16) + r
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float a = acc[mb][r];
+      a += __shfl_xor(a, 1);
+      a += __shfl_xor(a, 2);
+      a += __shfl_xor(a, 4);
+      a += __shfl_xor(a, 8);
+      acc[mb][r] = a;
+    }
+  if (m == 0) {
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) Ms[16 * mb + 4 * (lane >> 4) + r] = acc[mb][r] * (1.0f / 64.0f);
+  }
+  __syncthreads();
+}
+
 template <typename R>
 __global__ __launch_bounds__(64) void k_mlp_forward(Net net, SphView<R> v, const float2* __restrict__ nf,
                                                      const int* __restrict__ row, const double* __restrict__ pprev,
                                                      double scale, double* __restrict__ guess) {
-  __shared__ __bf16 H[16][kH + 8];
+  __shared__ __bf16 Et[64][8];
   __shared__ float Ms[kH];
   __shared__ __bf16 W3s[kH * kH];
   const int lane = threadIdx.x;
   for (int q = lane; q < kH * kH; q += 64) W3s[q] = net.wbf[OFF_W3 + q];
-  bf8 B1[4], B2[4][2];
-  load_fwd_operands(net, lane, B1, B2);
+  FwdOperands w;
+  load_fwd_t(net, lane, w);
   __syncthreads();
   for (int i = blockIdx.x; i < v.n; i += gridDim.x) {
     if (!row[i]) {
       if (lane == 0) guess[i] = pprev[i];
       continue;
     }
-    aggregate(net, v, nf, i, lane, B1, B2, H, Ms);
+    aggregate_fwd(v, nf, i, lane, w, Et, Ms);
     float z3;
     float y = node_forward(net, Ms, W3s, lane, z3);
     if (lane == 0) guess[i] = pprev[i] + scale * (double)y;

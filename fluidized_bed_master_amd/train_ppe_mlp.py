@@ -1,15 +1,20 @@
 """Train the learned PPE warm start and report the iteration counts of every
 guess.
 
-    python -m fluidized_bed_master_amd.train_ppe_mlp [--snapshots 400] [--out weights/ppe_mlp.npy]
+    python -m fluidized_bed_master_amd.train_ppe_mlp [--episodes 4] [--steps 100] [--out weights/ppe_mlp.npy]
 
-Workload: a 60 x 60 x 60 column (216,000 fluid at dx = 1 mm) with spheres of
+Workload: 60 x 60 x 60 columns (216,000 fluid at dx = 1 mm) with spheres of
 the benchmark's size and packing (d = 2 mm, phi = 0.35), through the ISPH
-step with a converged PPE (difference-form pressure gradient).  Every training snapshot is one time step whose mixed
-BiCGStab solve to rtol 1e-8 is followed by one Adam step of the network on
-(features before the solve, converged pressure), all on the device.  The
-evaluation then runs the same initial state with the zero, previous-step,
-extrapolated and learned guesses and prints one JSON line.
+step with a converged PPE (difference-form pressure gradient).  Training runs
+in episodes of `steps` time steps from rest, each on a column with its own
+seed: the reference's formulation has no free-surface pressure condition, so
+a PPE solved to convergence lets the pressure level drift and the column
+departs from rest after ~100 steps; the episodes stay in the regime the
+benchmark measures.  Every time step's mixed BiCGStab solve to rtol 1e-8 is
+followed by one Adam step of the network on (features before the solve,
+converged pressure), all on the device.  The evaluation then runs an unseen
+column with the zero, previous-step, extrapolated and learned guesses and
+prints one JSON line.
 """
 from __future__ import annotations
 
@@ -25,7 +30,8 @@ from . import abi, configs, learned
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--snapshots", type=int, default=400)
+    ap.add_argument("--episodes", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=100, help="time steps (training snapshots) per episode")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-steps", type=int, default=20)
     ap.add_argument("--eval-warm", type=int, default=20)
@@ -34,24 +40,29 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
 
     from .sophia import Sophia
-    params, parts = configs.c4_train()
-    params.pgrad_form = 1
 
-    def make():
+    def make(seed):
+        params, parts = configs.c4_train(seed=seed)
+        params.pgrad_form = 1
         return Sophia(params, parts, precision=abi.PREC_MIXED)
 
-    sim = make()
-    w0 = learned.init_params(seed=a.seed)
-    w, hist, secs = learned.train(sim, snapshots=a.snapshots, lr=a.lr, params=w0, log_every=50)
-    sim.close()
+    w = learned.init_params(seed=a.seed)
+    hist, secs = [], 0.0
+    for e in range(a.episodes):
+        sim = make(a.seed + 1 + e)
+        w, h, t = learned.train(sim, snapshots=a.steps, lr=a.lr, params=w, log_every=25)
+        sim.close()
+        hist += h
+        secs += t
     out = Path(a.out)
     out.parent.mkdir(parents=True, exist_ok=True)
     np.save(out, w.astype(np.float32))
-    ev = learned.evaluate(make, w, steps=a.eval_steps, warm=a.eval_warm)
+    ev = learned.evaluate(lambda: make(a.seed + 1000), w, steps=a.eval_steps, warm=a.eval_warm)
     losses = np.array([h["loss"] for h in hist])
     rep = {
-        "workload": f"60x60x60 column ({len(parts)} particles, 2 mm spheres phi 0.35), ISPH, converged mixed PPE rtol 1e-8",
-        "snapshots": a.snapshots,
+        "workload": "60x60x60 columns (2 mm spheres phi 0.35), ISPH, converged mixed PPE rtol 1e-8, "
+                    f"{a.episodes} episodes x {a.steps} steps from rest",
+        "snapshots": a.episodes * a.steps,
         "train_seconds": secs,
         "loss_first20": float(losses[:20].mean()),
         "loss_last20": float(losses[-20:].mean()),
