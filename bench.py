@@ -130,9 +130,10 @@ def main(argv=None) -> int:
                     help="preconditioner of the mixed-precision solve")
     ap.add_argument("--inner-rtol", type=float, default=0.01,
                     help="mixed solve: fold into the FP64 solution each time the FP32 residual drops by this factor")
-    ap.add_argument("--legs", default="fp64,fp64_pc,prev",
+    ap.add_argument("--legs", default="fp64,fp64_pc,prev,extrap",
                     help="comparison legs to run: fp64 (plain FP64 path), fp64_pc (FP64, same preconditioner), "
-                         "prev (mixed, previous-step guess)")
+                         "prev (mixed, previous-step guess), extrap (mixed, extrapolated guess without the "
+                         "learned correction)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -221,6 +222,17 @@ def main(argv=None) -> int:
                 "ppe_iters": float(np.mean([s["iterations"] for s in stp])),
                 "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in stp]))}
 
+    # the same mixed path without the learned correction: what the network adds
+    extrap = None
+    if "extrap" in legs and weights is not None:
+        ex_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=abi.GUESS_EXTRAPOLATE,
+                               rtol=1e-8, precond=precond, inner_rtol=a.inner_rtol)
+        dte, ste, _, _, _, _ = run_gpu(params, parts, abi.PREC_MIXED, ex_opts, a.steps, a.warmup)
+        extrap = {"value": n_total * a.steps / dte, "ms_per_step": dte / a.steps * 1e3,
+                  "ppe_ms": float(np.mean([s["ms"] for s in ste])),
+                  "ppe_iters": float(np.mean([s["iterations"] for s in ste])),
+                  "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in ste]))}
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_seconds)
@@ -263,6 +275,7 @@ def main(argv=None) -> int:
         "fp64_path": fp64,
         "fp64_same_precond": fp64_pc,
         "mixed_previous_guess": prev,
+        "mixed_extrapolated_guess": extrap,
         "speedup_vs_fp64_path": (value / fp64["value"]) if fp64 else None,
         "speedup_vs_fp64_same_precond": (value / fp64_pc["value"]) if fp64_pc else None,
         "kernel_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in sorted(rep.items())},
