@@ -85,7 +85,7 @@ __device__ inline void run_decode(const PpeOp& o, int i, int NI, int NJ, RunDeco
 // aggregate columns.  Non-rows write zeros.
 template <int FMT, int CM>
 __global__ __launch_bounds__(256) void k_coarse_rows(PpeOp o, AggGrid g, double* __restrict__ slots) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = particle_index();
   const int n = o.n;
   if (i >= n) return;
   double acc[8];
@@ -414,7 +414,7 @@ int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, T
   TimeScope ts(tm, "ppe_coarse_setup");
   const PpeOp o = ppe_op(w);
   const AggGrid g = agg_grid(c);
-  const int gr = grid_for(w.n);
+  const int gr = grid_xcd(w.n);
   if (w.fmt == kFmtRun8) {
     if (w.mixed) k_coarse_rows<kFmtRun8, 1><<<gr, 256, 0, s>>>(o, g, c.slots);
     else k_coarse_rows<kFmtRun8, 2><<<gr, 256, 0, s>>>(o, g, c.slots);

@@ -352,6 +352,23 @@ __device__ inline void kgc_apply(const DevParams& P, const R* cm, int n, int i, 
 }
 
 inline int grid_for(int n, int block = kBlock) { return (n + block - 1) / block; }
+// blocks for a per-particle kernel that gathers neighbour data: grid_for
+// rounded up to a multiple of 8, for particle_index()
+inline int grid_xcd(int n, int block = kBlock) {
+  const int g = grid_for(n, block);
+  return g < 8 ? g : (g + 7) & ~7;
+}
+
+// Particle of this thread.  Workgroups are dealt to the eight XCDs round
+// robin; with a grid that is a multiple of 8, block b is given logical block
+// (b mod 8) * (G / 8) + b / 8, so each XCD works on one contiguous eighth of
+// the cell-sorted particles and the neighbour fields its waves gather stay
+// in its own L2.  Other grids map linearly.
+__device__ inline int particle_index() {
+  const int G = gridDim.x;
+  const int b = (G & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3));
+  return b * (int)blockDim.x + (int)threadIdx.x;
+}
 
 // ---------------------------------------------------------------------------
 // launch wrappers implemented in the .hip files

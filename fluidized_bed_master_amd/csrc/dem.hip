@@ -17,7 +17,7 @@ namespace {
 // Shepard sums seen by each sphere and the sphere-phase porosity.
 template <typename R>
 __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc, CellView dc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= d.n) return;
   if (d.itype[i] > 2) return;
   const int pti = d.ptype[i];
@@ -61,7 +61,7 @@ __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc
 // Hertz normal spring + damping, Mindlin tangential spring + damping with a
 // Coulomb cap, torque, gravity; shear displacement history keyed by dem_idx.
 __global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
   if (i >= d.n) return;
   const int n = d.n, S = P.dem_slots;
   // the new history starts as a copy of the old one
@@ -175,7 +175,7 @@ __global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
 // drag (fluid regime) and the pressure-gradient force.
 template <typename R>
 __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
   if (i >= d.n) return;
   if (d.itype[i] != 1 || d.ptype[i] <= 1000) return;
   const double xi = d.x[i], yi = d.y[i], zi = d.z[i], uxi = d.ux[i], uyi = d.uy[i], uzi = d.uz[i];
@@ -281,7 +281,7 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
 // drag reaction distributed onto the fluid with the sphere's Shepard sum.
 template <typename R>
 __global__ void k_sph_reaction(DevParams P, SphView<R> v, DemView d, CellView dc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
   if (i >= v.n) return;
   const int t = v.ptype[i];
   if (v.itype[i] != 1 || t > 1000) return;
@@ -324,7 +324,7 @@ __global__ void k_sph_reaction(DevParams P, SphView<R> v, DemView d, CellView dc
 
 // KERNEL_clc_projection_dem (functions_TIME_ISPH2.cuh:214-308)
 __global__ void k_dem_predictor(DevParams P, DemView d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= d.n) return;
   if (d.itype[i] > 2) return;
   const double ux0 = d.ux[i], uy0 = d.uy[i], uz0 = d.uz[i];
@@ -342,7 +342,7 @@ __global__ void k_dem_predictor(DevParams P, DemView d) {
 // KERNEL_time_update_projection_dem (functions_TIME_ISPH2.cuh:624-776):
 // spheres carry no pressure acceleration, so u = u*; x from the average.
 __global__ void k_dem_corrector(DevParams P, DemView d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= d.n) return;
   if (d.itype[i] > 2) return;
   const int t = d.ptype[i];
@@ -371,14 +371,14 @@ template <typename R>
 int dem_porosity(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc,
                  const CellView& dc) {
   if (d.n <= 0) return 0;
-  k_porosity_dem<R><<<grid_for(d.n), kBlock, 0, s>>>(P, d, v, sc, dc);
+  k_porosity_dem<R><<<grid_xcd(d.n), kBlock, 0, s>>>(P, d, v, sc, dc);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
 int dem_contact(hipStream_t s, const DevParams& P, const DemView& d, const CellView& dc) {
   if (d.n <= 0) return 0;
-  k_dem_contact<<<grid_for(d.n), kBlock, 0, s>>>(P, d, dc);
+  k_dem_contact<<<grid_xcd(d.n), kBlock, 0, s>>>(P, d, dc);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -386,7 +386,7 @@ int dem_contact(hipStream_t s, const DevParams& P, const DemView& d, const CellV
 template <typename R>
 int dem_coupling(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc) {
   if (d.n <= 0) return 0;
-  k_dem_coupling<R><<<grid_for(d.n), kBlock, 0, s>>>(P, d, v, sc);
+  k_dem_coupling<R><<<grid_xcd(d.n), kBlock, 0, s>>>(P, d, v, sc);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -394,21 +394,21 @@ int dem_coupling(hipStream_t s, const DevParams& P, const DemView& d, const SphV
 template <typename R>
 int sph_dem_reaction(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dc) {
   if (v.n <= 0) return 0;
-  k_sph_reaction<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v, d, dc);
+  k_sph_reaction<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, d, dc);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
 int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d) {
   if (d.n <= 0) return 0;
-  k_dem_predictor<<<grid_for(d.n), kBlock, 0, s>>>(P, d);
+  k_dem_predictor<<<grid_xcd(d.n), kBlock, 0, s>>>(P, d);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d) {
   if (d.n <= 0) return 0;
-  k_dem_corrector<<<grid_for(d.n), kBlock, 0, s>>>(P, d);
+  k_dem_corrector<<<grid_xcd(d.n), kBlock, 0, s>>>(P, d);
   HIP_TRY(hipGetLastError());
   return 0;
 }

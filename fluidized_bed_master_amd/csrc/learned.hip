@@ -36,6 +36,20 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kFwdBlocks = 4096;
 
+// Particles of this block: eight contiguous chunks, one per XCD (workgroups
+// are dealt to XCDs round robin), so the particles an XCD works on at one
+// time are consecutive and share their neighbour-list lines and neighbour
+// fields in that XCD's L2.  gridDim.x is a multiple of 8.
+struct PartIter {
+  int i, end, stride;
+};
+__device__ inline PartIter xcd_particles(int n) {
+  const int x = blockIdx.x & 7, b = blockIdx.x >> 3, per = gridDim.x >> 3;
+  const int chunk = (n + 7) >> 3;
+  const int lo = x * chunk;
+  return PartIter{lo + b, min(n, lo + chunk), per};
+}
+
 __device__ inline f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 __device__ inline f4 mfma16(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
@@ -197,8 +211,6 @@ __device__ inline void aggregate(const Net& net, const SphView<R>& v, const floa
 struct FwdOperands {
   bf8 A1[4];      // W1 rows 16 nb + (l mod 16), inputs 0..5 (lanes 0-15)
   bf8 A2[4][2];   // W2 rows 16 mb + (l mod 16), hidden-1 units in the layer-1 result order
-  float b1[4][4]; // b1 of hidden-1 unit 16 nb + 4 (l / 16) + r
-  float b2[4][4]; // b2 of hidden-2 unit 16 mb + 4 (l / 16) + r
 };
 
 __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
@@ -208,11 +220,6 @@ __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
 #pragma unroll
     for (int j = 0; j < 8; j++)
       w.A1[nb][j] = (g == 0 && j < kEin) ? net.wbf[OFF_W1 + (16 * nb + m) * kEin + j] : (__bf16)0.0f;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      w.b1[nb][r] = net.master[OFF_B1 + 16 * nb + 4 * g + r];
-      w.b2[nb][r] = net.master[OFF_B2 + 16 * nb + 4 * g + r];
-    }
   }
 #pragma unroll
   for (int mb = 0; mb < 4; mb++)
@@ -225,9 +232,11 @@ __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
       }
 }
 
+// biases in LDS (bs[0..63] = b1, bs[64..127] = b2), read as one float4 per
+// block of four units: registers go to the weight operands, for occupancy
 template <typename R>
 __device__ inline void aggregate_fwd(const SphView<R>& v, const float2* nf, int i, int lane, const FwdOperands& w,
-                                     __bf16 (*Et)[8], float* Ms) {
+                                     const float* bs, __bf16 (*Et)[8], float* Ms) {
   const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
   const int m = lane & 15;
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i], ih = 1.0 / v.h[i];
@@ -268,8 +277,11 @@ __device__ inline void aggregate_fwd(const SphView<R>& v, const float2* nf, int 
       for (int nb = 0; nb < 4; nb++) {
         f4 z = {0.f, 0.f, 0.f, 0.f};
         z = mfma32(w.A1[nb], b, z);
-#pragma unroll
-        for (int r = 0; r < 4; r++) h1[nb][r] = fmaxf(z[r] + w.b1[nb][r], 0.f);
+        const float4 bb = *reinterpret_cast<const float4*>(bs + 16 * nb + 4 * (lane >> 4));
+        h1[nb][0] = fmaxf(z[0] + bb.x, 0.f);
+        h1[nb][1] = fmaxf(z[1] + bb.y, 0.f);
+        h1[nb][2] = fmaxf(z[2] + bb.z, 0.f);
+        h1[nb][3] = fmaxf(z[3] + bb.w, 0.f);
       }
       bf8 B2[2];
 #pragma unroll
@@ -283,8 +295,11 @@ __device__ inline void aggregate_fwd(const SphView<R>& v, const float2* nf, int 
         f4 z = {0.f, 0.f, 0.f, 0.f};
         z = mfma32(w.A2[mb][0], B2[0], z);
         z = mfma32(w.A2[mb][1], B2[1], z);
-#pragma unroll
-        for (int r = 0; r < 4; r++) acc[mb][r] += on ? fmaxf(z[r] + w.b2[mb][r], 0.f) : 0.f;
+        const float4 bb = *reinterpret_cast<const float4*>(bs + kH + 16 * mb + 4 * (lane >> 4));
+        acc[mb][0] += on ? fmaxf(z[0] + bb.x, 0.f) : 0.f;
+        acc[mb][1] += on ? fmaxf(z[1] + bb.y, 0.f) : 0.f;
+        acc[mb][2] += on ? fmaxf(z[2] + bb.z, 0.f) : 0.f;
+        acc[mb][3] += on ? fmaxf(z[3] + bb.w, 0.f) : 0.f;
       }
     }
   }
@@ -310,23 +325,27 @@ __device__ inline void aggregate_fwd(const SphView<R>& v, const float2* nf, int 
 }
 
 template <typename R>
-__global__ __launch_bounds__(64) void k_mlp_forward(Net net, SphView<R> v, const float2* __restrict__ nf,
+__global__ __launch_bounds__(64, 4) void k_mlp_forward(Net net, SphView<R> v, const float2* __restrict__ nf,
                                                      const int* __restrict__ row, const double* __restrict__ pprev,
                                                      double scale, double* __restrict__ guess) {
   __shared__ __bf16 Et[64][8];
   __shared__ float Ms[kH];
   __shared__ __bf16 W3s[kH * kH];
+  __shared__ __attribute__((aligned(16))) float bs[2 * kH];
   const int lane = threadIdx.x;
   for (int q = lane; q < kH * kH; q += 64) W3s[q] = net.wbf[OFF_W3 + q];
+  bs[lane] = net.master[OFF_B1 + lane];
+  bs[kH + lane] = net.master[OFF_B2 + lane];
   FwdOperands w;
   load_fwd_t(net, lane, w);
   __syncthreads();
-  for (int i = blockIdx.x; i < v.n; i += gridDim.x) {
+  for (PartIter it = xcd_particles(v.n); it.i < it.end; it.i += it.stride) {
+    const int i = it.i;
     if (!row[i]) {
       if (lane == 0) guess[i] = pprev[i];
       continue;
     }
-    aggregate_fwd(v, nf, i, lane, w, Et, Ms);
+    aggregate_fwd(v, nf, i, lane, w, bs, Et, Ms);
     float z3;
     float y = node_forward(net, Ms, W3s, lane, z3);
     if (lane == 0) guess[i] = pprev[i] + scale * (double)y;
@@ -374,7 +393,8 @@ __global__ __launch_bounds__(64) void k_mlp_backward(Net net, SphView<R> v, cons
   for (int k = 0; k < kH; k++) gW3[k] = 0.f;
   float gb3 = 0.f, gW4 = 0.f, gb4 = 0.f, loss = 0.f;
 
-  for (int i = blockIdx.x; i < v.n; i += gridDim.x) {
+  for (PartIter it = xcd_particles(v.n); it.i < it.end; it.i += it.stride) {
+    const int i = it.i;
     if (!row[i]) continue;
     // pass 1: forward
     aggregate(net, v, nf, i, lane, B1, B2, H, Ms);
@@ -687,7 +707,7 @@ int mlp_forward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, 
   }
   if (v.n <= 0) return 0;
   Net net{m.wbf, m.master};
-  int g = v.n < kFwdBlocks ? v.n : kFwdBlocks;
+  const int g = v.n < kFwdBlocks ? ((v.n + 7) / 8) * 8 : kFwdBlocks;
   k_mlp_forward<R><<<g, 64, 0, s>>>(net, v, m.nf, row, pprev, scale, guess);
   HIP_TRY(hipGetLastError());
   return 0;

@@ -172,7 +172,7 @@ __global__ void k_permute(int n, const uint32_t* __restrict__ order, PermuteArgs
 // test skip them).
 template <typename R>
 __global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   int c = 0;
   if (i < v.n) {
     const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
@@ -277,7 +277,7 @@ int fill_zero_bytes(hipStream_t s, void* p, size_t bytes) {
 template <typename R>
 int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt) {
   if (v.n <= 0) return 0;
-  k_neighbors<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v, cv, d_maxcnt);
+  k_neighbors<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, cv, d_maxcnt);
   HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -296,7 +296,7 @@ __device__ inline int pick9(const int (&a)[9], int r) {
 }
 
 __global__ void k_rowmask(DevParams P, int n, const int* ptype, const int* btype, const int* itype, int* row) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= n) return;
   int t = ptype[i];
   bool on = itype[i] <= 2 && !(P.open_boundary > 0 && btype[i] > 0) && t < 1000 && t > 0;
@@ -317,7 +317,7 @@ __global__ void k_cell_offsets(int n, const uint32_t* __restrict__ keys, int nc,
 
 template <typename R, int FMT, int MIXED>
 __global__ void k_assemble(DevParams P, SphView<R> v, AsmOut o) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int n = v.n;
   const size_t nn = (size_t)n;
@@ -976,7 +976,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
   w.rowmajor = P.z_index == 0;
   AsmOut o{v.n, P.NI, P.NJ, P.NK, keys, cell_off, w.row, w.ccnt, w.runc, w.idx, w.hi, w.lo, w.c64,
            w.diag, w.dinv, w.rhs, w.flags};
-  const int g = grid_for(v.n);
+  const int g = grid_xcd(v.n);
   k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, w.row);
   int fmt = P.z_index ? kFmtIdx32 : kFmtRun8;
   for (;;) {

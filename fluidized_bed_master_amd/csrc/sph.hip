@@ -18,7 +18,7 @@ namespace {
 // KERNEL_clc_mass_init_sph (functions_MASS.cuh:24-43): rho0 <- rho
 template <typename R>
 __global__ void k_mass_init(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.ptype[i] > 1000 || v.itype[i] > 2) return;
   v.rho0[i] = v.rho[i];
@@ -28,7 +28,7 @@ __global__ void k_mass_init(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int n = v.n;
   if (v.itype[i] > 2 || v.ptype[i] > 1000) {
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_porosity_sph(DevParams P, SphView<R> v, DemView d, CellView dc) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
   if (i >= v.n) return;
   if (v.itype[i] > 2) return;
   v.dpor_prev[i] = v.dpor[i];
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_porosity_sph(DevParams P, SphView<R>
 // KERNEL_clc_prep3D_prep_sph (functions_PREP2.cuh:1623-1679)
 template <typename R>
 __global__ void k_prep_prep(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2 || v.ptype[i] > 1000) return;
   int t = v.ptype[i];
@@ -143,7 +143,7 @@ __global__ void k_prep_prep(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2 || v.ptype[i] > 1000) return;
   const int n = v.n;
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int pti = v.ptype[i];
   if (v.itype[i] > 2) return;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
 template <typename R>
 __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
   if (i >= v.n) return;
   if (v.ptype[i] > 0) return;
   const double hd = v.h[i];
@@ -325,7 +325,7 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<R> v) {
   typedef double T;  // FP64 pair sums: see the note on pressure-gradient cancellation
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int pti = v.ptype[i];
   if (v.itype[i] > 2) return;
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<
 // KERNEL_clc_projection_sph (functions_TIME_ISPH2.cuh:104-212): predictor
 template <typename R>
 __global__ void k_predictor(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2) return;
   const int t = v.ptype[i];
@@ -382,7 +382,7 @@ __global__ void k_predictor(DevParams P, SphView<R> v) {
 // corrector with velocity clamp; positions integrated in FP64
 template <typename R>
 __global__ void k_corrector(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2) return;
   const int t = v.ptype[i];
@@ -417,7 +417,7 @@ __global__ void k_corrector(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int pti = v.ptype[i];
   if (v.itype[i] != 1 || pti > 1000 || pti == 0 || pti == 9) return;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
 
 template <typename R>
 __global__ void k_xsph_calc(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int pti = v.ptype[i];
   if (v.itype[i] != 1 || pti > 1000 || pti == 0 || pti == 9) return;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void k_monitor(int n, const int* ptype, const 
                                                  const U* uz, const double* rho, const U* ft, double* partials) {
   __shared__ double red[3][4];
   double m[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = particle_index(); i < n; i += gridDim.x * blockDim.x) {
     if (ptype[i] == 0 || itype[i] >= 2) continue;
     const double a = (double)ux[i], b = (double)uy[i], c = (double)uz[i];
     m[0] = fmax(m[0], sqrt(a * a + b * b + c * c));
@@ -510,7 +510,7 @@ int monitor_max(hipStream_t s, int n, const int* ptype, const int* itype, const 
                 const double* rho, const U* ft, double* scratch, double* out) {
   out[0] = out[1] = out[2] = 0.0;
   if (n <= 0) return 0;
-  int G = grid_for(n);
+  int G = grid_xcd(n);
   if (G > kMonBlocks) G = kMonBlocks;
   k_monitor<U><<<G, 256, 0, s>>>(n, ptype, itype, ux, uy, uz, rho, ft, scratch);
   k_monitor_fin<<<1, 64, 0, s>>>(scratch, G, scratch + 3 * kMonBlocks);
@@ -527,7 +527,7 @@ template int monitor_max<double>(hipStream_t, int, const int*, const int*, const
 #define LAUNCH(kern, ...)                                           \
   do {                                                              \
     if (v.n > 0) {                                                  \
-      kern<R><<<grid_for(v.n), kBlock, 0, s>>>(__VA_ARGS__);        \
+      kern<R><<<grid_xcd(v.n), kBlock, 0, s>>>(__VA_ARGS__);        \
       HIP_TRY(hipGetLastError());                                   \
     }                                                               \
     return 0;                                                       \
@@ -541,9 +541,9 @@ int sph_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const D
 }
 template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
-    k_prep_prep<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    k_prep_prep<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
-    k_prep<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    k_prep<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
   }
   return 0;
@@ -558,16 +558,16 @@ template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const
 template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_corrector, P, v); }
 template <typename R> int sph_xsph_only(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
-    k_xsph<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    k_xsph<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
   }
   return 0;
 }
 template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
-    k_xsph<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    k_xsph<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
-    k_xsph_calc<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    k_xsph_calc<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
   }
   return 0;

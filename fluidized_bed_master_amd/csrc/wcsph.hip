@@ -18,7 +18,7 @@ constexpr int kMoving = 9;  // MOVING (Parameters.cuh:12)
 // ftotal, drho and dtemp; saves the start-of-step state.
 template <typename R>
 __global__ void k_wc_predictor(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] == 3) return;
   const int t = v.ptype[i];
@@ -59,7 +59,7 @@ __global__ void k_wc_predictor(DevParams P, SphView<R> v) {
 // P3 reset after the predictor (WCSPH_Calc.cuh:59-60)
 template <typename R>
 __global__ void k_wc_reset(SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   const int n = v.n;
   v.fx[i] = 0; v.fy[i] = 0; v.fz[i] = 0; v.fmag[i] = 0;
@@ -73,7 +73,7 @@ __global__ void k_wc_reset(SphView<R> v) {
 // KERNEL_clc_mass_update (functions_MASS.cuh:45-60): m = rho vol.
 template <typename R>
 __global__ void k_volume_update(DevParams P, SphView<R> v, CellView cv, int count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2) return;
   if (P.eulerian && count > 0) return;
@@ -94,7 +94,7 @@ __global__ void k_volume_update(DevParams P, SphView<R> v, CellView cv, int coun
 
 template <typename R>
 __global__ void k_mass_update(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.ptype[i] > 1000 || v.itype[i] > 2) return;
   if (!P.eulerian) return;  // elix == eliy == 1
@@ -108,7 +108,7 @@ __global__ void k_mass_update(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_continuity(DevParams P, SphView<R> v) {
   typedef double T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2 || v.ptype[i] > 1000) return;
   if (P.open_boundary > 0 && v.btype[i] > 0) return;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_continuity(DevParams P, SphView<R> v
 // with B = 1.186 c0^2 / gamma (the reference applies no rho0 factor)
 template <typename R>
 __global__ void k_eos(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2 || v.ptype[i] > 1000 || v.btype[i] != 0) return;
   const double B = P.soundspeed * P.soundspeed * (1.186) / P.gamma;
@@ -171,7 +171,7 @@ __global__ void k_eos(DevParams P, SphView<R> v) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_wc_interaction(DevParams P, SphView<R> v) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] != 1 || v.ptype[i] > 1000) return;
   const int pti = v.ptype[i];
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_wc_interaction(DevParams P, SphView<
 // with conduction off, walls take temp0 + dtemp dt and fluid keeps T.
 template <typename R>
 __global__ void k_wc_corrector(DevParams P, SphView<R> v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] == 3) return;
   const int t = v.ptype[i];
@@ -319,7 +319,7 @@ __global__ void k_wc_corrector(DevParams P, SphView<R> v) {
 #define WC_LAUNCH(kern, ...)                                        \
   do {                                                              \
     if (v.n > 0) {                                                  \
-      kern<R><<<grid_for(v.n), kBlock, 0, s>>>(__VA_ARGS__);        \
+      kern<R><<<grid_xcd(v.n), kBlock, 0, s>>>(__VA_ARGS__);        \
       HIP_TRY(hipGetLastError());                                   \
     }                                                               \
     return 0;                                                       \
@@ -330,8 +330,8 @@ template <typename R> int wc_reset_p3(hipStream_t s, const SphView<R>& v) { WC_L
 template <typename R>
 int wc_volume_mass(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int count) {
   if (v.n <= 0) return 0;
-  k_volume_update<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v, cv, count);
-  k_mass_update<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+  k_volume_update<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, cv, count);
+  k_mass_update<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
   HIP_TRY(hipGetLastError());
   return 0;
 }
