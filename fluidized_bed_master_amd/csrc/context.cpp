@@ -694,21 +694,24 @@ struct Ctx : public sophia_ctx {
           st->guess_used = SOPHIA_GUESS_EXTRAPOLATE;
         }
       }
-      double scale = 0;
+      double scale = 0, rr_base = 0, known_rr = -1.0;
       if (want_mlp) {
-        TRY(mlp_prepare(pbase, &scale));
+        TRY(mlp_prepare(pbase, &scale, &rr_base));
         HIP_TRY(hipMemcpyAsync(mlp.pprev, pbase, (size_t)ns * 8, hipMemcpyDeviceToDevice, stream));
       }
       if (learned) {
         // learned correction with residual-based fallback to the extrapolation
-        TimeScope t(tm(), "mlp_forward");
-        TRY(mlp_forward<R>(stream, P, sv, mlp, ppe_w.row, pbase, scale, mlp.guess));
-        double r_l = 0, r_b = 0;
-        TRY(ppe_residual_of(stream, ppe_w, mlp.guess, &r_l));
-        TRY(ppe_residual_of(stream, ppe_w, pbase, &r_b));
+        // (the extrapolation's residual is the network's input r0)
+        {
+          TimeScope t(tm(), "mlp_forward");
+          TRY(mlp_forward<R>(stream, P, sv, mlp, ppe_w.row, pbase, scale, mlp.guess));
+        }
+        double r_l = 0, rr_l = 0;
+        TRY(ppe_residual_of(stream, ppe_w, mlp.guess, &r_l, &rr_l));
         st->r0_rel_learned = r_l;
-        if (r_l < r_b) {
+        if (rr_l < rr_base) {
           guess = mlp.guess;
+          known_rr = rr_l;  // w.r64 holds its residual
           st->guess_used = SOPHIA_GUESS_LEARNED;
         }
       }
@@ -716,7 +719,7 @@ struct Ctx : public sophia_ctx {
       sophia_ppe_opts oo = *o;
       if (oo.max_iter <= 0) oo.max_iter = 10000;
       if (oo.rtol <= 0) oo.rtol = 1e-8;
-      TRY(ppe_solve(stream, ppe_w, sv.pres, oo, *st, guess, tm()));
+      TRY(ppe_solve(stream, ppe_w, sv.pres, oo, *st, guess, tm(), known_rr));
       if (o->train && mlp.ready) {
         // one Adam step on (features before the solve, converged pressure)
         TimeScope t(tm(), "mlp_train");
@@ -1069,9 +1072,9 @@ struct Ctx : public sophia_ctx {
 
   // --- learned warm start -------------------------------------------------------
   // residual of the base guess and the network's node inputs
-  int mlp_prepare(const double* base, double* scale) {
+  int mlp_prepare(const double* base, double* scale, double* rr_base = nullptr) {
     TRY(ppe_apply64(stream, ppe_w, base, mlp.r0, true, nullptr));
-    return mlp_features<R>(stream, devparams(), sv, mlp, ppe_w.row, ppe_w.diag, scale);
+    return mlp_features<R>(stream, devparams(), sv, mlp, ppe_w.row, ppe_w.diag, scale, rr_base);
   }
   int mlp_check(int n) const {
     if (!ppe_assembled || n != ns) {

@@ -680,14 +680,16 @@ int mlp_reset_adam(hipStream_t s, Mlp& m) {
 
 template <typename R>
 int mlp_features(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, const int* row, const double* diag,
-                 double* scale) {
+                 double* scale, double* rr_base) {
   if (v.n <= 0) {
     *scale = 1;
+    if (rr_base) *rr_base = 0;
     return 0;
   }
   k_feature_sums<<<1, 1024, 0, s>>>(v.n, row, m.r0, diag, m.scal);
   HIP_TRY(hipMemcpyAsync(m.host_scal, m.scal, 3 * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (rr_base) *rr_base = m.host_scal[0];  // sum of r0^2 over the rows
   const double rows = m.host_scal[2] > 0 ? m.host_scal[2] : 1;
   double rms = sqrt(m.host_scal[0] / rows);
   const double dmean = m.host_scal[1] / rows;
@@ -753,7 +755,7 @@ int mlp_gather_previous(hipStream_t s, int n, const int* row, const double* pres
 
 #define INST(R)                                                                                                    \
   template int mlp_features<R>(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*, \
-                               double*);                                                                          \
+                               double*, double*);                                                                 \
   template int mlp_forward<R>(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*,  \
                               double, double*);                                                                   \
   template int mlp_backward<R>(hipStream_t, const DevParams&, const SphView<R>&, Mlp&, const int*, const double*, \

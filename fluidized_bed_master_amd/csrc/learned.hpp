@@ -49,7 +49,7 @@ int mlp_reset_adam(hipStream_t s, Mlp& m);
 // output scale is returned in *scale (synchronises)
 template <typename R>
 int mlp_features(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, const int* row, const double* diag,
-                 double* scale);
+                 double* scale, double* rr_base = nullptr);
 // guess_i = pprev_i + scale * y_i on rows, pprev_i elsewhere
 template <typename R>
 int mlp_forward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, const int* row, const double* pprev,

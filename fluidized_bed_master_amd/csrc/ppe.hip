@@ -316,7 +316,7 @@ __global__ void k_cell_offsets(int n, const uint32_t* __restrict__ keys, int nc,
 }
 
 template <typename R, int FMT, int MIXED>
-__global__ void k_assemble(DevParams P, SphView<R> v, AsmOut o) {
+__global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, AsmOut o) {
   int i = particle_index();
   if (i >= v.n) return;
   const int n = v.n;
@@ -360,26 +360,54 @@ __global__ void k_assemble(DevParams P, SphView<R> v, AsmOut o) {
   uint64_t rcw = 0;
   uint32_t word = 0;
   unsigned bad = 0;
-  for (int k = 0; k < cnt; k++) {
-    uint32_t e = v.nbr[(size_t)k * n + i];
-    int j = (int)(e & ~kNbrFlag);
+  // the list in batches of four: every field of the batch is gathered
+  // before any is used (straight-line loads, all in flight together), the
+  // next batch's indices are loaded, then the four entries run in list order
+  struct AsmNb {
+    double x, y, z, m, rho, por, pres;
+    R ux, uy, uz;
+    int pt, it, row;
+  };
+  uint32_t en[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) en[u] = cnt > 0 ? v.nbr[(size_t)min(u, cnt - 1) * n + i] : 0u;
+  for (int k0 = 0; k0 < cnt; k0 += 4) {
+    uint32_t eb[4];
+    AsmNb nb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      eb[u] = en[u];
+      const int j = (int)(eb[u] & ~kNbrFlag);
+      nb[u].x = v.x[j]; nb[u].y = v.y[j]; nb[u].z = v.z[j];
+      nb[u].m = v.m[j]; nb[u].rho = v.rho[j]; nb[u].por = v.por[j]; nb[u].pres = v.pres[j];
+      nb[u].ux = v.ux[j]; nb[u].uy = v.uy[j]; nb[u].uz = v.uz[j];
+      nb[u].pt = v.ptype[j]; nb[u].it = v.itype[j]; nb[u].row = o.row[j];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(k0 + 4 + u, cnt - 1) * n + i];
+#pragma unroll
+    for (int ub = 0; ub < 4; ub++) {
+    if (k0 + ub >= cnt) break;
+    const uint32_t e = eb[ub];
+    const int j = (int)(e & ~kNbrFlag);
+    const AsmNb& nq = nb[ub];
     if (e & kNbrFlag) continue;
-    if (!(v.ptype[j] < 1000 && v.itype[j] != 4)) continue;
-    double dx = xi - v.x[j], dy = yi - v.y[j], dz = zi - v.z[j];
+    if (!(nq.pt < 1000 && nq.it != 4)) continue;
+    double dx = xi - nq.x, dy = yi - nq.y, dz = zi - nq.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz) + 1e-20;
     double dw = knl_dw<double>(P.kernel_type, A, hd, r);
     double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
     kgc_apply(P, v.cm, n, i, gx, gy, gz);
-    double mj = v.m[j], rhoj = v.rho[j], porj = v.por[j];
-    bx += mj / rhoj * (pori * uxi - porj * (double)v.ux[j]) * gx;
-    by += mj / rhoj * (pori * uyi - porj * (double)v.uy[j]) * gy;
-    bz += mj / rhoj * (pori * uzi - porj * (double)v.uz[j]) * gz;
+    double mj = nq.m, rhoj = nq.rho, porj = nq.por;
+    bx += mj / rhoj * (pori * uxi - porj * (double)nq.ux) * gx;
+    by += mj / rhoj * (pori * uyi - porj * (double)nq.uy) * gy;
+    bz += mj / rhoj * (pori * uzi - porj * (double)nq.uz) * gz;
     double c = -(dx * gx + dy * gy + dz * gz) / r / r;
     double a = mj * ((pori * rhoi + porj * rhoj) / (rhoj * rhoi)) * c;
     Aij += a;
-    AP += a * v.pres[j];
-    if (!o.row[j]) {
-      lag += a * v.pres[j];  // non-row column: lives in the right-hand side
+    AP += a * nq.pres;
+    if (!nq.row) {
+      lag += a * nq.pres;  // non-row column: lives in the right-hand side
       continue;
     }
     if (a == 0.0) continue;
@@ -411,6 +439,7 @@ __global__ void k_assemble(DevParams P, SphView<R> v, AsmOut o) {
       o.c64[slot] = a;
     }
     kc++;
+    }
   }
   // pad the last quad so whole-quad loads read defined values
   if (kc & 3) {
@@ -1057,11 +1086,12 @@ int ppe_extrapolate(hipStream_t s, const PpeWork& w, const double* p, const doub
   return 0;
 }
 
-int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel) {
+int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel, double* rr_out) {
   double bn = 0, rr = 0;
   TRY(ppe_rhs_norm(s, w, &bn));
   TRY(ppe_apply64(s, w, x, w.r64, true, &rr));
   *rel = sqrt(rr) / (bn == 0 ? 1 : bn);
+  if (rr_out) *rr_out = rr;
   return 0;
 }
 
@@ -1155,7 +1185,7 @@ int bicgstab_dispatch(hipStream_t s, PpeWork& w, const CoarseWork* cw, double rn
 }  // namespace
 
 int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o, sophia_ppe_stats& st,
-              const double* guess_or_null, Timers* tm) {
+              const double* guess_or_null, Timers* tm, double known_rr) {
   const int n = w.n;
   if (n <= 0) return 0;
   const int g = grid_for(n);
@@ -1167,8 +1197,8 @@ int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o,
   double bn = 0;
   TRY(ppe_rhs_norm(s, w, &bn));
   if (bn == 0) bn = 1;
-  double rr = 0;
-  TRY(ppe_apply64(s, w, w.x64, w.r64, true, &rr));
+  double rr = known_rr;
+  if (!(guess_or_null && known_rr >= 0)) TRY(ppe_apply64(s, w, w.x64, w.r64, true, &rr));
   st.r0_rel = sqrt(rr) / bn;
   const double tol = o.rtol * bn;
   // reliable-update threshold: FP32 inner iterations fold into the FP64

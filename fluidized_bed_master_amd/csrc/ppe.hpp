@@ -79,9 +79,12 @@ int ppe_jacobi(hipStream_t s, const SphView<R>& v, PpeWork& w);
 // y = A x with the full-precision operator (FP64, or hi + lo); residual form
 // y = rhs - A x when residual; ||y||^2 over rows into *norm2 (synchronises)
 int ppe_apply64(hipStream_t s, PpeWork& w, const double* x, double* y, bool residual, double* norm2);
+// known_rr >= 0: w.r64 already holds b - A guess over the rows, with this
+// squared norm (the caller has just evaluated the guess)
 int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o, sophia_ppe_stats& st,
-              const double* guess_or_null, Timers* tm);
-int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel);
+              const double* guess_or_null, Timers* tm, double known_rr = -1.0);
+// ||b - A x|| / ||b|| into *rel; leaves b - A x in w.r64 and its squared norm in *rr_out
+int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel, double* rr_out = nullptr);
 // out = 3 p - 3 h1 + h2 (depth 2), 2 p - h1 (depth 1) or p on rows; p elsewhere
 int ppe_extrapolate(hipStream_t s, const PpeWork& w, const double* p, const double* h1, const double* h2, int depth,
                     double* out);
