@@ -170,6 +170,7 @@ SIGNATURES = {
     "sophia_monitors": (C.c_int, [_ctx, C.c_int32, _VP]),
     "sophia_wcsph_stage": (C.c_int, [_ctx, C.c_int32]),
     "sophia_ppe_layout": (C.c_int, [_ctx, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "sophia_ppe_coarse": (C.c_int, [_ctx, _P(C.c_int32), _VP, _VP]),
     "sophia_mlp_param_count": (C.c_int, [_P(C.c_int32)]),
     "sophia_mlp_set_params": (C.c_int, [_ctx, _VP, C.c_int32]),
     "sophia_mlp_get_params": (C.c_int, [_ctx, _VP, C.c_int32]),

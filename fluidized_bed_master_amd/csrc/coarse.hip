@@ -472,6 +472,14 @@ int coarse_apply(hipStream_t s, const CoarseWork& c, const PpeWork& w, const V* 
   return 0;
 }
 
+int coarse_export(hipStream_t s, const CoarseWork& c, float* E, float* X) {
+  const size_t row = (size_t)c.na * sizeof(float), pitch = (size_t)c.ld * sizeof(float);
+  if (E) HIP_TRY(hipMemcpy2DAsync(E, row, c.E, pitch, row, (size_t)c.na, hipMemcpyDeviceToHost, s));
+  if (X) HIP_TRY(hipMemcpy2DAsync(X, row, c.X, pitch, row, (size_t)c.na, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
 void coarse_free(CoarseWork& c) {
   auto F = [](void* p) {
     if (p) (void)hipFree(p);

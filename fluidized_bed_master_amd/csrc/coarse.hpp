@@ -48,5 +48,7 @@ template <typename V>
 int coarse_apply(hipStream_t s, const CoarseWork& c, const PpeWork& w, const V* v, V* out, const int* done,
                  const int* brk);
 void coarse_free(CoarseWork& c);
+// copy E and X (row-major na x na, FP32) to the host where non-null (synchronises)
+int coarse_export(hipStream_t s, const CoarseWork& c, float* E, float* X);
 
 }  // namespace sophia

@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "learned.hpp"
 #include "ppe.hpp"
+#include "coarse.hpp"
 #include "timers.hpp"
 
 static_assert(sizeof(part1) == 1272, "part1 layout must match class_Cuda_Particle_Array2.cuh:11-85");
@@ -121,6 +122,7 @@ struct sophia_ctx {
   virtual int get_ppe_system(double* diag, double* rhs, int32_t* row) const = 0;
   virtual int ppe_apply(const double* x, double* y, int n) = 0;
   virtual int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const = 0;
+  virtual int ppe_coarse(int32_t* dims, float* E, float* X) const = 0;
   virtual int monitors(int species, double* out) = 0;
   virtual int wcsph_stage(int stage) = 0;
   virtual int mlp_predict(double* out, int n) = 0;
@@ -1032,6 +1034,19 @@ struct Ctx : public sophia_ctx {
     return monitor_max<double>(stream, nd, dv.ptype, dv.itype, dv.ux, dv.uy, dv.uz, dv.rho, nullptr, mon_scratch,
                                out);
   }
+  int ppe_coarse(int32_t* dims, float* E, float* X) const override {
+    const CoarseWork* c = ppe_w.coarse;
+    if (!c || c->na == 0 || !c->valid) {
+      set_error("no two-level coarse operator: run a solve with precond = SOPHIA_PRECOND_TWO_LEVEL first");
+      return SOPHIA_E_STATE;
+    }
+    if (dims) {
+      dims[0] = c->AX;
+      dims[1] = c->AY;
+      dims[2] = c->AZ;
+    }
+    return coarse_export(stream, *c, E, X);
+  }
   int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const override {
     if (!ppe_assembled) {
       set_error("PPE not assembled yet");
@@ -1603,6 +1618,10 @@ int sophia_monitors(sophia_ctx* c, int32_t species, double* out) {
 int sophia_ppe_layout(const sophia_ctx* c, int64_t* nnz, int32_t* ib, int32_t* cb) {
   if (!c) return SOPHIA_E_ARG;
   return c->ppe_layout(nnz, ib, cb);
+}
+int sophia_ppe_coarse(const sophia_ctx* c, int32_t dims[3], float* E, float* X) {
+  if (!c || !dims) return SOPHIA_E_ARG;
+  return c->ppe_coarse(dims, E, X);
 }
 
 int sophia_knl_eval(int32_t kt, int32_t dim, double h, const double* r, int32_t n, double* w, double* dw) {

@@ -180,6 +180,17 @@ class Sophia:
         check(self.lib.sophia_monitors(self._ctx, species, ptr(out)), "sophia_monitors")
         return out
 
+    def ppe_coarse(self):
+        """(dims, E, X) of the last two-level solve: aggregates per axis and the
+        row-major FP32 coarse operator and its approximate inverse."""
+        dims = (C.c_int32 * 3)()
+        check(self.lib.sophia_ppe_coarse(self._ctx, dims, None, None), "sophia_ppe_coarse")
+        na = dims[0] * dims[1] * dims[2]
+        E = np.zeros((na, na), np.float32)
+        X = np.zeros((na, na), np.float32)
+        check(self.lib.sophia_ppe_coarse(self._ctx, dims, ptr(E), ptr(X)), "sophia_ppe_coarse")
+        return tuple(dims), E, X
+
     def ppe_layout(self) -> dict:
         nnz = C.c_int64()
         ib = C.c_int32()

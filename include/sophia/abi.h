@@ -293,6 +293,11 @@ int sophia_ppe_apply(sophia_ctx* ctx, const double* x, double* y, int32_t n);
  * (1: u8 offset inside a cell run, 4: u32 index) and per coefficient as read
  * by the inner iterations (4: FP32, 8: FP64).  Any may be NULL. */
 int sophia_ppe_layout(const sophia_ctx* ctx, int64_t* nnz, int32_t* index_bytes, int32_t* coef_bytes);
+/* Coarse operator of the last two-level solve (no reference counterpart;
+ * inspection): aggregates per axis into dims[3] (n_coarse = product), and,
+ * where non-NULL, E = Z^T A Z and its approximate inverse X, row-major
+ * n_coarse x n_coarse FP32 (aggregates without rows have E_II = 1). */
+int sophia_ppe_coarse(const sophia_ctx* ctx, int32_t dims[3], float* E, float* X);
 
 /* --- learned PPE warm start (message-passing MLP, BF16 MFMA) ------------- */
 /* The network predicts the pressure change of the PPE rows from the
