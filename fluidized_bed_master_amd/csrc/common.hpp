@@ -86,6 +86,7 @@ struct SphView {
   R *drho;                  // WCSPH continuity rate
   double *ppe1, *ppe2, *ppe3, *ppe4;
   R *xsx, *xsy, *xsz, *xst; // XSPH results
+  float4* pos4;      // FP32 copy of x, y, z (neighbour pre-filter), refreshed by NNPS
   // SPH-SPH neighbour list (slot major: nbr[k * n + i])
   uint32_t* nbr;
   int* nbr_cnt;

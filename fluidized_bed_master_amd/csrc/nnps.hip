@@ -171,21 +171,42 @@ __global__ void k_permute(int n, const uint32_t* __restrict__ order, PermuteArgs
 // sqrt(r2)+1e-20 < R carry kNbrFlag (those stages that use the +1e-20
 // test skip them).
 template <typename R>
-__global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt) {
+__global__ void k_pos4(SphView<R> v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  v.pos4[i] = make_float4((float)v.x[i], (float)v.y[i], (float)v.z[i], 0.f);
+}
+
+// Candidates of the 27 cells are tested on the FP32 copy first: farther than
+// Rs + tol or nearer than Rs - tol is decided there (one 16 B load); only the
+// band |r - Rs| <= tol takes the exact FP64 test, so the list, its order and
+// the r + 1e-20 < Rs flag are exactly those of the FP64 test.  tol bounds
+// the FP32 distance error: 2^-21 max|coordinate| + 2^-20 Rs, at least twice
+// the rounding of the coordinates, their differences and r^2.
+template <typename R>
+__global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt, double tol_abs) {
   int i = particle_index();
   int c = 0;
   if (i < v.n) {
     const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
     const double Rs = P.kappa * v.h[i];
+    const double tol = tol_abs + Rs * (1.0 / 1048576.0);
+    const float lo2 = (float)((Rs - tol) * (Rs - tol)), hi2 = (float)((Rs + tol) * (Rs + tol));
+    const float xf = (float)xi, yf = (float)yi, zf = (float)zi;
     const int n = v.n, cap = v.nbr_cap;
     for_cells(P, xi, yi, zi, cv, [&](int j) {
-      double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]);
-      if (r < Rs) {
-        uint32_t e = (uint32_t)j;
+      const float4 pj = v.pos4[j];
+      const float dx = xf - pj.x, dy = yf - pj.y, dz = zf - pj.z;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 > hi2) return;
+      uint32_t e = (uint32_t)j;
+      if (!(d2 < lo2)) {
+        const double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]);
+        if (!(r < Rs)) return;
         if (!(r + 1e-20 < Rs)) e |= kNbrFlag;
-        if (c < cap) v.nbr[(size_t)c * n + i] = e;
-        c++;
       }
+      if (c < cap) v.nbr[(size_t)c * n + i] = e;
+      c++;
     });
     v.nbr_cnt[i] = c;
   }
@@ -277,7 +298,10 @@ int fill_zero_bytes(hipStream_t s, void* p, size_t bytes) {
 template <typename R>
 int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt) {
   if (v.n <= 0) return 0;
-  k_neighbors<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, cv, d_maxcnt);
+  k_pos4<R><<<grid_for(v.n), kBlock, 0, s>>>(v);
+  const double m = fmax(fmax(fmax(fabs(P.x_min), fabs(P.x_max)), fmax(fabs(P.y_min), fabs(P.y_max))),
+                        fmax(fabs(P.z_min), fabs(P.z_max)));
+  k_neighbors<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, cv, d_maxcnt, m * (1.0 / 2097152.0));
   HIP_TRY(hipGetLastError());
   return 0;
 }
