@@ -395,6 +395,9 @@ int fill_zero_bytes(hipStream_t s, void* p, size_t bytes);
 
 template <typename R>
 int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt);
+// pos4 = FP32 copy of the current positions
+template <typename R>
+int refresh_pos4(hipStream_t s, const SphView<R>& v);
 
 // SPH stages
 template <typename R> int sph_mass_init(hipStream_t s, const DevParams& P, const SphView<R>& v);

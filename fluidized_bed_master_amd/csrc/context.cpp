@@ -716,6 +716,10 @@ struct Ctx : public sophia_ctx {
           guess = mlp.guess;
           known_rr = rr_l;  // w.r64 holds its residual
           st->guess_used = SOPHIA_GUESS_LEARNED;
+        } else {
+          // back to the extrapolation, whose residual is the network's input
+          HIP_TRY(hipMemcpyAsync(ppe_w.r64, mlp.r0, (size_t)ns * 8, hipMemcpyDeviceToDevice, stream));
+          known_rr = rr_base;
         }
       }
       TRY(rotate_pressure_history());

@@ -1,11 +1,15 @@
 // Learned PPE warm start on gfx950 MFMA.
 //
 // Network: edge MLP e_ij -> 64 -> 64 (ReLU), sum over the neighbour list
-// scaled by 1/64, node MLP 64 -> 64 (ReLU) -> 1.  One wave64 owns one
-// particle at a time (grid-stride over particles) and walks its neighbour
-// list in tiles of 16 edges.  The forward kernel computes each layer
-// transposed (weights as the A operand, see k_mlp_forward); the backward
-// kernel uses the layout below:
+// scaled by 1/64, node MLP 64 -> 64 (ReLU) -> 1.
+//
+// The forward kernel (k_mlp_forward) gives each wave 16 consecutive
+// particles and walks their neighbour lists slot by slot, with every layer
+// computed transposed so that the 16 MFMA columns are the 16 particles; the
+// per-particle sums then stay in registers (layout at the kernel).
+//
+// The backward kernel owns one particle per wave64 (grid-stride over
+// particles) and walks its neighbour list in tiles of 16 edges:
 //   layer 1  D[16 x 64] = E[16 x 32] W1^T[32 x 64]   4 x v_mfma_f32_16x16x32_bf16
 //            (the 6 edge inputs padded to K = 32 with zeros)
 //   layer 2  D[16 x 64] = H1[16 x 64] W2^T[64 x 64]  8 x v_mfma_f32_16x16x32_bf16
@@ -33,8 +37,6 @@ namespace {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-
-constexpr int kFwdBlocks = 4096;
 
 // Particles of this block: eight contiguous chunks, one per XCD (workgroups
 // are dealt to XCDs round robin), so the particles an XCD works on at one
@@ -209,7 +211,8 @@ __device__ inline void aggregate(const Net& net, const SphView<R>& v, const floa
 // the 16 lanes of each group.  Biases are added in FP32 after each MFMA, as
 // in the backward kernel's recomputation.
 struct FwdOperands {
-  bf8 A1[4];      // W1 rows 16 nb + (l mod 16), inputs 0..5 (lanes 0-15)
+  bf4 A1[4];      // W1 rows 16 nb + (l mod 16), inputs 4 (l / 16) .. +3 (16x16x16 layout): K = 0..5 the
+                  // edge inputs, K = 6, 7 the BF16 high and low parts of b1 (inputs 6, 7 are 1), rest 0
   bf8 A2[4][2];   // W2 rows 16 mb + (l mod 16), hidden-1 units in the layer-1 result order
 };
 
@@ -218,8 +221,13 @@ __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
 #pragma unroll
   for (int nb = 0; nb < 4; nb++) {
 #pragma unroll
-    for (int j = 0; j < 8; j++)
-      w.A1[nb][j] = (g == 0 && j < kEin) ? net.wbf[OFF_W1 + (16 * nb + m) * kEin + j] : (__bf16)0.0f;
+    for (int j = 0; j < 4; j++) {
+      const int k = 4 * g + j;
+      const float b1 = net.master[OFF_B1 + 16 * nb + m];
+      const __bf16 b1h = (__bf16)b1;
+      w.A1[nb][j] = k < kEin ? net.wbf[OFF_W1 + (16 * nb + m) * kEin + k]
+                  : k == kEin ? b1h : k == kEin + 1 ? (__bf16)(b1 - (float)b1h) : (__bf16)0.0f;
+    }
   }
 #pragma unroll
   for (int mb = 0; mb < 4; mb++)
@@ -232,124 +240,187 @@ __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
       }
 }
 
-// biases in LDS (bs[0..63] = b1, bs[64..127] = b2), read as one float4 per
-// block of four units: registers go to the weight operands, for occupancy
-template <typename R>
-__device__ inline void aggregate_fwd(const SphView<R>& v, const float2* nf, int i, int lane, const FwdOperands& w,
-                                     const float* bs, __bf16 (*Et)[8], float* Ms) {
-  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
-  const int m = lane & 15;
-  const double xi = v.x[i], yi = v.y[i], zi = v.z[i], ih = 1.0 / v.h[i];
-  float acc[4][4] = {};
-  for (int c0 = 0; c0 < cnt; c0 += 64) {
-    const int e = c0 + lane;
-    float ein[kEin] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    bool ok = false;
-    if (e < cnt) {
-      const uint32_t ent = v.nbr[(size_t)e * v.n + i];
-      if (!(ent & kNbrFlag)) {
-        const int j = (int)ent;
-        const double dx = (xi - v.x[j]) * ih, dy = (yi - v.y[j]) * ih, dz = (zi - v.z[j]) * ih;
-        const float2 f = nf[j];
-        ein[0] = (float)dx;
-        ein[1] = (float)dy;
-        ein[2] = (float)dz;
-        ein[3] = (float)sqrt(dx * dx + dy * dy + dz * dz);
-        ein[4] = f.x;
-        ein[5] = f.y;
-        ok = true;
-      }
-    }
-    const uint64_t valid = __ballot(ok);
-    __syncthreads();  // the previous chunk's tiles are done with Et
-#pragma unroll
-    for (int q = 0; q < kEin; q++) Et[lane][q] = (__bf16)ein[q];
-    Et[lane][6] = (__bf16)0.0f;
-    Et[lane][7] = (__bf16)0.0f;
-    __syncthreads();
-    for (int t = 0; t < 4 && c0 + 16 * t < cnt; t++) {
-      bf8 b;
-#pragma unroll
-      for (int q = 0; q < 8; q++) b[q] = lane < 16 ? Et[16 * t + m][q] : (__bf16)0.0f;
-      // layer 1: 16 hidden x 16 edges per block nb
-      float h1[4][4];
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) {
-        f4 z = {0.f, 0.f, 0.f, 0.f};
-        z = mfma32(w.A1[nb], b, z);
-        const float4 bb = *reinterpret_cast<const float4*>(bs + 16 * nb + 4 * (lane >> 4));
-        h1[nb][0] = fmaxf(z[0] + bb.x, 0.f);
-        h1[nb][1] = fmaxf(z[1] + bb.y, 0.f);
-        h1[nb][2] = fmaxf(z[2] + bb.z, 0.f);
-        h1[nb][3] = fmaxf(z[3] + bb.w, 0.f);
-      }
-      bf8 B2[2];
-#pragma unroll
-      for (int sk = 0; sk < 2; sk++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) B2[sk][j] = (__bf16)h1[2 * sk + j / 4][j & 3];
-      // layer 2 and the masked sum over this tile's edges (edge = lane mod 16)
-      const bool on = (valid >> (16 * t + m)) & 1ull;
-#pragma unroll
-      for (int mb = 0; mb < 4; mb++) {
-        f4 z = {0.f, 0.f, 0.f, 0.f};
-        z = mfma32(w.A2[mb][0], B2[0], z);
-        z = mfma32(w.A2[mb][1], B2[1], z);
-        const float4 bb = *reinterpret_cast<const float4*>(bs + kH + 16 * mb + 4 * (lane >> 4));
-        acc[mb][0] += on ? fmaxf(z[0] + bb.x, 0.f) : 0.f;
-        acc[mb][1] += on ? fmaxf(z[1] + bb.y, 0.f) : 0.f;
-        acc[mb][2] += on ? fmaxf(z[2] + bb.z, 0.f) : 0.f;
-        acc[mb][3] += on ? fmaxf(z[3] + bb.w, 0.f) : 0.f;
-      }
-    }
-  }
-  // sum over the 16 edge lanes of each group: hidden-2 unit 16 mb + 4 (l / 16) + r
-#pragma unroll
-  for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      float a = acc[mb][r];
-      a += __shfl_xor(a, 1);
-      a += __shfl_xor(a, 2);
-      a += __shfl_xor(a, 4);
-      a += __shfl_xor(a, 8);
-      acc[mb][r] = a;
-    }
-  if (m == 0) {
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) Ms[16 * mb + 4 * (lane >> 4) + r] = acc[mb][r] * (1.0f / 64.0f);
-  }
-  __syncthreads();
+// Lanes of one wave exchange the edge inputs through LDS.  A wave's LDS
+// instructions execute in issue order, so ordering the compiler's code
+// motion with wavefront-scope fences is all the synchronisation needed.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+constexpr int kFwdWaves = 4;    // waves per block of the forward kernel
+constexpr int kFwdGrid = 1024;  // blocks at most: 4 waves per SIMD on 256 CUs
+
+// Groups of 16 particles of this wave: eight contiguous chunks of groups,
+// one per XCD (blocks are dealt to XCDs round robin), so the groups an XCD
+// works on at one time are consecutive and share neighbour fields in that
+// XCD's L2.  gridDim.x is a multiple of 8.
+struct GroupIter {
+  int g, end, stride;
+};
+__device__ inline GroupIter xcd_groups(int ng) {
+  const int x = blockIdx.x & 7, per = gridDim.x >> 3;
+  const int chunk = (ng + 7) >> 3;
+  const int first = x * chunk + (blockIdx.x >> 3) * kFwdWaves + (int)(threadIdx.x >> 6);
+  return GroupIter{first, min(ng, (x + 1) * chunk), per * kFwdWaves};
+}
+
+// Forward kernel.  One wave owns a group of 16 consecutive particles and
+// walks their neighbour lists slot by slot, four slots per gather round
+// (lane l gathers the edge of particle l mod 16 at slot k0 + l / 16).  Each
+// layer is computed transposed, Z^T[hidden x 16] = W[hidden x K] E^T[K x 16],
+// with the 16 MFMA columns being the group's 16 particles at one slot:
+//   - a layer's result (lane l: hidden units 16 nb + 4 (l / 16) + r of
+//     particle l mod 16) is the next layer's B operand once the hidden
+//     dimension is enumerated in that order (K-step s of the next layer
+//     takes, in lane l, units 16 (2 s + j / 4) + 4 (l / 16) + j mod 4,
+//     j = 0..7; the A operand holds the weights in the same order);
+//   - the ReLU messages are summed over the slots in registers, so the
+//     aggregated message of particle l mod 16 ends up spread over the four
+//     lanes l mod 16 + 16 q with no cross-lane reduction, already in the
+//     node layer's B-operand order.
+// The node layer runs on MFMA as well: the FP32 message is split into BF16
+// high and low parts (hi + lo equals it to 2^-16 relative), so the result
+// matches an FP32 message against the BF16 weights W3.  b1 enters layer 1
+// as two extra inputs equal to 1 against its BF16 high and low parts; b2 and
+// b3 are added in FP32 after the MFMA.  Edge inputs use the FP32 positions
+// (pos4).
 template <typename R>
-__global__ __launch_bounds__(64, 4) void k_mlp_forward(Net net, SphView<R> v, const float2* __restrict__ nf,
-                                                     const int* __restrict__ row, const double* __restrict__ pprev,
-                                                     double scale, double* __restrict__ guess) {
-  __shared__ __bf16 Et[64][8];
-  __shared__ float Ms[kH];
-  __shared__ __bf16 W3s[kH * kH];
-  __shared__ __attribute__((aligned(16))) float bs[2 * kH];
-  const int lane = threadIdx.x;
-  for (int q = lane; q < kH * kH; q += 64) W3s[q] = net.wbf[OFF_W3 + q];
-  bs[lane] = net.master[OFF_B1 + lane];
-  bs[kH + lane] = net.master[OFF_B2 + lane];
+__global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphView<R> v,
+                                                                   const float2* __restrict__ nf,
+                                                                   const int* __restrict__ row,
+                                                                   const double* __restrict__ pprev, double scale,
+                                                                   double* __restrict__ guess) {
+  __shared__ bf8 A3s[4][2][64];                         // W3 as the node layer's A operand, per lane
+  __shared__ __attribute__((aligned(16))) float bs[3 * kH];  // b2, b3, W4 (W4 from its BF16 copy)
+  __shared__ bf8 Et[kFwdWaves][64];                     // edge inputs of one gather round, per wave
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int m = lane & 15, g4 = lane >> 4;
+  for (int q = tid; q < 4 * 2 * 64; q += blockDim.x) {
+    const int mb = q >> 7, sk = (q >> 6) & 1, l = q & 63;
+    bf8 a;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      a[j] = net.wbf[OFF_W3 + (16 * mb + (l & 15)) * kH + 16 * (2 * sk + j / 4) + 4 * (l >> 4) + (j & 3)];
+    A3s[mb][sk][l] = a;
+  }
+  if (tid < kH) {
+    bs[tid] = net.master[OFF_B2 + tid];
+    bs[kH + tid] = net.master[OFF_B3 + tid];
+    bs[2 * kH + tid] = (float)net.wbf[OFF_W4 + tid];
+  }
   FwdOperands w;
   load_fwd_t(net, lane, w);
   __syncthreads();
-  for (PartIter it = xcd_particles(v.n); it.i < it.end; it.i += it.stride) {
-    const int i = it.i;
-    if (!row[i]) {
-      if (lane == 0) guess[i] = pprev[i];
-      continue;
+  bf8* E = Et[tid >> 6];
+  const bf4* E4 = reinterpret_cast<const bf4*>(E);  // inputs 0-3 and 4-7 of each edge
+  const float b4 = net.master[OFF_B4];
+  const int ng = (v.n + 15) >> 4;
+  for (GroupIter it = xcd_groups(ng); it.g < it.end; it.g += it.stride) {
+    const int i = it.g * 16 + m;
+    const bool live = i < v.n;
+    const bool isrow = live && row[i] != 0;
+    const int cnt = isrow ? min(v.nbr_cnt[i], v.nbr_cap) : 0;
+    int maxc = cnt;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) maxc = max(maxc, __shfl_xor(maxc, off));
+    float y = 0.f;
+    if (__ballot(isrow) != 0) {  // wave-uniform
+      const float4 pi = isrow ? v.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float ih = isrow ? (float)(1.0 / (double)v.h[i]) : 0.f;
+      float acc[4][4] = {};
+      uint32_t nxt = g4 < cnt ? v.nbr[(size_t)g4 * v.n + i] : kNbrFlag;
+      for (int k0 = 0; k0 < maxc; k0 += 4) {
+        const uint32_t ent = nxt;
+        const int kn = k0 + 4 + g4;
+        if (k0 + 4 < maxc) nxt = kn < cnt ? v.nbr[(size_t)kn * v.n + i] : kNbrFlag;
+        const bool ok = !(ent & kNbrFlag);
+        bf8 e8;
+#pragma unroll
+        for (int q = 0; q < 6; q++) e8[q] = (__bf16)0.0f;
+        e8[6] = (__bf16)1.0f;  // bias inputs
+        e8[7] = (__bf16)1.0f;
+        if (ok) {
+          const int j = (int)ent;
+          const float4 pj = v.pos4[j];
+          const float2 f = nf[j];
+          const float dx = (pi.x - pj.x) * ih, dy = (pi.y - pj.y) * ih, dz = (pi.z - pj.z) * ih;
+          e8[0] = (__bf16)dx;
+          e8[1] = (__bf16)dy;
+          e8[2] = (__bf16)dz;
+          e8[3] = (__bf16)sqrtf(dx * dx + dy * dy + dz * dz);
+          e8[4] = (__bf16)f.x;
+          e8[5] = (__bf16)f.y;
+        }
+        const uint64_t valid = __ballot(ok);
+        wave_lds_sync();  // the previous round's reads of E are issued
+        E[lane] = e8;
+        wave_lds_sync();
+        for (int t = 0; t < 4 && k0 + t < maxc; t++) {
+          bf4 b;
+#pragma unroll
+          for (int q = 0; q < 4; q++) b[q] = (__bf16)0.0f;
+          if (g4 < 2) b = E4[2 * (16 * t + m) + g4];
+          // layer 1 (16 hidden x 16 particles per block nb, K = 16, bias
+          // included), ReLU, packed straight into the layer-2 B operand
+          bf8 B2[2];
+#pragma unroll
+          for (int nb = 0; nb < 4; nb++) {
+            f4 z = {0.f, 0.f, 0.f, 0.f};
+            z = mfma16(w.A1[nb], b, z);
+            B2[nb >> 1][4 * (nb & 1) + 0] = (__bf16)fmaxf(z[0], 0.f);
+            B2[nb >> 1][4 * (nb & 1) + 1] = (__bf16)fmaxf(z[1], 0.f);
+            B2[nb >> 1][4 * (nb & 1) + 2] = (__bf16)fmaxf(z[2], 0.f);
+            B2[nb >> 1][4 * (nb & 1) + 3] = (__bf16)fmaxf(z[3], 0.f);
+          }
+          // layer 2 and the masked sum over this slot's edges; b2 is read
+          // from LDS here rather than held in registers
+          const bool on = (valid >> (16 * t + m)) & 1ull;
+          asm volatile("" ::: "memory");
+#pragma unroll
+          for (int mb = 0; mb < 4; mb++) {
+            f4 z = {0.f, 0.f, 0.f, 0.f};
+            z = mfma32(w.A2[mb][0], B2[0], z);
+            z = mfma32(w.A2[mb][1], B2[1], z);
+            const float4 bb = *reinterpret_cast<const float4*>(bs + 16 * mb + 4 * g4);
+            acc[mb][0] += on ? fmaxf(z[0] + bb.x, 0.f) : 0.f;
+            acc[mb][1] += on ? fmaxf(z[1] + bb.y, 0.f) : 0.f;
+            acc[mb][2] += on ? fmaxf(z[2] + bb.z, 0.f) : 0.f;
+            acc[mb][3] += on ? fmaxf(z[3] + bb.w, 0.f) : 0.f;
+          }
+        }
+      }
+      // node layer: message M = acc / 64 as BF16 hi + lo parts
+      bf8 Bh[2], Bl[2];
+#pragma unroll
+      for (int sk = 0; sk < 2; sk++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const float mv = acc[2 * sk + j / 4][j & 3] * (1.0f / 64.0f);
+          const __bf16 hb = (__bf16)mv;
+          Bh[sk][j] = hb;
+          Bl[sk][j] = (__bf16)(mv - (float)hb);
+        }
+      float yp = 0.f;
+#pragma unroll
+      for (int mb = 0; mb < 4; mb++) {
+        const bf8 a0 = A3s[mb][0][lane], a1 = A3s[mb][1][lane];
+        f4 z = {0.f, 0.f, 0.f, 0.f};
+        z = mfma32(a0, Bl[0], z);
+        z = mfma32(a1, Bl[1], z);
+        z = mfma32(a0, Bh[0], z);
+        z = mfma32(a1, Bh[1], z);
+        const float4 bb = *reinterpret_cast<const float4*>(bs + kH + 16 * mb + 4 * g4);
+        const float4 ww = *reinterpret_cast<const float4*>(bs + 2 * kH + 16 * mb + 4 * g4);
+        yp += ww.x * fmaxf(z[0] + bb.x, 0.f) + ww.y * fmaxf(z[1] + bb.y, 0.f) + ww.z * fmaxf(z[2] + bb.z, 0.f) +
+              ww.w * fmaxf(z[3] + bb.w, 0.f);
+      }
+      yp += __shfl_xor(yp, 16);
+      yp += __shfl_xor(yp, 32);
+      y = yp + b4;
     }
-    aggregate_fwd(v, nf, i, lane, w, bs, Et, Ms);
-    float z3;
-    float y = node_forward(net, Ms, W3s, lane, z3);
-    if (lane == 0) guess[i] = pprev[i] + scale * (double)y;
-    __syncthreads();
+    if (g4 == 0 && live) guess[i] = isrow ? pprev[i] + scale * (double)y : pprev[i];
   }
 }
 
@@ -551,39 +622,44 @@ __global__ void k_to_bf16(const float* w, __bf16* o, int n) {
   if (p < n) o[p] = (__bf16)w[p];
 }
 
-// sums over rows of r0^2 and diag, row count -> scal[0..2]
-__global__ __launch_bounds__(1024) void k_feature_sums(int n, const int* row, const double* r0, const double* diag,
-                                                       double* scal) {
-  __shared__ double sh[3][16];
+// sums over rows of r0^2 and diag, and the row count: per-block partials
+// (grid-stride), then one block adds them in a fixed order -> scal[0..2]
+constexpr int kFeatBlocks = 256;
+__device__ inline double block_sum256(double a, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void k_feature_partials(int n, const int* row, const double* r0, const double* diag,
+                                                          double* part) {
+  __shared__ double sh[4];
   double a = 0, b = 0, c = 0;
-  for (int i = threadIdx.x; i < n; i += 1024)
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
     if (row[i]) {
       a += r0[i] * r0[i];
       b += diag[i];
       c += 1;
     }
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_xor(a, off);
-    b += __shfl_xor(b, off);
-    c += __shfl_xor(c, off);
-  }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][w] = a;
-    sh[1][w] = b;
-    sh[2][w] = c;
-  }
-  __syncthreads();
+  a = block_sum256(a, sh);
+  b = block_sum256(b, sh);
+  c = block_sum256(c, sh);
   if (threadIdx.x == 0) {
-    double s0 = 0, s1 = 0, s2 = 0;
-    for (int q = 0; q < 16; q++) {
-      s0 += sh[0][q];
-      s1 += sh[1][q];
-      s2 += sh[2][q];
-    }
-    scal[0] = s0;
-    scal[1] = s1;
-    scal[2] = s2;
+    part[3 * blockIdx.x + 0] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_feature_fin(const double* part, int nb, double* scal) {
+  __shared__ double sh[4];
+  for (int q = 0; q < 3; q++) {
+    double a = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) a += part[3 * k + q];
+    a = block_sum256(a, sh);
+    if (threadIdx.x == 0) scal[q] = a;
   }
 }
 
@@ -614,6 +690,7 @@ int mlp_alloc(Mlp& m, int n, int cap) {
     HIP_TRY(hipMalloc(&m.wbf, kNParam * 2));
     HIP_TRY(hipMalloc(&m.partials, (size_t)kMlpBwdBlocks * (kNParam + 1) * 4));
     HIP_TRY(hipMalloc(&m.scal, 8 * 8));
+    HIP_TRY(hipMalloc(&m.fpart, kFeatBlocks * 3 * 8));
     HIP_TRY(hipHostMalloc((void**)&m.host_scal, 8 * 8, hipHostMallocDefault));
   }
   if (m.n != n || !m.nf) {
@@ -635,7 +712,7 @@ int mlp_alloc(Mlp& m, int n, int cap) {
 
 void mlp_free(Mlp& m) {
   for (void* p : {(void*)m.master, (void*)m.grad, (void*)m.adam_m, (void*)m.adam_v, (void*)m.wbf, (void*)m.partials,
-                  (void*)m.scal, (void*)m.nf, (void*)m.r0, (void*)m.guess, (void*)m.target, (void*)m.pprev})
+                  (void*)m.scal, (void*)m.fpart, (void*)m.nf, (void*)m.r0, (void*)m.guess, (void*)m.target, (void*)m.pprev})
     if (p) (void)hipFree(p);
   if (m.host_scal) (void)hipHostFree(m.host_scal);
   m = Mlp{};
@@ -686,7 +763,9 @@ int mlp_features(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m,
     if (rr_base) *rr_base = 0;
     return 0;
   }
-  k_feature_sums<<<1, 1024, 0, s>>>(v.n, row, m.r0, diag, m.scal);
+  const int nb = std::min(kFeatBlocks, (v.n + 255) / 256);
+  k_feature_partials<<<nb, 256, 0, s>>>(v.n, row, m.r0, diag, m.fpart);
+  k_feature_fin<<<1, 256, 0, s>>>(m.fpart, nb, m.scal);
   HIP_TRY(hipMemcpyAsync(m.host_scal, m.scal, 3 * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (rr_base) *rr_base = m.host_scal[0];  // sum of r0^2 over the rows
@@ -709,8 +788,10 @@ int mlp_forward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, 
   }
   if (v.n <= 0) return 0;
   Net net{m.wbf, m.master};
-  const int g = v.n < kFwdBlocks ? ((v.n + 7) / 8) * 8 : kFwdBlocks;
-  k_mlp_forward<R><<<g, 64, 0, s>>>(net, v, m.nf, row, pprev, scale, guess);
+  TRY(refresh_pos4<R>(s, v));  // edge inputs use the FP32 positions
+  const int ng = (v.n + 15) / 16, waves = (ng + 7) / 8 * 8;
+  const int g = waves / kFwdWaves < kFwdGrid ? ((waves / kFwdWaves + 7) / 8) * 8 : kFwdGrid;
+  k_mlp_forward<R><<<g, 64 * kFwdWaves, 0, s>>>(net, v, m.nf, row, pprev, scale, guess);
   HIP_TRY(hipGetLastError());
   return 0;
 }

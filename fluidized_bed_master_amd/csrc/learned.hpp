@@ -36,6 +36,7 @@ struct Mlp {
   double* pprev = nullptr;   // pressure before the solve (FP64, sorted order)
   float* partials = nullptr; // per-block gradient partials (kMlpBwdBlocks x (kNParam + 1))
   double* scal = nullptr;    // device scalars: sum r^2, sum diag, rows, scale, loss
+  double* fpart = nullptr;   // per-block partials of the feature sums
   double* host_scal = nullptr;
 };
 

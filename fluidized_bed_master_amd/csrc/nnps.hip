@@ -305,6 +305,15 @@ int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, 
   HIP_TRY(hipGetLastError());
   return 0;
 }
+template <typename R>
+int refresh_pos4(hipStream_t s, const SphView<R>& v) {
+  if (v.n <= 0) return 0;
+  k_pos4<R><<<grid_for(v.n), kBlock, 0, s>>>(v);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+template int refresh_pos4<float>(hipStream_t, const SphView<float>&);
+template int refresh_pos4<double>(hipStream_t, const SphView<double>&);
 template int build_sph_neighbors<float>(hipStream_t, const DevParams&, const SphView<float>&, const CellView&, int*);
 template int build_sph_neighbors<double>(hipStream_t, const DevParams&, const SphView<double>&, const CellView&, int*);
 
