@@ -214,6 +214,7 @@ struct FwdOperands {
   bf4 A1[4];      // W1 rows 16 nb + (l mod 16), inputs 4 (l / 16) .. +3 (16x16x16 layout): K = 0..5 the
                   // edge inputs, K = 6, 7 the BF16 high and low parts of b1 (inputs 6, 7 are 1), rest 0
   bf8 A2[4][2];   // W2 rows 16 mb + (l mod 16), hidden-1 units in the layer-1 result order
+  f4 C2[4];       // b2 in the layer-2 result layout (units 16 mb + 4 (l / 16) + r), the accumulator start
 };
 
 __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
@@ -238,6 +239,25 @@ __device__ inline void load_fwd_t(const Net& net, int lane, FwdOperands& w) {
         const int u = 16 * (2 * sk + j / 4) + 4 * g + (j & 3);
         w.A2[mb][sk][j] = net.wbf[OFF_W2 + (16 * mb + m) * kH + u];
       }
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) w.C2[mb][r] = net.master[OFF_B2 + 16 * mb + 4 * g + r];
+}
+
+// ReLU through the sign bit: an integer max with 0 maps every negative
+// float (and -0) to +0 and leaves the rest unchanged, without the NaN
+// canonicalisation a float max needs
+__device__ inline float relu_bits(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+// ReLU of two BF16 values packed in one dword (integer max per half)
+__device__ inline uint32_t relu_pk_bf16(float a, float b) {
+  typedef short s2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+  typedef float fl2 __attribute__((ext_vector_type(2)));
+  const fl2 f = {a, b};
+  const s2 v = __builtin_bit_cast(s2, __builtin_convertvector(f, bf2));
+  const s2 z = {0, 0};
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
 }
 
 // Lanes of one wave exchange the edge inputs through LDS.  A wave's LDS
@@ -282,10 +302,13 @@ __device__ inline GroupIter xcd_groups(int ng) {
 //     node layer's B-operand order.
 // The node layer runs on MFMA as well: the FP32 message is split into BF16
 // high and low parts (hi + lo equals it to 2^-16 relative), so the result
-// matches an FP32 message against the BF16 weights W3.  b1 enters layer 1
-// as two extra inputs equal to 1 against its BF16 high and low parts; b2 and
-// b3 are added in FP32 after the MFMA.  Edge inputs use the FP32 positions
-// (pos4).
+// matches an FP32 message against the BF16 weights W3.  Each edge carries
+// two extra inputs equal to 1 (0 for a slot past the particle's list), and
+// b1 enters layer 1 as its BF16 high and low parts against them; a slot with
+// no edge then has h1 = 0 exactly.  Layer 2 starts its accumulation from b2
+// (FP32), so an absent edge adds exactly ReLU(b2), which is taken off once
+// per particle at the end: no per-slot masking.  b3 is added in FP32.  Edge
+// inputs use the FP32 positions (pos4).
 template <typename R>
 __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphView<R> v,
                                                                    const float2* __restrict__ nf,
@@ -293,7 +316,7 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
                                                                    const double* __restrict__ pprev, double scale,
                                                                    double* __restrict__ guess) {
   __shared__ bf8 A3s[4][2][64];                         // W3 as the node layer's A operand, per lane
-  __shared__ __attribute__((aligned(16))) float bs[3 * kH];  // b2, b3, W4 (W4 from its BF16 copy)
+  __shared__ __attribute__((aligned(16))) float bs[2 * kH];  // b3, W4 (W4 from its BF16 copy)
   __shared__ bf8 Et[kFwdWaves][64];                     // edge inputs of one gather round, per wave
   const int tid = threadIdx.x, lane = tid & 63;
   const int m = lane & 15, g4 = lane >> 4;
@@ -306,9 +329,8 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
     A3s[mb][sk][l] = a;
   }
   if (tid < kH) {
-    bs[tid] = net.master[OFF_B2 + tid];
-    bs[kH + tid] = net.master[OFF_B3 + tid];
-    bs[2 * kH + tid] = (float)net.wbf[OFF_W4 + tid];
+    bs[tid] = net.master[OFF_B3 + tid];
+    bs[kH + tid] = (float)net.wbf[OFF_W4 + tid];
   }
   FwdOperands w;
   load_fwd_t(net, lane, w);
@@ -330,18 +352,20 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
       const float4 pi = isrow ? v.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
       const float ih = isrow ? (float)(1.0 / (double)v.h[i]) : 0.f;
       float acc[4][4] = {};
+      int nvalid = 0;  // edges this lane gathered (particle l mod 16)
       uint32_t nxt = g4 < cnt ? v.nbr[(size_t)g4 * v.n + i] : kNbrFlag;
       for (int k0 = 0; k0 < maxc; k0 += 4) {
         const uint32_t ent = nxt;
         const int kn = k0 + 4 + g4;
         if (k0 + 4 < maxc) nxt = kn < cnt ? v.nbr[(size_t)kn * v.n + i] : kNbrFlag;
-        const bool ok = !(ent & kNbrFlag);
+        // an edge outside the list gets all-zero inputs, bias inputs
+        // included, so its layer-1 and layer-2 sums are exactly 0 and it adds
+        // nothing to the message
         bf8 e8;
 #pragma unroll
-        for (int q = 0; q < 6; q++) e8[q] = (__bf16)0.0f;
-        e8[6] = (__bf16)1.0f;  // bias inputs
-        e8[7] = (__bf16)1.0f;
-        if (ok) {
+        for (int q = 0; q < 8; q++) e8[q] = (__bf16)0.0f;
+        if (!(ent & kNbrFlag)) {
+          nvalid++;
           const int j = (int)ent;
           const float4 pj = v.pos4[j];
           const float2 f = nf[j];
@@ -349,47 +373,55 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
           e8[0] = (__bf16)dx;
           e8[1] = (__bf16)dy;
           e8[2] = (__bf16)dz;
-          e8[3] = (__bf16)sqrtf(dx * dx + dy * dy + dz * dz);
+          e8[3] = (__bf16)__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
           e8[4] = (__bf16)f.x;
           e8[5] = (__bf16)f.y;
+          e8[6] = (__bf16)1.0f;  // bias inputs
+          e8[7] = (__bf16)1.0f;
         }
-        const uint64_t valid = __ballot(ok);
         wave_lds_sync();  // the previous round's reads of E are issued
         E[lane] = e8;
         wave_lds_sync();
-        for (int t = 0; t < 4 && k0 + t < maxc; t++) {
+#pragma unroll 2
+        for (int t = 0; t < 4; t++) {
           bf4 b;
 #pragma unroll
           for (int q = 0; q < 4; q++) b[q] = (__bf16)0.0f;
           if (g4 < 2) b = E4[2 * (16 * t + m) + g4];
-          // layer 1 (16 hidden x 16 particles per block nb, K = 16, bias
+          // layer 1 (16 hidden x 16 particles per block nb, K = 16, b1
           // included), ReLU, packed straight into the layer-2 B operand
-          bf8 B2[2];
+          uint32_t B2u[2][4];
 #pragma unroll
           for (int nb = 0; nb < 4; nb++) {
             f4 z = {0.f, 0.f, 0.f, 0.f};
             z = mfma16(w.A1[nb], b, z);
-            B2[nb >> 1][4 * (nb & 1) + 0] = (__bf16)fmaxf(z[0], 0.f);
-            B2[nb >> 1][4 * (nb & 1) + 1] = (__bf16)fmaxf(z[1], 0.f);
-            B2[nb >> 1][4 * (nb & 1) + 2] = (__bf16)fmaxf(z[2], 0.f);
-            B2[nb >> 1][4 * (nb & 1) + 3] = (__bf16)fmaxf(z[3], 0.f);
+            B2u[nb >> 1][2 * (nb & 1) + 0] = relu_pk_bf16(z[0], z[1]);
+            B2u[nb >> 1][2 * (nb & 1) + 1] = relu_pk_bf16(z[2], z[3]);
           }
-          // layer 2 and the masked sum over this slot's edges; b2 is read
-          // from LDS here rather than held in registers
-          const bool on = (valid >> (16 * t + m)) & 1ull;
-          asm volatile("" ::: "memory");
+          const bf8 B20 = __builtin_bit_cast(bf8, B2u[0]), B21 = __builtin_bit_cast(bf8, B2u[1]);
+          // layer 2 from b2, ReLU and the sum over slots; an absent edge
+          // (h1 = 0) adds ReLU(b2), taken off after the last slot
 #pragma unroll
           for (int mb = 0; mb < 4; mb++) {
-            f4 z = {0.f, 0.f, 0.f, 0.f};
-            z = mfma32(w.A2[mb][0], B2[0], z);
-            z = mfma32(w.A2[mb][1], B2[1], z);
-            const float4 bb = *reinterpret_cast<const float4*>(bs + 16 * mb + 4 * g4);
-            acc[mb][0] += on ? fmaxf(z[0] + bb.x, 0.f) : 0.f;
-            acc[mb][1] += on ? fmaxf(z[1] + bb.y, 0.f) : 0.f;
-            acc[mb][2] += on ? fmaxf(z[2] + bb.z, 0.f) : 0.f;
-            acc[mb][3] += on ? fmaxf(z[3] + bb.w, 0.f) : 0.f;
+            f4 z = mfma32(w.A2[mb][0], B20, w.C2[mb]);
+            z = mfma32(w.A2[mb][1], B21, z);
+            acc[mb][0] += relu_bits(z[0]);
+            acc[mb][1] += relu_bits(z[1]);
+            acc[mb][2] += relu_bits(z[2]);
+            acc[mb][3] += relu_bits(z[3]);
           }
         }
+      }
+      {
+        // absent edges among the 4 * rounds slots processed per particle
+        int nv = nvalid;
+        nv += __shfl_xor(nv, 16);
+        nv += __shfl_xor(nv, 32);
+        const float absent = (float)(((maxc + 3) & ~3) - nv);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) acc[mb][r] -= absent * relu_bits(w.C2[mb][r]);
       }
       // node layer: message M = acc / 64 as BF16 hi + lo parts
       bf8 Bh[2], Bl[2];
@@ -411,8 +443,8 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
         z = mfma32(a1, Bl[1], z);
         z = mfma32(a0, Bh[0], z);
         z = mfma32(a1, Bh[1], z);
-        const float4 bb = *reinterpret_cast<const float4*>(bs + kH + 16 * mb + 4 * g4);
-        const float4 ww = *reinterpret_cast<const float4*>(bs + 2 * kH + 16 * mb + 4 * g4);
+        const float4 bb = *reinterpret_cast<const float4*>(bs + 16 * mb + 4 * g4);
+        const float4 ww = *reinterpret_cast<const float4*>(bs + kH + 16 * mb + 4 * g4);
         yp += ww.x * fmaxf(z[0] + bb.x, 0.f) + ww.y * fmaxf(z[1] + bb.y, 0.f) + ww.z * fmaxf(z[2] + bb.z, 0.f) +
               ww.w * fmaxf(z[3] + bb.w, 0.f);
       }

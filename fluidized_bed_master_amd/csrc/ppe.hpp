@@ -62,7 +62,10 @@ struct PpeWork {
   void* vec[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int vec_bytes = 0;
   struct CoarseWork* coarse = nullptr;  // two-level preconditioner, created on first use
+  unsigned mv_seq = 0;                  // inner matvec launches seen by the timers
 };
+
+constexpr unsigned kMvSample = 4;  // timers time one inner matvec launch in this many
 
 int ppe_workspace_alloc(PpeWork& w, int n, int cap, bool mixed);
 void ppe_workspace_free(PpeWork& w);

@@ -47,6 +47,10 @@ struct Timers {
     open_a = nullptr;
     if (pending.size() > 4096) collect();
   }
+  // count a launch without timing it (0 ms)
+  void count(const std::string& name) {
+    if (on) acc[name].first += 1;
+  }
   void collect() {
     if (pending.empty()) return;
     (void)hipStreamSynchronize(stream);

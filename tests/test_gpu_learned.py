@@ -48,9 +48,23 @@ def _oracle_inputs(g, p):
     return X, h, nf, rows, src, dst, sc.value, s1["pres"].copy()
 
 
-def test_forward_matches_oracle():
-    g, p = _assembled()
+def _weights(kind):
     w = learned.init_params(seed=11, out_scale=1.0)
+    if kind == "biased":
+        # biases of both signs: negative b2 units have ReLU(b2) = 0, positive
+        # ones exercise the forward's removal of absent-edge contributions
+        for k, (a, b) in enumerate([(mlp.OFF_B1, mlp.OFF_W2), (mlp.OFF_B2, mlp.OFF_W3), (mlp.OFF_B3, mlp.OFF_W4),
+                                    (mlp.OFF_B4, mlp.NPARAM)]):
+            w[a:b] = 0.3 * learned.normal(77 + k, b - a)
+    elif kind == "trained":
+        w = np.load(learned.WEIGHTS_DIR / "ppe_mlp.npy").astype(np.float32)
+    return w
+
+
+@pytest.mark.parametrize("kind", ["init", "biased", "trained"])
+def test_forward_matches_oracle(kind):
+    g, p = _assembled()
+    w = _weights(kind)
     g.mlp_set_params(w)
     X, h, nf, rows, src, dst, scale, pprev = _oracle_inputs(g, p)
     guess = g.mlp_predict()
@@ -58,7 +72,7 @@ def test_forward_matches_oracle():
     y_ref, _ = mlp.forward(w, X, h, nf, rows, src, dst, bf=True)
     m = rows.astype(bool)
     err = rel_err(y_gpu[m], y_ref[m])
-    _report("mlp_forward", {"rel_err": err, "rows": int(m.sum()), "edges": len(src)})
+    _report("mlp_forward_" + kind, {"rel_err": err, "rows": int(m.sum()), "edges": len(src)})
     assert err < 2e-3, err
     assert np.array_equal(guess[~m], pprev[~m])
 
