@@ -86,12 +86,28 @@ struct SphView {
   R *drho;                  // WCSPH continuity rate
   double *ppe1, *ppe2, *ppe3, *ppe4;
   R *xsx, *xsy, *xsz, *xst; // XSPH results
-  float4* pos4;      // FP32 copy of x, y, z (neighbour pre-filter), refreshed by NNPS
+  // Positions as FP32 offsets inside a grid of power-of-two spacing pgrid
+  // anchored at (pg0x, pg0y, pg0z), the grid cell packed in w (10 bits per
+  // axis); differences come from rel_diff.  Refreshed by NNPS.
+  float4* pos4;
   // SPH-SPH neighbour list (slot major: nbr[k * n + i])
   uint32_t* nbr;
   int* nbr_cnt;
   int nbr_cap;
+  double pg0x, pg0y, pg0z;
+  float pgrid;
 };
+
+// x_a - x_b from two pos4 records: the integer cell difference times the
+// power-of-two spacing (exact) plus the FP32 offset difference, one rounding
+// each; for records a few cells apart the error stays below 2^-21 pgrid
+__device__ inline float3 rel_diff(float4 a, float4 b, float g) {
+  const uint32_t ca = __float_as_uint(a.w), cb = __float_as_uint(b.w);
+  const float cx = (float)((int)(ca & 1023u) - (int)(cb & 1023u));
+  const float cy = (float)((int)((ca >> 10) & 1023u) - (int)((cb >> 10) & 1023u));
+  const float cz = (float)((int)(ca >> 20) - (int)(cb >> 20));
+  return make_float3(fmaf(cx, g, a.x - b.x), fmaf(cy, g, a.y - b.y), fmaf(cz, g, a.z - b.z));
+}
 
 struct DemView {
   int n;
@@ -395,7 +411,7 @@ int fill_zero_bytes(hipStream_t s, void* p, size_t bytes);
 
 template <typename R>
 int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt);
-// pos4 = FP32 copy of the current positions
+// pos4 records of the current positions
 template <typename R>
 int refresh_pos4(hipStream_t s, const SphView<R>& v);
 

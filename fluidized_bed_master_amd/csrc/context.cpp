@@ -101,6 +101,7 @@ struct sophia_ctx {
   virtual ~sophia_ctx() {}
   sophia_params p{};
   sophia_domain d{};
+  double pgrid = 1.0;  // pos4 grid spacing (power of two)
   int device = 0;
   int precision = SOPHIA_PREC_MIXED;
   hipStream_t stream = nullptr;
@@ -251,6 +252,8 @@ struct Ctx : public sophia_ctx {
     v.ux = sux.cur; v.uy = suy.cur; v.uz = suz.cur; v.m = sm.cur; v.rho = srho.cur; v.temp = stemp.cur;
     v.flt_s = sflt.cur; v.dpor = sdpor.cur;
     v.ptype = sptype.cur; v.btype = sbtype.cur; v.itype = sitype.cur;
+    v.pg0x = d.x_min; v.pg0y = d.y_min; v.pg0z = d.z_min;
+    v.pgrid = (float)pgrid;
     DemView& w = dv;
     w.n = nd;
     w.x = dx_.cur; w.y = dy_.cur; w.z = dz_.cur; w.ux = dux.cur; w.uy = duy.cur; w.uz = duz.cur;
@@ -372,6 +375,10 @@ struct Ctx : public sophia_ctx {
       d.x_min = a; d.x_max = b; d.y_min = c; d.y_max = e; d.z_min = f; d.z_max = g;
     }
     d.dcell = 1.1 * p.kappa * P[0].h;
+    // pos4 grid: the smallest power-of-two spacing with at most 1000 cells
+    // along the longest side
+    const double span = std::max({d.x_max - d.x_min, d.y_max - d.y_min, d.z_max - d.z_min, 1e-30});
+    pgrid = std::ldexp(1.0, (int)std::ceil(std::log2(span / 1000.0)));
     d.NI = (int)((d.x_max - d.x_min) / d.dcell) + 1;
     d.NJ = (int)((d.y_max - d.y_min) / d.dcell) + 1;
     d.NK = (int)((d.z_max - d.z_min) / d.dcell) + 1;

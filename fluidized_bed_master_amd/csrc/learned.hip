@@ -308,7 +308,7 @@ __device__ inline GroupIter xcd_groups(int ng) {
 // no edge then has h1 = 0 exactly.  Layer 2 starts its accumulation from b2
 // (FP32), so an absent edge adds exactly ReLU(b2), which is taken off once
 // per particle at the end: no per-slot masking.  b3 is added in FP32.  Edge
-// inputs use the FP32 positions (pos4).
+// inputs use the pos4 position records.
 template <typename R>
 __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphView<R> v,
                                                                    const float2* __restrict__ nf,
@@ -369,7 +369,8 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
           const int j = (int)ent;
           const float4 pj = v.pos4[j];
           const float2 f = nf[j];
-          const float dx = (pi.x - pj.x) * ih, dy = (pi.y - pj.y) * ih, dz = (pi.z - pj.z) * ih;
+          const float3 d = rel_diff(pi, pj, v.pgrid);
+          const float dx = d.x * ih, dy = d.y * ih, dz = d.z * ih;
           e8[0] = (__bf16)dx;
           e8[1] = (__bf16)dy;
           e8[2] = (__bf16)dz;
@@ -820,7 +821,7 @@ int mlp_forward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, 
   }
   if (v.n <= 0) return 0;
   Net net{m.wbf, m.master};
-  TRY(refresh_pos4<R>(s, v));  // edge inputs use the FP32 positions
+  TRY(refresh_pos4<R>(s, v));  // edge inputs use the pos4 records
   const int ng = (v.n + 15) / 16, waves = (ng + 7) / 8 * 8;
   const int g = waves / kFwdWaves < kFwdGrid ? ((waves / kFwdWaves + 7) / 8) * 8 : kFwdGrid;
   k_mlp_forward<R><<<g, 64 * kFwdWaves, 0, s>>>(net, v, m.nf, row, pprev, scale, guess);

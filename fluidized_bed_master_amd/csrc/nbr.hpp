@@ -1,22 +1,40 @@
 // Batched neighbour-list traversal shared by the pair kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace sophia {
 
-// Fields of one neighbour, gathered before any of them is used.
+// Fields of one neighbour j of particle i, gathered before any of them is
+// used; dx, dy, dz = x_i - x_j.
 template <typename R>
 struct Nb {
-  double x, y, z, m, rho, pres, h;
+  double dx, dy, dz, m, rho, pres, h;
   R ux, uy, uz, temp, grx, gry, grz;
   double rref, por;
   int pt;
 };
-template <typename R>
-__device__ inline Nb<R> load_nb(const SphView<R>& v, int j) {
+
+// Particle i's side of the differences: FP64 position, or its pos4 record.
+struct NbOrigin {
+  double x, y, z;
+  float4 p4;
+};
+
+// REL (FP32 fields only): the differences come from the pos4 records (one
+// 16 B gather, error below 2^-21 of the grid spacing) instead of three FP64
+// gathers, for stages that use them in FP32.  Otherwise FP64 differences.
+template <bool REL, typename R>
+__device__ inline Nb<R> load_nb(const SphView<R>& v, int j, const NbOrigin& o) {
   Nb<R> q;
-  q.x = v.x[j]; q.y = v.y[j]; q.z = v.z[j];
+  if constexpr (REL && std::is_same<R, float>::value) {
+    const float3 d = rel_diff(o.p4, v.pos4[j], v.pgrid);
+    q.dx = d.x; q.dy = d.y; q.dz = d.z;
+  } else {
+    q.dx = o.x - v.x[j]; q.dy = o.y - v.y[j]; q.dz = o.z - v.z[j];
+  }
   q.m = v.m[j]; q.rho = v.rho[j]; q.pres = v.pres[j]; q.h = v.h[j];
   q.ux = v.ux[j]; q.uy = v.uy[j]; q.uz = v.uz[j]; q.temp = v.temp[j]; q.rref = v.rho_ref[j];
   q.grx = v.grx[j]; q.gry = v.gry[j]; q.grz = v.grz[j]; q.por = v.por[j];
@@ -31,11 +49,17 @@ __device__ inline Nb<R> load_nb(const SphView<R>& v, int j) {
 // a body never reads are dropped by the compiler.  Slots past the count
 // repeat the last entry with on = false; bodies add `on ? term : 0` (a
 // select: exact, and NaN-safe for the unused slots).
-template <typename R, class F>
+template <bool REL = false, typename R, class F>
 __device__ inline void for_nbrs4(const SphView<R>& v, int i, bool eps_rule, F&& f) {
   const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
   if (cnt <= 0) return;
   const size_t n = (size_t)v.n;
+  NbOrigin o;
+  if constexpr (REL && std::is_same<R, float>::value) {
+    o.p4 = v.pos4[i];
+  } else {
+    o.x = v.x[i]; o.y = v.y[i]; o.z = v.z[i];
+  }
   uint32_t en[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(u, cnt - 1) * n + i];
@@ -45,7 +69,7 @@ __device__ inline void for_nbrs4(const SphView<R>& v, int i, bool eps_rule, F&& 
     for (int u = 0; u < 4; u++) e[u] = en[u];
     Nb<R> q[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) q[u] = load_nb(v, (int)(e[u] & ~kNbrFlag));
+    for (int u = 0; u < 4; u++) q[u] = load_nb<REL>(v, (int)(e[u] & ~kNbrFlag), o);
 #pragma unroll
     for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(k + 4 + u, cnt - 1) * n + i];
 #pragma unroll

@@ -170,19 +170,32 @@ __global__ void k_permute(int n, const uint32_t* __restrict__ order, PermuteArgs
 // reference's visiting order.  Entries that pass sqrt(r2) < R but fail
 // sqrt(r2)+1e-20 < R carry kNbrFlag (those stages that use the +1e-20
 // test skip them).
+// pos4 record: grid cell c = floor((x - x0) / g) per axis (clamped to
+// 0..1023) and the FP32 offset x - (x0 + c g), computed in FP64
+__device__ inline uint32_t pos_cell(double x, double x0, double g, float& off) {
+  const int c = min(max((int)floor((x - x0) / g), 0), 1023);
+  off = (float)(x - (x0 + (double)c * g));
+  return (uint32_t)c;
+}
 template <typename R>
 __global__ void k_pos4(SphView<R> v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
-  v.pos4[i] = make_float4((float)v.x[i], (float)v.y[i], (float)v.z[i], 0.f);
+  const double g = v.pgrid;
+  float ox, oy, oz;
+  const uint32_t cx = pos_cell(v.x[i], v.pg0x, g, ox);
+  const uint32_t cy = pos_cell(v.y[i], v.pg0y, g, oy);
+  const uint32_t cz = pos_cell(v.z[i], v.pg0z, g, oz);
+  v.pos4[i] = make_float4(ox, oy, oz, __uint_as_float(cx | cy << 10 | cz << 20));
 }
 
-// Candidates of the 27 cells are tested on the FP32 copy first: farther than
-// Rs + tol or nearer than Rs - tol is decided there (one 16 B load); only the
-// band |r - Rs| <= tol takes the exact FP64 test, so the list, its order and
-// the r + 1e-20 < Rs flag are exactly those of the FP64 test.  tol bounds
-// the FP32 distance error: 2^-21 max|coordinate| + 2^-20 Rs, at least twice
-// the rounding of the coordinates, their differences and r^2.
+// Candidates of the 27 cells are tested on the pos4 records first: farther
+// than Rs + tol or nearer than Rs - tol is decided there (one 16 B load);
+// only the band |r - Rs| <= tol takes the exact FP64 test, so the list, its
+// order and the r + 1e-20 < Rs flag are exactly those of the FP64 test.  tol
+// bounds the FP32 distance error: 2^-21 max|coordinate| + 2^-20 Rs, well
+// above the rounding of the record differences (2^-21 of the grid spacing,
+// itself below the domain extent) and of r^2.
 template <typename R>
 __global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt, double tol_abs) {
   int i = particle_index();
@@ -192,12 +205,12 @@ __global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt,
     const double Rs = P.kappa * v.h[i];
     const double tol = tol_abs + Rs * (1.0 / 1048576.0);
     const float lo2 = (float)((Rs - tol) * (Rs - tol)), hi2 = (float)((Rs + tol) * (Rs + tol));
-    const float xf = (float)xi, yf = (float)yi, zf = (float)zi;
+    const float4 pi = v.pos4[i];
+    const float g = v.pgrid;
     const int n = v.n, cap = v.nbr_cap;
     for_cells(P, xi, yi, zi, cv, [&](int j) {
-      const float4 pj = v.pos4[j];
-      const float dx = xf - pj.x, dy = yf - pj.y, dz = zf - pj.z;
-      const float d2 = dx * dx + dy * dy + dz * dz;
+      const float3 dd = rel_diff(pi, v.pos4[j], g);
+      const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
       if (d2 > hi2) return;
       uint32_t e = (uint32_t)j;
       if (!(d2 < lo2)) {

@@ -39,8 +39,8 @@ __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
   const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rs = (T)(P.kappa * hd);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   T xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
-  for_nbrs4(v, i, false, [&](const Nb<R>& q, bool on) {
-    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
+  for_nbrs4<true>(v, i, false, [&](const Nb<R>& q, bool on) {
+    T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz);
     const bool use = on && q.pt <= 1000 && r > 0 && r < Rs;
     T dw = knl_dw<T>(P.kernel_type, A, h, r);
@@ -87,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_porosity_sph(DevParams P, SphView<R>
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const int pti = v.ptype[i];
   T flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
-  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
-    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
+  for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
+    T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
     T w = knl_w<T>(P.kernel_type, A, h, r);
     int ptj = q.pt;
@@ -155,9 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   const double ratio_i = v.rho[i] / v.rho_ref[i];
   const bool anto = P.delsph == 2;
   T flt = 0, rx = 0, ry = 0, rz = 0;
-  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+  for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
     const bool use = on && q.pt <= 1000;
-    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
+    T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
     T mj = q.m, rhoj = q.rho;
     if (P.filt_step) flt += use ? mj / rhoj * knl_w<T>(P.kernel_type, A, h, r) : (T)0;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
     const bool use = on && q.pt < 1000;
     // geometry in FP64: the pressure-gradient term cancels a hydrostatic
     // offset of order H/h times its result
-    const double dxd = xi - q.x, dyd = yi - q.y, dzd = zi - q.z;
+    const double dxd = q.dx, dyd = q.dy, dzd = q.dz;
     const double rd = sqrt(dxd * dxd + dyd * dyd + dzd * dzd) + 1e-20;
     const double dwd = knl_dw<double>(P.kernel_type, Ad, hd, rd);
     double gxd = dwd * dxd / rd, gyd = dwd * dyd / rd, gzd = dwd * dzd / rd;
@@ -338,9 +338,9 @@ __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<
   const T rhoi = v.rho[i], pori = v.por[i];
   T tx = 0, ty = 0, tz = 0;
   if (P.fp_solve) {
-    for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+    for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
       const bool use = on && q.pt < 1000;
-      T dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
+      T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
       T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
       T dw = knl_dw<T>(P.kernel_type, A, h, r);
       T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
@@ -426,10 +426,10 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i], ti = v.temp[i], c = (T)P.c_xsph;
   T tx = 0, ty = 0, tz = 0, tt = 0, fl = 0;
-  for_nbrs4(v, i, false, [&](const Nb<R>& q, bool on) {
+  for_nbrs4<true>(v, i, false, [&](const Nb<R>& q, bool on) {
     int ptj = q.pt;
     const bool use = on && (ptj == 1 || ptj == 3);
-    T dx = (T)(xi - q.x), dy = (T)(yi - q.y), dz = (T)(zi - q.z);
+    T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz);
     T w = knl_w<T>(P.kernel_type, A, h, r);
     T vj = (T)q.m / (T)q.rho;

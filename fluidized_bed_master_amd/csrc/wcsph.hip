@@ -123,9 +123,9 @@ __global__ __launch_bounds__(kBlock) void k_continuity(DevParams P, SphView<R> v
   const T el = P.eulerian ? 0.0 : 1.0;
   const bool dsph = P.delsph > 0, anto = P.delsph == 2;
   T tx = 0, ty = 0, tz = 0, px = 0, py = 0, pz = 0, tR = 0;
-  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+  for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
     const bool use = on && q.pt <= 1000;
-    const T dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
+    const T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     const T r = sqrt(dx * dx + dy * dy + dz * dz) + 1e-20;
     const T dw = knl_dw<T>(P.kernel_type, A, h, r);
     T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
@@ -140,9 +140,9 @@ __global__ __launch_bounds__(kBlock) void k_continuity(DevParams P, SphView<R> v
     pz += use ? uzi * mj / rhoj * (rhoi / pori - rhoj / porj) * gz : 0.0;
     if (dsph) {
       const T rrj = q.rref;
-      T phi = (gxi + (T)q.grx) * (q.x - xi);
-      phi += (gyi + (T)q.gry) * (q.y - yi);
-      phi += (gzi + (T)q.grz) * (q.z - zi);
+      T phi = (gxi + (T)q.grx) * -dx;
+      phi += (gyi + (T)q.gry) * -dy;
+      phi += (gzi + (T)q.grz) * -dz;
       phi = (-0.5 * phi) * (anto ? 1.0 : 0.0) + (rhoj / rrj - rhoi / rri);
       T trr = -2 * P.delta_sph * h * rri * P.soundspeed * (mj / rhoj);
       trr *= phi * dw / r;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_wc_interaction(DevParams P, SphView<
   double ptx = 0, pty = 0, ptz = 0, px = 0, py = 0, pz = 0;
   for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
     const bool use = on && q.pt <= 1000;
-    const double dxd = xi - q.x, dyd = yi - q.y, dzd = zi - q.z;
+    const double dxd = q.dx, dyd = q.dy, dzd = q.dz;
     const double rd = sqrt(dxd * dxd + dyd * dyd + dzd * dzd) + 1e-20;
     const double dwd = knl_dw<double>(P.kernel_type, Ad, hd, rd);
     double gxd = dwd * dxd / rd, gyd = dwd * dyd / rd, gzd = dwd * dzd / rd;
