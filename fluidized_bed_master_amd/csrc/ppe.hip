@@ -352,7 +352,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   }
   const double hd = v.h[i];
   const double A = knl_tmpA<double>(P.kernel_type, P.dim, hd);
-  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  // geometry: FP64 positions, or with FP32 fields the pos4 records (one 16 B
+  // gather per neighbour, difference error below 2^-21 of the grid spacing)
+  constexpr bool REL = std::is_same<R, float>::value;
+  const double xi = REL ? 0.0 : v.x[i], yi = REL ? 0.0 : v.y[i], zi = REL ? 0.0 : v.z[i];
+  const float4 pi4 = REL ? v.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const double uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i];
   const double rhoi = v.rho[i], pori = v.por[i];
   double bx = 0, by = 0, bz = 0, Aij = 0, AP = 0, lag = 0;
@@ -363,8 +367,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   // the list in batches of four: every field of the batch is gathered
   // before any is used (straight-line loads, all in flight together), the
   // next batch's indices are loaded, then the four entries run in list order
+  typedef typename std::conditional<REL, float, double>::type GT;
   struct AsmNb {
-    double x, y, z, m, rho, por, pres;
+    GT x, y, z;  // x_i - x_j
+    double m, rho, por, pres;
     R ux, uy, uz;
     int pt, it, row;
   };
@@ -378,7 +384,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     for (int u = 0; u < 4; u++) {
       eb[u] = en[u];
       const int j = (int)(eb[u] & ~kNbrFlag);
-      nb[u].x = v.x[j]; nb[u].y = v.y[j]; nb[u].z = v.z[j];
+      if constexpr (REL) {
+        const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid);
+        nb[u].x = d.x; nb[u].y = d.y; nb[u].z = d.z;
+      } else {
+        nb[u].x = xi - v.x[j]; nb[u].y = yi - v.y[j]; nb[u].z = zi - v.z[j];
+      }
       nb[u].m = v.m[j]; nb[u].rho = v.rho[j]; nb[u].por = v.por[j]; nb[u].pres = v.pres[j];
       nb[u].ux = v.ux[j]; nb[u].uy = v.uy[j]; nb[u].uz = v.uz[j];
       nb[u].pt = v.ptype[j]; nb[u].it = v.itype[j]; nb[u].row = o.row[j];
@@ -393,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     const AsmNb& nq = nb[ub];
     if (e & kNbrFlag) continue;
     if (!(nq.pt < 1000 && nq.it != 4)) continue;
-    double dx = xi - nq.x, dy = yi - nq.y, dz = zi - nq.z;
+    double dx = nq.x, dy = nq.y, dz = nq.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz) + 1e-20;
     double dw = knl_dw<double>(P.kernel_type, A, hd, r);
     double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
