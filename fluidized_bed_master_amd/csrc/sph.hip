@@ -206,10 +206,10 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   const double Ad = knl_tmpA<double>(P.kernel_type, P.dim, hd);
   T tx = 0, ty = 0, tz = 0, Rc = 0, ex = 0, ey = 0, ez = 0, et = 0;
   double px = 0, py = 0, pz = 0;
-  for_nbrs4(v, i, true, [&](const Nb<R>& q, bool on) {
+  auto term = [&](const Nb<R>& q, bool on) {
     const bool use = on && q.pt < 1000;
-    // geometry in FP64: the pressure-gradient term cancels a hydrostatic
-    // offset of order H/h times its result
+    // geometry in FP64 arithmetic: the symmetric pressure-gradient term
+    // cancels a hydrostatic offset of order H/h times its result
     const double dxd = q.dx, dyd = q.dy, dzd = q.dz;
     const double rd = sqrt(dxd * dxd + dyd * dyd + dzd * dzd) + 1e-20;
     const double dwd = knl_dw<double>(P.kernel_type, Ad, hd, rd);
@@ -267,7 +267,12 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
       H /= (r + (T)1e-10) * rhoi * rhoj * (kci + kcj);
       Rc += use ? H : (T)0;
     }
-  });
+  };
+  // the difference-form pressure gradient carries no hydrostatic offset, so
+  // with FP32 fields the position records give the geometry; the symmetric
+  // form keeps FP64 geometry
+  if (P.pgrad_form) for_nbrs4<true>(v, i, true, term);
+  else for_nbrs4<false>(v, i, true, term);
   if (P.fg_solve) tz += -(T)P.gravity;
   v.fx[i] = tx - ex;
   v.fy[i] = ty - ey;
@@ -337,20 +342,24 @@ __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i], pid = v.pres[i];
   const T rhoi = v.rho[i], pori = v.por[i];
   T tx = 0, ty = 0, tz = 0;
+  auto term = [&](const Nb<R>& q, bool on) {
+    const bool use = on && q.pt < 1000;
+    T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
+    T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
+    T dw = knl_dw<T>(P.kernel_type, A, h, r);
+    T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+    kgc_apply(P, v.cm, n, i, gx, gy, gz);
+    const T pp = P.pgrad_form ? q.pres - pid : pid + q.pres;
+    T Cp = -pori * q.m * pp / (rhoi * q.rho);
+    tx += use ? Cp * gx : (T)0;
+    ty += use ? Cp * gy : (T)0;
+    tz += use ? Cp * gz : (T)0;
+  };
   if (P.fp_solve) {
-    for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
-      const bool use = on && q.pt < 1000;
-      T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
-      T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
-      T dw = knl_dw<T>(P.kernel_type, A, h, r);
-      T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
-      kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      const T pp = P.pgrad_form ? q.pres - pid : pid + q.pres;
-      T Cp = -pori * q.m * pp / (rhoi * q.rho);
-      tx += use ? Cp * gx : (T)0;
-      ty += use ? Cp * gy : (T)0;
-      tz += use ? Cp * gz : (T)0;
-    });
+    // the difference form carries no hydrostatic offset, so geometry from
+    // the position records is enough; the symmetric form keeps FP64 geometry
+    if (P.pgrad_form) for_nbrs4<true>(v, i, true, term);
+    else for_nbrs4<false>(v, i, true, term);
   }
   v.fpx[i] = (R)tx;
   v.fpy[i] = (R)ty;
