@@ -1010,6 +1010,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
   }
   w.keys = keys;
   w.cell_off = cell_off;
+  w.bn = -1;
   w.NI = P.NI;
   w.NJ = P.NJ;
   w.ncell = P.NI * P.NJ * P.NK;
@@ -1080,12 +1081,17 @@ int ppe_rhs_norm(hipStream_t s, PpeWork& w, double* bn) {
     *bn = 0;
     return 0;
   }
+  if (w.bn >= 0) {  // the right-hand side is fixed from assembly to assembly
+    *bn = w.bn;
+    return 0;
+  }
   k_rhs_norm<<<w.nbg, kRedThreads, 0, s>>>(w.n, w.row, w.rhs, w.partials);
   launch_fin(s, w, kFinNorm);
   HIP_TRY(hipGetLastError());
   BiState h;
   TRY(read_state(s, w, &h));
   *bn = sqrt(h.norm);
+  w.bn = *bn;
   return 0;
 }
 
@@ -1261,8 +1267,9 @@ int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o,
     if (done != 2) break;  // converged or out of iterations; 2 = breakdown -> restart
     if (it == 0) break;
   }
-  // final full-precision residual
-  TRY(ppe_apply64(s, w, w.x64, w.r64, true, &rr));
+  // rr is the full-precision residual of x64 as it stands: every BiCGStab
+  // run ends with a fold followed by the FP64 residual of the folded x64
+  // (k_bi_true), and without a run it is the residual of the guess
   st.r_rel = sqrt(rr) / bn;
   st.iterations = total;
   st.outer = folds;
