@@ -128,7 +128,7 @@ def main(argv=None) -> int:
     ap.add_argument("--weights", default=str(ROOT / "fluidized_bed_master_amd" / "weights" / "ppe_mlp.npy"))
     ap.add_argument("--precond", default="two_level", choices=["two_level", "jacobi"],
                     help="preconditioner of the mixed-precision solve")
-    ap.add_argument("--inner-rtol", type=float, default=0.01,
+    ap.add_argument("--inner-rtol", type=float, default=0.001,
                     help="mixed solve: fold into the FP64 solution each time the FP32 residual drops by this factor")
     ap.add_argument("--legs", default="fp64,fp64_pc,prev,extrap",
                     help="comparison legs to run: fp64 (plain FP64 path), fp64_pc (FP64, same preconditioner), "

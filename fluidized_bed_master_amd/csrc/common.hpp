@@ -88,7 +88,8 @@ struct SphView {
   R *xsx, *xsy, *xsz, *xst; // XSPH results
   // Positions as FP32 offsets inside a grid of power-of-two spacing pgrid
   // anchored at (pg0x, pg0y, pg0z), the grid cell packed in w (10 bits per
-  // axis); differences come from rel_diff.  Refreshed by NNPS.
+  // axis); differences come from rel_diff.  Rebuilt by NNPS and after every
+  // position update (ISPH corrector, WCSPH predictor and corrector).
   float4* pos4;
   // SPH-SPH neighbour list (slot major: nbr[k * n + i])
   uint32_t* nbr;

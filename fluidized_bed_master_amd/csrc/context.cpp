@@ -650,7 +650,10 @@ struct Ctx : public sophia_ctx {
       TRY(dem_predictor(stream, P, dv));
     } else {
       TimeScope t(tm(), "corrector");
-      if (sph_on()) TRY(sph_corrector<R>(stream, P, sv));
+      if (sph_on()) {
+        TRY(sph_corrector<R>(stream, P, sv));
+        TRY(refresh_pos4<R>(stream, sv));  // the position records follow every move
+      }
       TRY(dem_corrector(stream, P, dv));
       after_corrector = true;
     }
@@ -972,9 +975,15 @@ struct Ctx : public sophia_ctx {
           // NNPS once (Eulerian); P3 starts at zero, so the predictor may
           // follow the sort: it is per particle
           TRY(nnps());
-          if (p.time_type == 1) TRY(wc_predictor<R>(stream, P, sv));
+          if (p.time_type == 1) {
+            TRY(wc_predictor<R>(stream, P, sv));
+            TRY(refresh_pos4<R>(stream, sv));
+          }
         } else {
-          if (p.time_type == 1) TRY(wc_predictor<R>(stream, P, sv));
+          if (p.time_type == 1) {
+            TRY(wc_predictor<R>(stream, P, sv));
+            TRY(refresh_pos4<R>(stream, sv));
+          }
           TRY(wc_reset_p3<R>(stream, sv));
         }
         after_corrector = false;
@@ -1017,6 +1026,7 @@ struct Ctx : public sophia_ctx {
       }
       case 7:
         TRY(wc_corrector<R>(stream, P, sv));
+        TRY(refresh_pos4<R>(stream, sv));
         after_corrector = true;
         break;
       case 8:

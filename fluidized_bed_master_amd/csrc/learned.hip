@@ -821,7 +821,6 @@ int mlp_forward(hipStream_t s, const DevParams& P, const SphView<R>& v, Mlp& m, 
   }
   if (v.n <= 0) return 0;
   Net net{m.wbf, m.master};
-  TRY(refresh_pos4<R>(s, v));  // edge inputs use the pos4 records
   const int ng = (v.n + 15) / 16, waves = (ng + 7) / 8 * 8;
   const int g = waves / kFwdWaves < kFwdGrid ? ((waves / kFwdWaves + 7) / 8) * 8 : kFwdGrid;
   k_mlp_forward<R><<<g, 64 * kFwdWaves, 0, s>>>(net, v, m.nf, row, pprev, scale, guess);

@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
 // take a Shepard interpolation of fluid p, T and eps-weighted rho over the
 // 27 cells within 2*kappa*h, with W(2h) normalised by A(h).
 template <typename R>
-__global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
+__global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs) {
   typedef R T;
   int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
   if (i >= v.n) return;
@@ -302,8 +302,24 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv) {
   // density ratios at that level
   double tp = 0, fl = 0, tr = 0;
   T tt = 0;
+  // FP32 fields: candidates are tested on the position records first, as in
+  // the neighbour build; only the band |r - R2| <= tol takes the exact FP64
+  // distance, so membership is that of the FP64 test, and the kernel weight
+  // of the others takes the record distance (FP32 weights either way)
+  constexpr bool REL = std::is_same<R, float>::value;
+  const double tol = tol_abs + R2 * (1.0 / 1048576.0);
+  const float lo2 = (float)((R2 - tol) * (R2 - tol)), hi2 = (float)((R2 + tol) * (R2 + tol));
+  const float4 pi4 = REL ? v.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   for_cells(P, xi, yi, zi, cv, [&](int j) {
-    double rd = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+    double rd;
+    if constexpr (REL) {
+      const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid);
+      const float d2 = d.x * d.x + d.y * d.y + d.z * d.z;
+      if (d2 > hi2) return;  // r >= R2 for certain
+      rd = d2 < lo2 ? (double)sqrtf(d2) + 1e-20 : dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+    } else {
+      rd = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+    }
     int ptj = v.ptype[j];
     if (!(ptj < 1000)) return;
     if (!(rd < R2)) return;
@@ -560,7 +576,9 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_prep, P, v); }
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_advection, P, v); }
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv) {
-  LAUNCH(k_neumann, P, v, cv);
+  const double m = fmax(fmax(fmax(fabs(P.x_min), fabs(P.x_max)), fmax(fabs(P.y_min), fabs(P.y_max))),
+                        fmax(fabs(P.z_min), fabs(P.z_max)));
+  LAUNCH(k_neumann, P, v, cv, m * (1.0 / 2097152.0));
 }
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_predictor, P, v); }
