@@ -87,7 +87,10 @@ class Params(C.Structure):
         ("eulerian_sph", C.c_int32), ("fix_xsph_position", C.c_int32), ("dem_max_contacts", C.c_int32),
         ("sph_max_neighbors", C.c_int32), ("domain_fixed", C.c_int32), ("periodic_mask", C.c_int32),
         ("domain_min", C.c_double * 3), ("domain_max", C.c_double * 3),
-        ("pgrad_form", C.c_int32), ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
+        ("pgrad_form", C.c_int32),
+        ("dem_walls", C.c_int32), ("wall_cyl", C.c_double * 3), ("wall_box", C.c_double * 6),
+        ("wall_cone", C.c_double * 5), ("wall_rest", C.c_double * 2), ("wall_mu", C.c_double * 2),
+        ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
     ]
 
     def as_dict(self):

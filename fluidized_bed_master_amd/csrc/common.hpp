@@ -440,6 +440,13 @@ template <typename R> int dem_coupling(hipStream_t s, const DevParams& P, const 
 template <typename R> int sph_dem_reaction(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dcells);
 int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
+// DEM wall boundaries (sophia_params.dem_walls and wall_*)
+struct DemWallParams {
+  int mask;                  // bit 0 cylinder, 1 box, 2 truncated cone
+  double cyl[3], box[6], cone[5];
+  double rest_n, rest_s, mu_s, mu_rs;
+};
+int dem_walls(hipStream_t s, const DevParams& P, const DemView& d, const DemWallParams& W);
 
 // WCSPH stages (wcsph.hip)
 template <typename R> int wc_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v);

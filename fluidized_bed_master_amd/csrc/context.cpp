@@ -655,6 +655,20 @@ struct Ctx : public sophia_ctx {
         TRY(refresh_pos4<R>(stream, sv));  // the position records follow every move
       }
       TRY(dem_corrector(stream, P, dv));
+      if (p.dem_walls) {
+        // DEM walls after the time update (ISPH_Calc_seperate.cuh:439-441)
+        DemWallParams W;
+        W.mask = p.dem_walls;
+        std::memcpy(W.cyl, p.wall_cyl, sizeof(W.cyl));
+        std::memcpy(W.box, p.wall_box, sizeof(W.box));
+        std::memcpy(W.cone, p.wall_cone, sizeof(W.cone));
+        W.rest_n = p.wall_rest[0];
+        W.rest_s = p.wall_rest[1];
+        W.mu_s = p.wall_mu[0];
+        W.mu_rs = p.wall_mu[1];
+        TimeScope t(tm(), "dem_walls");
+        TRY(dem_walls(stream, P, dv, W));
+      }
       after_corrector = true;
     }
     return 0;
@@ -1235,6 +1249,18 @@ void sophia_params_default(sophia_params* p) {
   p->fix_xsph_position = 0;
   p->dem_max_contacts = 16;
   p->sph_max_neighbors = 0;
+  // DEM walls off; geometry and contact constants of functions_DEM_BC.cuh
+  p->dem_walls = 0;
+  const double cyl[3] = {0.0, 0.0, 0.0125};                                      // :254-257
+  const double box[6] = {0.0, 0.0, -0.00176647182, 1.15, 1.015, 0.03353294364};  // :1503-1509
+  const double cone[5] = {0.0, 0.0, 0.0125, 0.0018, 0.01853294364};              // :723-724, 4240-4242
+  std::memcpy(p->wall_cyl, cyl, sizeof(cyl));
+  std::memcpy(p->wall_box, box, sizeof(box));
+  std::memcpy(p->wall_cone, cone, sizeof(cone));
+  p->wall_rest[0] = 0.9;   // rest_n (:1)
+  p->wall_rest[1] = -0.9;  // rest_s (:2)
+  p->wall_mu[0] = 0.10;    // mu_s (:4)
+  p->wall_mu[1] = 0.2;     // mu_rs (:5)
 }
 
 // read_solv_input (function_init.cuh:216-615): whitespace tokens; a key is

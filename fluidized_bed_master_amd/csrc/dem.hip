@@ -365,7 +365,164 @@ __global__ void k_dem_corrector(DevParams P, DemView d) {
   d.uz[i] = uzc;
 }
 
+// --- DEM wall boundaries (functions_DEM_BC.cuh) -----------------------------
+// State of one sphere while the walls act on it in turn.
+struct WallBody {
+  double x, y, z, ux, uy, uz, wx, wy, wz;  // position, velocity, angular velocity
+  double fx, fy, fz, tx, ty, tz;            // ftotal (per mass), torque
+  double m, ri, rad;
+};
+
+// Contact response shared by the three walls (the common tail of
+// KERNEL_treat_DEM_{cylinder,box,truncated_cone}_z_dem, e.g. :609-725):
+// n is the unit normal pointing into the container.  Slow normal approach
+// (|u.n| < 50 dt): sliding friction mu_s m (f.n) along the contact slip and
+// rolling friction mu_rs m (f.n) along the spin slip, none when the wall
+// pulls (f.n > 0); the normal velocity is removed.  Otherwise an impulse
+// with restitution rest_n (normal) and rest_s (tangential).
+__device__ inline void wall_response(const DemWallParams& W, double dt, double nx, double ny, double nz,
+                                     WallBody& b) {
+  const double rad = b.rad, m = b.m, ri = b.ri;
+  const double un = b.ux * nx + b.uy * ny + b.uz * nz;
+  // velocity of the contact point and its spin part
+  const double cwx = -rad * (b.wy * nz - b.wz * ny);
+  const double cwy = -rad * (b.wz * nx - b.wx * nz);
+  const double cwz = -rad * (b.wx * ny - b.wy * nx);
+  const double cx = b.ux + cwx, cy = b.uy + cwy, cz = b.uz + cwz;
+  const double cn = cx * nx + cy * ny + cz * nz;
+  const double cwn = cwx * nx + cwy * ny + cwz * nz;
+  double sx = cx - cn * nx, sy = cy - cn * ny, sz = cz - cn * nz;       // slip
+  double qx = cwx - cwn * nx, qy = cwy - cwn * ny, qz = cwz - cwn * nz;  // spin slip
+  sx = fabs(sx) < 1e-10 ? 0.0 : sx;
+  sy = fabs(sy) < 1e-10 ? 0.0 : sy;
+  sz = fabs(sz) < 1e-10 ? 0.0 : sz;
+  qx = fabs(qx) < 1e-10 ? 0.0 : qx;
+  qy = fabs(qy) < 1e-10 ? 0.0 : qy;
+  qz = fabs(qz) < 1e-10 ? 0.0 : qz;
+  const double smag = sqrt(sx * sx + sy * sy + sz * sz);
+  const double qmag = sqrt(qx * qx + qy * qy + qz * qz);
+  const double ex = sx / (smag + 1e-20), ey = sy / (smag + 1e-20), ez = sz / (smag + 1e-20);
+  const double gx = qx / (qmag + 1e-20), gy = qy / (qmag + 1e-20), gz = qz / (qmag + 1e-20);
+  const double k = rad / ri;
+  if (fabs(un) < 50.0 * dt) {
+    const double fn = b.fx * nx + b.fy * ny + b.fz * nz;
+    const bool on = !(fn > 0.0);
+    const double a = on ? W.mu_s * m * fn : 0.0, r = on ? W.mu_rs * m * fn : 0.0;
+    const double ffx = a * ex, ffy = a * ey, ffz = a * ez;
+    const double tfx = ffx + r * gx, tfy = ffy + r * gy, tfz = ffz + r * gz;
+    b.fx += ffx / m;
+    b.fy += ffy / m;
+    b.fz += ffz / m;
+    const double mx = -k * (ny * tfz - nz * tfy), my = -k * (nz * tfx - nx * tfz), mz = -k * (nx * tfy - ny * tfx);
+    b.tx += mx;
+    b.ty += my;
+    b.tz += mz;
+    const double vx = b.ux + (ffx / m) * dt, vy = b.uy + (ffy / m) * dt, vz = b.uz + (ffz / m) * dt;
+    const double vn = vx * nx + vy * ny + vz * nz;
+    b.ux = vx - vn * nx;
+    b.uy = vy - vn * ny;
+    b.uz = vz - vn * nz;
+    b.wx += mx * dt;
+    b.wy += my * dt;
+    b.wz += mz * dt;
+  } else {
+    const double jn = -m * (1.0 + W.rest_n) * cn;
+    const double js = -1.0 / ((1.0 / m) + (rad * rad / ri)) * (1.0 + W.rest_s) * smag;
+    const double jx = jn * nx + js * ex, jy = jn * ny + js * ey, jz = jn * nz + js * ez;
+    b.ux += (1.0 / m) * jx;
+    b.uy += (1.0 / m) * jy;
+    b.uz += (1.0 / m) * jz;
+    b.wx -= k * (ny * jz - nz * jy);
+    b.wy -= k * (nz * jx - nx * jz);
+    b.wz -= k * (nx * jy - ny * jx);
+  }
+}
+
+// KERNEL_treat_DEM_cylinder_z_dem (:491-729): a sphere beyond the vertical
+// cylinder of radius R about (xc, yc) is moved onto the wall radially.
+__device__ inline void wall_cylinder(const DemWallParams& W, double dt, WallBody& b) {
+  const double xc = W.cyl[0], yc = W.cyl[1], R = W.cyl[2], rad = b.rad;
+  const double rx = b.x - xc, ry = b.y - yc;
+  if (rx * rx + ry * ry - (R - rad) * (R - rad) <= 0.0) return;  // inside
+  const double d = sqrt(rx * rx + ry * ry);
+  const double cpx = xc + (R - rad) * rx / d, cpy = yc + (R - rad) * ry / d;
+  b.x = cpx;
+  b.y = cpy;
+  const double nx = -(cpx - xc) / (R - rad), ny = -(cpy - yc) / (R - rad), nz = 0.0;
+  if (sqrt(nx * nx + ny * ny + nz * nz) < 0.5) return;
+  wall_response(W, dt, nx, ny, nz, b);
+}
+
+// KERNEL_treat_DEM_box_z_dem (:1770-2027): a sphere outside the box is
+// clamped onto it; the normal is the sign pattern of the move.  The x sign
+// is cleared on |cp_y - x| < 1e-12 as in the reference (:1877), so a box
+// centred off x = 0 lets the rounding of xc + (x - xc) into the normal.
+__device__ inline void wall_box(const DemWallParams& W, double dt, WallBody& b) {
+  const double xc = W.box[0], yc = W.box[1], zc = W.box[2], rad = b.rad;
+  const double hx = 0.5 * W.box[3] - rad, hy = 0.5 * W.box[4] - rad, hz = 0.5 * W.box[5] - rad;
+  const double x = b.x, y = b.y, z = b.z;
+  const double F = fmax(fmax(fabs(x - xc) - hx, fabs(y - yc) - hy), fabs(z - zc) - hz);
+  if (F <= 0.0) return;  // inside
+  const double cpx = xc + fmin(hx, fmax(-hx, x - xc));
+  const double cpy = yc + fmin(hy, fmax(-hy, y - yc));
+  const double cpz = zc + fmin(hz, fmax(-hz, z - zc));
+  b.x = cpx;
+  b.y = cpy;
+  b.z = cpz;
+  double sx = (cpx - x) / (fabs(cpx - x) + 1e-20);
+  double sy = (cpy - y) / (fabs(cpy - y) + 1e-20);
+  double sz = (cpz - z) / (fabs(cpz - z) + 1e-20);
+  if (fabs(cpy - y) < 1e-12) sy = 0.0;
+  if (fabs(cpy - x) < 1e-12) sx = 0.0;
+  if (fabs(cpz - z) < 1e-12) sz = 0.0;
+  const double sm = sqrt(sx * sx + sy * sy + sz * sz) + 1e-20;
+  const double nx = sx / sm, ny = sy / sm, nz = sz / sm;
+  if (sqrt(nx * nx + ny * ny + nz * nz) < 0.5) return;
+  wall_response(W, dt, nx, ny, nz, b);
+}
+
+// KERNEL_treat_DEM_truncated_cone_z_dem (:4485-4725): the cone narrows from
+// radius r1 to r2 over height h down to z = -h; a sphere above z = rad - h
+// and outside the cone surface is moved onto it vertically.
+__device__ inline void wall_cone(const DemWallParams& W, double dt, WallBody& b) {
+  const double xc = W.cone[0], yc = W.cone[1], r1 = W.cone[2], r2 = W.cone[3], h = W.cone[4];
+  const double rad = b.rad, x = b.x, y = b.y, z = b.z;
+  if (z <= -h + rad) return;
+  const double L = sqrt((r1 - r2) * (r1 - r2) + h * h);
+  const double sn = h / L, cs = fabs(r1 - r2) / L, tn = sn / cs;
+  const double rr = r2 + (h + z) / tn - rad / sn;
+  if ((x - xc) * (x - xc) + (y - yc) * (y - yc) - rr * rr <= 0.0) return;  // inside
+  const double d = sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc));
+  b.z = tn * (d + rad / sn - r2) - h;
+  const double nx = -sn * (x - xc) / d, ny = -sn * (y - yc) / d, nz = cs;
+  wall_response(W, dt, nx, ny, nz, b);
+}
+
+// Walls in the reference's order, one thread per sphere (i_type 1 only).
+__global__ void k_dem_walls(DevParams P, DemView d, DemWallParams W) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  if (d.itype[i] != 1 || d.ptype[i] <= 1000) return;
+  WallBody b{d.x[i],  d.y[i],  d.z[i],  d.ux[i],  d.uy[i],  d.uz[i],  d.wx[i], d.wy[i], d.wz[i],
+             d.fx[i], d.fy[i], d.fz[i], d.tqx[i], d.tqy[i], d.tqz[i], d.m[i],  d.ri[i], d.rad[i]};
+  if (W.mask & 1) wall_cylinder(W, P.dt_dem, b);
+  if (W.mask & 2) wall_box(W, P.dt_dem, b);
+  if (W.mask & 4) wall_cone(W, P.dt_dem, b);
+  d.x[i] = b.x; d.y[i] = b.y; d.z[i] = b.z;
+  d.ux[i] = b.ux; d.uy[i] = b.uy; d.uz[i] = b.uz;
+  d.wx[i] = b.wx; d.wy[i] = b.wy; d.wz[i] = b.wz;
+  d.fx[i] = b.fx; d.fy[i] = b.fy; d.fz[i] = b.fz;
+  d.tqx[i] = b.tx; d.tqy[i] = b.ty; d.tqz[i] = b.tz;
+}
+
 }  // namespace
+
+int dem_walls(hipStream_t s, const DevParams& P, const DemView& d, const DemWallParams& W) {
+  if (d.n <= 0 || W.mask == 0) return 0;
+  k_dem_walls<<<grid_for(d.n), kBlock, 0, s>>>(P, d, W);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
 
 template <typename R>
 int dem_porosity(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc,

@@ -136,6 +136,19 @@ typedef struct sophia_params {
    * 1: sum_j m_j (p_j - p_i)/(rho_i rho_j) grad~W_ij (zero for a constant
    *    pressure; keeps a converged PPE solve stable, see the design notes) */
   int32_t pgrad_form;
+  /* DEM wall boundaries after the DEM time update (ISPH_Calc_seperate.cuh:
+   * 439-441), applied in this order: bit 0 cylinder about z
+   * (KERNEL_treat_DEM_cylinder_z_dem, functions_DEM_BC.cuh:491-729), bit 1
+   * box (KERNEL_treat_DEM_box_z_dem, :1770-2027), bit 2 truncated cone
+   * about z (KERNEL_treat_DEM_truncated_cone_z_dem, :4485-4725).  The
+   * reference calls all three every step with hard-coded geometry; here they
+   * are off by default (0) and the geometry defaults to those values. */
+  int32_t dem_walls;
+  double wall_cyl[3];   /* centre x, y; radius (xz_cylinder, yz_cylinder, rz_cylinder: :254-257) */
+  double wall_box[6];   /* centre x, y, z; side lengths x, y, z (xz_box .. Lzz_box: :1503-1509) */
+  double wall_cone[5];  /* centre x, y; top radius r1, bottom radius r2, height h (:723-724, 4240-4242) */
+  double wall_rest[2];  /* restitution: normal rest_n, tangential rest_s (:1-2) */
+  double wall_mu[2];    /* friction: sliding mu_s, rolling mu_rs (:4-5) */
   int32_t reserved_i;
   double reserved[7];
 } sophia_params;

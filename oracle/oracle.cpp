@@ -35,6 +35,10 @@
  *   coupling reaction         momentum balance sum_i m_i da_i = -m_a F_a
  *   predictor/corrector/XSPH  exact restatement, checked against hand-written
  *                             per-particle formulas in the tests
+ *   DEM walls                 head-on impact: u_n -> -rest_n u_n, position
+ *                             on the wall, spin unchanged; resting contact:
+ *                             normal velocity removed, friction opposes slip;
+ *                             all three geometries
  *   parity unpinned           advection Eulerian term and Neumann wall
  *                             interpolation beyond their closed-form special
  *                             cases (uniform field reproduced exactly)
@@ -1205,6 +1209,120 @@ static void corrector(orc_state& s, Species& sp, double dt, bool is_sph) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* DEM walls: KERNEL_treat_DEM_cylinder_z_dem (functions_DEM_BC.cuh:491-729),*/
+/* KERNEL_treat_DEM_box_z_dem (:1770-2027), KERNEL_treat_DEM_truncated_      */
+/* cone_z_dem (:4485-4725), called in this order after the DEM time update   */
+/* (ISPH_Calc_seperate.cuh:439-441).  Geometry from sophia_params.wall_*.    */
+/* ------------------------------------------------------------------------ */
+namespace {
+struct V3 {
+  double x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(double k, V3 a) { return {k * a.x, k * a.y, k * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double zero_small(double v) { return std::fabs(v) < 1e-10 ? 0.0 : v; }
+
+// the tail shared by the three kernels (e.g. :609-725); n points inwards
+void wall_contact(const sophia_params& p, double dt, V3 n, part1& T, part3& o) {
+  const double rad = T.rad, m = T.m, ri = T.ri;
+  V3 u{T.ux, T.uy, T.uz}, w{T.wx, T.wy, T.wz};
+  const double un_mag = dot(u, n);
+  V3 uc_w = (-rad) * cross(w, n);   // spin part of the contact velocity
+  V3 uc = u + uc_w;
+  const double ucn = dot(uc, n), ucwn = dot(uc_w, n);
+  V3 ucs = uc - ucn * n, ucws = uc_w - ucwn * n;
+  ucs = {zero_small(ucs.x), zero_small(ucs.y), zero_small(ucs.z)};
+  ucws = {zero_small(ucws.x), zero_small(ucws.y), zero_small(ucws.z)};
+  const double ucs_mag = std::sqrt(dot(ucs, ucs)), ucws_mag = std::sqrt(dot(ucws, ucws));
+  const V3 s{ucs.x / (ucs_mag + 1e-20), ucs.y / (ucs_mag + 1e-20), ucs.z / (ucs_mag + 1e-20)};
+  const V3 sw{ucws.x / (ucws_mag + 1e-20), ucws.y / (ucws_mag + 1e-20), ucws.z / (ucws_mag + 1e-20)};
+  if (std::fabs(un_mag) < 50 * dt) {  // rolling on the surface
+    const double fn = o.ftotalx * n.x + o.ftotaly * n.y + o.ftotalz * n.z;
+    V3 ff = (p.wall_mu[0] * m * fn) * s, frf = (p.wall_mu[1] * m * fn) * sw;
+    if (fn > 0.0) ff = frf = {0.0, 0.0, 0.0};
+    o.ftotalx += ff.x / m; o.ftotaly += ff.y / m; o.ftotalz += ff.z / m;
+    const V3 tq = (-(rad / ri)) * cross(n, ff + frf);
+    o.torqx += tq.x; o.torqy += tq.y; o.torqz += tq.z;
+    const V3 u_c{u.x + (ff.x / m) * dt, u.y + (ff.y / m) * dt, u.z + (ff.z / m) * dt};
+    const V3 un = dot(u_c, n) * n;
+    T.ux = u_c.x - un.x; T.uy = u_c.y - un.y; T.uz = u_c.z - un.z;
+    T.wx = w.x + tq.x * dt; T.wy = w.y + tq.y * dt; T.wz = w.z + tq.z * dt;
+  } else {  // impulse
+    const V3 Jn = (-m * (1 + p.wall_rest[0]) * ucn) * n;
+    const V3 Js = (-1 / ((1 / m) + (rad * rad / ri)) * (1 + p.wall_rest[1]) * ucs_mag) * s;
+    const V3 J = Jn + Js;
+    T.ux = u.x + (1 / m) * J.x; T.uy = u.y + (1 / m) * J.y; T.uz = u.z + (1 / m) * J.z;
+    const V3 dw = (rad / ri) * cross(n, J);
+    T.wx = w.x - dw.x; T.wy = w.y - dw.y; T.wz = w.z - dw.z;
+  }
+}
+
+void wall_cylinder_z(const sophia_params& p, double dt, part1& T, part3& o) {
+  const double xc = p.wall_cyl[0], yc = p.wall_cyl[1], R = p.wall_cyl[2], rad = T.rad;
+  const double F = (T.x - xc) * (T.x - xc) + (T.y - yc) * (T.y - yc) - (R - rad) * (R - rad);
+  if (F <= 0) return;
+  const double dist = std::sqrt((T.x - xc) * (T.x - xc) + (T.y - yc) * (T.y - yc));
+  const double cpx = xc + (R - rad) * (T.x - xc) / dist;
+  const double cpy = yc + (R - rad) * (T.y - yc) / dist;
+  T.x = cpx;
+  T.y = cpy;
+  const V3 n{-(cpx - xc) / (R - rad), -(cpy - yc) / (R - rad), 0.0};
+  if (std::sqrt(dot(n, n)) < 0.5) return;
+  wall_contact(p, dt, n, T, o);
+}
+
+void wall_box_z(const sophia_params& p, double dt, part1& T, part3& o) {
+  const double* b = p.wall_box;
+  const double rad = T.rad, x = T.x, y = T.y, z = T.z;
+  const double ax = 0.5 * b[3] - rad, ay = 0.5 * b[4] - rad, az = 0.5 * b[5] - rad;
+  const double F = std::fmax(std::fmax(std::fabs(x - b[0]) - ax, std::fabs(y - b[1]) - ay), std::fabs(z - b[2]) - az);
+  if (F <= 0) return;
+  const double cpx = b[0] + std::fmin(ax, std::fmax(-ax, x - b[0]));
+  const double cpy = b[1] + std::fmin(ay, std::fmax(-ay, y - b[1]));
+  const double cpz = b[2] + std::fmin(az, std::fmax(-az, z - b[2]));
+  T.x = cpx; T.y = cpy; T.z = cpz;
+  double sgx = (cpx - x) / (std::fabs(cpx - x) + 1e-20);
+  double sgy = (cpy - y) / (std::fabs(cpy - y) + 1e-20);
+  double sgz = (cpz - z) / (std::fabs(cpz - z) + 1e-20);
+  if (std::fabs(cpy - y) < 1e-12) sgy = 0.0;
+  if (std::fabs(cpy - x) < 1e-12) sgx = 0.0;  // cp_y against x, as at :1877
+  if (std::fabs(cpz - z) < 1e-12) sgz = 0.0;
+  const double mag = std::sqrt(sgx * sgx + sgy * sgy + sgz * sgz) + 1e-20;
+  const V3 n{sgx / mag, sgy / mag, sgz / mag};
+  if (std::sqrt(dot(n, n)) < 0.5) return;
+  wall_contact(p, dt, n, T, o);
+}
+
+void wall_truncated_cone_z(const sophia_params& p, double dt, part1& T, part3& o) {
+  const double xc = p.wall_cone[0], yc = p.wall_cone[1], r1 = p.wall_cone[2], r2 = p.wall_cone[3], h = p.wall_cone[4];
+  const double rad = T.rad, x = T.x, y = T.y, z = T.z;
+  if (z <= -h + rad) return;
+  const double sin_t = h / std::sqrt((r1 - r2) * (r1 - r2) + h * h);
+  const double cos_t = std::fabs(r1 - r2) / std::sqrt((r1 - r2) * (r1 - r2) + h * h);
+  const double tan_t = sin_t / cos_t;
+  const double rw = r2 + (h + z) / tan_t - rad / sin_t;
+  const double rxy = std::sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc));
+  if ((x - xc) * (x - xc) + (y - yc) * (y - yc) - rw * rw <= 0) return;
+  T.z = tan_t * (rxy + rad / sin_t - r2) - h;
+  const V3 n{-sin_t * (x - xc) / rxy, -sin_t * (y - yc) / rxy, cos_t};
+  wall_contact(p, dt, n, T, o);
+}
+}  // namespace
+
+static void dem_walls(orc_state& s) {
+  Species& sp = s.dem;
+  for (int i = 0; i < sp.n(); i++) {
+    if (sp.SP1[i].i_type != 1 || sp.SP1[i].p_type <= 1000) continue;
+    if (s.p.dem_walls & 1) wall_cylinder_z(s.p, s.p.dt, sp.P1[i], sp.P3[i]);
+    if (s.p.dem_walls & 2) wall_box_z(s.p, s.p.dt, sp.P1[i], sp.P3[i]);
+    if (s.p.dem_walls & 4) wall_truncated_cone_z(s.p, s.p.dt, sp.P1[i], sp.P3[i]);
+  }
+}
+
 void orc_time(orc_state* s, int32_t phase) {
   int stride = std::max(1, s->p.decouple_stride);
   bool sph_on = (s->count % stride) == 0;
@@ -1215,6 +1333,7 @@ void orc_time(orc_state* s, int32_t phase) {
   } else {
     if (sph_on) corrector(*s, s->sph, stride * s->p.dt, true);
     corrector(*s, s->dem, s->p.dt, false);
+    if (s->p.dem_walls) dem_walls(*s);
     s->after_corrector = true;
   }
 }

@@ -316,3 +316,54 @@ def test_two_level_solve_is_deterministic():
         out.append(g.get_particles(abi.SPH)[0]["pres"].copy())
         g.close()
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_dem_walls_match_oracle(prec):
+    """DEM walls (cylinder, box, truncated cone; functions_DEM_BC.cuh) after
+    the DEM time update, with the geometry set to cut through the seeded
+    sphere bed so that every wall moves spheres: two full steps against the
+    oracle, DEM state and accumulators."""
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.small_isph_dem()
+    dem = parts["p_type"] > 1000
+    xs, ys, zs, rad = parts["x"][dem], parts["y"][dem], parts["z"][dem], parts["rad"][dem].max()
+    cx, cy = float(xs.mean()), float(ys.mean())
+    rmax = float(np.hypot(xs - cx, ys - cy).max())
+    p.dem_walls = 7
+    p.wall_cyl[0], p.wall_cyl[1], p.wall_cyl[2] = cx, cy, 0.85 * rmax + rad
+    zc, span = float(zs.mean()), float(zs.max() - zs.min())
+    # box centred on x = y = 0 as in the reference: its x sign test compares
+    # cp_y with x (functions_DEM_BC.cuh:1877), so with another x centre the
+    # rounding of xc + (x - xc) leaks into the normal and the result turns
+    # chaotic in the last bit of x
+    for k, v in enumerate([0.0, 0.0, zc, 1.0, 1.0, 0.7 * span + 2 * rad]):
+        p.wall_box[k] = v
+    # cone about the bed axis whose wall radius at the bed's mid height is
+    # 0.7 rmax (apex side below the bed)
+    h, r2 = 0.02, 0.3 * rmax
+    tn = (h + zc) / (0.7 * rmax + rad - r2)
+    r1 = r2 + h / tn
+    for k, v in enumerate([cx, cy, r1, r2, h]):
+        p.wall_cone[k] = v
+    g, o = Sophia(p, parts, precision=prec), Oracle(p, parts)
+    p0 = configs.small_isph_dem()[0]
+    o0 = Oracle(p0, parts)  # the same column without walls
+    opts = abi.ppe_opts(method=abi.PPE_JACOBI)
+    tol = TOL[prec]
+    rep = {}
+    for step in range(2):
+        g.step(opts)
+        o.step(opts)
+        o0.step(opts)
+        a1, _, a3 = g.get_particles(abi.DEM)
+        b1, _, b3 = o.get_particles(abi.DEM)
+        compare_fields(a1, b1, ["x", "y", "z", "ux", "uy", "uz", "wx", "wy", "wz"], tol * (10 if step else 1),
+                       f"step{step}/dem", report=rep)
+        compare_fields(a3, b3, ["ftotalx", "ftotaly", "ftotalz", "torqx", "torqy", "torqz"],
+                       tol * (10 if step else 1), f"step{step}/dem/p3", report=rep)
+    moved = int(np.sum(o0.get_particles(abi.DEM)[0]["z"] != b1["z"]) + np.sum(o0.get_particles(abi.DEM)[0]["x"] != b1["x"]))
+    rep["spheres_moved_by_walls"] = moved
+    _report("dem_walls_" + ("fp64" if prec == abi.PREC_FP64 else "mixed"), rep)
+    assert moved > 0

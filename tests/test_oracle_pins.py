@@ -464,3 +464,118 @@ def test_monitors_exclusion_rule():
     assert not counted[fluid[0]] and not counted[wall]
     assert mon[1] == pytest.approx(parts["rho"][counted].max())
     assert mon[1] < 5.0e4
+
+
+# ---------------------------------------------------------------------------
+# DEM walls (functions_DEM_BC.cuh): KERNEL_treat_DEM_{cylinder,box,
+# truncated_cone}_z_dem after the DEM time update
+# ---------------------------------------------------------------------------
+def _one_sphere(pos, u=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0), r=1e-3, rho=2500.0, walls=0, gravity=False):
+    p = configs.config_params(1e-3, 1e-6)
+    p.fg_solve = 1 if gravity else 0
+    p.dem_walls = walls
+    parts = np.zeros(4, abi.PART1)
+    parts["i_type"] = 1
+    parts["h"] = 1.3e-3
+    # two SPH walls spanning the domain, the sphere, a last wall
+    parts["p_type"][[0, 1, 3]] = 0
+    parts["x"][:2] = [-0.03, 0.03]
+    parts["y"][:2] = [-0.03, 0.03]
+    parts["z"][:2] = [-0.03, 0.03]
+    parts["m"][[0, 1, 3]] = 1e-6
+    parts["rho"][[0, 1, 3]] = 1000
+    m = rho * 4 / 3 * math.pi * r ** 3
+    q = 2
+    parts["x"][q], parts["y"][q], parts["z"][q] = pos
+    parts["ux"][q], parts["uy"][q], parts["uz"][q] = u
+    parts["wx"][q], parts["wy"][q], parts["wz"][q] = w
+    parts["rad"][q] = r
+    parts["m"][q] = m
+    parts["ri"][q] = 0.4 * m * r * r
+    parts["rho"][q] = rho
+    parts["p_type"][q] = abi.PT_DEM
+    parts["dem_idx"][q] = 1
+    return p, parts, m
+
+
+def _run_walls(p, parts, setup=None):
+    o = Oracle(p, parts)
+    if setup:
+        setup(o)
+    o.nnps()
+    o.dem_interaction()
+    o.predict()
+    o.correct()
+    d1, _, d3 = o.get_particles(abi.DEM)
+    return d1[0], d3[0]
+
+
+def _box(p, c=(0.0, 0.0, 0.0), L=(0.02, 0.02, 0.02)):
+    for k in range(3):
+        p.wall_box[k] = c[k]
+        p.wall_box[3 + k] = L[k]
+
+
+def test_dem_box_head_on_impact_restitution():
+    """Falling onto the floor faster than 50 dt: u_n -> -rest_n u_n, the
+    sphere sits on the floor, no tangential velocity or spin appears."""
+    p, parts, m = _one_sphere((0.0, 0.0, -0.0095), u=(0.0, 0.0, -1.0), walls=2)
+    _box(p)
+    q, _ = _run_walls(p, parts)
+    assert q["z"] == -(0.5 * 0.02 - 1e-3)
+    assert abs(q["uz"] - p.wall_rest[0]) < 1e-12
+    assert q["ux"] == 0 and q["uy"] == 0
+    assert q["wx"] == 0 and q["wy"] == 0 and q["wz"] == 0
+
+
+def test_dem_cylinder_radial_impact_with_spin():
+    """Radial impact on the cylinder: the normal part reflects with rest_n;
+    the contact slip from the spin, rad wz, takes the tangential impulse
+    (1 + rest_s) slip / (1/m + rad^2/ri) (= m (1 + rest_s) slip / 3.5 for a
+    solid sphere), and the spin changes by -(rad/ri) n x J."""
+    r, wz = 1e-3, 100.0
+    p, parts, m = _one_sphere((0.0095, 0.0, 0.0), u=(1.0, 0.0, 0.0), w=(0.0, 0.0, wz), r=r, walls=1)
+    p.wall_cyl[0], p.wall_cyl[1], p.wall_cyl[2] = 0.0, 0.0, 0.01
+    q, _ = _run_walls(p, parts)
+    assert abs(q["x"] - 0.009) < 1e-15 and q["y"] == 0
+    assert abs(q["ux"] + p.wall_rest[0]) < 1e-12
+    slip = r * wz  # contact-point velocity along +y for n = -x
+    js = -(1 + p.wall_rest[1]) * slip / (1 / m + r * r / (0.4 * m * r * r))
+    assert abs(q["uy"] - js / m) < 1e-12 * abs(js / m)
+    # n = -x, J = (Jx, Js, 0): -(rad/ri)(n x J)_z = -(rad/ri)(-Js)
+    assert abs(q["wz"] - (wz + r / (0.4 * m * r * r) * js)) < 1e-9 * wz
+
+
+def test_dem_box_resting_contact_friction():
+    """Resting on the floor under gravity (|u_n| < 50 dt): the normal velocity
+    is removed, sliding friction mu_s m (f.n) opposes the slip and its torque
+    -(rad/ri) n x F spins the sphere forward."""
+    r, us = 1e-3, 0.01
+    p, parts, m = _one_sphere((0.0, 0.0, -0.0090001), u=(us, 0.0, 0.0), r=r, walls=2, gravity=True)
+    _box(p)
+    q, o3 = _run_walls(p, parts)
+    dt = p.dt
+    fn = -p.gravity
+    ff = p.wall_mu[0] * m * fn            # along +x (the slip), negative
+    assert q["uz"] == 0.0
+    assert abs(q["ux"] - (us + ff / m * dt)) < 1e-12 * us
+    assert abs(o3["ftotalx"] - ff / m) < 1e-12
+    tq = (r / (0.4 * m * r * r)) * (-ff)  # -(rad/ri) (z x ff x)_y, with z x x = y
+    assert abs(o3["torqy"] - tq) < 1e-9 * abs(tq)
+    assert abs(q["wy"] - o3["torqy"] * dt) < 1e-12 * abs(o3["torqy"] * dt)
+
+
+def test_dem_truncated_cone_projection_lands_on_the_wall():
+    """A sphere outside the cone wall is moved vertically onto it: at the new
+    height the wall radius r2 + (h + z)/tan - rad/sin equals its distance
+    from the axis; the normal is a unit vector."""
+    r = 5e-4
+    p, parts, m = _one_sphere((0.01, 0.0, -0.005), u=(0.2, 0.0, 0.0), r=r, walls=4)
+    q, _ = _run_walls(p, parts)
+    xc, yc, r1, r2, h = p.wall_cone
+    L = math.hypot(r1 - r2, h)
+    sn, cs = h / L, abs(r1 - r2) / L
+    tn = sn / cs
+    d = math.hypot(q["x"] - xc, q["y"] - yc)
+    assert abs(r2 + (h + q["z"]) / tn - r / sn - d) < 1e-12
+    assert q["z"] != -0.005
