@@ -90,6 +90,7 @@ class Params(C.Structure):
         ("pgrad_form", C.c_int32),
         ("dem_walls", C.c_int32), ("wall_cyl", C.c_double * 3), ("wall_box", C.c_double * 6),
         ("wall_cone", C.c_double * 5), ("wall_rest", C.c_double * 2), ("wall_mu", C.c_double * 2),
+        ("ob_band", C.c_double * 4), ("ob_inlet_u", C.c_double), ("ob_inlet_temp", C.c_double),
         ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
     ]
 

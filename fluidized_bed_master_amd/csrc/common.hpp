@@ -440,6 +440,14 @@ template <typename R> int dem_coupling(hipStream_t s, const DevParams& P, const 
 template <typename R> int sph_dem_reaction(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dcells);
 int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
+// open boundaries (sophia_params.open_boundary = 1, ob_*)
+struct OpenBoundary {
+  double L[4];  // z bands: inlet [L1, L2), outlet [L3, L4)
+  double inlet_u, inlet_temp;
+};
+template <typename R>
+int sph_open_boundary(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv,
+                      const OpenBoundary& ob);
 // DEM wall boundaries (sophia_params.dem_walls and wall_*)
 struct DemWallParams {
   int mask;                  // bit 0 cylinder, 1 box, 2 truncated cone

@@ -138,6 +138,29 @@ struct sophia_ctx {
 
 namespace {
 
+// c_initial_inner_outer_particle_single_sph/_dem (functions_NNPS.cuh:8-76,
+// 138-200), with open boundaries: every particle starts as i_type 1,
+// buffer_type 0; fluid (1 <= p_type < 1000) with z in [L1, L2) becomes an
+// inlet buffer particle (buffer_type 1, i_type 2), in [L3, L4) an outlet
+// one (buffer_type 2, i_type 2); both tests carry the reference's -1e-6
+void tag_open_boundary(const sophia_params& p, std::vector<part1>& P) {
+  const double* L = p.ob_band;
+  for (part1& q : P) {
+    q.i_type = 1;
+    q.buffer_type = 0;
+    if (q.p_type < 1000 && q.p_type >= 1) {
+      if (q.z >= (L[0] - 1e-6) && q.z < (L[1] - 1e-6)) {
+        q.buffer_type = 1;
+        q.i_type = 2;
+      }
+      if (q.z >= (L[2] - 1e-6) && q.z < (L[3] - 1e-6)) {
+        q.buffer_type = 2;
+        q.i_type = 2;
+      }
+    }
+  }
+}
+
 int ceil_log2(uint32_t v) {
   int b = 0;
   while ((1ull << b) < (uint64_t)v) b++;
@@ -284,6 +307,12 @@ struct Ctx : public sophia_ctx {
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
     setup_domain(P, n);
+    std::vector<part1> tagged;
+    if (p.open_boundary == 1) {
+      tagged.assign(P, P + n);
+      tag_open_boundary(p, tagged);
+      P = tagged.data();
+    }
     std::vector<int> si, di;
     for (int i = 0; i < n; i++) (P[i].p_type <= 1000 ? si : di).push_back(i);
     ns = (int)si.size();
@@ -655,6 +684,15 @@ struct Ctx : public sophia_ctx {
         TRY(refresh_pos4<R>(stream, sv));  // the position records follow every move
       }
       TRY(dem_corrector(stream, P, dv));
+      if (sph_on() && p.open_boundary == 1) {
+        // inlet/outlet buffer state from the interior (ISPH_Calc_seperate.cuh:410-426)
+        TimeScope t(tm(), "open_boundary");
+        OpenBoundary ob;
+        std::memcpy(ob.L, p.ob_band, sizeof(ob.L));
+        ob.inlet_u = p.ob_inlet_u;
+        ob.inlet_temp = p.ob_inlet_temp;
+        TRY(sph_open_boundary<R>(stream, P, sv, CellView{scs, sce}, ob));
+      }
       if (p.dem_walls) {
         // DEM walls after the time update (ISPH_Calc_seperate.cuh:439-441)
         DemWallParams W;
@@ -1261,6 +1299,11 @@ void sophia_params_default(sophia_params* p) {
   p->wall_rest[1] = -0.9;  // rest_s (:2)
   p->wall_mu[0] = 0.10;    // mu_s (:4)
   p->wall_mu[1] = 0.2;     // mu_rs (:5)
+  // open-boundary bands and inlet state (functions_NNPS.cuh:2-5; functions_BC.cuh:961-964)
+  const double band[4] = {-10.505, -0.021, 0.006, 10.25};
+  std::memcpy(p->ob_band, band, sizeof(band));
+  p->ob_inlet_u = 24.045;
+  p->ob_inlet_temp = 823.0;
 }
 
 // read_solv_input (function_init.cuh:216-615): whitespace tokens; a key is

@@ -15,12 +15,16 @@ namespace sophia {
 
 namespace {
 
-// KERNEL_clc_mass_init_sph (functions_MASS.cuh:24-43): rho0 <- rho
+// KERNEL_clc_mass_init_sph (functions_MASS.cuh:24-43): vol0 = vol =
+// (h/1.6)^dim, rho0 <- rho
 template <typename R>
 __global__ void k_mass_init(DevParams P, SphView<R> v) {
   int i = particle_index();
   if (i >= v.n) return;
   if (v.ptype[i] > 1000 || v.itype[i] > 2) return;
+  const double vol0 = pow(v.h[i] / P.h_over_dx, (double)P.dim);
+  v.vol0[i] = vol0;
+  v.vol[i] = vol0;
   v.rho0[i] = v.rho[i];
 }
 
@@ -342,6 +346,67 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs
   }
 }
 
+// KERNEL_open_boundary_extrapolation3D_1_sph + _calc_sph (functions_BC.cuh:
+// 787-984, 1204-1217), open_boundary = 1: each inlet/outlet buffer particle
+// takes Shepard means of the interior fluid (p_type 1, buffer_type 0) around
+// its ghost node, mirrored in z about L2 (inlet) or L3 (outlet), within
+// kappa h of it over the 27 cells of the ghost node's cell.  Inlet: p = mean
+// p, rho = eps rho_air, u = (0, 0, 2 u_in) - mean u, T = T_in.  Outlet:
+// p = 0, rho = eps mean(rho/eps), u = mean u, T = mean T.  m = rho_old vol0
+// (rho_old vol for Eulerian fluid).  The reference's gradient sums enter
+// with a factor 0 and are left out.  A buffer particle reads only interior
+// particles and its own fields, so the extrapolation and its copy-back
+// (the _calc kernel) are one pass.
+template <typename R>
+__global__ void k_open_boundary(DevParams P, SphView<R> v, CellView cv, OpenBoundary ob) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int bt = v.btype[i];
+  if (bt <= 0 || v.ptype[i] >= 1000) return;
+  const double hd = v.h[i];
+  const double A = knl_tmpA<double>(P.kernel_type, P.dim, hd);
+  const double Rs = P.kappa * hd;
+  const double xg = v.x[i], yg = v.y[i], zi = v.z[i];
+  const double zg = bt == 1 ? 2 * ob.L[1] - zi : (bt == 2 ? 2 * ob.L[2] - zi : zi);
+  double trho = 0, tpres = 0, tux = 0, tuy = 0, tuz = 0, ttemp = 0, tflt = 0;
+  for_cells(P, xg, yg, zg, cv, [&](int j) {
+    if (!(v.ptype[j] < 1000)) return;
+    const double r = dist_exact(xg - v.x[j], yg - v.y[j], zg - v.z[j]) + 1e-20;
+    if (!(r < Rs)) return;
+    if (!(v.ptype[j] == 1 && v.btype[j] == 0)) return;
+    const double w = knl_w<double>(P.kernel_type, A, hd, r);
+    const double volj = v.vol[j];
+    trho += v.rho[j] / v.por[j] * w * volj;
+    tpres += volj * v.pres[j] * w;
+    tuz += volj * (double)v.uz[j] * w;
+    tux += volj * (double)v.ux[j] * w;
+    tuy += volj * (double)v.uy[j] * w;
+    ttemp += volj * (double)v.temp[j] * w;
+    tflt += volj * w;
+  });
+  const double pori = v.por[i], rho_old = v.rho[i];
+  double m = rho_old * v.vol0[i];
+  if (P.eulerian && v.ptype[i] > 0) m = rho_old * v.vol[i];
+  if (bt == 1) {
+    v.pres[i] = tpres / tflt;
+    v.rho[i] = pori * P.rho_fluid;
+    v.uz[i] = (R)(ob.inlet_u * 2 - tuz / tflt);
+    v.ux[i] = (R)(0 - tux / tflt);
+    v.uy[i] = (R)(0 - tuy / tflt);
+    v.temp[i] = (R)ob.inlet_temp;
+  } else if (bt == 2) {
+    v.pres[i] = 0.0;
+    v.rho[i] = pori * (trho / tflt);
+    v.uz[i] = (R)(tuz / tflt);
+    v.ux[i] = (R)(tux / tflt);
+    v.uy[i] = (R)(tuy / tflt);
+    v.temp[i] = (R)(ttemp / tflt);
+  } else {
+    return;
+  }
+  v.m[i] = m;
+}
+
 // KERNEL_pressureforce3D_sph (functions_PPE2.cuh:2154-2335)
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<R> v) {
@@ -581,6 +646,11 @@ template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const S
   LAUNCH(k_neumann, P, v, cv, m * (1.0 / 2097152.0));
 }
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
+template <typename R>
+int sph_open_boundary(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv,
+                      const OpenBoundary& ob) {
+  LAUNCH(k_open_boundary, P, v, cv, ob);
+}
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_predictor, P, v); }
 template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_corrector, P, v); }
 template <typename R> int sph_xsph_only(hipStream_t s, const DevParams& P, const SphView<R>& v) {
@@ -609,6 +679,8 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
   template int sph_pressure_force<R>(hipStream_t, const DevParams&, const SphView<R>&);                     \
+  template int sph_open_boundary<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&,      \
+                                    const OpenBoundary&);                                                   \
   template int sph_predictor<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_corrector<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_xsph<R>(hipStream_t, const DevParams&, const SphView<R>&);                                \

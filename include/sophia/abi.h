@@ -149,6 +149,12 @@ typedef struct sophia_params {
   double wall_cone[5];  /* centre x, y; top radius r1, bottom radius r2, height h (:723-724, 4240-4242) */
   double wall_rest[2];  /* restitution: normal rest_n, tangential rest_s (:1-2) */
   double wall_mu[2];    /* friction: sliding mu_s, rolling mu_rs (:4-5) */
+  /* open boundaries (open_boundary = 1): z bands [L1, L2) inlet and
+   * [L3, L4) outlet (functions_NNPS.cuh:2-5, 26-37), inlet velocity and
+   * temperature that the reference hard-codes (functions_BC.cuh:961-964) */
+  double ob_band[4];
+  double ob_inlet_u;
+  double ob_inlet_temp;
   int32_t reserved_i;
   double reserved[7];
 } sophia_params;

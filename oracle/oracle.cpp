@@ -35,6 +35,9 @@
  *   coupling reaction         momentum balance sum_i m_i da_i = -m_a F_a
  *   predictor/corrector/XSPH  exact restatement, checked against hand-written
  *                             per-particle formulas in the tests
+ *   open boundary             uniform interior fluid: the outlet copies it,
+ *                             the inlet mirrors its velocity about u_in;
+ *                             ghost node mirrored about the band plane
  *   DEM walls                 head-on impact: u_n -> -rest_n u_n, position
  *                             on the wall, spin unchanged; resting contact:
  *                             normal velocity removed, friction opposes slip;
@@ -376,6 +379,18 @@ orc_state* orc_create(const sophia_params* p, const part1* particles, int32_t n)
   // read_sph_dem (function_init.cuh:893-911): SPH = p_type <= 1000
   for (int i = 0; i < n; i++) {
     part1 q = particles[i];
+    if (p->open_boundary == 1) {
+      // c_initial_inner_outer_particle_single_sph/_dem (functions_NNPS.cuh:
+      // 8-76, 138-200): reset, then the inlet [L1, L2) and outlet [L3, L4)
+      // z bands of the fluid become buffer particles
+      const double* L = p->ob_band;
+      q.i_type = 1;
+      q.buffer_type = 0;
+      if (q.p_type < 1000 && q.p_type >= 1) {
+        if (q.z >= (L[0] - 1e-6) && q.z < (L[1] - 1e-6)) { q.buffer_type = 1; q.i_type = 2; }
+        if (q.z >= (L[2] - 1e-6) && q.z < (L[3] - 1e-6)) { q.buffer_type = 2; q.i_type = 2; }
+      }
+    }
     if (q.p_type <= 1000) s->sph.P1.push_back(q);
     else s->dem.P1.push_back(q);
   }
@@ -1323,6 +1338,74 @@ static void dem_walls(orc_state& s) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Open boundary: KERNEL_open_boundary_extrapolation3D_1_sph (functions_BC. */
+/* cuh:787-984) then _calc_sph (:1204-1217), every decouple_stride steps    */
+/* after the time update (ISPH_Calc_seperate.cuh:410-426)                   */
+/* ------------------------------------------------------------------------ */
+static void open_boundary(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int n = sp.n();
+  struct OBC { double pres, rho, m, ux, uy, uz, temp; bool set; };
+  std::vector<OBC> out(n, OBC{0, 0, 0, 0, 0, 0, 0, false});
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& q = sp.P1[i];
+    if (q.buffer_type <= 0 || q.p_type >= 1000) continue;
+    const double h = q.h, A = knl_tmpA(p.kernel_type, p.dim, h), range = p.kappa * h;
+    // ghost node mirrored about the inlet (L2) or outlet (L3) plane
+    const double xg = q.x, yg = q.y;
+    double zg = q.z;
+    if (q.buffer_type == 1) zg = 2 * p.ob_band[1] - q.z;
+    else if (q.buffer_type == 2) zg = 2 * p.ob_band[2] - q.z;
+    double trho = 0, tpres = 0, tuz = 0, tux = 0, tuy = 0, ttemp = 0, tflt = 0;
+    for_candidates(s, sp, xg, yg, zg, [&](int j) {
+      const part1& r = sp.P1[j];
+      if (!(r.p_type < 1000)) return;
+      const double d = dist_of(xg - r.x, yg - r.y, zg - r.z) + 1e-20;
+      if (!(d < range)) return;
+      if (!(r.p_type == 1 && r.buffer_type == 0)) return;
+      const double w = knl_w(p.kernel_type, A, h, d);
+      trho += r.rho / r.DEMpor * w * r.vol;
+      tpres += r.vol * r.pres * w;
+      tuz += r.vol * r.uz * w;
+      tux += r.vol * r.ux * w;
+      tuy += r.vol * r.uy * w;
+      ttemp += r.vol * r.temp * w;
+      tflt += r.vol * w;
+    });
+    OBC o;
+    o.set = true;
+    o.m = q.rho * q.vol0;
+    if (q.elix < 0.9999 && q.p_type > 0) o.m = q.rho * q.vol;
+    if (q.buffer_type == 1) {
+      o.pres = tpres / tflt;
+      o.rho = q.DEMpor * p.rho_fluid;
+      o.uz = p.ob_inlet_u * 2 - tuz / tflt;
+      o.ux = 0 - tux / tflt;
+      o.uy = 0 - tuy / tflt;
+      o.temp = p.ob_inlet_temp;
+    } else if (q.buffer_type == 2) {
+      o.pres = 0.0;
+      o.rho = q.DEMpor * (trho / tflt);
+      o.uz = tuz / tflt;
+      o.ux = tux / tflt;
+      o.uy = tuy / tflt;
+      o.temp = ttemp / tflt;
+    } else {
+      o.set = false;
+    }
+    out[i] = o;
+  }
+  for (int i = 0; i < n; i++) {
+    if (!out[i].set) continue;
+    part1& q = sp.P1[i];
+    q.pres = out[i].pres; q.rho = out[i].rho; q.m = out[i].m;
+    q.uz = out[i].uz; q.ux = out[i].ux; q.uy = out[i].uy; q.temp = out[i].temp;
+  }
+}
+
 void orc_time(orc_state* s, int32_t phase) {
   int stride = std::max(1, s->p.decouple_stride);
   bool sph_on = (s->count % stride) == 0;
@@ -1333,6 +1416,7 @@ void orc_time(orc_state* s, int32_t phase) {
   } else {
     if (sph_on) corrector(*s, s->sph, stride * s->p.dt, true);
     corrector(*s, s->dem, s->p.dt, false);
+    if (sph_on && s->p.open_boundary == 1) open_boundary(*s);
     if (s->p.dem_walls) dem_walls(*s);
     s->after_corrector = true;
   }

@@ -367,3 +367,36 @@ def test_dem_walls_match_oracle(prec):
     rep["spheres_moved_by_walls"] = moved
     _report("dem_walls_" + ("fp64" if prec == abi.PREC_FP64 else "mixed"), rep)
     assert moved > 0
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_open_boundary_matches_oracle(prec):
+    """Open boundaries (open_boundary = 1): the two bottom fluid layers are
+    inlet buffer particles, the two top ones outlet buffer particles; two
+    full steps (Jacobi PPE, buffer particles outside the system) against the
+    oracle, every SPH field."""
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.small_isph_dem()
+    p.open_boundary = 1
+    p.ob_band[0], p.ob_band[1], p.ob_band[2], p.ob_band[3] = -1.0, 0.002, 0.010, 1.0
+    g, o = Sophia(p, parts, precision=prec), Oracle(p, parts)
+    opts = abi.ppe_opts(method=abi.PPE_JACOBI)
+    tol = TOL[prec]
+    rep = {}
+    for step in range(2):
+        g.step(opts)
+        o.step(opts)
+        a1, _, _ = g.get_particles(abi.SPH)
+        b1, _, _ = o.get_particles(abi.SPH)
+        assert np.array_equal(a1["buffer_type"], b1["buffer_type"])
+        assert np.array_equal(a1["i_type"], b1["i_type"])
+        buf = b1["buffer_type"] > 0
+        compare_fields(a1, b1, ["x", "y", "z", "ux", "uy", "uz", "pres", "rho", "m", "temp"],
+                       tol * (10 if step else 1), f"step{step}/sph", report=rep)
+        compare_fields(a1, b1, ["ux", "uy", "uz", "pres", "rho", "m"], tol * (10 if step else 1),
+                       f"step{step}/sph/buffer", mask=buf, report=rep)
+    rep["inlet"] = int((b1["buffer_type"] == 1).sum())
+    rep["outlet"] = int((b1["buffer_type"] == 2).sum())
+    _report("open_boundary_" + ("fp64" if prec == abi.PREC_FP64 else "mixed"), rep)
+    assert rep["inlet"] == 200 and rep["outlet"] == 200

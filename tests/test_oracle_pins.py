@@ -579,3 +579,58 @@ def test_dem_truncated_cone_projection_lands_on_the_wall():
     d = math.hypot(q["x"] - xc, q["y"] - yc)
     assert abs(r2 + (h + q["z"]) / tn - r / sn - d) < 1e-12
     assert q["z"] != -0.005
+
+
+# ---------------------------------------------------------------------------
+# open boundary (functions_NNPS.cuh:8-76; functions_BC.cuh:787-984, 1204-1217)
+# ---------------------------------------------------------------------------
+def test_open_boundary_uniform_interior_closed_form():
+    """Uniform interior fluid: the outlet layers copy the interior velocity,
+    temperature and density and take p = 0; the inlet layers take the
+    interior pressure, rho_air eps, T_in and the velocity mirrored about
+    (0, 0, u_in): (-ux, -uy, 2 u_in - uz).  Band tagging from the z bands."""
+    dx, n, nz = 1e-3, 8, 12
+    p = configs.config_params(dx, 1e-6)
+    p.fg_solve = p.fp_solve = p.fv_solve = 0
+    p.open_boundary = 1
+    p.ob_band[0], p.ob_band[1], p.ob_band[2], p.ob_band[3] = -1.0, 0.0015, 0.0095, 1.0
+    g = np.arange(n) * dx
+    X, Y, Z = np.meshgrid(g, g, np.arange(nz) * dx, indexing="ij")
+    parts = np.zeros(X.size, abi.PART1)
+    parts["x"], parts["y"], parts["z"] = X.ravel(), Y.ravel(), Z.ravel()
+    u = (0.1, -0.05, 0.2)
+    parts["ux"], parts["uy"], parts["uz"] = u
+    parts["h"] = 1.3 * dx
+    parts["m"] = 1000.0 * dx ** 3
+    parts["rho"] = 1000.0
+    parts["pres"] = 1234.5
+    parts["temp"] = 310.0
+    parts["p_type"] = 1
+    parts["i_type"] = 1
+    parts["DEMpor"] = 1.0
+    o = Oracle(p, parts)
+    o.nnps()
+    o.prep()
+    o.predict()
+    o.correct()
+    s1, _, _ = o.get_particles(abi.SPH)
+    inlet = s1["z"] < 0.0015
+    outlet = s1["z"] >= 0.0095
+    assert inlet.sum() == 2 * n * n and outlet.sum() == 2 * n * n
+    assert (s1["buffer_type"][inlet] == 1).all() and (s1["buffer_type"][outlet] == 2).all()
+    assert (s1["i_type"][inlet | outlet] == 2).all() and (s1["buffer_type"][~(inlet | outlet)] == 0).all()
+    tol = 1e-12
+    # outlet: interior means
+    assert np.all(s1["pres"][outlet] == 0.0)
+    for f, v in zip(("ux", "uy", "uz"), u):
+        assert np.abs(s1[f][outlet] - v).max() < tol * abs(v)
+    assert np.abs(s1["temp"][outlet] - 310.0).max() < tol * 310
+    assert np.abs(s1["rho"][outlet] - 1000.0).max() < tol * 1000
+    # inlet: mirrored velocity, interior pressure, prescribed state
+    assert np.abs(s1["pres"][inlet] - 1234.5).max() < tol * 1234.5
+    assert np.abs(s1["ux"][inlet] + u[0]).max() < tol and np.abs(s1["uy"][inlet] + u[1]).max() < tol
+    assert np.abs(s1["uz"][inlet] - (2 * p.ob_inlet_u - u[2])).max() < tol * p.ob_inlet_u
+    assert np.all(s1["temp"][inlet] == p.ob_inlet_temp)
+    assert np.all(s1["rho"][inlet] == 1.0 * p.rho_fluid)
+    # mass from the pre-update density and vol0 = (h / h_over_dx)^3
+    assert np.abs(s1["m"][inlet | outlet] - 1000.0 * (1.3 * dx / 1.3) ** 3).max() < 1e-12 * 1000.0 * dx ** 3
