@@ -130,10 +130,11 @@ def main(argv=None) -> int:
                     help="preconditioner of the mixed-precision solve")
     ap.add_argument("--inner-rtol", type=float, default=0.001,
                     help="mixed solve: fold into the FP64 solution each time the FP32 residual drops by this factor")
-    ap.add_argument("--legs", default="fp64,fp64_pc,prev,extrap",
+    ap.add_argument("--legs", default="fp64,fp64_pc,prev,extrap,bc",
                     help="comparison legs to run: fp64 (plain FP64 path), fp64_pc (FP64, same preconditioner), "
                          "prev (mixed, previous-step guess), extrap (mixed, extrapolated guess without the "
-                         "learned correction)")
+                         "learned correction), bc (open boundaries and a DEM wall on the same column: "
+                         "cost of the two boundary kernels)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -239,6 +240,37 @@ def main(argv=None) -> int:
                   "ppe_iters": float(np.mean([s["iterations"] for s in ste])),
                   "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in ste]))}
 
+    boundaries = None
+    if "bc" in legs:
+        # the same step with open boundaries (bottom and top 2 mm of the fluid
+        # as inlet and outlet buffers) and a DEM box wall around the column;
+        # reports what the two boundary kernels cost per step
+        import copy
+        pb = copy.copy(params)
+        fl = parts["p_type"] == 1
+        z0, z1 = float(parts["z"][fl].min()), float(parts["z"][fl].max())
+        pb.open_boundary = 1
+        for k, v in enumerate([-1.0, z0 + 2e-3, z1 - 2e-3, 1.0]):
+            pb.ob_band[k] = v
+        pb.ob_inlet_u = 0.0
+        pb.ob_inlet_temp = 293.0
+        dm = parts["p_type"] > 1000
+        lo = [float(parts[c][dm].min()) for c in ("x", "y", "z")]
+        hi = [float(parts[c][dm].max()) for c in ("x", "y", "z")]
+        rad = float(parts["rad"][dm].max())
+        pb.dem_walls = 2
+        for k in range(3):
+            pb.wall_box[k] = 0.0 if k < 2 else 0.5 * (lo[k] + hi[k])
+            pb.wall_box[3 + k] = (2 * max(abs(lo[k]), abs(hi[k])) if k < 2 else hi[k] - lo[k]) + 4 * rad
+        steps_bc = max(1, min(a.steps, 3))
+        dtb, stb, repb, _, _, _ = run_gpu(pb, parts, abi.PREC_MIXED, mixed_opts, steps_bc, 1, timers=True,
+                                          mlp_weights=weights)
+        boundaries = {"ms_per_step": dtb / steps_bc * 1e3,
+                      "open_boundary_ms": repb.get("open_boundary", (0, 0.0))[1] / steps_bc,
+                      "dem_walls_ms": repb.get("dem_walls", (0, 0.0))[1] / steps_bc,
+                      "buffer_particles": int(((parts["z"] < z0 + 2e-3) | (parts["z"] >= z1 - 2e-3))[fl].sum()),
+                      "steps": steps_bc}
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_seconds)
@@ -282,6 +314,7 @@ def main(argv=None) -> int:
         "fp64_same_precond": fp64_pc,
         "mixed_previous_guess": prev,
         "mixed_extrapolated_guess": extrap,
+        "boundaries": boundaries,
         "speedup_vs_fp64_path": (value / fp64["value"]) if fp64 else None,
         "speedup_vs_fp64_same_precond": (value / fp64_pc["value"]) if fp64_pc else None,
         "kernel_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in sorted(rep.items())},

@@ -28,6 +28,7 @@ PPE_JACOBI, PPE_BICGSTAB = 0, 1
 PPE_FP64, PPE_MIXED_IR = 0, 1
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
+VTK_FLUID, VTK_DEM = 0, 1
 PREDICT, CORRECT = 0, 1
 ERRORS = {
     -1: "SOPHIA_E_ARG", -2: "SOPHIA_E_HIP", -3: "SOPHIA_E_IO", -4: "SOPHIA_E_STATE",
@@ -172,6 +173,8 @@ SIGNATURES = {
     "sophia_get_ppe_system": (C.c_int, [_ctx, _VP, _VP, _VP]),
     "sophia_ppe_apply": (C.c_int, [_ctx, _VP, _VP, C.c_int32]),
     "sophia_monitors": (C.c_int, [_ctx, C.c_int32, _VP]),
+    "sophia_write_vtk": (C.c_int, [_ctx, C.c_char_p, C.c_int32]),
+    "sophia_write_restart": (C.c_int, [_ctx, C.c_char_p]),
     "sophia_wcsph_stage": (C.c_int, [_ctx, C.c_int32]),
     "sophia_ppe_layout": (C.c_int, [_ctx, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "sophia_ppe_coarse": (C.c_int, [_ctx, _P(C.c_int32), _VP, _VP]),

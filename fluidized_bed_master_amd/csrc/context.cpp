@@ -1639,6 +1639,142 @@ int sophia_count(const sophia_ctx* c, int32_t species, int32_t* n) {
   return 0;
 }
 
+extern "C++" {
+namespace {
+
+// FloatSwap / IntSwap (functions_OUTPUT.cuh:68-121): each value as a
+// big-endian 4-byte word; doubles are rounded to float first
+void put_be32(FILE* f, uint32_t u) {
+  const unsigned char b[4] = {(unsigned char)(u >> 24), (unsigned char)(u >> 16), (unsigned char)(u >> 8),
+                              (unsigned char)u};
+  std::fwrite(b, 1, 4, f);
+}
+void put_float(FILE* f, double v) {
+  const float x = (float)v;
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  put_be32(f, u);
+}
+
+struct Records {
+  std::vector<part1> p1;
+  std::vector<part2> p2;
+  std::vector<part3> p3;
+};
+int fetch(const sophia_ctx* c, int species, Records& r) {
+  const int n = species == SOPHIA_SPH ? c->d.n_sph : c->d.n_dem;
+  r.p1.assign(n, part1{});
+  r.p2.assign(n, part2{});
+  r.p3.assign(n, part3{});
+  if (n == 0) return 0;
+  return c->get_particles(species, r.p1.data(), r.p2.data(), r.p3.data());
+}
+
+// one float field over the selected particles
+template <class G>
+void vtk_field(FILE* f, const char* name, int count, const Records& r, bool (*sel)(const part1&), G get) {
+  std::fprintf(f, "%s\t1\t%d\tfloat\n", name, count);
+  for (size_t i = 0; i < r.p1.size(); i++)
+    if (sel(r.p1[i])) put_float(f, get(i));
+}
+bool sel_fluid(const part1& q) { return q.p_type == 1; }
+bool sel_dem(const part1& q) { return q.p_type > 1000; }
+
+}  // namespace
+}  // extern "C++"
+
+int sophia_write_vtk(const sophia_ctx* c, const char* path, int32_t kind) {
+  if (!c || !path || (kind != SOPHIA_VTK_FLUID && kind != SOPHIA_VTK_DEM)) return SOPHIA_E_ARG;
+  Records r;
+  TRY(fetch(c, kind == SOPHIA_VTK_FLUID ? SOPHIA_SPH : SOPHIA_DEM, r));
+  bool (*sel)(const part1&) = kind == SOPHIA_VTK_FLUID ? sel_fluid : sel_dem;
+  int count = 0;
+  for (const part1& q : r.p1) count += kind == SOPHIA_VTK_FLUID ? (q.p_type == 1) : (q.dem_idx > 0);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    set_error("cannot open %s for writing", path);
+    return SOPHIA_E_IO;
+  }
+  std::fprintf(f, "# vtk DataFile Version 3.0\n");
+  std::fprintf(f, "Print out results in vtk format\n");
+  std::fprintf(f, "BINARY\n");
+  std::fprintf(f, "DATASET POLYDATA\n");
+  std::fprintf(f, "POINTS\t%d\tfloat\n", count);
+  for (const part1& q : r.p1)
+    if (sel(q)) {
+      put_float(f, q.x);
+      put_float(f, q.y);
+      put_float(f, q.z);
+    }
+  std::fprintf(f, "POINT_DATA\t%d\n", count);
+  const auto& P1 = r.p1;
+  const auto& P3 = r.p3;
+  if (kind == SOPHIA_VTK_FLUID) {
+    // save_plot_fluid_vtk_bin_fluid_sph (:445-659)
+    std::fprintf(f, "FIELD FieldData\t9\n");
+    vtk_field(f, "porosity", count, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+    vtk_field(f, "conv_rate", count, r, sel, [&](size_t i) { return P1[i].Q_sdf; });
+    vtk_field(f, "pressure", count, r, sel, [&](size_t i) { return P1[i].pres; });
+    vtk_field(f, "cond_rate", count, r, sel, [&](size_t i) { return P1[i].Q_f; });
+    vtk_field(f, "dtemp", count, r, sel, [&](size_t i) { return P3[i].dtemp; });
+    std::fprintf(f, "buffer_type\t1\t%d\tint\n", count);
+    for (const part1& q : P1)
+      if (sel(q)) put_be32(f, (uint32_t)q.buffer_type);
+    vtk_field(f, "density", count, r, sel, [&](size_t i) { return P1[i].rho; });
+    vtk_field(f, "temperature", count, r, sel, [&](size_t i) { return P1[i].temp; });
+    vtk_field(f, "uz", count, r, sel, [&](size_t i) { return P1[i].uz; });
+  } else {
+    // save_plot_fluid_vtk_bin_solid_dem (:1135-1446)
+    std::fprintf(f, "FIELD FieldData\t10\n");
+    vtk_field(f, "itype", count, r, sel, [&](size_t i) { return (double)P1[i].i_type; });
+    vtk_field(f, "mag_uijf", count, r, sel, [&](size_t i) { return P1[i].test6; });
+    vtk_field(f, "Euler", count, r, sel, [&](size_t i) { return P1[i].eliz; });
+    vtk_field(f, "porosity", count, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+    vtk_field(f, "uz", count, r, sel, [&](size_t i) { return P1[i].uz; });
+    vtk_field(f, "Q_sd", count, r, sel, [&](size_t i) { return P1[i].Q_sd; });
+    vtk_field(f, "ftotal_z", count, r, sel, [&](size_t i) { return P3[i].ftotalz; });
+    vtk_field(f, "temperature_surface", count, r, sel, [&](size_t i) { return P1[i].temp; });
+    vtk_field(f, "rad", count, r, sel, [&](size_t i) { return P1[i].rad; });
+    vtk_field(f, "dtemp_dt", count, r, sel, [&](size_t i) { return P3[i].dtemp; });
+  }
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  if (!ok) {
+    set_error("write error on %s", path);
+    return SOPHIA_E_IO;
+  }
+  return 0;
+}
+
+int sophia_write_restart(const sophia_ctx* c, const char* path) {
+  if (!c || !path) return SOPHIA_E_ARG;
+  Records sph, dem;
+  TRY(fetch(c, SOPHIA_SPH, sph));
+  TRY(fetch(c, SOPHIA_DEM, dem));
+  FILE* f = std::fopen(path, "w");
+  if (!f) {
+    set_error("cannot open %s for writing", path);
+    return SOPHIA_E_IO;
+  }
+  std::fprintf(f, "1 2 3 4 5 6 7 8 9 10 11 12 13 26\n");
+  for (const Records* r : {&sph, &dem})
+    for (size_t i = 0; i < r->p1.size(); i++) {
+      const part1& q = r->p1[i];
+      std::fprintf(f, "%f\t%f\t%f\t", q.x, q.y, q.z);
+      std::fprintf(f, "%f\t%f\t%f\t", q.ux, q.uy, q.uz);
+      std::fprintf(f, "%e\t%d\t%e\t", q.m, q.p_type, q.h);
+      std::fprintf(f, "%f\t%f\t%f\t", q.temp, q.pres, q.rho);
+      std::fprintf(f, "%f\t%f\n", r->p2[i].rho_ref, q.flt_s);
+    }
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  if (!ok) {
+    set_error("write error on %s", path);
+    return SOPHIA_E_IO;
+  }
+  return 0;
+}
+
 int sophia_nnps(sophia_ctx* c) { CHECK_CTX(c); return c->nnps(); }
 int sophia_prep(sophia_ctx* c) { CHECK_CTX(c); return c->prep(); }
 int sophia_interaction(sophia_ctx* c) { CHECK_CTX(c); return c->interaction(); }

@@ -174,6 +174,15 @@ class Sophia:
         """One stage (0-8) of the WCSPH step (solver_type 0)."""
         check(self.lib.sophia_wcsph_stage(self._ctx, stage), "sophia_wcsph_stage")
 
+    def write_vtk(self, path, kind: int = 0) -> None:
+        """Legacy binary VTK of the fluid (kind 0) or the spheres (kind 1),
+        in the reference writers' layout (functions_OUTPUT.cuh)."""
+        check(self.lib.sophia_write_vtk(self._ctx, str(path).encode(), kind), "sophia_write_vtk")
+
+    def write_restart(self, path) -> None:
+        """restart.txt of save_restart (functions_OUTPUT.cuh:1926-1955)."""
+        check(self.lib.sophia_write_restart(self._ctx, str(path).encode()), "sophia_write_restart")
+
     def monitors(self, species: int = SPH) -> np.ndarray:
         """max |u|, max rho, max ftotal of one species."""
         out = np.zeros(3)

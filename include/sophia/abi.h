@@ -297,6 +297,24 @@ int sophia_wcsph_stage(sophia_ctx* ctx, int32_t stage);
 int sophia_monitors(sophia_ctx* ctx, int32_t species, double out[3]);
 /* number of particles of a species */
 int sophia_count(const sophia_ctx* ctx, int32_t species, int32_t* n);
+/* Output files of the current state, in sorted order as the reference writes
+ * its arrays.  Legacy VTK 3.0 binary POLYDATA, big-endian 4-byte words
+ * (FloatSwap / IntSwap, functions_OUTPUT.cuh:68-121):
+ *   SOPHIA_VTK_FLUID  save_plot_fluid_vtk_bin_fluid_sph (:445-659): fluid
+ *                     particles (p_type 1), points and 9 fields;
+ *   SOPHIA_VTK_DEM    save_plot_fluid_vtk_bin_solid_dem (:1135-1446): spheres
+ *                     (p_type > 1000; the point count is the spheres with
+ *                     dem_idx > 0, as the reference counts), 10 fields. */
+#define SOPHIA_VTK_FLUID 0
+#define SOPHIA_VTK_DEM 1
+int sophia_write_vtk(const sophia_ctx* ctx, const char* path, int32_t kind);
+/* restart.txt of save_restart (functions_OUTPUT.cuh:1926-1955): a header of
+ * input column codes "1 2 3 4 5 6 7 8 9 10 11 12 13 26" and one line per
+ * particle (SPH, then spheres): x y z ux uy uz (%f), m (%e), p_type (%d),
+ * h (%e), temp pres rho rho_ref flt_s (%f), tab-separated.  sophia_create
+ * reads it back as an input file (column 13 is skipped, as the reference's
+ * reader does). */
+int sophia_write_restart(const sophia_ctx* ctx, const char* path);
 /* evaluate W and dW/dr on the device (calc_kernel_wij / calc_kernel_dwij) */
 int sophia_knl_eval(int32_t kernel_type, int32_t dim, double h, const double* r,
                     int32_t n, double* w, double* dw);

@@ -116,3 +116,24 @@ def test_create_rejects_bad_arguments():
     with pytest.raises(abi.SophiaError) as e:
         Sophia(p, parts)
     assert e.value.rc == -1
+
+
+def test_reference_dem_vtk_layout_fixture():
+    """The committed layout of a reference DEM output file (made by
+    tests/golden/make_vtk_layout.py from plotdata/demDEM_*_0stp.vtk): the
+    byte length is the header lines plus 4-byte words, all big-endian
+    (positions of order 1e-3 m, radius 0.4 mm), and the field sequence is the
+    one save_plot_fluid_vtk_bin_solid_dem writes."""
+    import json
+    lay = json.load(open(GOLD / "reference_dem_vtk_layout.json"))
+    n = lay["n_points"]
+    text = sum(len(l) + 1 for l in lay["header"]) + len(lay["point_data"]) + 1 + len(lay["field_line"]) + 1
+    text += sum(len(f["line"]) + 1 for f in lay["fields"])
+    words = 3 * n + sum(f["count"] * f["ncomp"] for f in lay["fields"])
+    assert lay["bytes"] == text + 4 * words
+    assert [f["name"] for f in lay["fields"]] == ["itype", "mag_uijf", "Euler", "porosity", "uz", "Q_sd", "ftotal_z",
+                                                  "temperature_surface", "rad", "dtemp_dt"]
+    assert all(f["type"] == "float" and f["ncomp"] == 1 and f["count"] == n for f in lay["fields"])
+    assert all(abs(x) < 0.1 for x in lay["points_head"])
+    rad = [f for f in lay["fields"] if f["name"] == "rad"][0]["head"]
+    assert all(abs(r - 4e-4) < 1e-9 for r in rad)
