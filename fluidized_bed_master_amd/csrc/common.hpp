@@ -27,6 +27,15 @@ constexpr uint32_t kEmpty = 0xffffffffu;  // empty cell marker
 constexpr int kMaxHist = 16;              // DEM shear-history slots
 constexpr int kBlock = 256;               // default workgroup size (4 waves)
 constexpr uint32_t kNbrFlag = 0x80000000u;  // set on list entries that fail r+1e-20 < R
+// The SPH neighbour list is stored in chunks of 8 consecutive slots of one
+// particle (32 bytes): slot k of particle i of n sits at nbr_slot(k, i, n).
+// The builder writes whole chunks, so no 32-byte sector is written piecemeal,
+// and pads a particle's last chunk with its last entry: a 16-byte load of
+// slots 4q..4q+3 with 4q < count reads valid particle indices only.
+constexpr int kNbrChunk = 8;
+__host__ __device__ inline size_t nbr_slot(int k, int i, int n) {
+  return (((size_t)(k >> 3) * (size_t)n + (size_t)i) << 3) + (size_t)(k & 7);
+}
 
 // error reporting
 void set_error(const char* fmt, ...);
@@ -91,7 +100,8 @@ struct SphView {
   // axis); differences come from rel_diff.  Rebuilt by NNPS and after every
   // position update (ISPH corrector, WCSPH predictor and corrector).
   float4* pos4;
-  // SPH-SPH neighbour list (slot major: nbr[k * n + i])
+  // SPH-SPH neighbour list (chunked slots: nbr[nbr_slot(k, i, n)]); nbr_cap is
+  // a multiple of 8
   uint32_t* nbr;
   int* nbr_cnt;
   int nbr_cap;

@@ -289,6 +289,7 @@ struct Ctx : public sophia_ctx {
 
   int alloc_nbr(int cap) {
     if (sv.nbr) arena.drop(sv.nbr);
+    cap = (cap + kNbrChunk - 1) / kNbrChunk * kNbrChunk;  // whole chunks
     sv.nbr_cap = cap;
     TRY(arena.alloc(&sv.nbr, (size_t)cap * std::max(ns, 1)));
     ppe_workspace_free(ppe_w);
@@ -375,7 +376,7 @@ struct Ctx : public sophia_ctx {
     TRY(arena.alloc(&maxcnt, 1));
     int cap = p.sph_max_neighbors > 0 ? p.sph_max_neighbors : 96;
     TRY(alloc_nbr(cap));
-    TRY(mlp_alloc(mlp, ns, cap));
+    TRY(mlp_alloc(mlp, ns, sv.nbr_cap));
     // upload
     refresh_views();
     TRY(upload(P, si, di));

@@ -94,7 +94,7 @@ __device__ inline EdgeTile<R> edge_tile(const SphView<R>& v, const float2* nf, i
   if (lane < 16) {
     const int e = c * 16 + lane;
     if (e < cnt) {
-      const uint32_t ent = v.nbr[(size_t)e * v.n + i];
+      const uint32_t ent = v.nbr[nbr_slot(e, i, v.n)];
       if (!(ent & kNbrFlag)) {
         const int j = (int)ent;
         const double ih = 1.0 / v.h[i];
@@ -353,11 +353,11 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
       const float ih = isrow ? (float)(1.0 / (double)v.h[i]) : 0.f;
       float acc[4][4] = {};
       int nvalid = 0;  // edges this lane gathered (particle l mod 16)
-      uint32_t nxt = g4 < cnt ? v.nbr[(size_t)g4 * v.n + i] : kNbrFlag;
+      uint32_t nxt = g4 < cnt ? v.nbr[nbr_slot(g4, i, v.n)] : kNbrFlag;
       for (int k0 = 0; k0 < maxc; k0 += 4) {
         const uint32_t ent = nxt;
         const int kn = k0 + 4 + g4;
-        if (k0 + 4 < maxc) nxt = kn < cnt ? v.nbr[(size_t)kn * v.n + i] : kNbrFlag;
+        if (k0 + 4 < maxc) nxt = kn < cnt ? v.nbr[nbr_slot(kn, i, v.n)] : kNbrFlag;
         // an edge outside the list gets all-zero inputs, bias inputs
         // included, so its layer-1 and layer-2 sums are exactly 0 and it adds
         // nothing to the message

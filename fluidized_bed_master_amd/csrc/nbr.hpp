@@ -60,18 +60,17 @@ __device__ inline void for_nbrs4(const SphView<R>& v, int i, bool eps_rule, F&& 
   } else {
     o.x = v.x[i]; o.y = v.y[i]; o.z = v.z[i];
   }
-  uint32_t en[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(u, cnt - 1) * n + i];
+  const int last4 = (cnt - 1) & ~3;
+  uint4 en4 = *(const uint4*)(v.nbr + nbr_slot(0, i, (int)n));
   for (int k = 0; k < cnt; k += 4) {
+    const uint32_t en[4] = {en4.x, en4.y, en4.z, en4.w};
     uint32_t e[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) e[u] = en[u];
     Nb<R> q[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) q[u] = load_nb<REL>(v, (int)(e[u] & ~kNbrFlag), o);
-#pragma unroll
-    for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(k + 4 + u, cnt - 1) * n + i];
+    en4 = *(const uint4*)(v.nbr + nbr_slot(min(k + 4, last4), i, (int)n));
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const bool on = (k + u < cnt) && !(eps_rule && (e[u] & kNbrFlag));

@@ -208,6 +208,18 @@ __global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt,
     const float4 pi = v.pos4[i];
     const float g = v.pgrid;
     const int n = v.n, cap = v.nbr_cap;
+    // the last 8 entries, oldest first; a full chunk goes out as two 16-byte stores
+    uint32_t r[kNbrChunk] = {};
+    auto put = [&](int k0) {
+      uint4* p = (uint4*)(v.nbr + nbr_slot(k0, i, n));
+      p[0] = make_uint4(r[0], r[1], r[2], r[3]);
+      p[1] = make_uint4(r[4], r[5], r[6], r[7]);
+    };
+    auto shift_in = [&](uint32_t e) {
+#pragma unroll
+      for (int t = 0; t < kNbrChunk - 1; t++) r[t] = r[t + 1];
+      r[kNbrChunk - 1] = e;
+    };
     for_cells(P, xi, yi, zi, cv, [&](int j) {
       const float3 dd = rel_diff(pi, v.pos4[j], g);
       const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
@@ -218,9 +230,16 @@ __global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt,
         if (!(r < Rs)) return;
         if (!(r + 1e-20 < Rs)) e |= kNbrFlag;
       }
-      if (c < cap) v.nbr[(size_t)c * n + i] = e;
+      shift_in(e);
       c++;
+      if ((c & (kNbrChunk - 1)) == 0 && c <= cap) put(c - kNbrChunk);
     });
+    const int m = c & (kNbrChunk - 1);
+    if (m != 0 && c < cap) {  // pad the last chunk with the last entry
+      const uint32_t last = r[kNbrChunk - 1];
+      for (int t = m; t < kNbrChunk; t++) shift_in(last);
+      put(c - m);
+    }
     v.nbr_cnt[i] = c;
   }
   // one atomic per wave for the largest count (capacity check on the host)

@@ -374,11 +374,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     R ux, uy, uz;
     int pt, it, row;
   };
-  uint32_t en[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) en[u] = cnt > 0 ? v.nbr[(size_t)min(u, cnt - 1) * n + i] : 0u;
+  const int last4 = (cnt - 1) & ~3;
+  uint4 en4 = cnt > 0 ? *(const uint4*)(v.nbr + nbr_slot(0, i, (int)n)) : make_uint4(0u, 0u, 0u, 0u);
   for (int k0 = 0; k0 < cnt; k0 += 4) {
     uint32_t eb[4];
+    const uint32_t en[4] = {en4.x, en4.y, en4.z, en4.w};
     AsmNb nb[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -394,8 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
       nb[u].ux = v.ux[j]; nb[u].uy = v.uy[j]; nb[u].uz = v.uz[j];
       nb[u].pt = v.ptype[j]; nb[u].it = v.itype[j]; nb[u].row = o.row[j];
     }
-#pragma unroll
-    for (int u = 0; u < 4; u++) en[u] = v.nbr[(size_t)min(k0 + 4 + u, cnt - 1) * n + i];
+    en4 = *(const uint4*)(v.nbr + nbr_slot(min(k0 + 4, last4), i, (int)n));
 #pragma unroll
     for (int ub = 0; ub < 4; ub++) {
     if (k0 + ub >= cnt) break;
