@@ -219,6 +219,49 @@ __device__ inline void for_cells(const DevParams& P, double x, double y, double 
   }
 }
 
+// The same 27 cells scanned by a whole wave: every lane passes the same point,
+// and the candidates of each row of three cells (fixed dz, dy) are dealt out
+// over the 64 lanes, so each lane sees a subset and the caller adds the lanes'
+// partial sums with wave_sum.  For work items too few to fill the chip one
+// thread each (DEM spheres).
+template <class F>
+__device__ inline void wave_for_cells(const DevParams& P, double x, double y, double z, const CellView& cv, F&& f) {
+  int ic, jc, kc;
+  cell_of(P, x, y, z, ic, jc, kc);
+  const uint32_t lane = threadIdx.x & 63;
+  for (int dz = -1; dz <= 1; dz++) {
+    int c = kc + dz;
+    if (c < 0 || c > P.NK - 1) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      int b = jc + dy;
+      if (b < 0 || b > P.NJ - 1) continue;
+      uint32_t s[3], len[3];
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++) {
+        s[dx] = 0;
+        len[dx] = 0;
+        int a = ic + dx - 1;
+        if (a < 0 || a > P.NI - 1) continue;
+        int k = cell_key(P, a, b, c);
+        uint32_t st = cv.start[k];
+        if (st == kEmpty) continue;
+        s[dx] = st;
+        len[dx] = cv.end[k] - st;
+      }
+      const uint32_t l01 = len[0] + len[1], tot = l01 + len[2];
+      for (uint32_t t = lane; t < tot; t += 64)
+        f((int)(t < len[0] ? s[0] + t : t < l01 ? s[1] + (t - len[0]) : s[2] + (t - l01)));
+    }
+  }
+}
+
+// sum over the 64 lanes; the butterfly gives every lane the same bits
+__device__ inline double wave_sum(double a) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+  return a;
+}
+
 // the (2 rng + 1)^3 cells around (x,y,z), same visiting order as for_cells
 template <class F>
 __device__ inline void for_cells_r(const DevParams& P, double x, double y, double z, const CellView& cv, int rng,

@@ -17,7 +17,7 @@ namespace {
 // Shepard sums seen by each sphere and the sphere-phase porosity.
 template <typename R>
 __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc, CellView dc) {
-  int i = particle_index();
+  const int i = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);  // one wave per sphere
   if (i >= d.n) return;
   if (d.itype[i] > 2) return;
   const int pti = d.ptype[i];
@@ -25,7 +25,7 @@ __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc
   const double xi = d.x[i], yi = d.y[i], zi = d.z[i];
   double fltd = 0, fltd2 = 0, flt = 0, por = 0;
   if (v.n > 0) {
-    for_cells(P, xi, yi, zi, sc, [&](int j) {
+    wave_for_cells(P, xi, yi, zi, sc, [&](int j) {
       double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
       if (!(r < Rs)) return;
       if (!(pti > 1000)) return;
@@ -41,7 +41,7 @@ __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc
       }
     });
   }
-  for_cells(P, xi, yi, zi, dc, [&](int j) {
+  wave_for_cells(P, xi, yi, zi, dc, [&](int j) {
     double r = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]) + 1e-20;
     if (!(r < Rs)) return;
     if (pti > 1000 && d.ptype[j] > 1000) {
@@ -49,6 +49,11 @@ __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc
       por += knl_w<double>(P.kernel_type, A, h, r) * 4.18879 * rj * rj * rj;
     }
   });
+  fltd = wave_sum(fltd);
+  fltd2 = wave_sum(fltd2);
+  flt = wave_sum(flt);
+  por = wave_sum(por);
+  if ((threadIdx.x & 63) != 0) return;
   if (pti > 1000) {
     d.flt_s[i] = flt;
     d.flt_sd[i] = fltd;
@@ -175,7 +180,7 @@ __global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
 // drag (fluid regime) and the pressure-gradient force.
 template <typename R>
 __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
+  const int i = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);  // one wave per sphere
   if (i >= d.n) return;
   if (d.itype[i] != 1 || d.ptype[i] <= 1000) return;
   const double xi = d.x[i], yi = d.y[i], zi = d.z[i], uxi = d.ux[i], uyi = d.uy[i], uzi = d.uz[i];
@@ -184,7 +189,7 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
   const double h = d.h[i], A = knl_tmpA<double>(P.kernel_type, P.dim, h), Rs = P.kappa * h;
   double tx = 0, ty = 0, tz = 0, ufx = 0, ufy = 0, ufz = 0, uax = 0, uay = 0, uaz = 0, por = 0;
   if (v.n > 0) {
-    for_cells(P, xi, yi, zi, sc, [&](int j) {
+    wave_for_cells(P, xi, yi, zi, sc, [&](int j) {
       const int t = v.ptype[j];
       const double porj = (double)v.por[j];
       if (!(t <= 1000 && porj < 1.0)) return;
@@ -212,6 +217,11 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
       }
     });
   }
+  tx = wave_sum(tx); ty = wave_sum(ty); tz = wave_sum(tz);
+  ufx = wave_sum(ufx); ufy = wave_sum(ufy); ufz = wave_sum(ufz);
+  uax = wave_sum(uax); uay = wave_sum(uay); uaz = wave_sum(uaz);
+  por = wave_sum(por);
+  if ((threadIdx.x & 63) != 0) return;
   const double pori = por / (flt_fluid + 1.0e-15);
   const double sf = flt_fluid + 1.0e-15, sa = flt_air + 1.0e-15, sm = flt_fluid + flt_air + 1.0e-15;
   const double rfx = ufx / sf - uxi, rfy = ufy / sf - uyi, rfz = ufz / sf - uzi;
@@ -528,7 +538,7 @@ template <typename R>
 int dem_porosity(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc,
                  const CellView& dc) {
   if (d.n <= 0) return 0;
-  k_porosity_dem<R><<<grid_xcd(d.n), kBlock, 0, s>>>(P, d, v, sc, dc);
+  k_porosity_dem<R><<<(d.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(P, d, v, sc, dc);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -543,7 +553,7 @@ int dem_contact(hipStream_t s, const DevParams& P, const DemView& d, const CellV
 template <typename R>
 int dem_coupling(hipStream_t s, const DevParams& P, const DemView& d, const SphView<R>& v, const CellView& sc) {
   if (d.n <= 0) return 0;
-  k_dem_coupling<R><<<grid_xcd(d.n), kBlock, 0, s>>>(P, d, v, sc);
+  k_dem_coupling<R><<<(d.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(P, d, v, sc);
   HIP_TRY(hipGetLastError());
   return 0;
 }
