@@ -219,16 +219,16 @@ __device__ inline void for_cells(const DevParams& P, double x, double y, double 
   }
 }
 
-// The same 27 cells scanned by a whole wave: every lane passes the same point,
-// and the candidates of each row of three cells (fixed dz, dy) are dealt out
-// over the 64 lanes, so each lane sees a subset and the caller adds the lanes'
-// partial sums with wave_sum.  For work items too few to fill the chip one
-// thread each (DEM spheres).
-template <class F>
-__device__ inline void wave_for_cells(const DevParams& P, double x, double y, double z, const CellView& cv, F&& f) {
+// The same 27 cells scanned by a group of G aligned lanes: every lane of the
+// group passes the same point, and the candidates of each row of three cells
+// (fixed dz, dy) are dealt out over the G lanes, so each lane sees a subset
+// and the caller adds the lanes' partial sums with group_sum<G>.  For work
+// items too few to fill the chip one thread each (DEM spheres, walls).
+template <int G, class F>
+__device__ inline void group_for_cells(const DevParams& P, double x, double y, double z, const CellView& cv, F&& f) {
   int ic, jc, kc;
   cell_of(P, x, y, z, ic, jc, kc);
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & (G - 1);
   for (int dz = -1; dz <= 1; dz++) {
     int c = kc + dz;
     if (c < 0 || c > P.NK - 1) continue;
@@ -249,18 +249,25 @@ __device__ inline void wave_for_cells(const DevParams& P, double x, double y, do
         len[dx] = cv.end[k] - st;
       }
       const uint32_t l01 = len[0] + len[1], tot = l01 + len[2];
-      for (uint32_t t = lane; t < tot; t += 64)
+      for (uint32_t t = lane; t < tot; t += G)
         f((int)(t < len[0] ? s[0] + t : t < l01 ? s[1] + (t - len[0]) : s[2] + (t - l01)));
     }
   }
 }
+template <class F>
+__device__ inline void wave_for_cells(const DevParams& P, double x, double y, double z, const CellView& cv, F&& f) {
+  group_for_cells<64>(P, x, y, z, cv, static_cast<F&&>(f));
+}
 
-// sum over the 64 lanes; the butterfly gives every lane the same bits
-__device__ inline double wave_sum(double a) {
+// sum over a group of G aligned lanes; the butterfly gives every lane of the
+// group the same bits
+template <int G, typename T>
+__device__ inline T group_sum(T a) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+  for (int off = G / 2; off > 0; off >>= 1) a += __shfl_xor(a, off);
   return a;
 }
+__device__ inline double wave_sum(double a) { return group_sum<64>(a); }
 
 // the (2 rng + 1)^3 cells around (x,y,z), same visiting order as for_cells
 template <class F>

@@ -291,10 +291,14 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
 // KERNEL_Neumann_boundary3D_sph (functions_BC.cuh:139-274): wall particles
 // take a Shepard interpolation of fluid p, T and eps-weighted rho over the
 // 27 cells within 2*kappa*h, with W(2h) normalised by A(h).
+constexpr int kNeumannLanes = 8;
+
 template <typename R>
 __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs) {
   typedef R T;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;  // linear: work concentrates near walls/spheres
+  // kNeumannLanes lanes per particle: wall particles are a minority that
+  // clusters in cell order, one thread each leaves the chip nearly idle
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kNeumannLanes);
   if (i >= v.n) return;
   if (v.ptype[i] > 0) return;
   const double hd = v.h[i];
@@ -314,7 +318,7 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs
   const double tol = tol_abs + R2 * (1.0 / 1048576.0);
   const float lo2 = (float)((R2 - tol) * (R2 - tol)), hi2 = (float)((R2 + tol) * (R2 + tol));
   const float4 pi4 = REL ? v.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  for_cells(P, xi, yi, zi, cv, [&](int j) {
+  group_for_cells<kNeumannLanes>(P, xi, yi, zi, cv, [&](int j) {
     double rd;
     if constexpr (REL) {
       const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid);
@@ -336,6 +340,11 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs
     tt += wv * (T)v.temp[j];
     tr += v.rho[j] / v.por[j] * (double)wv;
   });
+  tp = group_sum<kNeumannLanes>(tp);
+  fl = group_sum<kNeumannLanes>(fl);
+  tt = group_sum<kNeumannLanes>(tt);
+  tr = group_sum<kNeumannLanes>(tr);
+  if ((threadIdx.x & (kNeumannLanes - 1)) != 0) return;
   if (fl < 1e-6) {
     v.rho[i] = v.rho_ref[i];
     v.pres[i] = 0.0;
@@ -643,7 +652,12 @@ template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv) {
   const double m = fmax(fmax(fmax(fabs(P.x_min), fabs(P.x_max)), fmax(fabs(P.y_min), fabs(P.y_max))),
                         fmax(fabs(P.z_min), fabs(P.z_max)));
-  LAUNCH(k_neumann, P, v, cv, m * (1.0 / 2097152.0));
+  if (v.n > 0) {
+    const size_t nt = (size_t)v.n * kNeumannLanes;
+    k_neumann<R><<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, v, cv, m * (1.0 / 2097152.0));
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
 }
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
 template <typename R>
