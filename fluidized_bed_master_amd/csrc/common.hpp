@@ -453,7 +453,7 @@ __device__ inline int particle_index() {
 // ---------------------------------------------------------------------------
 struct SortScratch {
   uint32_t *keys_a, *keys_b, *vals_a, *vals_b;
-  uint32_t* hist;   // 256 * max_tiles
+  uint32_t* hist;   // 256 * max_tiles, then 256 digit totals
   int max_tiles;
 };
 

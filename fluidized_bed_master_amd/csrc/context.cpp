@@ -371,7 +371,7 @@ struct Ctx : public sophia_ctx {
       TRY(arena.alloc(&sc.vals_a, m));
       TRY(arena.alloc(&sc.vals_b, m));
       sc.max_tiles = (int)((m + 4095) / 4096);
-      TRY(arena.alloc(&sc.hist, (size_t)256 * sc.max_tiles));
+      TRY(arena.alloc(&sc.hist, (size_t)256 * sc.max_tiles + 256));
     }
     TRY(arena.alloc(&maxcnt, 1));
     int cap = p.sph_max_neighbors > 0 ? p.sph_max_neighbors : 96;

@@ -55,28 +55,48 @@ __global__ __launch_bounds__(kSortThreads) void k_hist(int n, int shift, const u
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of m entries by one workgroup of 1024 threads
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ a, int m) {
-  __shared__ uint32_t part[1024];
-  int t = threadIdx.x;
-  int per = (m + 1023) / 1024;
-  int lo = t * per, hi = min(lo + per, m);
-  uint32_t s = 0;
-  for (int i = lo; i < hi; i++) s += a[i];
-  part[t] = s;
+// Exclusive scan of the digit-major histogram hist[d * ntiles + tile] in two
+// launches of 256 workgroups: k_scan_rows scans each digit's row in place and
+// writes the row total to tot[d]; k_scan_add adds the totals of the digits
+// below d to row d.
+__global__ __launch_bounds__(256) void k_scan_rows(uint32_t* __restrict__ a, int m, uint32_t* __restrict__ tot) {
+  __shared__ uint32_t ws[4];
+  uint32_t* row = a + (size_t)blockIdx.x * m;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t carry = 0;
+  for (int base = 0; base < m; base += 256) {
+    const int i = base + t;
+    const uint32_t x = i < m ? row[i] : 0u;
+    uint32_t s = x;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(s, off);
+      if (lane >= off) s += y;
+    }
+    if (lane == 63) ws[w] = s;
+    __syncthreads();
+    uint32_t wp = 0;
+    for (int q = 0; q < w; q++) wp += ws[q];
+    const uint32_t all = ws[0] + ws[1] + ws[2] + ws[3];
+    if (i < m) row[i] = carry + wp + s - x;
+    carry += all;
+    __syncthreads();
+  }
+  if (t == 0) tot[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ a, int m, const uint32_t* __restrict__ tot) {
+  __shared__ uint32_t ws[4];
+  const int t = threadIdx.x, d = blockIdx.x;
+  uint32_t v = t < d ? tot[t] : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((t & 63) == 0) ws[t >> 6] = v;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[t] - s;  // exclusive prefix of this thread's chunk
-  for (int i = lo; i < hi; i++) {
-    uint32_t v = a[i];
-    a[i] = run;
-    run += v;
-  }
+  const uint32_t off = ws[0] + ws[1] + ws[2] + ws[3];
+  if (off == 0) return;
+  uint32_t* row = a + (size_t)d * m;
+  for (int i = t; i < m; i += 256) row[i] += off;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_scatter(int n, int shift, const uint32_t* __restrict__ kin,
@@ -275,7 +295,8 @@ int radix_sort_pairs(hipStream_t s, int n, int nbits, SortScratch& sc, uint32_t*
     }
     for (int shift = 0; shift < nbits; shift += 8) {
       k_hist<<<ntiles, kSortThreads, 0, s>>>(n, shift, ka, sc.hist, ntiles);
-      k_scan<<<1, 1024, 0, s>>>(sc.hist, 256 * ntiles);
+      k_scan_rows<<<256, 256, 0, s>>>(sc.hist, ntiles, sc.hist + (size_t)256 * sc.max_tiles);
+      k_scan_add<<<256, 256, 0, s>>>(sc.hist, ntiles, sc.hist + (size_t)256 * sc.max_tiles);
       k_scatter<<<ntiles, kSortThreads, 0, s>>>(n, shift, ka, va, kb, vb, sc.hist, ntiles);
       HIP_TRY(hipGetLastError());
       uint32_t* t = ka; ka = kb; kb = t;
