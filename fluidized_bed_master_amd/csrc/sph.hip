@@ -368,7 +368,9 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs
 // (the _calc kernel) are one pass.
 template <typename R>
 __global__ void k_open_boundary(DevParams P, SphView<R> v, CellView cv, OpenBoundary ob) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // kNeumannLanes lanes per particle, as for the walls: buffer particles are
+  // a thin band in cell order
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kNeumannLanes);
   if (i >= v.n) return;
   const int bt = v.btype[i];
   if (bt <= 0 || v.ptype[i] >= 1000) return;
@@ -378,7 +380,7 @@ __global__ void k_open_boundary(DevParams P, SphView<R> v, CellView cv, OpenBoun
   const double xg = v.x[i], yg = v.y[i], zi = v.z[i];
   const double zg = bt == 1 ? 2 * ob.L[1] - zi : (bt == 2 ? 2 * ob.L[2] - zi : zi);
   double trho = 0, tpres = 0, tux = 0, tuy = 0, tuz = 0, ttemp = 0, tflt = 0;
-  for_cells(P, xg, yg, zg, cv, [&](int j) {
+  group_for_cells<kNeumannLanes>(P, xg, yg, zg, cv, [&](int j) {
     if (!(v.ptype[j] < 1000)) return;
     const double r = dist_exact(xg - v.x[j], yg - v.y[j], zg - v.z[j]) + 1e-20;
     if (!(r < Rs)) return;
@@ -393,6 +395,14 @@ __global__ void k_open_boundary(DevParams P, SphView<R> v, CellView cv, OpenBoun
     ttemp += volj * (double)v.temp[j] * w;
     tflt += volj * w;
   });
+  trho = group_sum<kNeumannLanes>(trho);
+  tpres = group_sum<kNeumannLanes>(tpres);
+  tuz = group_sum<kNeumannLanes>(tuz);
+  tux = group_sum<kNeumannLanes>(tux);
+  tuy = group_sum<kNeumannLanes>(tuy);
+  ttemp = group_sum<kNeumannLanes>(ttemp);
+  tflt = group_sum<kNeumannLanes>(tflt);
+  if ((threadIdx.x & (kNeumannLanes - 1)) != 0) return;
   const double pori = v.por[i], rho_old = v.rho[i];
   double m = rho_old * v.vol0[i];
   if (P.eulerian && v.ptype[i] > 0) m = rho_old * v.vol[i];
@@ -663,7 +673,12 @@ template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, 
 template <typename R>
 int sph_open_boundary(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv,
                       const OpenBoundary& ob) {
-  LAUNCH(k_open_boundary, P, v, cv, ob);
+  if (v.n > 0) {
+    const size_t nt = (size_t)v.n * kNeumannLanes;
+    k_open_boundary<R><<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, v, cv, ob);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
 }
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_predictor, P, v); }
 template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_corrector, P, v); }
