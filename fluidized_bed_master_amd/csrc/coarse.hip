@@ -343,9 +343,16 @@ int alloc_coarse(CoarseWork& c, const PpeWork& w) {
     return SOPHIA_E_CAPACITY;
   }
   void* blas = c.blas;
+  hipStream_t side = c.side;
+  hipEvent_t ev_in = c.ev_in, ev_rows = c.ev_rows;
   c.blas = nullptr;
+  c.side = nullptr;
+  c.ev_in = c.ev_rows = nullptr;
   coarse_free(c);
   c.blas = blas;
+  c.side = side;
+  c.ev_in = ev_in;
+  c.ev_rows = ev_rows;
   c.NI = NI; c.NJ = NJ; c.NK = NK;
   c.AX = AX; c.AY = AY; c.AZ = AZ; c.na = na; c.ld = ld;
   const size_t m = (size_t)ld * ld * sizeof(float);
@@ -399,6 +406,39 @@ int ns_update(hipStream_t s, CoarseWork& c) {
 
 }  // namespace
 
+// E = Z^T A Z: per-particle aggregate-slot sums, then one row of E per aggregate
+static int coarse_rows(hipStream_t s, CoarseWork& c, const PpeWork& w) {
+  const PpeOp o = ppe_op(w);
+  const AggGrid g = agg_grid(c);
+  const int gr = grid_xcd(w.n);
+  if (w.fmt == kFmtRun8) {
+    if (w.mixed) k_coarse_rows<kFmtRun8, 1><<<gr, 256, 0, s>>>(o, g, c.slots);
+    else k_coarse_rows<kFmtRun8, 2><<<gr, 256, 0, s>>>(o, g, c.slots);
+  } else {
+    if (w.mixed) k_coarse_rows<kFmtIdx32, 1><<<gr, 256, 0, s>>>(o, g, c.slots);
+    else k_coarse_rows<kFmtIdx32, 2><<<gr, 256, 0, s>>>(o, g, c.slots);
+  }
+  k_coarse_reduce<<<c.na, 64, 0, s>>>(o, g, c.slots, c.E, c.ld);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w) {
+  if (!w.rowmajor || w.n <= 0) return 0;
+  TRY(alloc_coarse(c, w));
+  if (!c.side) {
+    HIP_TRY(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c.ev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c.ev_rows, hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventRecord(c.ev_in, s));
+  HIP_TRY(hipStreamWaitEvent(c.side, c.ev_in, 0));
+  TRY(coarse_rows(c.side, c, w));
+  HIP_TRY(hipEventRecord(c.ev_rows, c.side));
+  c.rows_pending = true;
+  return 0;
+}
+
 int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, Timers* tm) {
   *usable = false;
   if (!w.rowmajor || w.n <= 0) return 0;
@@ -412,18 +452,12 @@ int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, T
   }
   BLAS_TRY(rocblas_set_stream((rocblas_handle)c.blas, s));
   TimeScope ts(tm, "ppe_coarse_setup");
-  const PpeOp o = ppe_op(w);
-  const AggGrid g = agg_grid(c);
-  const int gr = grid_xcd(w.n);
-  if (w.fmt == kFmtRun8) {
-    if (w.mixed) k_coarse_rows<kFmtRun8, 1><<<gr, 256, 0, s>>>(o, g, c.slots);
-    else k_coarse_rows<kFmtRun8, 2><<<gr, 256, 0, s>>>(o, g, c.slots);
+  if (c.rows_pending) {
+    HIP_TRY(hipStreamWaitEvent(s, c.ev_rows, 0));
+    c.rows_pending = false;
   } else {
-    if (w.mixed) k_coarse_rows<kFmtIdx32, 1><<<gr, 256, 0, s>>>(o, g, c.slots);
-    else k_coarse_rows<kFmtIdx32, 2><<<gr, 256, 0, s>>>(o, g, c.slots);
+    TRY(coarse_rows(s, c, w));
   }
-  k_coarse_reduce<<<c.na, 64, 0, s>>>(o, g, c.slots, c.E, c.ld);
-  HIP_TRY(hipGetLastError());
   const int gsz = (int)std::min<size_t>(((size_t)c.na * c.ld + 255) / 256, 4096);
   c.ns_steps = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -487,6 +521,12 @@ void coarse_free(CoarseWork& c) {
   F(c.E); F(c.X); F(c.X2); F(c.U); F(c.c); F(c.y); F(c.partials); F(c.slots);
   if (c.host) (void)hipHostFree(c.host);
   if (c.blas) (void)rocblas_destroy_handle((rocblas_handle)c.blas);
+  if (c.side) {
+    (void)hipStreamSynchronize(c.side);
+    (void)hipStreamDestroy(c.side);
+  }
+  if (c.ev_in) (void)hipEventDestroy(c.ev_in);
+  if (c.ev_rows) (void)hipEventDestroy(c.ev_rows);
   c = CoarseWork{};
 }
 

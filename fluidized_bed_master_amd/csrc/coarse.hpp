@@ -37,7 +37,16 @@ struct CoarseWork {
   bool valid = false;      // X is an accepted inverse of some earlier E
   double res = 0;          // ||I - E X||_F / sqrt(na) of X against the current E
   int ns_steps = 0;        // Newton-Schulz steps of the last setup
+  // E built ahead on a second stream (default priority) while the solver
+  // stream computes the initial guess and its residual
+  hipStream_t side = nullptr;
+  hipEvent_t ev_in = nullptr, ev_rows = nullptr;
+  bool rows_pending = false;
 };
+
+// Start building E from the operator just assembled on stream s, on the side
+// stream; the next coarse_setup on s waits for it instead of building E.
+int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w);
 
 // Build E from the assembled operator (row-major cell keys) and bring X to
 // ||I - E X|| <= kCoarseTol.  Returns 0 with *usable = false when X cannot be
