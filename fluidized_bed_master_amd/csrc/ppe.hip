@@ -363,6 +363,33 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   int kc = 0, run = 0;
   uint64_t rcw = 0;
   uint32_t word = 0;
+  uint32_t qi4[4] = {0u, 0u, 0u, 0u};
+  float qh4[4] = {0.f, 0.f, 0.f, 0.f}, ql4[4] = {0.f, 0.f, 0.f, 0.f};
+  double qc4[4] = {0.0, 0.0, 0.0, 0.0};
+  auto push_quad = [&](uint32_t jj, float h, float l, double c) {
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      qi4[t] = qi4[t + 1];
+      qh4[t] = qh4[t + 1];
+      ql4[t] = ql4[t + 1];
+      qc4[t] = qc4[t + 1];
+    }
+    qi4[3] = jj;
+    qh4[3] = h;
+    ql4[3] = l;
+    qc4[3] = c;
+  };
+  auto store_quad = [&](int q) {
+    const size_t qs = (size_t)q * nn + i;
+    if (FMT == kFmtIdx32) reinterpret_cast<uint4*>(o.idx)[qs] = make_uint4(qi4[0], qi4[1], qi4[2], qi4[3]);
+    if (MIXED) {
+      reinterpret_cast<float4*>(o.hi)[qs] = make_float4(qh4[0], qh4[1], qh4[2], qh4[3]);
+      reinterpret_cast<float4*>(o.lo)[qs] = make_float4(ql4[0], ql4[1], ql4[2], ql4[3]);
+    } else {
+      reinterpret_cast<double2*>(o.c64)[2 * qs] = make_double2(qc4[0], qc4[1]);
+      reinterpret_cast<double2*>(o.c64)[2 * qs + 1] = make_double2(qc4[2], qc4[3]);
+    }
+  };
   unsigned bad = 0;
   // the list in batches of four: every field of the batch is gathered
   // before any is used (straight-line loads, all in flight together), the
@@ -421,9 +448,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
       continue;
     }
     if (a == 0.0) continue;
-    // store column kc
+    // store column kc: the quad's values collect in registers (oldest first)
+    // and leave as whole 16-byte stores
     const int q = kc >> 2, u = kc & 3;
-    const size_t slot = ((size_t)q * nn + i) * 4 + u;
     if (FMT == kFmtRun8) {
       while (run < 9 && j >= pick9(E, run)) run++;
       int off = run < 9 ? j - pick9(B, run) : -1;
@@ -438,32 +465,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
         o.idx[(size_t)q * nn + i] = word;
         word = 0;
       }
-    } else {
-      o.idx[slot] = (uint32_t)j;
     }
-    if (MIXED) {
-      float h = (float)a;
-      o.hi[slot] = h;
-      o.lo[slot] = (float)(a - (double)h);
-    } else {
-      o.c64[slot] = a;
+    {
+      const float h = (float)a;
+      push_quad((uint32_t)j, h, (float)(a - (double)h), a);
     }
     kc++;
+    if ((kc & 3) == 0) store_quad(q);
     }
   }
   // pad the last quad so whole-quad loads read defined values
   if (kc & 3) {
     const int q = kc >> 2;
-    for (int u = kc & 3; u < 4; u++) {
-      const size_t slot = ((size_t)q * nn + i) * 4 + u;
-      if (FMT == kFmtIdx32) o.idx[slot] = (uint32_t)i;
-      if (MIXED) {
-        o.hi[slot] = 0.f;
-        o.lo[slot] = 0.f;
-      } else {
-        o.c64[slot] = 0.0;
-      }
-    }
+    for (int u = kc & 3; u < 4; u++) push_quad((uint32_t)i, 0.f, 0.f, 0.0);
+    store_quad(q);
     if (FMT == kFmtRun8) o.idx[(size_t)q * nn + i] = word;
   }
   if (bad) atomicOr(o.flags, bad);
