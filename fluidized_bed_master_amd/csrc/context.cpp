@@ -742,9 +742,11 @@ struct Ctx : public sophia_ctx {
       TRY(ppe_assemble<R>(stream, P, sv, ppe_w, skeys, scoff));
     }
     ppe_assembled = true;
-    if (o->method == SOPHIA_PPE_BICGSTAB && o->precond == SOPHIA_PRECOND_TWO_LEVEL && o->zero_mean == 0) {
+    if (o->method == SOPHIA_PPE_BICGSTAB && o->precond == SOPHIA_PRECOND_TWO_LEVEL && o->zero_mean == 0 &&
+        !(o->guess == SOPHIA_GUESS_LEARNED && mlp.ready)) {
       // the coarse matrix of the two-level preconditioner builds on a side
-      // stream while the guess and its residual are computed on this one
+      // stream while the guess and its residual are computed on this one (not
+      // beside the learned forward, which keeps every CU busy by itself)
       if (!ppe_w.coarse) ppe_w.coarse = new CoarseWork();
       TRY(coarse_rows_early(stream, *ppe_w.coarse, ppe_w));
     }
