@@ -469,6 +469,13 @@ struct PermuteField {
 int permute_fields(hipStream_t s, int n, const uint32_t* order, const PermuteField* fields, int nf);
 int fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
 int fill_zero_bytes(hipStream_t s, void* p, size_t bytes);
+// several fills in one launch (one grid row per span)
+struct FillSpan {
+  uint32_t* p;
+  size_t n;  // 32-bit words
+  uint32_t v;
+};
+int fill_u32_many(hipStream_t s, const FillSpan* spans, int count);
 
 template <typename R>
 int build_sph_neighbors(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv, int* d_maxcnt);

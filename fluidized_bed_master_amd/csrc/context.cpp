@@ -539,12 +539,16 @@ struct Ctx : public sophia_ctx {
       // rewritten each step for the particles their kernels compute; the
       // others (walls, skipped types) keep their input value, which is zero
       // (the DEM drag reads pgf of walls)
-      for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz, sv.xsx,
-                   sv.xsy, sv.xsz, sv.xst})
-        TRY(fill_zero_bytes(stream, a, (size_t)ns * sizeof(R)));
-      TRY(fill_zero_bytes(stream, sv.cm, (size_t)6 * ns * sizeof(R)));
       // PPE1..4 (P1) are written for PPE rows only; the rest keep zero
-      for (double* a : {sv.ppe1, sv.ppe2, sv.ppe3, sv.ppe4}) TRY(fill_zero_bytes(stream, a, (size_t)ns * 8));
+      {
+        std::vector<FillSpan> z;
+        for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz, sv.xsx,
+                     sv.xsy, sv.xsz, sv.xst})
+          z.push_back({(uint32_t*)a, (size_t)ns * sizeof(R) / 4, 0u});
+        z.push_back({(uint32_t*)sv.cm, (size_t)6 * ns * sizeof(R) / 4, 0u});
+        for (double* a : {sv.ppe1, sv.ppe2, sv.ppe3, sv.ppe4}) z.push_back({(uint32_t*)a, (size_t)ns * 2, 0u});
+        TRY(fill_u32_many(stream, z.data(), (int)z.size()));
+      }
       // neighbour list with capacity check
       for (int attempt = 0; attempt < 2; attempt++) {
         TRY(fill_u32(stream, (uint32_t*)maxcnt, 0u, 1));
@@ -588,8 +592,11 @@ struct Ctx : public sophia_ctx {
       std::swap(hist_idx[0], hist_idx[1]);
       std::swap(hist_ds[0], hist_ds[1]);
       refresh_views();
-      for (double* a : {dv.fx, dv.fy, dv.fz, dv.tqx, dv.tqy, dv.tqz})
-        TRY(fill_zero_bytes(stream, a, (size_t)nd * 8));
+      {
+        std::vector<FillSpan> z;
+        for (double* a : {dv.fx, dv.fy, dv.fz, dv.tqx, dv.tqy, dv.tqz}) z.push_back({(uint32_t*)a, (size_t)nd * 2, 0u});
+        TRY(fill_u32_many(stream, z.data(), (int)z.size()));
+      }
     } else {
       TRY(fill_u32(stream, dcs, kEmpty, (size_t)d.num_cells));
     }

@@ -168,6 +168,20 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+constexpr int kMaxFill = 24;
+struct FillArgs {
+  uint32_t* p[kMaxFill];
+  size_t n[kMaxFill];
+  uint32_t v[kMaxFill];
+};
+
+__global__ void k_fill_many(FillArgs a) {
+  uint32_t* p = a.p[blockIdx.y];
+  const size_t n = a.n[blockIdx.y];
+  const uint32_t v = a.v[blockIdx.y];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 constexpr int kMaxPermute = 48;
 struct PermuteArgs {
   PermuteField f[kMaxPermute];
@@ -309,9 +323,8 @@ int radix_sort_pairs(hipStream_t s, int n, int nbits, SortScratch& sc, uint32_t*
 }
 
 int cell_ranges(hipStream_t s, int n, int num_cells, const uint32_t* keys, uint32_t* start, uint32_t* end) {
-  int rc = fill_u32(s, start, kEmpty, (size_t)num_cells);
-  if (rc) return rc;
-  rc = fill_u32(s, end, 0u, (size_t)num_cells);
+  const FillSpan f[2] = {{start, (size_t)num_cells, kEmpty}, {end, (size_t)num_cells, 0u}};
+  int rc = fill_u32_many(s, f, 2);
   if (rc) return rc;
   if (n <= 0) return 0;
   k_ranges<<<grid_for(n), kBlock, 0, s>>>(n, keys, start, end);
@@ -337,6 +350,26 @@ int fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
   if (blocks > 65536) blocks = 65536;
   k_fill_u32<<<(unsigned)blocks, kBlock, 0, s>>>(p, v, n);
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int fill_u32_many(hipStream_t s, const FillSpan* spans, int count) {
+  for (int off = 0; off < count; off += kMaxFill) {
+    const int k = count - off < kMaxFill ? count - off : kMaxFill;
+    FillArgs a;
+    size_t nmax = 0;
+    for (int q = 0; q < k; q++) {
+      a.p[q] = spans[off + q].p;
+      a.n[q] = spans[off + q].n;
+      a.v[q] = spans[off + q].v;
+      if (a.n[q] > nmax) nmax = a.n[q];
+    }
+    if (nmax == 0) continue;
+    size_t blocks = (nmax + kBlock - 1) / kBlock;
+    if (blocks > 2048) blocks = 2048;
+    k_fill_many<<<dim3((unsigned)blocks, (unsigned)k), kBlock, 0, s>>>(a);
+    HIP_TRY(hipGetLastError());
+  }
   return 0;
 }
 
