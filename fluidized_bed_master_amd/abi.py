@@ -30,6 +30,7 @@ GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
 VTK_FLUID, VTK_DEM = 0, 1
 PREDICT, CORRECT = 0, 1
+COMPAT_NO_SPIN, COMPAT_NUSSELT, COMPAT_HEAT_OVERWRITE, COMPAT_CP_ARGS, COMPAT_ALL = 1, 2, 4, 8, 15
 ERRORS = {
     -1: "SOPHIA_E_ARG", -2: "SOPHIA_E_HIP", -3: "SOPHIA_E_IO", -4: "SOPHIA_E_STATE",
     -5: "SOPHIA_E_NOMEM", -6: "SOPHIA_E_CAPACITY", -7: "SOPHIA_E_NUMERIC",
@@ -92,6 +93,11 @@ class Params(C.Structure):
         ("dem_walls", C.c_int32), ("wall_cyl", C.c_double * 3), ("wall_box", C.c_double * 6),
         ("wall_cone", C.c_double * 5), ("wall_rest", C.c_double * 2), ("wall_mu", C.c_double * 2),
         ("ob_band", C.c_double * 4), ("ob_inlet_u", C.c_double), ("ob_inlet_temp", C.c_double),
+        ("reference_compat", C.c_int32),
+        ("pr_fluid", C.c_double), ("pr_air", C.c_double), ("dem_k_mat", C.c_double * 2),
+        ("dem_alpha_mat", C.c_double * 2), ("dem_mat_radius", C.c_double),
+        ("heat_flux", C.c_double), ("heat_flux_box", C.c_double * 6),
+        ("free_surface", C.c_int32), ("fs_threshold", C.c_double),
         ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
     ]
 

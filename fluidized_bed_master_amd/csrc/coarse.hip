@@ -423,7 +423,7 @@ static int coarse_rows(hipStream_t s, CoarseWork& c, const PpeWork& w) {
   return 0;
 }
 
-int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w) {
+int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w, Timers* tm) {
   if (!w.rowmajor || w.n <= 0) return 0;
   TRY(alloc_coarse(c, w));
   if (!c.side) {
@@ -433,9 +433,28 @@ int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w) {
   }
   HIP_TRY(hipEventRecord(c.ev_in, s));
   HIP_TRY(hipStreamWaitEvent(c.side, c.ev_in, 0));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (tm) {
+    t0 = tm->get();
+    t1 = tm->get();
+    HIP_TRY(hipEventRecord(t0, c.side));
+  }
   TRY(coarse_rows(c.side, c, w));
+  if (tm) {
+    HIP_TRY(hipEventRecord(t1, c.side));
+    tm->add_span("ppe_coarse_rows_side", t0, t1);
+  }
   HIP_TRY(hipEventRecord(c.ev_rows, c.side));
   c.rows_pending = true;
+  c.rows_gen = w.gen;
+  c.e_gen = w.gen;
+  return 0;
+}
+
+int coarse_settle(hipStream_t s, CoarseWork& c) {
+  if (!c.rows_pending) return 0;
+  HIP_TRY(hipStreamWaitEvent(s, c.ev_rows, 0));
+  c.rows_pending = false;
   return 0;
 }
 
@@ -452,11 +471,12 @@ int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, T
   }
   BLAS_TRY(rocblas_set_stream((rocblas_handle)c.blas, s));
   TimeScope ts(tm, "ppe_coarse_setup");
-  if (c.rows_pending) {
-    HIP_TRY(hipStreamWaitEvent(s, c.ev_rows, 0));
-    c.rows_pending = false;
-  } else {
+  const bool early = c.rows_pending && c.rows_gen == w.gen;
+  TRY(coarse_settle(s, c));
+  if (!early) {
+    // no side-stream build of this operator: build E here
     TRY(coarse_rows(s, c, w));
+    c.e_gen = w.gen;
   }
   const int gsz = (int)std::min<size_t>(((size_t)c.na * c.ld + 255) / 256, 4096);
   c.ns_steps = 0;
@@ -483,6 +503,7 @@ int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, T
       c.ns_steps++;
     }
     c.res = res;
+    c.x_gen = c.e_gen;
     if (ok) {
       c.valid = true;
       *usable = true;

@@ -42,11 +42,20 @@ struct CoarseWork {
   hipStream_t side = nullptr;
   hipEvent_t ev_in = nullptr, ev_rows = nullptr;
   bool rows_pending = false;
+  uint64_t rows_gen = 0;  // operator generation (PpeWork::gen) of the pending side-stream build
+  uint64_t e_gen = 0;     // operator generation E was built from
+  uint64_t x_gen = 0;     // generation of the E that X was last refreshed against
 };
 
 // Start building E from the operator just assembled on stream s, on the side
-// stream; the next coarse_setup on s waits for it instead of building E.
-int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w);
+// stream; the next coarse_setup on s for the same operator generation waits
+// for it instead of building E.
+int coarse_rows_early(hipStream_t s, CoarseWork& c, const PpeWork& w, Timers* tm);
+// Make stream s wait for a pending side-stream build and clear it.  Called at
+// the end of every PPE stage, so that no later kernel on s (NNPS, the next
+// assembly) rewrites what the side stream still reads, whether or not the
+// solve reached coarse_setup.
+int coarse_settle(hipStream_t s, CoarseWork& c);
 
 // Build E from the assembled operator (row-major cell keys) and bring X to
 // ||I - E X|| <= kCoarseTol.  Returns 0 with *usable = false when X cannot be

@@ -66,6 +66,12 @@ struct DevParams {
   double rho_fluid, rho_water, h_over_dx, mu_fluid, mu_water, k_fluid, k_dem, cp_fluid, cp_dem;
   double dem_G, dem_nu, dem_friction, dem_damping, dem_hertz, drag_switch;
   double eos_b, gamma, delta_sph, p_ref;
+  int compat;             // sophia_params.reference_compat bits
+  int free_surface;
+  double fs_threshold;
+  double pr_fluid, pr_air, dem_k_mat[2], dem_alpha_mat[2], dem_mat_radius;
+  double heat_flux;       // W/m^2 through flux walls (0 = off)
+  double heat_flux_box[6];
 };
 
 // ---------------------------------------------------------------------------
@@ -81,6 +87,7 @@ struct SphView {
   int *ptype, *btype, *itype;
   // scratch
   double* por;  // porosity eps (FP64: pressure-gradient sums cancel against it)
+  double* flux_s;  // fluid Shepard sum of each flux wall (0 elsewhere)
   R *dpor_prev;
   double *rho_ref, *rho0;  // reference / start-of-step density (P2; the Tait EOS amplifies their rounding)
   double *vol, *vol0;      // WCSPH particle volume and its initial value (P1.vol, P1.vol0)
@@ -136,6 +143,8 @@ struct DemView {
   double *fx, *fy, *fz, *tqx, *tqy, *tqz;
   double *ux0, *uy0, *uz0;
   double *test6;
+  double *dtemp, *temp0;     // temperature rate (P3.dtemp) and start-of-step temperature (P2.temp0)
+  double *hrad, *hcon;       // radiative and contact parts of dtemp (P1.test1, P1.test2)
 };
 
 struct CellView {
@@ -492,6 +501,8 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
 // driver integrates density by continuity)
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// fluid Shepard sums of the flux walls (no-op without a wall heat flux)
+template <typename R> int sph_flux_walls(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv);
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v);

@@ -263,6 +263,18 @@ struct Ctx : public sophia_ctx {
     P.gamma = p.gamma;
     P.delta_sph = p.delta_sph;
     P.p_ref = p.p_ref;
+    P.compat = p.reference_compat;
+    P.free_surface = p.free_surface;
+    P.fs_threshold = p.fs_threshold;
+    P.pr_fluid = p.pr_fluid;
+    P.pr_air = p.pr_air;
+    for (int k = 0; k < 2; k++) {
+      P.dem_k_mat[k] = p.dem_k_mat[k];
+      P.dem_alpha_mat[k] = p.dem_alpha_mat[k];
+    }
+    P.dem_mat_radius = p.dem_mat_radius;
+    P.heat_flux = p.con_solve == 1 ? p.heat_flux : 0.0;
+    for (int k = 0; k < 6; k++) P.heat_flux_box[k] = p.heat_flux_box[k];
     return P;
   }
 
@@ -334,6 +346,7 @@ struct Ctx : public sophia_ctx {
     // SPH scratch
     SphView<R>& v = sv;
     TRY(arena.alloc(&v.por, NS));
+    TRY(arena.alloc(&v.flux_s, NS));
     for (double** f : {&v.rho_ref, &v.rho0, &v.vol, &v.vol0, &v.x0, &v.y0, &v.z0}) TRY(arena.alloc(f, NS));
     for (R** f : {&v.dpor_prev, &v.grx, &v.gry, &v.grz, &v.fx, &v.fy, &v.fz, &v.fmag,
                   &v.fpx, &v.fpy, &v.fpz, &v.pgx, &v.pgy, &v.pgz, &v.dtemp, &v.temp0, &v.ux0, &v.uy0, &v.uz0,
@@ -354,7 +367,8 @@ struct Ctx : public sophia_ctx {
     }
     DemView& w = dv;
     for (double** f : {&w.flt_s, &w.flt_sd, &w.flt_sd2, &w.por, &w.fax, &w.fay, &w.faz, &w.fbx, &w.fby, &w.fbz,
-                       &w.fx, &w.fy, &w.fz, &w.tqx, &w.tqy, &w.tqz, &w.ux0, &w.uy0, &w.uz0, &w.test6})
+                       &w.fx, &w.fy, &w.fz, &w.tqx, &w.tqy, &w.tqz, &w.ux0, &w.uy0, &w.uz0, &w.test6,
+                       &w.dtemp, &w.temp0, &w.hrad, &w.hcon})
       TRY(arena.alloc(f, ND));
     // cells and sort scratch
     size_t NC = (size_t)d.num_cells;
@@ -512,6 +526,9 @@ struct Ctx : public sophia_ctx {
   }
 
   int nnps() override {
+    // a side-stream coarse build left by a PPE stage that returned early
+    // finishes before the sort rewrites its inputs
+    if (ppe_w.coarse) TRY(coarse_settle(stream, *ppe_w.coarse));
     const bool on = sph_on();
     DevParams P = devparams();
     if (on && ns > 0) {
@@ -594,7 +611,8 @@ struct Ctx : public sophia_ctx {
       refresh_views();
       {
         std::vector<FillSpan> z;
-        for (double* a : {dv.fx, dv.fy, dv.fz, dv.tqx, dv.tqy, dv.tqz}) z.push_back({(uint32_t*)a, (size_t)nd * 2, 0u});
+        for (double* a : {dv.fx, dv.fy, dv.fz, dv.tqx, dv.tqy, dv.tqz, dv.dtemp})
+          z.push_back({(uint32_t*)a, (size_t)nd * 2, 0u});
         TRY(fill_u32_many(stream, z.data(), (int)z.size()));
       }
     } else {
@@ -637,6 +655,7 @@ struct Ctx : public sophia_ctx {
       TimeScope t(tm(), "prep");
       TRY(sph_prep<R>(stream, P, sv));
     }
+    if (on && P.heat_flux != 0.0) TRY(sph_flux_walls<R>(stream, P, sv));
     return 0;
   }
 
@@ -755,7 +774,7 @@ struct Ctx : public sophia_ctx {
       // stream while the guess and its residual are computed on this one (not
       // beside the learned forward, which keeps every CU busy by itself)
       if (!ppe_w.coarse) ppe_w.coarse = new CoarseWork();
-      TRY(coarse_rows_early(stream, *ppe_w.coarse, ppe_w));
+      TRY(coarse_rows_early(stream, *ppe_w.coarse, ppe_w, tm()));
     }
     if (o->method == SOPHIA_PPE_JACOBI) {
       TRY(rotate_pressure_history());
@@ -822,6 +841,9 @@ struct Ctx : public sophia_ctx {
       TimeScope t(tm(), "pressure_force");
       TRY(sph_pressure_force<R>(stream, P, sv));
     }
+    // the side-stream coarse build of this operator, if any, completes before
+    // anything later on the stream rewrites its inputs
+    if (ppe_w.coarse) TRY(coarse_settle(stream, *ppe_w.coarse));
     HIP_TRY(hipEventRecord(ev1, stream));
     HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0;
@@ -955,7 +977,8 @@ struct Ctx : public sophia_ctx {
     const double* src[] = {dv.x, dv.y, dv.z, dv.ux, dv.uy, dv.uz, dv.wx, dv.wy, dv.wz, dv.m, dv.ri, dv.rad,
                            dv.h, dv.rho, dv.temp, dv.fdx, dv.fdy, dv.fdz, dv.qsd, dv.flt_s, dv.flt_sd, dv.flt_sd2,
                            dv.por, dv.fax, dv.fay, dv.faz, dv.fbx, dv.fby, dv.fbz, dv.fx, dv.fy, dv.fz,
-                           dv.tqx, dv.tqy, dv.tqz, dv.ux0, dv.uy0, dv.uz0, dv.test6};
+                           dv.tqx, dv.tqy, dv.tqz, dv.ux0, dv.uy0, dv.uz0, dv.test6,
+                           dv.hrad, dv.hcon, dv.dtemp, dv.temp0};
     const int NF = sizeof(src) / sizeof(src[0]);
     std::vector<double> a[NF];
     for (int k = 0; k < NF; k++) TRY(get(a[k], src[k], n));
@@ -976,6 +999,7 @@ struct Ctx : public sophia_ctx {
         q.flt_s = A(19); q.flt_sd = A(20); q.flt_sd_2 = A(21); q.DEMpor = A(22);
         q.Fdx_da = A(23); q.Fdy_da = A(24); q.Fdz_da = A(25); q.Fdx_b = A(26); q.Fdy_b = A(27); q.Fdz_b = A(28);
         q.test6 = A(38);
+        q.test1 = A(39); q.test2 = A(40);
         q.elix = q.eliy = q.eliz = 1.0;
         for (int k = 0; k < kMaxHist; k++) {
           q.ct_idx[k] = hi[(size_t)k * n + i];
@@ -986,12 +1010,14 @@ struct Ctx : public sophia_ctx {
         part2& q = p2[i];
         std::memset(&q, 0, sizeof(q));
         q.ux0 = A(35); q.uy0 = A(36); q.uz0 = A(37);
+        q.temp0 = A(42);
       }
       if (p3) {
         part3& q = p3[i];
         std::memset(&q, 0, sizeof(q));
         q.ftotalx = A(29); q.ftotaly = A(30); q.ftotalz = A(31);
         q.torqx = A(32); q.torqy = A(33); q.torqz = A(34);
+        q.dtemp = A(41);
       }
     }
     return 0;
@@ -1128,6 +1154,12 @@ struct Ctx : public sophia_ctx {
     const CoarseWork* c = ppe_w.coarse;
     if (!c || c->na == 0 || !c->valid) {
       set_error("no two-level coarse operator: run a solve with precond = SOPHIA_PRECOND_TWO_LEVEL first");
+      return SOPHIA_E_STATE;
+    }
+    if (c->e_gen != c->x_gen) {
+      // the last solve started from a guess that already met rtol: its E was
+      // built ahead of a setup that never ran, so X belongs to an earlier E
+      set_error("coarse operator rebuilt for a solve that needed no iterations; its inverse is not refreshed");
       return SOPHIA_E_STATE;
     }
     if (dims) {
@@ -1320,6 +1352,18 @@ void sophia_params_default(sophia_params* p) {
   std::memcpy(p->ob_band, band, sizeof(band));
   p->ob_inlet_u = 24.045;
   p->ob_inlet_temp = 823.0;
+  p->reference_compat = 0;
+  // heat constants (functions_SPH_DEM_COUPLING2.cuh:708-718, 997, 1041, 1081; functions_PROP.cuh:279-340)
+  p->pr_fluid = 0.7;
+  p->pr_air = 1.0;
+  p->dem_k_mat[0] = 0.84;
+  p->dem_k_mat[1] = 55.0;
+  p->dem_alpha_mat[0] = 2.5e-6;
+  p->dem_alpha_mat[1] = 1.77e-5;
+  p->dem_mat_radius = 0.0012;
+  p->heat_flux = 0.0;
+  p->free_surface = 0;
+  p->fs_threshold = 0.3;
 }
 
 // read_solv_input (function_init.cuh:216-615): whitespace tokens; a key is
@@ -1551,6 +1595,11 @@ int sophia_create_from_arrays(const sophia_params* p, const part1* particles, in
   }
   if (p->dim != 2 && p->dim != 3) {
     set_error("create: dim must be 2 or 3");
+    return SOPHIA_E_ARG;
+  }
+  if (p->solver_type != 0 && p->dim != 3) {
+    // the ISPH-DEM stages exist in 3D only (KERNEL_*3D_sph / _dem)
+    set_error("create: the ISPH-DEM step is 3D only (dim = %d)", p->dim);
     return SOPHIA_E_ARG;
   }
   if (p->kappa <= 0 || p->dt <= 0) {
@@ -1918,6 +1967,7 @@ int sophia_mlp_features(sophia_ctx* c, float* nf, int32_t n, double* scale) {
 
 int sophia_timers_enable(sophia_ctx* c, int32_t on) {
   CHECK_CTX(c);
+  c->timers.off();
   c->timers.reset();
   c->timers.on = on != 0;
   return 0;

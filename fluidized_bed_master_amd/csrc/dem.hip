@@ -62,6 +62,41 @@ __global__ void k_porosity_dem(DevParams P, DemView d, SphView<R> v, CellView sc
   }
 }
 
+// emissivity of spheres (functions_PROP.cuh:420-440, graphite fit above 700 K)
+__device__ inline double dem_emissivity(double T) {
+  return T >= 700.0 ? 0.505586 + 7.92943e-4 * T - 5.228643e-7 * T * T + 1.10479e-10 * T * T * T : 0.842337;
+}
+
+// Radiative rate of sphere i from sphere j across the facing caps of a
+// double cone (functions_DEM_INTERACTION2.cuh:1085-1152), per heat capacity
+// of i's outer shell rc = rho_i V_shell c_p.  The cone radius is
+// 0.56 (2 R*) (1 - 0.401038)^(-1/3); the reference's integer -1/3 makes the
+// last factor 1 (SOPHIA_COMPAT_NUSSELT).
+__device__ inline double dem_radiation(const DevParams& P, double radi, double radj, double dist, double Ti,
+                                       double Tj, double rc) {
+  const double radij = 1 / (1 / radi + 1 / radj);
+  const double ai = radi / dist, aj = radj / dist, dA = aj - ai;
+  const double cone = (P.compat & SOPHIA_COMPAT_NUSSELT) ? 1.0 : 1.1863156050445258;  // (1 - 0.401038)^(-1/3)
+  const double dc_r = 0.56 * (2 * radij) * cone;
+  const double dc_ar = 3.14159 * dc_r * dc_r;
+  const double di_max = 0.5 * (sqrt(1 + 4.0 * dc_ar / (3.14159 * dist * dist * (1 - dA * dA))) - dA);
+  const double cos_i = (1 - dA * dA) / (2 * di_max) - dA;
+  const double cos_j = (1 - dA * dA) / (2 * (di_max + dA)) + dA;
+  const double sin_i = sqrt(1 - cos_i * cos_i), sin_j = sqrt(1 - cos_j * cos_j);
+  const double ri_cr = radi * sin_i, rj_cr = radj * sin_j;
+  const double d_cr = dist - radi * cos_i - radj * cos_j;
+  const double z_cr = 1 + (4 * rj_cr * rj_cr + d_cr * d_cr) / (ri_cr * ri_cr);
+  const double F_cr = 0.5 * (z_cr - sqrt(z_cr * z_cr - 4.0 * rj_cr * rj_cr / (ri_cr * ri_cr)));
+  const double F_ij = (1 + cos_i) * F_cr * 0.5;
+  const double ar_i = 2.0 * 3.14159 * radi * radi * (1 - cos_i);
+  const double ar_j = 2.0 * 3.14159 * radj * radj * (1 - cos_j);
+  const double ar_ratio = ar_j / ar_i;
+  const double Y_ij = F_ij + (1 - F_ij) * (ar_ratio / (1 + ar_ratio));
+  const double emi = dem_emissivity(Ti), emj = dem_emissivity(Tj);
+  const double em_inv = (1 - emi) / (emi * ar_i) + (1 - emj) / (emj * ar_j) + 1 / (Y_ij * ar_i);
+  return -5.6704e-8 * (Ti * Ti * Ti * Ti - Tj * Tj * Tj * Tj) / (em_inv * rc);
+}
+
 // KERNEL_DEM_interaction3D_dem (functions_DEM_INTERACTION2.cuh:888-1766):
 // Hertz normal spring + damping, Mindlin tangential spring + damping with a
 // Coulomb cap, torque, gravity; shear displacement history keyed by dem_idx.
@@ -80,6 +115,13 @@ __global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
   const double wxi = d.wx[i], wyi = d.wy[i], wzi = d.wz[i], radi = d.rad[i], mi = d.m[i], rii = d.ri[i];
   const double G = P.dem_G, nu = P.dem_nu, E = G * (2 * (1 + nu));
   double tx = 0, ty = 0, tz = 0, qx = 0, qy = 0, qz = 0;
+  // heat (con_solve): the outer shell (thickness rad/6) exchanges heat with
+  // neighbouring spheres by radiation and through contact circles
+  const bool heat = P.con_solve == 1;
+  const double Ti = d.temp[i], dr2 = radi / 6.0;
+  const double rc = d.rho[i] * ((4.0 / 3.0) * 3.141592 * (radi * radi * radi - (radi - dr2) * (radi - dr2) * (radi - dr2))) *
+                    P.cp_dem;
+  double h_rad = 0, h_con = 0;
   for_cells(P, xi, yi, zi, dc, [&](int j) {
     if (!(d.ptype[j] > 1000)) return;
     const double radj = d.rad[j];
@@ -87,8 +129,15 @@ __global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
     const double dist = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]);
     const double ovlp = radi + radj - dist;
     if (!(dist < 3.5 * radi && dist > 0.0)) return;
+    if (heat) h_rad += dem_radiation(P, radi, radj, dist, Ti, d.temp[j], rc);
     if (!(dist < 2.01 * radi)) return;
     if (ovlp > 0) {
+      if (heat) {
+        // contact conduction (:1300-1315), harmonic-mean conductivity
+        const double kc_ij = 1 / (1 / P.k_dem + 1 / P.k_dem);
+        const double radij_h = 1 / (1 / radi + 1 / radj);
+        h_con += -4.0 * kc_ij * sqrt((2 * ovlp) * (radij_h - 0.5 * ovlp)) * (Ti - d.temp[j]) / rc;
+      }
       double dsx = 0, dsy = 0, dsz = 0;
       int slot = -1;
       for (int k = 0; k < S; k++)
@@ -167,6 +216,13 @@ __global__ void k_dem_contact(DevParams P, DemView d, CellView dc) {
     }
   });
   if (P.fg_solve) tz += -P.gravity;
+  if (heat) {
+    // (:1725-1729) without the inner-shell terms: the shells temp1..3 are
+    // never advanced on this path
+    d.hrad[i] = h_rad;
+    d.hcon[i] = h_con;
+    d.dtemp[i] = h_rad + h_con;
+  }
   d.fx[i] = tx;
   d.fy[i] = ty;
   d.fz[i] = tz;
@@ -188,6 +244,7 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
   const double mi = d.m[i], flt_air = d.flt_s[i], flt_fluid = d.flt_sd[i];
   const double h = d.h[i], A = knl_tmpA<double>(P.kernel_type, P.dim, h), Rs = P.kappa * h;
   double tx = 0, ty = 0, tz = 0, ufx = 0, ufy = 0, ufz = 0, uax = 0, uay = 0, uaz = 0, por = 0;
+  double tf = 0, ta = 0;  // kernel-weighted fluid / air temperature sums (:806-867)
   if (v.n > 0) {
     wave_for_cells(P, xi, yi, zi, sc, [&](int j) {
       const int t = v.ptype[j];
@@ -202,12 +259,14 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
         ufy += (double)v.uy[j] * mj * w / rhoj;
         ufz += (double)v.uz[j] * mj * w / rhoj;
         por += porj * mj * w / rhoj;
+        if (P.con_solve == 1) tf += (double)v.temp[j] * mj * w / rhoj;
       }
       if (t == 3 && flt_air > 0.000001) {
         uax += (double)v.ux[j] * mj * w / rhoj;
         uay += (double)v.uy[j] * mj * w / rhoj;
         uaz += (double)v.uz[j] * mj * w / rhoj;
         por += porj * mj * w / rhoj;
+        if (P.con_solve == 1) ta += (double)v.temp[j] * mj * w / rhoj;
       }
       if (t != 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 3 && flt_fluid > 0.01) {
         double c = (mj / rhoj) * w * voli / (mi * flt_fluid + 1.0e-15);
@@ -221,6 +280,10 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
   ufx = wave_sum(ufx); ufy = wave_sum(ufy); ufz = wave_sum(ufz);
   uax = wave_sum(uax); uay = wave_sum(uay); uaz = wave_sum(uaz);
   por = wave_sum(por);
+  if (P.con_solve == 1) {
+    tf = wave_sum(tf);
+    ta = wave_sum(ta);
+  }
   if ((threadIdx.x & 63) != 0) return;
   const double pori = por / (flt_fluid + 1.0e-15);
   const double sf = flt_fluid + 1.0e-15, sa = flt_air + 1.0e-15, sm = flt_fluid + flt_air + 1.0e-15;
@@ -236,6 +299,14 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
   const double nu_f = P.mu_fluid / P.rho_fluid;
   double dfx = 0, dfy = 0, dfz = 0, dax = 0, day = 0, daz = 0;
   const bool any = (flt_fluid + flt_air) > 0.00001;
+  // convective heat (:996-1106): Ranz-Marshall Nu = 2 + 0.6 Re^0.5 Pr^(1/3)
+  // (Pr^0 under SOPHIA_COMPAT_NUSSELT), h = Nu k / d over the specific
+  // surface 6/d; the sphere's k and diffusivity by material (:708-718)
+  const double Ti = d.temp[i], rhoi = d.rho[i];
+  const int mat = radi < P.dem_mat_radius ? 1 : 0;
+  const double ki = P.dem_k_mat[mat], cpi = ki / (P.dem_alpha_mat[mat] * rhoi), a_ht = 6.0 / di;
+  const bool int_third = (P.compat & SOPHIA_COMPAT_NUSSELT) != 0;
+  double dq_vol = 0;
   if (magf > 1.0e-8 && any && f_fluid > P.drag_switch) {
     double Re = di * magf * pori / (nu_f + 1.0e-15);
     double sq = 0.63 + 4.8 / sqrt(Re);
@@ -250,6 +321,8 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
     d.fdx[i] = dfx;
     d.fdy[i] = dfy;
     d.fdz[i] = dfz;
+    const double Nu = 2.0 + 0.6 * pow(Re, 0.5) * (int_third ? 1.0 : cbrt(P.pr_fluid));
+    dq_vol = Nu * P.k_fluid / di * a_ht * (tf / sf - Ti);
   }
   if (maga > 1.0e-8 && any && f_air > P.drag_switch) {
     double Re = di * pori * maga / (nu_f + 1.0e-15);
@@ -261,6 +334,8 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
     dax = k * rax * maga * e / mi;
     day = k * ray * maga * e / mi;
     daz = k * raz * maga * e / mi;
+    const double Nu = 2.0 + 0.6 * pow(Re, 0.5) * (int_third ? 1.0 : cbrt(P.pr_air));
+    dq_vol = Nu * ki / di * a_ht * (ta / sa - Ti);
   }
   if ((maga > 1.0e-8 || magf > 1.0e-8) && any && f_air <= P.drag_switch && f_fluid <= P.drag_switch) {
     double rho_m = P.rho_fluid * f_air + P.rho_water * f_fluid;
@@ -275,6 +350,8 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
     dax = k * rmx * magm * e / mi;
     day = k * rmy * magm * e / mi;
     daz = k * rmz * magm * e / mi;
+    const double Nu = 2.0 + 0.6 * pow(Re, 0.5) * (int_third ? 1.0 : cbrt(P.pr_air));
+    dq_vol = Nu * ki / di * a_ht * ((tf + ta) / sm - Ti);
   }
   d.fx[i] = d.fx[i] + dfx / pori + tx;
   d.fy[i] = d.fy[i] + dfy / pori + ty;
@@ -285,6 +362,11 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
   d.fbx[i] = tx;
   d.fby[i] = ty;
   d.fbz[i] = tz;
+  if (P.con_solve == 1) {
+    const double conv = dq_vol / (rhoi * cpi);  // [K/s]
+    d.dtemp[i] = (P.compat & SOPHIA_COMPAT_HEAT_OVERWRITE) ? conv : d.dtemp[i] + conv;
+    d.qsd[i] = dq_vol * mi / rhoi;  // [W]
+  }
 }
 
 // KERNEL_SPH_coupling3D_sph (functions_SPH_DEM_COUPLING2.cuh:1815-2019):
@@ -302,7 +384,11 @@ __global__ void k_sph_reaction(DevParams P, SphView<R> v, DemView d, CellView dc
   if (d.n > 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 0 && (t == 1 || t == 3)) {
     const double A = knl_tmpA<double>(P.kernel_type, P.dim, hd);
     const double rhoi = v.rho[i];
-    const double cpi = t > 1000 ? P.cp_dem : P.cp_fluid;
+    // heat_capacity(ptypei, tempi) at :1860 swaps its arguments: the
+    // temperature is read as the particle type (SOPHIA_COMPAT_CP_ARGS)
+    const double cpi = (P.compat & SOPHIA_COMPAT_CP_ARGS)
+                           ? ((unsigned)(double)v.temp[i] > 1000u ? P.cp_dem : P.cp_fluid)
+                           : (t > 1000 ? P.cp_dem : P.cp_fluid);
     for_cells(P, xi, yi, zi, dc, [&](int j) {
       if (!(d.ptype[j] > 1000)) return;
       double r = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]);
@@ -347,6 +433,7 @@ __global__ void k_dem_predictor(DevParams P, DemView d) {
   d.ux0[i] = ux0;
   d.uy0[i] = uy0;
   d.uz0[i] = uz0;
+  if (P.con_solve == 1 && d.ptype[i] > 0) d.temp0[i] = d.temp[i];
 }
 
 // KERNEL_time_update_projection_dem (functions_TIME_ISPH2.cuh:624-776):
@@ -373,6 +460,15 @@ __global__ void k_dem_corrector(DevParams P, DemView d) {
   d.ux[i] = uxc;
   d.uy[i] = uyc;
   d.uz[i] = uzc;
+  if (t > 1000 && !(P.compat & SOPHIA_COMPAT_NO_SPIN)) {
+    // spin from the contact torque per inertia, as the WCSPH corrector does
+    // (functions_TIME.cuh:753-755); the ISPH path leaves it out
+    const double bz = (d.btype[i] == 0);
+    d.wx[i] += d.tqx[i] * bz * P.dt_dem;
+    d.wy[i] += d.tqy[i] * bz * P.dt_dem;
+    d.wz[i] += d.tqz[i] * bz * P.dt_dem;
+  }
+  if (P.con_solve == 1 && t > 0) d.temp[i] = d.temp0[i] + d.dtemp[i] * P.dt_dem * (double)(t != -1);
 }
 
 // --- DEM wall boundaries (functions_DEM_BC.cuh) -----------------------------

@@ -15,6 +15,7 @@ struct Nb {
   R ux, uy, uz, temp, grx, gry, grz;
   double rref, por;
   int pt;
+  int j;  // the neighbour's index, for fields a few kernels read on condition
 };
 
 // Particle i's side of the differences: FP64 position, or its pos4 record.
@@ -39,6 +40,7 @@ __device__ inline Nb<R> load_nb(const SphView<R>& v, int j, const NbOrigin& o) {
   q.ux = v.ux[j]; q.uy = v.uy[j]; q.uz = v.uz[j]; q.temp = v.temp[j]; q.rref = v.rho_ref[j];
   q.grx = v.grx[j]; q.gry = v.gry[j]; q.grz = v.grz[j]; q.por = v.por[j];
   q.pt = v.ptype[j];
+  q.j = j;
   return q;
 }
 

@@ -1025,6 +1025,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
   w.keys = keys;
   w.cell_off = cell_off;
   w.bn = -1;
+  w.gen++;
   w.NI = P.NI;
   w.NJ = P.NJ;
   w.ncell = P.NI * P.NJ * P.NK;

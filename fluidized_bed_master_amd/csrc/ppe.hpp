@@ -64,6 +64,7 @@ struct PpeWork {
   struct CoarseWork* coarse = nullptr;  // two-level preconditioner, created on first use
   unsigned mv_seq = 0;                  // inner matvec launches seen by the timers
   double bn = -1;                       // ||rhs|| over the rows of this assembly (< 0: not yet taken)
+  uint64_t gen = 0;                     // assemblies so far: tags data derived from one operator
 };
 
 constexpr unsigned kMvSample = 4;  // timers time one inner matvec launch in this many

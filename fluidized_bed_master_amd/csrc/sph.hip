@@ -183,10 +183,35 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   }
 }
 
+// Flux walls (sophia_params.heat_flux): each wall particle inside the flux
+// box sums V_k W(r, h_w) over the fluid of its neighbour list.  The fluid
+// particle i then receives q A_w W_iw / S_w from every flux wall w in its
+// list, so the walls' power q A_w reaches the fluid in full.
+__device__ inline bool flux_receiver(const DevParams& P, int pt, int it, int bt) {
+  return it <= 2 && !(P.open_boundary > 0 && bt > 0) && pt > 0 && pt < 1000;
+}
+template <typename R>
+__global__ void k_flux_walls(DevParams P, SphView<R> v) {
+  const int i = particle_index();
+  if (i >= v.n) return;
+  double S = 0;
+  const double x = v.x[i], y = v.y[i], z = v.z[i];
+  const double* b = P.heat_flux_box;
+  if (v.ptype[i] == 0 && x >= b[0] && y >= b[1] && z >= b[2] && x <= b[3] && y <= b[4] && z <= b[5]) {
+    const double h = v.h[i], A = knl_tmpA<double>(P.kernel_type, P.dim, h);
+    for_nbrs4<false>(v, i, false, [&](const Nb<R>& q, bool on) {
+      const bool use = on && flux_receiver(P, q.pt, v.itype[q.j], v.btype[q.j]);
+      const double r = sqrt(q.dx * q.dx + q.dy * q.dy + q.dz * q.dz);
+      S += use ? q.m / q.rho * knl_w<double>(P.kernel_type, A, h, r) : 0.0;
+    });
+  }
+  v.flux_s[i] = S;
+}
+
 // KERNEL_advection_force3D_sph (functions_PPE2.cuh:1251-1607): viscous,
 // artificial viscosity, Eulerian advection, conduction, gravity and the
 // pressure-gradient force density used by the coupling.
-template <typename R>
+template <typename R, bool FLUX>
 __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v) {
   typedef R T;
   int i = particle_index();
@@ -209,9 +234,17 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   const double rhoid = v.rho[i], porid = v.por[i];
   const double Ad = knl_tmpA<double>(P.kernel_type, P.dim, hd);
   T tx = 0, ty = 0, tz = 0, Rc = 0, ex = 0, ey = 0, ez = 0, et = 0;
-  double px = 0, py = 0, pz = 0;
+  double px = 0, py = 0, pz = 0, fl = 0;
   auto term = [&](const Nb<R>& q, bool on) {
     const bool use = on && q.pt < 1000;
+    if constexpr (FLUX) {
+      // wall heat flux from a flux wall j: q A_j W_ij / S_j
+      const double S = v.flux_s[q.j];
+      if (use && S > 0.0) {
+        const double rw = sqrt(q.dx * q.dx + q.dy * q.dy + q.dz * q.dz), a = q.h / P.h_over_dx;
+        fl += P.heat_flux * (a * a) * knl_w<double>(P.kernel_type, Ad, hd, rw) / S;
+      }
+    }
     // geometry in FP64 arithmetic: the symmetric pressure-gradient term
     // cancels a hydrostatic offset of order H/h times its result
     const double dxd = q.dx, dyd = q.dy, dzd = q.dz;
@@ -281,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   v.fx[i] = tx - ex;
   v.fy[i] = ty - ey;
   v.fz[i] = tz - ez;
-  if (P.con_solve) v.dtemp[i] = Rc / cpi * (T)(pti != -1) - et;
+  if (P.con_solve) v.dtemp[i] = Rc / cpi * (T)(pti != -1) - et + (T)(fl / (rhoid * (double)cpi));
   v.pgx[i] = (R)px;
   v.pgy[i] = (R)py;
   v.pgz[i] = (R)pz;
@@ -658,7 +691,21 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
   return 0;
 }
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_prep, P, v); }
-template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_advection, P, v); }
+template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n > 0) {
+    if (P.heat_flux != 0.0) k_advection<R, true><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
+    else k_advection<R, false><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+template <typename R> int sph_flux_walls(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n > 0 && P.heat_flux != 0.0) {
+    k_flux_walls<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv) {
   const double m = fmax(fmax(fmax(fabs(P.x_min), fabs(P.x_max)), fmax(fabs(P.y_min), fabs(P.y_max))),
                         fmax(fabs(P.z_min), fabs(P.z_max)));
@@ -706,6 +753,7 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                               \
   template int sph_prep_sums<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_flux_walls<R>(hipStream_t, const DevParams&, const SphView<R>&);                         \
   template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
   template int sph_pressure_force<R>(hipStream_t, const DevParams&, const SphView<R>&);                     \
   template int sph_open_boundary<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&,      \

@@ -20,8 +20,9 @@ struct Timers {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<long, double>> acc;  // name -> (launches, ms)
-  hipEvent_t open_a = nullptr;
-  std::string open_name;
+  // open scopes, innermost last: a scope nested in another closes first and
+  // both are timed
+  std::vector<Pending> open;
 
   hipEvent_t get() {
     if (!pool.empty()) {
@@ -35,17 +36,22 @@ struct Timers {
   }
   void begin(const char* name) {
     if (!on) return;
-    open_a = get();
-    open_name = name;
-    (void)hipEventRecord(open_a, stream);
+    Pending p{name, get(), nullptr};
+    (void)hipEventRecord(p.a, stream);
+    open.push_back(p);
   }
   void end() {
-    if (!on || !open_a) return;
-    hipEvent_t b = get();
-    (void)hipEventRecord(b, stream);
-    pending.push_back({open_name, open_a, b});
-    open_a = nullptr;
-    if (pending.size() > 4096) collect();
+    if (!on || open.empty()) return;
+    Pending p = open.back();
+    open.pop_back();
+    p.b = get();
+    (void)hipEventRecord(p.b, stream);
+    pending.push_back(p);
+    if (pending.size() > 4096 && open.empty()) collect();
+  }
+  // time [a, b) recorded by the caller on another stream under this name
+  void add_span(const char* name, hipEvent_t a, hipEvent_t b) {
+    if (on) pending.push_back({name, a, b});
   }
   // count a launch without timing it (0 ms)
   void count(const std::string& name) {
@@ -69,7 +75,14 @@ struct Timers {
     collect();
     acc.clear();
   }
+  // stop timing: events of scopes still open go back to the pool
+  void off() {
+    for (auto& p : open) pool.push_back(p.a);
+    open.clear();
+    on = false;
+  }
   ~Timers() {
+    for (auto& p : open) (void)hipEventDestroy(p.a);
     for (auto& p : pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);

@@ -155,9 +155,54 @@ typedef struct sophia_params {
   double ob_band[4];
   double ob_inlet_u;
   double ob_inlet_temp;
+  /* Reference quirks reproduced on request (bit set = as the reference;
+   * 0 = corrected, the default):
+   *   bit 0 SOPHIA_COMPAT_NO_SPIN: the ISPH step never integrates DEM spin
+   *         from the contact torque (KERNEL_time_update_projection_dem,
+   *         functions_TIME_ISPH2.cuh:624-776, has no omega update; the WCSPH
+   *         corrector does, functions_TIME.cuh:753-755);
+   *   bit 1 SOPHIA_COMPAT_NUSSELT: pow(Pr, 1/3) with integer 1/3 = Pr^0
+   *         (functions_SPH_DEM_COUPLING2.cuh:998, 1042, 1082);
+   *   bit 2 SOPHIA_COMPAT_HEAT_OVERWRITE: the coupling's convective rate
+   *         replaces the DEM contact/radiative rate instead of adding to it
+   *         (:1103 writes P3_dem.dtemp after KERNEL_DEM_interaction3D_dem
+   *         set it, functions_DEM_INTERACTION2.cuh:1725-1729);
+   *   bit 3 SOPHIA_COMPAT_CP_ARGS: heat_capacity(p_type, temp) with swapped
+   *         arguments in the SPH reaction heat (:1860). */
+  int32_t reference_compat;
+  /* heat transfer (con_solve = 1) */
+  double pr_fluid;          /* Pr of the fluid-regime Nusselt number (0.7, :997) */
+  double pr_air;            /* Pr of the air and mixture regimes (1.0, :1041, :1081) */
+  double dem_k_mat[2];      /* sphere conductivity of material 0, 1 (0.84, 55.0: functions_PROP.cuh:279-308) */
+  double dem_alpha_mat[2];  /* sphere diffusivity of material 0, 1 (2.5e-6, 1.77e-5: :312-340) */
+  double dem_mat_radius;    /* spheres with rad < this are material 1 (0.0012, functions_SPH_DEM_COUPLING2.cuh:708) */
+  /* wall heat flux into the fluid, W/m^2 (0 = off; no reference
+   * counterpart in the compiled kernels, HEAT_FLUX = 0 at
+   * functions_DEM_INTERACTION2.cuh:1).  Wall particles (p_type 0) inside
+   * heat_flux_box [xmin ymin zmin xmax ymax zmax] are flux walls of face
+   * area (h / h_over_dx)^2 each; their power is spread over the fluid by
+   * the kernel, normalised by each wall's fluid Shepard sum, so the fluid
+   * receives exactly heat_flux times the total face area. */
+  double heat_flux;
+  double heat_flux_box[6];
+  /* free surface: fluid rows detected as surface (KERNEL_clc_surface_detect3D,
+   * functions_PST.cuh:263-320: |mean unit vector to neighbours| > fs_threshold
+   * or fewer than 10 neighbours) take the Dirichlet row p = p_ref, the
+   * commented-out lines of functions_PPE2.cuh:533-534.  Neighbours counted
+   * are every SPH particle within kappa*h (walls included), where the
+   * reference counts only its own p_type (which marks fluid next to walls
+   * as surface).  0 = off (reference: no surface rows). */
+  int32_t free_surface;
+  double fs_threshold;      /* 0.3 (functions_PST.cuh:319) */
   int32_t reserved_i;
   double reserved[7];
 } sophia_params;
+
+#define SOPHIA_COMPAT_NO_SPIN 1
+#define SOPHIA_COMPAT_NUSSELT 2
+#define SOPHIA_COMPAT_HEAT_OVERWRITE 4
+#define SOPHIA_COMPAT_CP_ARGS 8
+#define SOPHIA_COMPAT_ALL 15
 
 /* Domain geometry after find_minmax (function_init.cuh:913-950) and the
  * cell sizing of ISPH.cuh:93-113. */

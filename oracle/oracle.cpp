@@ -90,6 +90,8 @@ struct orc_state {
   int64_t count = 0;
   double time = 0.0;
   bool after_corrector = false;
+  // fluid Shepard sum of each flux wall (sorted SPH order; 0 elsewhere)
+  std::vector<double> flux_S;
   // PPE system of the last assembly
   std::vector<int32_t> row_mask;
   std::vector<PPERow> rows;
@@ -217,6 +219,19 @@ static double conductivity(const sophia_params& p, int ptype) {
 }
 static double heat_capacity(const sophia_params& p, int ptype) {
   return std::abs(ptype) > 1000 ? p.cp_dem : p.cp_fluid;
+}
+// emissivity of spheres (functions_PROP.cuh:420-440, graphite fit above 700 K)
+static double dem_emissivity(double T) {
+  if (T >= 700.0) return 0.505586 + 7.92943e-4 * T - 5.228643e-7 * T * T + 1.10479e-10 * T * T * T;
+  return 0.842337;
+}
+// Pr^(1/3) of the Nusselt correlation; the reference's integer 1/3 gives Pr^0
+static double pr_third(const sophia_params& p, double pr) {
+  return (p.reference_compat & SOPHIA_COMPAT_NUSSELT) ? 1.0 : std::cbrt(pr);
+}
+static bool in_flux_box(const sophia_params& p, const part1& q) {
+  const double* b = p.heat_flux_box;
+  return q.p_type == 0 && q.x >= b[0] && q.y >= b[1] && q.z >= b[2] && q.x <= b[3] && q.y <= b[4] && q.z <= b[5];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -718,6 +733,36 @@ static void prep3d_sph(orc_state& s) {
   }
 }
 
+/* Flux walls (sophia_params.heat_flux; no counterpart in the compiled      */
+/* reference): S_w = sum over fluid k within kappa*h_w of V_k W(r_kw, h_w),  */
+/* the normaliser that makes the fluid receive heat_flux times the walls'   */
+/* face area exactly (sum_i V_i q A_w W_iw / S_w = q A_w).                  */
+static bool flux_receiver(const sophia_params& p, const part1& q) {
+  return q.i_type <= kITypeCrt && !(p.open_boundary > 0 && q.buffer_type > 0) && q.p_type > 0 && q.p_type < 1000;
+}
+static void flux_walls(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  s.flux_S.assign(sp.n(), 0.0);
+  if (!(p.con_solve == 1 && p.heat_flux != 0.0)) return;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (!in_flux_box(p, qi)) continue;
+    const double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
+    double S = 0;
+    for_candidates(s, sp, qi.x, qi.y, qi.z, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!flux_receiver(p, qj)) return;
+      double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z);
+      if (!(r < R)) return;
+      S += qj.m / qj.rho * knl_w(kt, A, hi, r);
+    });
+    s.flux_S[i] = S;
+  }
+}
+
 void orc_prep(orc_state* s) {
   int stride = std::max(1, s->p.decouple_stride);
   bool sph_on = (s->count % stride) == 0;
@@ -729,6 +774,7 @@ void orc_prep(orc_state* s) {
     if (sph_on) {
       prep_prep_sph(*s);
       prep3d_sph(*s);
+      flux_walls(*s);
     }
   }
 }
@@ -756,11 +802,17 @@ void orc_interaction(orc_state* s) {
     double kci = conductivity(p, qi.p_type), cpi = heat_capacity(p, qi.p_type);
     double visi = viscosity(p) + sp.P3[i].vis_t;
     double tx = 0, ty = 0, tz = 0, Rc = 0, px = 0, py = 0, pz = 0;
-    double ex = 0, ey = 0, ez = 0, et = 0;
+    double ex = 0, ey = 0, ez = 0, et = 0, fl = 0;
+    const bool flux = p.con_solve == 1 && p.heat_flux != 0.0 && (int)s->flux_S.size() == sp.n();
     for_candidates(*s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type < 1000)) return;
       double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      if (flux && s->flux_S[j] > 0.0) {
+        // wall heat flux: q A_w W_iw / S_w (the wall's face area from its own h)
+        double rw = dist_of(dx, dy, dz);
+        if (rw < R) fl += p.heat_flux * std::pow(qj.h / p.h_over_dx, 2) * knl_w(kt, A, hi, rw) / s->flux_S[j];
+      }
       double r = dist_of(dx, dy, dz) + 1e-20;
       if (!(r < R)) return;
       double dw = knl_dw(kt, A, hi, r);
@@ -820,7 +872,7 @@ void orc_interaction(orc_state* s) {
     o.ftotalx = 1 * (tx - ex);
     o.ftotaly = 1 * (ty - ey);
     o.ftotalz = 1 * (tz - ez);
-    if (p.con_solve) o.dtemp = Rc / cpi * (qi.p_type != -1) - et;
+    if (p.con_solve) o.dtemp = Rc / cpi * (qi.p_type != -1) - et + fl / (rhoi * cpi);
     o.fsx = o.fsy = o.fsz = 0;
     qi.pgf_x = px;
     qi.pgf_y = py;
@@ -851,6 +903,13 @@ void orc_dem_interaction(orc_state* s) {
     double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
     double wxi = qi.wx, wyi = qi.wy, wzi = qi.wz, radi = qi.rad, mi = qi.m, rii = qi.ri;
     double tx = 0, ty = 0, tz = 0, qx = 0, qy = 0, qz = 0;
+    // heat (con_solve, :948-1030, 1080-1194, 1300-1315): the outer shell of
+    // thickness rad/6 exchanges heat with neighbouring spheres
+    const bool heat = p.con_solve == 1;
+    const double tempi = qi.temp, rhoi = qi.rho, kci = p.k_dem, cpi = p.cp_dem, emi = dem_emissivity(tempi);
+    const double dr2 = radi / 6.0;
+    const double vol_matrix = (4.0 / 3.0) * 3.141592 * (radi * radi * radi - (radi - dr2) * (radi - dr2) * (radi - dr2));
+    double dt_rad = 0, dt_con = 0;
     for_candidates(*s, dm, xi, yi, zi, [&](int j) {
       const part1& qj = dm.SP1[j];
       if (!(qj.p_type > 1000)) return;
@@ -859,8 +918,41 @@ void orc_dem_interaction(orc_state* s) {
       double dist = dist_of(xi - qj.x, yi - qj.y, zi - qj.z);
       double ovlp = radi + radj - dist;
       if (!(dist < 3.5 * radi && dist > 0.0)) return;
+      const double radij_h = 1 / (1 / radi + 1 / radj);
+      if (heat) {
+        // radiation between the facing caps of a double cone (:1085-1152);
+        // its radius 0.56 (2 R*) (1 - 0.401038)^(-1/3), with the reference's
+        // integer -1/3 giving exponent 0 in compat mode
+        const double tempj = qj.temp, emj = dem_emissivity(tempj);
+        const double ai = radi / dist, aj = radj / dist, dA = aj - ai;
+        const double cone = (p.reference_compat & SOPHIA_COMPAT_NUSSELT) ? 1.0 : std::pow(1 - 0.401038, -1.0 / 3.0);
+        const double dc_r = 0.56 * (2 * radij_h) * cone;
+        const double dc_ar = 3.14159 * dc_r * dc_r;
+        const double di_max = 0.5 * (std::sqrt(1 + 4.0 * dc_ar / (3.14159 * dist * dist * (1 - dA * dA))) - dA);
+        const double cos_i = (1 - dA * dA) / (2 * di_max) - dA;
+        const double cos_j = (1 - dA * dA) / (2 * (di_max + dA)) + dA;
+        const double sin_i = std::sqrt(1 - cos_i * cos_i), sin_j = std::sqrt(1 - cos_j * cos_j);
+        const double ri_cr = radi * sin_i, rj_cr = radj * sin_j;
+        const double d_cr = dist - radi * cos_i - radj * cos_j;
+        const double z_cr = 1 + (4 * rj_cr * rj_cr + d_cr * d_cr) / (ri_cr * ri_cr);
+        const double F_cr = 0.5 * (z_cr - std::sqrt(z_cr * z_cr - 4.0 * rj_cr * rj_cr / (ri_cr * ri_cr)));
+        const double F_ij = (1 + cos_i) * F_cr * 0.5;
+        const double ar_i = 2.0 * 3.14159 * radi * radi * (1 - cos_i);
+        const double ar_j = 2.0 * 3.14159 * radj * radj * (1 - cos_j);
+        const double ar_ratio = ar_j / ar_i;
+        const double Y_ij = F_ij + (1 - F_ij) * (ar_ratio / (1 + ar_ratio));
+        const double em_inv = (1 - emi) / (emi * ar_i) + (1 - emj) / (emj * ar_j) + 1 / (Y_ij * ar_i);
+        const double t4 = tempi * tempi * tempi * tempi - tempj * tempj * tempj * tempj;
+        dt_rad += -5.6704e-8 * t4 / (em_inv * (rhoi * vol_matrix) * cpi);
+      }
       if (!(dist < 2.01 * radi)) return;
       if (dist > 0 && ovlp > 0) {
+        if (heat) {
+          // contact conduction through the overlap circle (:1300-1315)
+          const double kc_ij = 1 / (1 / kci + 1 / p.k_dem);
+          dt_con += -4.0 * kc_ij * std::sqrt((2 * ovlp) * (radij_h - 0.5 * ovlp)) * (tempi - qj.temp) /
+                    ((rhoi * vol_matrix) * cpi);
+        }
         double dsx = 0, dsy = 0, dsz = 0;
         int slot = -1;
         for (int k = 0; k < slots; k++)
@@ -946,6 +1038,13 @@ void orc_dem_interaction(orc_state* s) {
     });
     if (p.fg_solve) tz += -p.gravity;
     part3& o = dm.P3[i];
+    if (heat) {
+      // (:1725-1729) without the inner-shell terms: this path never advances
+      // the shell temperatures temp1..3, so their conduction is left out
+      dm.SP1[i].test1 = dt_rad;  // the kernel writes its read buffer (P1[i].test1, :1723-1724)
+      dm.SP1[i].test2 = dt_con;
+      o.dtemp = dt_rad + dt_con;
+    }
     o.ftotalx = tx;
     o.ftotaly = ty;
     o.ftotalz = tz;
@@ -976,6 +1075,7 @@ static void dem_coupling(orc_state& s) {
     double fx0 = o.ftotalx, fy0 = o.ftotaly, fz0 = o.ftotalz;
     double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
     double tx = 0, ty = 0, tz = 0, ufx = 0, ufy = 0, ufz = 0, uax = 0, uay = 0, uaz = 0, por = 0;
+    double tf = 0, ta = 0;  // kernel-weighted fluid and air temperature sums (:806-867)
     if (sp.n() > 0) {
       for_candidates(s, sp, xi, yi, zi, [&](int j) {
         const part1& qj = sp.SP1[j];
@@ -990,12 +1090,14 @@ static void dem_coupling(orc_state& s) {
           ufy += qj.uy * mj * w / rhoj;
           ufz += qj.uz * mj * w / rhoj;
           por += qj.DEMpor * mj * w / rhoj;
+          tf += qj.temp * mj * w / rhoj;
         }
         if (t == 3 && flt_air > 0.000001) {
           uax += qj.ux * mj * w / rhoj;
           uay += qj.uy * mj * w / rhoj;
           uaz += qj.uz * mj * w / rhoj;
           por += qj.DEMpor * mj * w / rhoj;
+          ta += qj.temp * mj * w / rhoj;
         }
         if (t != 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 3) {
           if (flt_fluid > 0.01) {
@@ -1023,6 +1125,16 @@ static void dem_coupling(orc_state& s) {
     qi.test6 = magf;
     double dfx = 0, dfy = 0, dfz = 0, dax = 0, day = 0, daz = 0;
     qi.DEMpor = pori;
+    // convective heat (:996-1106): Ranz-Marshall Nu = 2 + 0.6 Re^0.5 Pr^(1/3),
+    // h = Nu k / d over the specific surface 6/d; the sphere's k and
+    // diffusivity by material (:708-718); no branch taken means no exchange
+    const double tempi = qi.temp, rhoi = qi.rho;
+    const int mat = radi < p.dem_mat_radius ? 1 : 0;
+    const double ki = p.dem_k_mat[mat], cpi = ki / (p.dem_alpha_mat[mat] * rhoi);
+    const double a_ht = 6.0 / di;
+    const double T_f = tf / (flt_fluid + 1.0e-15) - tempi, T_a = ta / (flt_air + 1.0e-15) - tempi;
+    const double T_m = (tf + ta) / (flt_fluid + flt_air + 1.0e-15) - tempi;
+    double dq_vol = 0;
     double nu_f = p.mu_fluid / p.rho_fluid;  // vis_kif = VISCOSITY_AIR/DENSITY_AIR
     double rho_f = p.rho_fluid;
     if (magf > 1.0e-8 && (flt_fluid + flt_air) > 0.00001 && f_fluid > p.drag_switch) {
@@ -1037,6 +1149,8 @@ static void dem_coupling(orc_state& s) {
       qi.Fdx_df = dfx;
       qi.Fdy_df = dfy;
       qi.Fdz_df = dfz;
+      const double Nu = 2.0 + 0.6 * std::pow(Re, 0.5) * pr_third(p, p.pr_fluid);
+      dq_vol = Nu * p.k_fluid / di * a_ht * T_f;
     }
     if (maga > 1.0e-8 && (flt_fluid + flt_air) > 0.00001 && f_air > p.drag_switch) {
       double Re = di * pori * maga / (nu_f + 1.0e-15);
@@ -1047,6 +1161,8 @@ static void dem_coupling(orc_state& s) {
       dax = k * rax * maga * e / mi;
       day = k * ray * maga * e / mi;
       daz = k * raz * maga * e / mi;
+      const double Nu = 2.0 + 0.6 * std::pow(Re, 0.5) * pr_third(p, p.pr_air);
+      dq_vol = Nu * ki / di * a_ht * T_a;
     }
     if ((maga > 1.0e-8 || magf > 1.0e-8) && (flt_fluid + flt_air) > 0.00001 && f_air <= p.drag_switch &&
         f_fluid <= p.drag_switch) {
@@ -1061,6 +1177,8 @@ static void dem_coupling(orc_state& s) {
       dax = k * rmx * magm * e / mi;
       day = k * rmy * magm * e / mi;
       daz = k * rmz * magm * e / mi;
+      const double Nu = 2.0 + 0.6 * std::pow(Re, 0.5) * pr_third(p, p.pr_air);
+      dq_vol = Nu * ki / di * a_ht * T_m;
     }
     o.ftotalx = fx0 + dfx / pori + tx;
     o.ftotaly = fy0 + dfy / pori + ty;
@@ -1071,6 +1189,11 @@ static void dem_coupling(orc_state& s) {
     qi.Fdx_b = tx;
     qi.Fdy_b = ty;
     qi.Fdz_b = tz;
+    if (p.con_solve == 1) {
+      const double conv = dq_vol / (rhoi * cpi);  // [K/s]
+      o.dtemp = (p.reference_compat & SOPHIA_COMPAT_HEAT_OVERWRITE) ? conv : o.dtemp + conv;
+      qi.Q_sd = dq_vol * mi / rhoi;  // [W]
+    }
   }
 }
 
@@ -1087,7 +1210,11 @@ static void sph_coupling(orc_state& s) {
     int t = qi.p_type;
     part3& o = sp.P3[i];
     double fx0 = o.ftotalx, fy0 = o.ftotaly, fz0 = o.ftotalz, dti = o.dtemp;
-    double cpi = heat_capacity(p, t);
+    // heat_capacity(ptypei, tempi) at :1860 swaps its arguments: the
+    // temperature is read as the particle type
+    double cpi = (p.reference_compat & SOPHIA_COMPAT_CP_ARGS)
+                     ? (std::abs((long)(unsigned)qi.temp) > 1000 ? p.cp_dem : p.cp_fluid)
+                     : heat_capacity(p, t);
     double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
     double tx = 0, ty = 0, tz = 0, tt = 0;
     if (dm.n() > 0) {
@@ -1193,6 +1320,14 @@ static void corrector(orc_state& s, Species& sp, double dt, bool is_sph) {
       T.x = xc; T.y = yc; T.z = zc;
     }
     T.ux = uxc; T.uy = uyc; T.uz = uzc;
+    if (!is_sph && t > 1000 && !(p.reference_compat & SOPHIA_COMPAT_NO_SPIN)) {
+      // spin from the contact torque per inertia, as the WCSPH corrector
+      // does (functions_TIME.cuh:753-755); the ISPH path leaves it out
+      const double bz = (q.buffer_type == 0);
+      T.wx = q.wx + o.torqx * bz * dt;
+      T.wy = q.wy + o.torqy * bz * dt;
+      T.wz = q.wz + o.torqz * bz * dt;
+    }
     if (p.con_solve == 1 && t > 0) {
       T.temp = q2.temp0 + o.dtemp * dt * (t != -1);
     } else {
