@@ -23,6 +23,7 @@ SOPHIA_OK = 0
 SPH, DEM = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 KNL_GAUSSIAN, KNL_QUINTIC, KNL_QUARTIC, KNL_WENDLAND2, KNL_WENDLAND4, KNL_WENDLAND6 = range(6)
+KNL_NOALLOC_MAX = 65536  # SOPHIA_KNL_NOALLOC_MAX
 PT_BOUNDARY, PT_FLUID, PT_MOVING, PT_DEM = 0, 1, 9, 1001
 PPE_JACOBI, PPE_BICGSTAB = 0, 1
 PPE_FP64, PPE_MIXED_IR = 0, 1
@@ -175,6 +176,7 @@ SIGNATURES = {
     "sophia_get_particles": (C.c_int, [_ctx, C.c_int32, _VP, _VP, _VP]),
     "sophia_count": (C.c_int, [_ctx, C.c_int32, _P(C.c_int32)]),
     "sophia_knl_eval": (C.c_int, [C.c_int32, C.c_int32, C.c_double, _VP, C.c_int32, _VP, _VP]),
+    "sophia_knl_eval_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_double, _VP, C.c_int32, _VP, _VP]),
     "sophia_set_pressure": (C.c_int, [_ctx, _VP, C.c_int32]),
     "sophia_get_ppe_system": (C.c_int, [_ctx, _VP, _VP, _VP]),
     "sophia_ppe_apply": (C.c_int, [_ctx, _VP, _VP, C.c_int32]),
@@ -239,6 +241,10 @@ class SophiaError(RuntimeError):
         msg = lib.sophia_last_error().decode(errors="replace")
         super().__init__(f"{where} -> {ERRORS.get(rc, rc)}: {msg}")
         self.rc = rc
+
+
+def last_error() -> str:
+    return load_library().sophia_last_error().decode(errors="replace")
 
 
 def check(rc: int, where: str) -> None:

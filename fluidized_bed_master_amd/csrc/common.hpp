@@ -547,5 +547,11 @@ template <typename U>
 int monitor_max(hipStream_t s, int n, const int* ptype, const int* itype, const U* ux, const U* uy, const U* uz,
                 const double* rho, const U* ft, double* scratch, double* out);
 int knl_eval_device(hipStream_t s, int kt, int dim, double h, const double* r, int n, double* w, double* dw);
+// allocation-free variant on a file-scope device array (nnps.hip): at most
+// kKnlScratchMax radii per call; calls that overlap on one device race
+constexpr int kKnlScratchMax = 65536;
+constexpr int kMaxDevices = 64;
+int knl_scratch_init(int device);
+int knl_eval_noalloc(int kt, int dim, double h, const double* r, int n, double* w, double* dw);
 
 }  // namespace sophia

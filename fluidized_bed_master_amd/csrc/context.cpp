@@ -1939,6 +1939,11 @@ int sophia_knl_eval(int32_t kt, int32_t dim, double h, const double* r, int32_t 
   return rc;
 }
 
+int sophia_knl_eval_noalloc(int32_t kt, int32_t dim, double h, const double* r, int32_t n, double* w, double* dw) {
+  if (!r || !w || !dw || n < 0) return SOPHIA_E_ARG;
+  return sophia::knl_eval_noalloc(kt, dim, h, r, n, w, dw);
+}
+
 // learned warm start
 int sophia_mlp_param_count(int32_t* n) {
   if (!n) return SOPHIA_E_ARG;

@@ -15,6 +15,8 @@
 // reference's SortPairs of (key, identity index).
 #include "common.hpp"
 
+#include <mutex>
+
 namespace sophia {
 
 namespace {
@@ -407,6 +409,52 @@ int knl_eval_device(hipStream_t s, int kt, int dim, double h, const double* r, i
   if (n <= 0) return 0;
   k_knl<<<grid_for(n), kBlock, 0, s>>>(kt, dim, h, r, n, w, dw);
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Allocation-free kernel evaluation: the three staging arrays (r, W, dW/dr)
+// live in one file-scope device array, kKnlScratchMax values each.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in, not a device
+// address); every device holds its own copy of the array.
+__device__ __align__(256) double scratch[3 * kKnlScratchMax];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int knl_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d outside the %d devices the scratch table covers", device, kMaxDevices);
+    return SOPHIA_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return 0;
+}
+
+int knl_eval_noalloc(int kt, int dim, double h, const double* r, int n, double* w, double* dw) {
+  if (n > kKnlScratchMax) {
+    set_error("sophia_knl_eval_noalloc: n = %d exceeds the scratch capacity of %d values", n, kKnlScratchMax);
+    return SOPHIA_E_CAPACITY;
+  }
+  if (n <= 0) return 0;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  TRY(knl_scratch_init(dev));
+  double* base = g_scratch_ptr[dev];
+  double *dr = base, *dwv = base + kKnlScratchMax, *ddw = base + 2 * kKnlScratchMax;
+  const size_t bytes = (size_t)n * sizeof(double);
+  HIP_TRY(hipMemcpy(dr, r, bytes, hipMemcpyHostToDevice));
+  // every output value is written by the kernel, so nothing needs clearing
+  k_knl<<<grid_for(n), kBlock, 0, nullptr>>>(kt, dim, h, dr, n, dwv, ddw);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(w, dwv, bytes, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(dw, ddw, bytes, hipMemcpyDeviceToHost));
   return 0;
 }
 

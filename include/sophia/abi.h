@@ -363,6 +363,16 @@ int sophia_write_restart(const sophia_ctx* ctx, const char* path);
 /* evaluate W and dW/dr on the device (calc_kernel_wij / calc_kernel_dwij) */
 int sophia_knl_eval(int32_t kernel_type, int32_t dim, double h, const double* r,
                     int32_t n, double* w, double* dw);
+/* The same evaluation with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Staging
+ * for r, W and dW/dr is a file-scope device array of 3 x 65536 doubles
+ * (1.5 MiB per device, held until the process ends); n > 65536 returns
+ * SOPHIA_E_CAPACITY.  Every call on one device shares that array, so calls
+ * that overlap on one device (from several host threads) race. */
+#define SOPHIA_KNL_NOALLOC_MAX 65536
+int sophia_knl_eval_noalloc(int32_t kernel_type, int32_t dim, double h, const double* r,
+                            int32_t n, double* w, double* dw);
 /* overwrite the SPH pressure (sorted order) e.g. to pose a PPE problem */
 int sophia_set_pressure(sophia_ctx* ctx, const double* p, int32_t n);
 /* PPE system of the current step in sorted order: diag = sum_j a_ij,
