@@ -41,7 +41,11 @@ def _matvec_bytes(layout: dict, n_sph: int, fp32: bool) -> float:
     return layout["nnz"] * (coef + layout["index_bytes"]) + n_sph * (28.0 + 3 * vec)
 
 
-def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp_weights=None):
+def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp_weights=None, keep_pressure=False):
+    """warmup untimed steps, then `steps` timed ones between two device
+    synchronisations.  With keep_pressure the final pressure of the PPE rows
+    (fluid particles, in the particles' original numbering) is returned in
+    the last slot, for comparisons between legs run from the same input."""
     from fluidized_bed_master_amd.sophia import Sophia
     from fluidized_bed_master_amd import abi
     g = Sophia(params, parts, precision=precision)
@@ -62,8 +66,15 @@ def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp
     rep = g.timers_report() if timers else {}
     layout = g.ppe_layout()
     n_sph, n_dem = g.count(abi.SPH), g.count(abi.DEM)
+    pres = None
+    if keep_pressure:
+        a, _, a3 = g.get_particles(abi.SPH)
+        # the SPH records are in cell order; key them by position to compare legs
+        fl = (a["p_type"] >= 1) & (a["p_type"] < 1000) & (a3["lbl_surf"] == 0)
+        order = np.lexsort((a["x"][fl], a["y"][fl], a["z"][fl]))
+        pres = a["pres"][fl][order]
     g.close()
-    return dt, st_timed, rep, layout, n_sph, n_dem
+    return dt, st_timed, rep, layout, n_sph, n_dem, pres
 
 
 PMC_FILE = ROOT / "profiles" / "ppe_matvec_fp32_c5_1m_pmc.txt"
@@ -88,31 +99,53 @@ def _pmc_traffic():
         return None
 
 
-def cpu_baseline(target_seconds: float = 15.0):
-    """The FP64 C++/OpenMP oracle (same formulas, same BiCGStab) timed on a
-    bounded sample of the same workload kind: a 40 x 40 x 40 ISPH-DEM column."""
+def cpu_baseline(params, parts, target_seconds: float = 15.0):
+    """The FP64 C++/OpenMP oracle (same formulas, same BiCGStab with the
+    Jacobi preconditioner, previous-step guess, rtol 1e-8) on the benchmark's
+    own input: one untimed step, then whole steps until target_seconds have
+    passed (at least one), with every host thread the process may use;
+    beside it a single-thread run of the same step on a 24^3-fluid column of
+    the same kind, bounded the same way."""
+    import copy
     from fluidized_bed_master_amd import abi, configs
     from oracle import oracle
-    p, parts = configs.isph_dem_column(40, 40, 40, 1.0e-3, d_dem=2.0e-3, dem_height=0.012, seed=5, dt=2.0e-6)
-    o = oracle.Oracle(p, parts)
     opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8)
-    o.step(opts)  # first step (count 0 extras) untimed
-    n = len(parts)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        o.step(opts)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= target_seconds or steps >= 200:
-            break
+
+    def timed(p, pa, secs):
+        o = oracle.Oracle(p, pa)
+        o.step(opts)  # first step (count 0 extras) untimed
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            o.step(opts)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= secs or steps >= 200:
+                break
+        o.close()
+        return len(pa) * steps / el, steps, el
+
+    oracle.set_threads(0)
+    threads = oracle.get_threads()
+    v, steps, el = timed(params, parts, target_seconds)
+    ps, pas = configs.isph_dem_column(24, 24, 24, 1.0e-3, d_dem=2.0e-3, dem_height=0.008, seed=5, dt=params.dt)
+    ps = copy.copy(ps)
+    for f in ("con_solve", "heat_flux", "free_surface", "pgrad_form"):
+        setattr(ps, f, getattr(params, f))
+    if params.heat_flux != 0.0:
+        configs.heated(ps, pas, params.heat_flux)
+    oracle.set_threads(1)
+    v1, steps1, el1 = timed(ps, pas, max(5.0, target_seconds / 2))
+    oracle.set_threads(0)
     return {
-        "value": n * steps / el,
+        "value": v,
         "unit": "particle-steps/s",
-        "cores": oracle.get_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": f"40x40x40 ISPH-DEM column ({n} particles), {steps} steps, BiCGStab FP64 rtol 1e-8, "
-                  f"{el:.1f} s",
+        "sample": f"the benchmark input itself ({len(parts)} particles), {steps} step(s) after one untimed, "
+                  f"{el:.1f} s on {threads} threads; BiCGStab FP64, Jacobi preconditioner, rtol 1e-8",
+        "single_thread": {"value": v1, "unit": "particle-steps/s", "cores": 1,
+                          "sample": f"24x24x24-fluid ISPH-DEM column ({len(pas)} particles), same settings, "
+                                    f"{steps1} step(s), {el1:.1f} s"},
     }
 
 
@@ -125,6 +158,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-heat", action="store_true", help="heat transfer off (free-surface rows stay on)")
+    ap.add_argument("--pgrad-form", type=int, default=0, choices=[0, 1],
+                    help="pressure gradient: 0 the reference's symmetric sum V_j (p_i + p_j) grad W, "
+                         "1 the difference form sum V_j (p_j - p_i) grad W")
     ap.add_argument("--weights", default=str(ROOT / "fluidized_bed_master_amd" / "weights" / "ppe_mlp.npy"))
     ap.add_argument("--precond", default="two_level", choices=["two_level", "jacobi"],
                     help="preconditioner of the mixed-precision solve")
@@ -148,10 +185,15 @@ def main(argv=None) -> int:
     gen = {"c5_1m": configs.c5_1m, "c5": configs.c5, "c2_isph": configs.c2_isph,
            "small": lambda: configs.small_isph_dem()}[a.config]
     params, parts = gen()
-    # a PPE converged to rtol 1e-8 is only stable with the difference-form
-    # pressure gradient; the reference's symmetric form is kept for its
-    # one-sweep Jacobi update
-    params.pgrad_form = 1
+    # C5's heat transfer (conduction, radiation, convective exchange, a
+    # 5 kW/m^2 side-wall flux) and free-surface PPE rows, under which the
+    # converged solve with the reference's symmetric pressure gradient stays
+    # at rest
+    if not a.no_heat:
+        configs.heated(params, parts)
+    else:
+        params.free_surface = 1
+    params.pgrad_form = a.pgrad_form
     n_total = len(parts)
 
     weights = None
@@ -166,8 +208,8 @@ def main(argv=None) -> int:
     # operator and vectors, previous-step guess
     fp64_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8)
 
-    dt, st, rep, layout, n_sph, n_dem = run_gpu(params, parts, abi.PREC_MIXED, mixed_opts, a.steps, a.warmup,
-                                               timers=True, mlp_weights=weights)
+    dt, st, rep, layout, n_sph, n_dem, pres_head = run_gpu(params, parts, abi.PREC_MIXED, mixed_opts, a.steps, a.warmup,
+                                                     timers=True, mlp_weights=weights, keep_pressure=True)
     value = n_total * a.steps / dt
     ms_step = dt / a.steps * 1e3
     ppe_ms = float(np.mean([s["ms"] for s in st]))
@@ -202,18 +244,26 @@ def main(argv=None) -> int:
 
     fp64 = None
     if "fp64" in legs:
-        dt64, st64, _, _, _, _ = run_gpu(params, parts, abi.PREC_FP64, fp64_opts, a.steps, a.warmup)
+        dt64, st64, _, _, _, _, pres64 = run_gpu(params, parts, abi.PREC_FP64, fp64_opts, a.steps, a.warmup,
+                                                 keep_pressure=True)
         fp64 = {"value": n_total * a.steps / dt64, "ms_per_step": dt64 / a.steps * 1e3,
                 "ppe_ms": float(np.mean([s["ms"] for s in st64])),
                 "ppe_iters": float(np.mean([s["iterations"] for s in st64])),
-                "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in st64]))}
+                "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in st64])),
+                "ppe_converged_all": bool(all(s["converged"] for s in st64)),
+                "ppe_r_rel_max": float(np.max([s["r_rel"] for s in st64]))}
+        # the headline's pressure against the FP64 path's after the same
+        # warm-up + timed steps from the same input (fluid rows): the
+        # FP64 path matches the FP64 oracle to 1e-9 in the parity tests
+        if pres64 is not None and pres_head is not None and len(pres64) == len(pres_head):
+            fp64["headline_p_rel_err"] = float(np.max(np.abs(pres_head - pres64)) / np.max(np.abs(pres64)))
 
     # the FP64 path with the same preconditioner: what precision alone buys
     fp64_pc = None
     if "fp64_pc" in legs and precond != abi.PRECOND_JACOBI:
         pc_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8,
                                precond=precond)
-        dtq, stq, _, _, _, _ = run_gpu(params, parts, abi.PREC_FP64, pc_opts, a.steps, a.warmup)
+        dtq, stq, _, _, _, _, _ = run_gpu(params, parts, abi.PREC_FP64, pc_opts, a.steps, a.warmup)
         fp64_pc = {"value": n_total * a.steps / dtq, "ms_per_step": dtq / a.steps * 1e3,
                    "ppe_ms": float(np.mean([s["ms"] for s in stq])),
                    "ppe_iters": float(np.mean([s["iterations"] for s in stq])),
@@ -224,7 +274,7 @@ def main(argv=None) -> int:
     if "prev" in legs:
         prev_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=abi.GUESS_PREVIOUS,
                                  rtol=1e-8, precond=precond, inner_rtol=a.inner_rtol)
-        dtp, stp, _, _, _, _ = run_gpu(params, parts, abi.PREC_MIXED, prev_opts, a.steps, a.warmup)
+        dtp, stp, _, _, _, _, _ = run_gpu(params, parts, abi.PREC_MIXED, prev_opts, a.steps, a.warmup)
         prev = {"ms_per_step": dtp / a.steps * 1e3, "ppe_ms": float(np.mean([s["ms"] for s in stp])),
                 "ppe_iters": float(np.mean([s["iterations"] for s in stp])),
                 "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in stp]))}
@@ -234,7 +284,7 @@ def main(argv=None) -> int:
     if "extrap" in legs and weights is not None:
         ex_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_MIXED_IR, guess=abi.GUESS_EXTRAPOLATE,
                                rtol=1e-8, precond=precond, inner_rtol=a.inner_rtol)
-        dte, ste, _, _, _, _ = run_gpu(params, parts, abi.PREC_MIXED, ex_opts, a.steps, a.warmup)
+        dte, ste, _, _, _, _, _ = run_gpu(params, parts, abi.PREC_MIXED, ex_opts, a.steps, a.warmup)
         extrap = {"value": n_total * a.steps / dte, "ms_per_step": dte / a.steps * 1e3,
                   "ppe_ms": float(np.mean([s["ms"] for s in ste])),
                   "ppe_iters": float(np.mean([s["iterations"] for s in ste])),
@@ -263,7 +313,7 @@ def main(argv=None) -> int:
             pb.wall_box[k] = 0.0 if k < 2 else 0.5 * (lo[k] + hi[k])
             pb.wall_box[3 + k] = (2 * max(abs(lo[k]), abs(hi[k])) if k < 2 else hi[k] - lo[k]) + 4 * rad
         steps_bc = max(1, min(a.steps, 3))
-        dtb, stb, repb, _, _, _ = run_gpu(pb, parts, abi.PREC_MIXED, mixed_opts, steps_bc, 1, timers=True,
+        dtb, stb, repb, _, _, _, _ = run_gpu(pb, parts, abi.PREC_MIXED, mixed_opts, steps_bc, 1, timers=True,
                                           mlp_weights=weights)
         boundaries = {"ms_per_step": dtb / steps_bc * 1e3,
                       "open_boundary_ms": repb.get("open_boundary", (0, 0.0))[1] / steps_bc,
@@ -273,7 +323,7 @@ def main(argv=None) -> int:
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(a.cpu_seconds)
+        cpu = cpu_baseline(params, parts, a.cpu_seconds)
 
     out = {
         "metric": "particle-steps/s ISPH-DEM 1M; PPE ms learned+FP32 vs FP64 @rtol 1e-8",
@@ -291,14 +341,18 @@ def main(argv=None) -> int:
         "config": {
             "workload": f"{a.config}: ISPH-DEM column, {n_sph - int(np.sum(parts['p_type'] == 0))} fluid + "
                         f"{int(np.sum(parts['p_type'] == 0))} wall + {n_dem} DEM, dx=1mm, h=1.3dx, "
-                        f"Wendland C2, dt={params.dt:g}",
+                        f"Wendland C2, dt={params.dt:g}, "
+                        + (f"heat on (conduction, sphere radiation and contact conduction, Ranz-Marshall "
+                           f"exchange, {params.heat_flux:g} W/m^2 side-wall flux)" if params.con_solve else "heat off")
+                        + ", free-surface PPE rows",
             "particles": n_total,
             "ppe": "BiCGStab rtol 1e-8, FP32 iterations with FP64 reliable updates, "
                    + ("two-level (Jacobi + 3x3x3-cell aggregate coarse correction) preconditioner, "
                       if precond == abi.PRECOND_TWO_LEVEL else "Jacobi preconditioner, ")
                    + ("learned correction of the extrapolated guess" if weights is not None
                       else "quadratic extrapolation of the last three pressures"),
-            "pressure_gradient": "difference form sum_j V_j (p_j - p_i) grad~W_ij",
+            "pressure_gradient": ("difference form sum_j V_j (p_j - p_i) grad~W_ij" if params.pgrad_form else
+                                  "the reference's symmetric form sum_j V_j (p_i + p_j) grad~W_ij"),
             "parallelism": "single GPU",
         },
         "ppe_ms_per_step": ppe_ms,
@@ -306,6 +360,8 @@ def main(argv=None) -> int:
         "ppe_folds_per_step": float(np.mean([s["outer"] for s in st])),
         "ppe_inner_rtol": a.inner_rtol,
         "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in st])),
+        "ppe_converged_all": bool(all(s["converged"] for s in st)),
+        "ppe_r_rel_max": float(np.max([s["r_rel"] for s in st])),
         "ppe_guess_learned_frac": float(np.mean([s["guess_used"] == abi.GUESS_LEARNED for s in st])),
         "ppe_precond_used": sorted({int(s["precond_used"]) for s in st}),
         "ppe_n_coarse": int(st[-1]["n_coarse"]),

@@ -98,7 +98,7 @@ class Params(C.Structure):
         ("pr_fluid", C.c_double), ("pr_air", C.c_double), ("dem_k_mat", C.c_double * 2),
         ("dem_alpha_mat", C.c_double * 2), ("dem_mat_radius", C.c_double),
         ("heat_flux", C.c_double), ("heat_flux_box", C.c_double * 6),
-        ("free_surface", C.c_int32), ("fs_threshold", C.c_double),
+        ("free_surface", C.c_int32), ("fs_threshold", C.c_double), ("prop_table", C.c_int32),
         ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
     ]
 
@@ -133,6 +133,7 @@ class PpeStats(C.Structure):
         ("r0_rel", C.c_double), ("r0_rel_learned", C.c_double), ("r_rel", C.c_double), ("ms", C.c_double),
         ("n_rows", C.c_int32), ("nnz", C.c_int32), ("loss", C.c_double),
         ("precond_used", C.c_int32), ("n_coarse", C.c_int32), ("coarse_res", C.c_double),
+        ("r0_rel_base", C.c_double),
     ]
 
     def as_dict(self):
@@ -155,7 +156,10 @@ SIGNATURES = {
     "sophia_read_input": (C.c_int, [C.c_char_p, _VP, C.c_int32, _P(C.c_int32)]),
     "sophia_read_ptype": (C.c_int, [C.c_char_p, _VP, C.c_int32, _P(C.c_int32)]),
     "sophia_create_from_arrays": (C.c_int, [_P(Params), _VP, C.c_int32, C.c_int32, C.c_int32, _P(_ctx)]),
-    "sophia_create": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, _P(_ctx)]),
+    "sophia_create": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, _P(_ctx)]),
+    "sophia_set_table": (C.c_int, [_ctx, _VP]),
+    "sophia_read_table": (C.c_int, [C.c_char_p, _VP]),
+    "sophia_table_eval": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_double, _P(C.c_double)]),
     "sophia_destroy": (C.c_int, [_ctx]),
     "sophia_last_error": (C.c_char_p, []),
     "sophia_get_domain": (C.c_int, [_ctx, _P(Domain)]),

@@ -14,6 +14,9 @@ Workloads (sizes in metres):
   c5()                   0.1 x 0.1 x 0.2 fill: 2,000,000 fluid + 60,000 spheres
   c2_isph()              0.05 x 0.05 x 0.15 fill at dx = 1 mm: 375,000 fluid +
                          8,000 spheres d = 3 mm (ISPH PPE on C2's particle set)
+  heated(p, parts)       C5's heating on any column: conduction on, a wall heat
+                         flux through the innermost side-wall layer over the
+                         fill height, free-surface PPE rows
   small_isph_dem(...)    parity-test sizes
 """
 from __future__ import annotations
@@ -230,3 +233,25 @@ def c2_isph(seed: int = 2):
     layers = math.ceil(8000 / (int(0.05 // s) ** 2))
     return isph_dem_column(50, 50, 150, dx, d_dem=3.0e-3, dem_height=layers * s + 1e-9, max_dem=8000,
                            seed=seed, dt=5.0e-7)
+
+
+def heated(p: Params, parts: np.ndarray, flux: float = 5000.0) -> Params:
+    """C5's heat transfer on a column from isph_dem_column: particle and
+    sphere conduction, radiation and convective exchange (con_solve = 1), a
+    wall heat flux (W/m^2) through the innermost layer of the four side
+    walls over the fill height, and Dirichlet free-surface rows in the PPE
+    (walls counted as neighbours), which keep a converged solve at rest."""
+    fl = parts["p_type"] == PT_FLUID
+    wall = parts["p_type"] == PT_BOUNDARY
+    dx = float(parts["h"][0]) / p.h_over_dx
+    lo = [float(parts[c][fl].min()) - 1.25 * dx for c in ("x", "y")]
+    hi = [float(parts[c][fl].max()) + 1.25 * dx for c in ("x", "y")]
+    zt = float(parts["z"][fl].max()) + 0.25 * dx
+    p.con_solve = 1
+    p.heat_flux = flux
+    box = [lo[0], lo[1], 0.0, hi[0], hi[1], zt]
+    for k, v in enumerate(box):
+        p.heat_flux_box[k] = v
+    p.free_surface = 1
+    assert np.any(wall & (parts["z"] > 0) & (parts["x"] >= lo[0]) & (parts["x"] <= hi[0]))
+    return p

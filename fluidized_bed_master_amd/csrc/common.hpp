@@ -72,7 +72,50 @@ struct DevParams {
   double pr_fluid, pr_air, dem_k_mat[2], dem_alpha_mat[2], dem_mat_radius;
   double heat_flux;       // W/m^2 through flux walls (0 = off)
   double heat_flux_box[6];
+  // property tables (prop_table): per |p_type| < tab_types the block
+  // tab[k*50 + tab_index[t] ...] of size tab_size[t], k = T, h, k, mu
+  int prop_table, tab_types;
+  int tab_index[SOPHIA_TABLE_TYPES], tab_size[SOPHIA_TABLE_TYPES];
+  const double* tab;
 };
+
+// Piecewise-linear interpolation of ys over the ascending xs (n >= 2),
+// extended linearly beyond both ends with the end segments: the reference's
+// interp2 (functions_PROP.cuh:143-175), evaluated in its operation order
+// ((y1 - y0) / (x1 - x0) * (x - x0) + y0) so that host and device agree.
+__host__ __device__ inline double interp_table(const double* xs, const double* ys, int n, double x) {
+  int a = 0;
+  if (xs[n - 1] < x) {
+    a = n - 2;
+  } else if (!(x <= xs[0])) {
+    a = n - 2;
+    for (int i = 0; i < n - 1; i++)
+      if (xs[i] < x && x <= xs[i + 1]) {
+        a = i;
+        break;
+      }
+  }
+  double y = ys[a + 1] - ys[a];
+  y /= (xs[a + 1] - xs[a]);
+  y *= (x - xs[a]);
+  y += ys[a];
+  return y;
+}
+
+// viscosity and conductivity of an SPH particle: the table of its |p_type|
+// when property tables are on and it has one, else the constants
+__device__ inline double prop_mu(const DevParams& P, int pt, double T) {
+  const int t = pt < 0 ? -pt : pt;
+  if (P.prop_table && t < P.tab_types && P.tab_size[t] >= 2)
+    return interp_table(P.tab + P.tab_index[t], P.tab + 3 * SOPHIA_TABLE_VALUES + P.tab_index[t], P.tab_size[t], T);
+  return P.mu_fluid;
+}
+__device__ inline double prop_k(const DevParams& P, int pt, double T) {
+  const int t = pt < 0 ? -pt : pt;
+  if (P.prop_table && t < P.tab_types && P.tab_size[t] >= 2)
+    return interp_table(P.tab + P.tab_index[t], P.tab + 2 * SOPHIA_TABLE_VALUES + P.tab_index[t], P.tab_size[t], T);
+  return t > 1000 ? P.k_dem : P.k_fluid;
+}
 
 // ---------------------------------------------------------------------------
 // Device state.  Persistent fields travel with the particles through the
@@ -88,6 +131,8 @@ struct SphView {
   // scratch
   double* por;  // porosity eps (FP64: pressure-gradient sums cancel against it)
   double* flux_s;  // fluid Shepard sum of each flux wall (0 elsewhere)
+  int* surf;       // free-surface label (P3.lbl_surf), written before the PPE when free_surface > 0
+  R *visp, *condp; // per-particle viscosity and conductivity from the property tables (prop_table)
   R *dpor_prev;
   double *rho_ref, *rho0;  // reference / start-of-step density (P2; the Tait EOS amplifies their rounding)
   double *vol, *vol0;      // WCSPH particle volume and its initial value (P1.vol, P1.vol0)
@@ -503,6 +548,10 @@ template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v);
 // fluid Shepard sums of the flux walls (no-op without a wall heat flux)
 template <typename R> int sph_flux_walls(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// per-particle viscosity and conductivity from the property tables (prop_table)
+template <typename R> int sph_props(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// free-surface labels and their Dirichlet pressure before the PPE (free_surface > 0)
+template <typename R> int sph_surface(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv);
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const SphView<R>& v);

@@ -126,6 +126,7 @@ struct sophia_ctx {
   virtual int ppe_coarse(int32_t* dims, float* E, float* X) const = 0;
   virtual int monitors(int species, double* out) = 0;
   virtual int wcsph_stage(int stage) = 0;
+  virtual int set_table(const sophia_table& t) = 0;
   virtual int mlp_predict(double* out, int n) = 0;
   virtual int mlp_train(const double* target, int n, double lr, double* loss, float* grad, int n_params, bool step) = 0;
   virtual int mlp_set(const float* w, int n) = 0;
@@ -195,6 +196,11 @@ struct Ctx : public sophia_ctx {
   PpeWork ppe_w;
   bool ppe_assembled = false;
   int n_rows = 0;
+  // property tables (data.txt) and their device copy: T, h, k, mu blocks of
+  // SOPHIA_TABLE_VALUES each
+  sophia_table table{};
+  bool has_table = false;
+  double* dtab = nullptr;
   Mlp mlp;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -275,6 +281,13 @@ struct Ctx : public sophia_ctx {
     P.dem_mat_radius = p.dem_mat_radius;
     P.heat_flux = p.con_solve == 1 ? p.heat_flux : 0.0;
     for (int k = 0; k < 6; k++) P.heat_flux_box[k] = p.heat_flux_box[k];
+    P.prop_table = (p.prop_table && dtab) ? 1 : 0;
+    P.tab_types = has_table ? SOPHIA_TABLE_TYPES : 0;
+    for (int k = 0; k < SOPHIA_TABLE_TYPES; k++) {
+      P.tab_index[k] = table.index[k];
+      P.tab_size[k] = table.size[k];
+    }
+    P.tab = dtab;
     return P;
   }
 
@@ -347,6 +360,9 @@ struct Ctx : public sophia_ctx {
     SphView<R>& v = sv;
     TRY(arena.alloc(&v.por, NS));
     TRY(arena.alloc(&v.flux_s, NS));
+    TRY(arena.alloc(&v.surf, NS));
+    TRY(arena.alloc(&v.visp, NS));
+    TRY(arena.alloc(&v.condp, NS));
     for (double** f : {&v.rho_ref, &v.rho0, &v.vol, &v.vol0, &v.x0, &v.y0, &v.z0}) TRY(arena.alloc(f, NS));
     for (R** f : {&v.dpor_prev, &v.grx, &v.gry, &v.grz, &v.fx, &v.fy, &v.fz, &v.fmag,
                   &v.fpx, &v.fpy, &v.fpz, &v.pgx, &v.pgy, &v.pgz, &v.dtemp, &v.temp0, &v.ux0, &v.uy0, &v.uz0,
@@ -564,6 +580,7 @@ struct Ctx : public sophia_ctx {
           z.push_back({(uint32_t*)a, (size_t)ns * sizeof(R) / 4, 0u});
         z.push_back({(uint32_t*)sv.cm, (size_t)6 * ns * sizeof(R) / 4, 0u});
         for (double* a : {sv.ppe1, sv.ppe2, sv.ppe3, sv.ppe4}) z.push_back({(uint32_t*)a, (size_t)ns * 2, 0u});
+        z.push_back({(uint32_t*)sv.surf, (size_t)ns, 0u});
         TRY(fill_u32_many(stream, z.data(), (int)z.size()));
       }
       // neighbour list with capacity check
@@ -628,6 +645,10 @@ struct Ctx : public sophia_ctx {
       set_error("stage called before sophia_nnps");
       return SOPHIA_E_STATE;
     }
+    if (p.prop_table && !has_table) {
+      set_error("property_table is on and no table was installed (sophia_set_table)");
+      return SOPHIA_E_STATE;
+    }
     return 0;
   }
 
@@ -639,6 +660,7 @@ struct Ctx : public sophia_ctx {
     bool on = sph_on();
     if (p.dim != 3) return 0;
     if (on) TRY(sph_mass_init<R>(stream, P, sv));
+    if (on && P.prop_table) TRY(sph_props<R>(stream, P, sv));
     if (on && p.kgc_solve > 0) {
       TimeScope t(tm(), "kgc");
       TRY(sph_kgc<R>(stream, P, sv));
@@ -763,6 +785,11 @@ struct Ctx : public sophia_ctx {
       TimeScope t(tm(), "neumann");
       TRY(sph_neumann<R>(stream, P, sv, sc));
     }
+    if (p.free_surface > 0) {
+      // surface labels and their Dirichlet pressure (ISPH_Calc_seperate.cuh:289-296)
+      TimeScope t(tm(), "surface");
+      TRY(sph_surface<R>(stream, P, sv));
+    }
     {
       TimeScope t(tm(), "ppe_assemble");
       TRY(ppe_assemble<R>(stream, P, sv, ppe_w, skeys, scoff));
@@ -809,6 +836,7 @@ struct Ctx : public sophia_ctx {
         double r_l = 0, rr_l = 0;
         TRY(ppe_residual_of(stream, ppe_w, mlp.guess, &r_l, &rr_l));
         st->r0_rel_learned = r_l;
+        st->r0_rel_base = rr_l > 0 ? r_l * std::sqrt(rr_base / rr_l) : 0.0;
         if (rr_l < rr_base) {
           guess = mlp.guess;
           known_rr = rr_l;  // w.r64 holds its residual
@@ -927,8 +955,9 @@ struct Ctx : public sophia_ctx {
       for (int k = 0; k < NF; k++) TRY(get(a[k], src[k], n));
       std::vector<R> cm;
       TRY(get(cm, sv.cm, (size_t)6 * n));
-      std::vector<int> rows;
+      std::vector<int> rows, surf;
       if (ppe_assembled) TRY(get(rows, (const int*)ppe_w.row, n));
+      TRY(get(surf, (const int*)sv.surf, n));
       for (int i = 0; i < n; i++) {
         auto A = [&](int k) { return (double)a[k][i]; };
         if (p1) {
@@ -964,6 +993,7 @@ struct Ctx : public sophia_ctx {
           std::memset(&q, 0, sizeof(q));
           q.ftotalx = A(15); q.ftotaly = A(16); q.ftotalz = A(17); q.ftotal = A(18);
           q.fpx = A(19); q.fpy = A(20); q.fpz = A(21); q.dtemp = A(25); q.drho = A(34);
+          q.lbl_surf = surf[i] ? 1.0 : 0.0;
           double c[6];
           for (int k = 0; k < 6; k++) c[k] = (double)cm[(size_t)k * n + i];
           q.Cm[0][0] = c[0]; q.Cm[0][1] = c[1]; q.Cm[0][2] = c[2];
@@ -1115,6 +1145,7 @@ struct Ctx : public sophia_ctx {
       }
       case 6: {
         TimeScope t(tm(), "wc_interaction");
+        if (P.prop_table) TRY(sph_props<R>(stream, P, sv));
         TRY(wc_interaction<R>(stream, P, sv));
         break;
       }
@@ -1135,6 +1166,38 @@ struct Ctx : public sophia_ctx {
         set_error("WCSPH stage %d out of range 0..8", stage);
         return SOPHIA_E_ARG;
     }
+    return 0;
+  }
+
+  int set_table(const sophia_table& t) override {
+    if (t.n_tables < 0 || t.n_tables > SOPHIA_TABLE_TYPES || t.n_values < 0 || t.n_values > SOPHIA_TABLE_VALUES) {
+      set_error("set_table: %d blocks, %d values (at most %d, %d)", t.n_tables, t.n_values, SOPHIA_TABLE_TYPES,
+                SOPHIA_TABLE_VALUES);
+      return SOPHIA_E_ARG;
+    }
+    for (int k = 0; k < SOPHIA_TABLE_TYPES; k++) {
+      if (t.size[k] == 0) continue;  // no block for this type: constants
+      if (t.size[k] < 2 || t.index[k] < 0 || t.index[k] + t.size[k] > t.n_values) {
+        set_error("set_table: block %d (first %d, %d values) outside the %d values", k, t.index[k], t.size[k],
+                  t.n_values);
+        return SOPHIA_E_ARG;
+      }
+      for (int j = t.index[k] + 1; j < t.index[k] + t.size[k]; j++)
+        if (!(t.T[j] > t.T[j - 1])) {
+          set_error("set_table: temperatures of block %d are not ascending", k);
+          return SOPHIA_E_ARG;
+        }
+    }
+    if (!dtab) TRY(arena.alloc(&dtab, (size_t)4 * SOPHIA_TABLE_VALUES));
+    std::vector<double> h((size_t)4 * SOPHIA_TABLE_VALUES);
+    std::memcpy(h.data(), t.T, sizeof(t.T));
+    std::memcpy(h.data() + SOPHIA_TABLE_VALUES, t.h, sizeof(t.h));
+    std::memcpy(h.data() + 2 * SOPHIA_TABLE_VALUES, t.k, sizeof(t.k));
+    std::memcpy(h.data() + 3 * SOPHIA_TABLE_VALUES, t.mu, sizeof(t.mu));
+    HIP_TRY(hipMemcpyAsync(dtab, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    table = t;
+    has_table = true;
     return 0;
   }
 
@@ -1364,6 +1427,7 @@ void sophia_params_default(sophia_params* p) {
   p->heat_flux = 0.0;
   p->free_surface = 0;
   p->fs_threshold = 0.3;
+  p->prop_table = 0;
 }
 
 // read_solv_input (function_init.cuh:216-615): whitespace tokens; a key is
@@ -1416,6 +1480,8 @@ int sophia_read_solv(const char* path, sophia_params* p) {
       p->fva_solve = yes(at(i + 2));
     } else if (k == "gravitational-force") {
       p->fg_solve = yes(at(i + 2));
+    } else if (k == "property_table(YES/NO):") {
+      p->prop_table = yes(at(i + 1));
     } else if (k == "Conduction(YES/NO):") {
       p->con_solve = yes(at(i + 1));
     } else if (k == "XSPH") {
@@ -1584,6 +1650,116 @@ int sophia_read_ptype(const char* path, int32_t* out, int32_t capacity, int32_t*
   return 0;
 }
 
+int sophia_read_table(const char* path, sophia_table* out) {
+  if (!path || !out) return SOPHIA_E_ARG;
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    set_error("cannot open %s", path);
+    return SOPHIA_E_IO;
+  }
+  std::memset(out, 0, sizeof(*out));
+  char line[1000];
+  int block = -1, start = 0, rows = 0, len = 0;
+  int err = 0;
+  auto fail = [&](const char* fmt, int a) {
+    set_error(fmt, path, a);
+    err = SOPHIA_E_IO;
+  };
+  // one row: comma-separated numbers, written from the block's first slot
+  auto parse_row = [&](char* text, double* to) -> int {
+    int c = 0;
+    for (char* tok = std::strtok(text, ","); tok; tok = std::strtok(nullptr, ",")) {
+      char* e = nullptr;
+      double v = std::strtod(tok, &e);
+      if (e == tok) continue;  // the newline after the last comma
+      if (start + c >= SOPHIA_TABLE_VALUES) return -1;
+      to[start + c] = v;
+      c++;
+    }
+    return c;
+  };
+  while (!err && std::fgets(line, sizeof(line), f)) {
+    char* t = line;
+    while (*t == ' ' || *t == '\t') t++;
+    if (block < 0) {
+      if (!std::strncmp(t, "#p_type", 7)) {
+        int num = std::atoi(t + 7);
+        if (num < 0 || num >= SOPHIA_TABLE_TYPES) {
+          fail("%s: block #p_type %d outside 0..9", num);
+          break;
+        }
+        if (out->size[num] != 0) {
+          fail("%s: two blocks for p_type %d", num);
+          break;
+        }
+        block = num;
+        rows = 0;
+        len = SOPHIA_TABLE_VALUES;
+      }
+      continue;
+    }
+    if (!std::strncmp(t, "#end", 4)) {
+      if (rows != 15 || len < 2) {
+        fail("%s: block %d needs the rows //T, //h, //k and //mu with at least 2 values each", block);
+        break;
+      }
+      out->index[block] = start;
+      out->size[block] = len;
+      start += len;
+      out->n_tables++;
+      out->n_values = start;
+      block = -1;
+      continue;
+    }
+    double* dst = nullptr;
+    int bit = 0;
+    if (!std::strncmp(t, "//T", 3)) dst = out->T, bit = 1;
+    else if (!std::strncmp(t, "//h", 3)) dst = out->h, bit = 2;
+    else if (!std::strncmp(t, "//k", 3)) dst = out->k, bit = 4;
+    else if (!std::strncmp(t, "//mu", 4)) dst = out->mu, bit = 8;
+    else continue;
+    if (!std::fgets(line, sizeof(line), f)) {
+      fail("%s: block %d ends inside a row", block);
+      break;
+    }
+    int c = parse_row(line, dst);
+    if (c < 0) {
+      fail("%s: more than %d table values", SOPHIA_TABLE_VALUES);
+      break;
+    }
+    rows |= bit;
+    len = std::min(len, c);
+  }
+  std::fclose(f);
+  if (!err && block >= 0) {
+    set_error("%s: block %d has no #end", path, block);
+    err = SOPHIA_E_IO;
+  }
+  return err;
+}
+
+int sophia_table_eval(const sophia_table* t, int32_t p_type, int32_t what, double x, double* y) {
+  if (!t || !y) return SOPHIA_E_ARG;
+  const int pt = std::abs(p_type);
+  if (pt >= SOPHIA_TABLE_TYPES || t->size[pt] < 2) {
+    set_error("table: no block for p_type %d", p_type);
+    return SOPHIA_E_ARG;
+  }
+  const int i0 = t->index[pt], n = t->size[pt];
+  const double *xs = t->T + i0, *ys = nullptr;
+  switch (what) {
+    case SOPHIA_PROP_MU: ys = t->mu + i0; break;
+    case SOPHIA_PROP_K: ys = t->k + i0; break;
+    case SOPHIA_PROP_H: ys = t->h + i0; break;
+    case SOPHIA_PROP_T: xs = t->h + i0; ys = t->T + i0; break;
+    default:
+      set_error("table: unknown property %d", what);
+      return SOPHIA_E_ARG;
+  }
+  *y = sophia::interp_table(xs, ys, n, x);
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // context lifetime
 // ---------------------------------------------------------------------------
@@ -1647,8 +1823,12 @@ int sophia_create_from_arrays(const sophia_params* p, const part1* particles, in
   return 0;
 }
 
-int sophia_create(const char* solv_path, const char* input_path, int32_t device, int32_t precision,
-                  sophia_ctx** out) {
+int sophia_create(const char* solv_path, const char* input_path, const char* ptype_path, const char* data_path,
+                  int32_t device, int32_t precision, sophia_ctx** out) {
+  if (!solv_path || !input_path || !out) {
+    set_error("create: null path or output");
+    return SOPHIA_E_ARG;
+  }
   sophia_params p;
   sophia_params_default(&p);
   int rc = sophia_read_solv(solv_path, &p);
@@ -1659,7 +1839,42 @@ int sophia_create(const char* solv_path, const char* input_path, int32_t device,
   std::vector<part1> v((size_t)n);
   rc = sophia_read_input(input_path, v.data(), n, &n);
   if (rc) return rc;
-  return sophia_create_from_arrays(&p, v.data(), n, device, precision, out);
+  if (ptype_path) {
+    int32_t m = 0;
+    rc = sophia_read_ptype(ptype_path, nullptr, 0, &m);
+    if (rc) return rc;
+    std::vector<int32_t> pt((size_t)m);
+    rc = sophia_read_ptype(ptype_path, pt.data(), m, &m);
+    if (rc) return rc;
+    if (m != n) {
+      set_error("create: %s lists %d types for %d particles of %s", ptype_path, m, n, input_path);
+      return SOPHIA_E_ARG;
+    }
+    for (int i = 0; i < n; i++)
+      if (pt[i] != v[i].p_type) {
+        set_error("create: particle %d has p_type %d in %s and %d in %s", i, pt[i], ptype_path, v[i].p_type,
+                  input_path);
+        return SOPHIA_E_ARG;
+      }
+  }
+  sophia_table tab;
+  if (data_path) {
+    rc = sophia_read_table(data_path, &tab);
+    if (rc) return rc;
+  } else if (p.prop_table) {
+    set_error("create: %s asks for property tables and no data.txt was given", solv_path);
+    return SOPHIA_E_ARG;
+  }
+  rc = sophia_create_from_arrays(&p, v.data(), n, device, precision, out);
+  if (rc) return rc;
+  if (data_path) {
+    rc = sophia_set_table(*out, &tab);
+    if (rc) {
+      sophia_destroy(*out);
+      *out = nullptr;
+    }
+  }
+  return rc;
 }
 
 int sophia_destroy(sophia_ctx* ctx) {
@@ -1693,6 +1908,11 @@ int sophia_get_clock(const sophia_ctx* c, int64_t* count, double* time) {
   if (time) *time = c->time;
   return 0;
 }
+int sophia_set_table(sophia_ctx* c, const sophia_table* t) {
+  if (!c || !t) return SOPHIA_E_ARG;
+  return c->set_table(*t);
+}
+
 int sophia_sync(sophia_ctx* c) {
   CHECK_CTX(c);
   HIP_TRY(hipStreamSynchronize(c->stream));

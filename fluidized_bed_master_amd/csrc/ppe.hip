@@ -295,11 +295,13 @@ __device__ inline int pick9(const int (&a)[9], int r) {
   return v;
 }
 
-__global__ void k_rowmask(DevParams P, int n, const int* ptype, const int* btype, const int* itype, int* row) {
+__global__ void k_rowmask(DevParams P, int n, const int* ptype, const int* btype, const int* itype, const int* surf,
+                          int* row) {
   int i = particle_index();
   if (i >= n) return;
   int t = ptype[i];
-  bool on = itype[i] <= 2 && !(P.open_boundary > 0 && btype[i] > 0) && t < 1000 && t > 0;
+  bool on = itype[i] <= 2 && !(P.open_boundary > 0 && btype[i] > 0) && t < 1000 && t > 0 &&
+            !(P.free_surface > 0 && surf[i] != 0);
   row[i] = on ? 1 : 0;
 }
 
@@ -1033,7 +1035,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
   AsmOut o{v.n, P.NI, P.NJ, P.NK, keys, cell_off, w.row, w.ccnt, w.runc, w.idx, w.hi, w.lo, w.c64,
            w.diag, w.dinv, w.rhs, w.flags};
   const int g = grid_xcd(v.n);
-  k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, w.row);
+  k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, v.surf, w.row);
   int fmt = P.z_index ? kFmtIdx32 : kFmtRun8;
   for (;;) {
     k_zero_u32<<<1, 64, 0, s>>>(w.flags, 4);

@@ -208,6 +208,18 @@ __global__ void k_flux_walls(DevParams P, SphView<R> v) {
   v.flux_s[i] = S;
 }
 
+// viscosity() and conductivity() of every SPH particle at its temperature
+// (functions_PROP.cuh:228-264) when property tables are on: one table
+// interpolation per particle, read by the pair sums for both ends of a pair
+template <typename R>
+__global__ void k_props(DevParams P, SphView<R> v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const double T = (double)v.temp[i];
+  v.visp[i] = (R)prop_mu(P, v.ptype[i], T);
+  v.condp[i] = (R)prop_k(P, v.ptype[i], T);
+}
+
 // KERNEL_advection_force3D_sph (functions_PPE2.cuh:1251-1607): viscous,
 // artificial viscosity, Eulerian advection, conduction, gravity and the
 // pressure-gradient force density used by the coupling.
@@ -227,7 +239,8 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i];
   const double pid = v.pres[i];
   const T mi = (T)v.m[i], rhoi = (T)v.rho[i], pori = (T)v.por[i], tempi = v.temp[i];
-  const T visi = (T)P.mu_fluid, kci = (T)(pti > 1000 ? P.k_dem : P.k_fluid);
+  const bool tab = P.prop_table != 0;
+  const T visi = tab ? v.visp[i] : (T)P.mu_fluid, kci = tab ? v.condp[i] : (T)(pti > 1000 ? P.k_dem : P.k_fluid);
   const T cpi = (T)(pti > 1000 ? P.cp_dem : P.cp_fluid);
   const T eli = P.eulerian ? (T)0 : (T)1;
   const bool euler = eli < (T)1;
@@ -264,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
     const T mj = (T)q.m, rhoj = (T)q.rho;
     const T uxj = q.ux, uyj = q.uy, uzj = q.uz;
     if (P.fv_solve) {
-      T visj = (T)P.mu_fluid;
+      T visj = tab ? v.visp[q.j] : (T)P.mu_fluid;
       T Cv = dx * gx + dy * gy + dz * gz;
       Cv *= (visi * visj) / (visi + visj);
       Cv *= 4 * pori * (mj / (rhoi * rhoj));
@@ -299,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
       }
     }
     if (P.con_solve) {
-      T kcj = (T)(q.pt > 1000 ? P.k_dem : P.k_fluid);
+      T kcj = tab ? v.condp[q.j] : (T)(q.pt > 1000 ? P.k_dem : P.k_fluid);
       T H = 4 * pori * mj * kcj * kci * (tempi - (T)q.temp) * dw;
       H /= (r + (T)1e-10) * rhoi * rhoj * (kci + kcj);
       Rc += use ? H : (T)0;
@@ -460,6 +473,50 @@ __global__ void k_open_boundary(DevParams P, SphView<R> v, CellView cv, OpenBoun
 }
 
 // KERNEL_pressureforce3D_sph (functions_PPE2.cuh:2154-2335)
+// KERNEL_clc_surface_detect3D (functions_PST.cuh:263-320), which the
+// reference calls just before its PPE (commented out at
+// ISPH_Calc_seperate.cuh:289-296): the mean of the unit vectors (x_i -
+// x_j)/(r + 1e-10) over the neighbours within kappa h, divided by their count
+// non (self included), marks surface particles where its length exceeds
+// fs_threshold or non < 10.  free_surface = 2 counts neighbours of the
+// particle's own p_type, as the reference does; free_surface = 1 counts every
+// SPH particle, walls included, so that fluid against a wall is not surface.
+// Surface particles take the Dirichlet pressure p_ref (the commented-out rows
+// of functions_PPE2.cuh:533-534) and leave the PPE rows.  FP64 sums in the
+// list's (the reference's) order with no contraction, so labels match the
+// oracle's exactly.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_surface(DevParams P, SphView<R> v) {
+#pragma clang fp contract(off)
+  int i = particle_index();
+  if (i >= v.n) return;
+  const int pti = v.ptype[i];
+  const bool fluid = v.itype[i] <= 2 && !(P.open_boundary > 0 && v.btype[i] > 0) && pti < 1000 && pti > 0;
+  if (!fluid) {
+    v.surf[i] = 0;
+    return;
+  }
+  double tx = 0, ty = 0, tz = 0;
+  int non = 0;
+  for_nbrs4<false>(v, i, false, [&](const Nb<R>& q, bool on) {
+#pragma clang fp contract(off)
+    const bool use = on && (P.free_surface == 2 ? q.pt == pti : q.pt < 1000);
+    const double rd = 1.0 / (dist_exact(q.dx, q.dy, q.dz) + 1e-10);
+    const double ax = q.dx * rd, ay = q.dy * rd, az = q.dz * rd;
+    tx += use ? ax : 0.0;
+    ty += use ? ay : 0.0;
+    tz += use ? az : 0.0;
+    non += use ? 1 : 0;
+  });
+  const double rn = 1.0 / ((double)non + 1e-10);
+  tx = tx * rn;
+  ty = ty * rn;
+  tz = tz * rn;
+  const bool surf = dist_exact(tx, ty, tz) > P.fs_threshold || non < 10;
+  v.surf[i] = surf ? 1 : 0;
+  if (surf) v.pres[i] = P.p_ref;
+}
+
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<R> v) {
   typedef double T;  // FP64 pair sums: see the note on pressure-gradient cancellation
@@ -716,6 +773,14 @@ template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const S
   }
   return 0;
 }
+template <typename R> int sph_surface(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_surface, P, v); }
+template <typename R> int sph_props(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n > 0 && P.prop_table) {
+    k_props<R><<<grid_for(v.n), kBlock, 0, s>>>(P, v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
 template <typename R>
 int sph_open_boundary(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv,
@@ -755,6 +820,8 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_flux_walls<R>(hipStream_t, const DevParams&, const SphView<R>&);                         \
   template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
+  template int sph_surface<R>(hipStream_t, const DevParams&, const SphView<R>&);                            \
+  template int sph_props<R>(hipStream_t, const DevParams&, const SphView<R>&);                              \
   template int sph_pressure_force<R>(hipStream_t, const DevParams&, const SphView<R>&);                     \
   template int sph_open_boundary<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&,      \
                                     const OpenBoundary&);                                                   \

@@ -183,9 +183,10 @@ __global__ __launch_bounds__(kBlock) void k_wc_interaction(DevParams P, SphView<
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i], tempi = v.temp[i];
   const double pid = v.pres[i], rhoid = v.rho[i], porid = v.por[i];
   const T mi = (T)v.m[i], rhoi = (T)rhoid, pori = (T)porid;
-  const T visi = (T)P.mu_fluid;
+  const bool tab = P.prop_table != 0;
+  const T visi = tab ? v.visp[i] : (T)P.mu_fluid;
   const T eta = (T)(0.001 * hd);
-  const T kci = (T)(pti > 1000 || pti < -1000 ? P.k_dem : P.k_fluid);
+  const T kci = tab ? v.condp[i] : (T)(pti > 1000 || pti < -1000 ? P.k_dem : P.k_fluid);
   const T cpi = (T)(pti > 1000 || pti < -1000 ? P.cp_dem : P.cp_fluid);
   const T el = P.eulerian ? (T)0 : (T)1;
   const bool euler = el < (T)1;
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_wc_interaction(DevParams P, SphView<
     const T mj = (T)q.m, rhoj = (T)q.rho;
     const T uxj = q.ux, uyj = q.uy, uzj = q.uz;
     if (P.fv_solve && pti > 0) {
-      T visj = (T)P.mu_fluid;
+      T visj = tab ? v.visp[q.j] : (T)P.mu_fluid;
       T Cv = dx * gx + dy * gy + dz * gz;
       Cv *= (visi * visj) / (visi + visj);
       Cv *= 4 * pori * (mj / (rhoi * rhoj));
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_wc_interaction(DevParams P, SphView<
     }
     if (P.con_solve) {
       const int ptj = q.pt;
-      T kcj = (T)(ptj > 1000 || ptj < -1000 ? P.k_dem : P.k_fluid);
+      T kcj = tab ? v.condp[q.j] : (T)(ptj > 1000 || ptj < -1000 ? P.k_dem : P.k_fluid);
       T H = (T)4.0 * mj * kcj * kci * (tempi - (T)q.temp) * dw;
       H /= cpi * (r + eta * eta) * rhoi * rhoj * (kci + kcj);
       Rc += use ? H : (T)0;

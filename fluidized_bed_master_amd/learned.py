@@ -90,16 +90,25 @@ def evaluate(make_sim, params: np.ndarray, steps: int = 20, warm: int = 0,
                                  precond=precond)
         for _ in range(warm):
             sim.step(warm_opts)
-        its, r0, fb, ms = [], [], 0, []
+        its, r0, fb, ms, r0l, r0b = [], [], 0, [], [], []
         for _ in range(steps):
             st = sim.step(opts)
             its.append(st["iterations"])
             ms.append(st["ms"])
             r0.append(st["r0_rel"])
-            if guess == abi.GUESS_LEARNED and st["guess_used"] != abi.GUESS_LEARNED:
-                fb += 1
+            if guess == abi.GUESS_LEARNED:
+                # the network's own guess and the extrapolation it corrects,
+                # on the same state, whichever the solve started from
+                r0l.append(st["r0_rel_learned"])
+                r0b.append(st["r0_rel_base"])
+                if st["guess_used"] != abi.GUESS_LEARNED:
+                    fb += 1
         sim.close()
         res[name] = {"iterations": its, "r0_rel": r0, "ppe_ms": ms, "fallbacks": fb,
                      "mean_iterations": float(np.mean(its)), "mean_r0_rel": float(np.mean(r0)),
                      "mean_ppe_ms": float(np.mean(ms))}
+        if guess == abi.GUESS_LEARNED:
+            ratio = np.array(r0l) / np.maximum(np.array(r0b), 1e-300)
+            res[name].update({"r0_rel_learned": r0l, "r0_rel_base": r0b, "kept_frac": 1.0 - fb / steps,
+                              "mean_r0_ratio_learned_to_base": float(np.mean(ratio))})
     return res

@@ -33,14 +33,21 @@ class Sophia:
         self.precision = precision
 
     @classmethod
-    def from_files(cls, solv: str, input_txt: str, device: int = 0, precision: int = PREC_MIXED) -> "Sophia":
+    def from_files(cls, solv: str, input_txt: str, ptype: str | None = None, data: str | None = None,
+                   device: int = 0, precision: int = PREC_MIXED) -> "Sophia":
+        """The reference's start-up from solv.txt, input.txt and, when given,
+        p_type.txt and data.txt."""
         self = cls.__new__(cls)
         self.lib = abi.load_library()
         self._ctx = C.c_void_p()
-        check(self.lib.sophia_create(solv.encode(), input_txt.encode(), device, precision, C.byref(self._ctx)),
-              "sophia_create")
+        enc = (lambda x: None if x is None else x.encode())
+        check(self.lib.sophia_create(solv.encode(), input_txt.encode(), enc(ptype), enc(data), device, precision,
+                                     C.byref(self._ctx)), "sophia_create")
         self.precision = precision
         return self
+
+    def set_table(self, table: "Table") -> None:
+        check(self.lib.sophia_set_table(self._ctx, C.byref(table)), "sophia_set_table")
 
     def close(self) -> None:
         if getattr(self, "_ctx", None):
@@ -268,6 +275,31 @@ def knl_eval(kernel_type: int, dim: int, h: float, r: np.ndarray):
     dw = np.zeros_like(r)
     check(lib.sophia_knl_eval(kernel_type, dim, h, ptr(r), len(r), ptr(w), ptr(dw)), "sophia_knl_eval")
     return w, dw
+
+
+class Table(C.Structure):
+    """sophia_table: data.txt property blocks by p_type."""
+    _fields_ = [("n_tables", C.c_int32), ("n_values", C.c_int32),
+                ("index", C.c_int32 * 10), ("size", C.c_int32 * 10),
+                ("T", C.c_double * 50), ("h", C.c_double * 50), ("k", C.c_double * 50), ("mu", C.c_double * 50)]
+
+    def block(self, p_type: int) -> dict:
+        a, n = self.index[p_type], self.size[p_type]
+        return {f: np.array(getattr(self, f)[a:a + n]) for f in ("T", "h", "k", "mu")}
+
+    def eval(self, p_type: int, what: int, x: float) -> float:
+        y = C.c_double()
+        check(abi.load_library().sophia_table_eval(C.byref(self), p_type, what, x, C.byref(y)), "sophia_table_eval")
+        return y.value
+
+
+PROP_MU, PROP_K, PROP_H, PROP_T = 0, 1, 2, 3
+
+
+def read_table(path: str) -> Table:
+    t = Table()
+    check(abi.load_library().sophia_read_table(path.encode(), C.byref(t)), "sophia_read_table")
+    return t
 
 
 def read_solv(path: str) -> Params:

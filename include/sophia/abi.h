@@ -194,6 +194,13 @@ typedef struct sophia_params {
    * as surface).  0 = off (reference: no surface rows). */
   int32_t free_surface;
   double fs_threshold;      /* 0.3 (functions_PST.cuh:319) */
+  /* property_table(YES/NO) of solv.txt (function_init.cuh:261-262, vii[2]):
+   * 1 = viscosity and conductivity of SPH particles whose |p_type| has a
+   * table in data.txt are interpolated in temperature (viscosity() and
+   * conductivity(), functions_PROP.cuh:228-264); needs the table, passed
+   * to sophia_create or sophia_set_table.  Types without a table (spheres,
+   * |p_type| >= 10) keep the constants. */
+  int32_t prop_table;
   int32_t reserved_i;
   double reserved[7];
 } sophia_params;
@@ -266,6 +273,7 @@ typedef struct sophia_ppe_stats {
   int32_t precond_used;     /* SOPHIA_PRECOND_* actually used */
   int32_t n_coarse;         /* coarse unknowns (two-level) */
   double coarse_res;        /* ||I - E X||_F / sqrt(n_coarse) of the coarse inverse X used */
+  double r0_rel_base;       /* same for the extrapolated guess the learned one corrects (if tried) */
 } sophia_ppe_stats;
 
 typedef struct sophia_ctx sophia_ctx;
@@ -280,14 +288,57 @@ int sophia_read_input(const char* path, part1* out, int32_t capacity, int32_t* n
 /* p_type.txt: one integer per line (ISPH.cuh via gpu_count_boundary_numbers) */
 int sophia_read_ptype(const char* path, int32_t* out, int32_t capacity, int32_t* n);
 
+/* data.txt property tables (read_table, function_init.cuh:966-1182): blocks
+ * "#p_type N" ... "#end", each with comma-separated rows after the lines
+ * "//T", "//h", "//k" and "//mu" (temperature K, enthalpy J/kg,
+ * conductivity W/m/K, viscosity Pa s), stored one after the other.  A
+ * block's length is its shortest row: the reference takes the last row's
+ * count (count2, :1108), and its own data.txt has a 6-value T row over
+ * 5-value rows, whose sixth value the next block overwrites.  Blocks are
+ * filed under their own p_type (0..9); the reference files them by reading
+ * order (:1117-1124), which for its data.txt leaves the "#p_type 9" block
+ * with an unset size.  At most SOPHIA_TABLE_VALUES values over all blocks
+ * (table_size, Parameters.cuh:131); types with no block have size 0. */
+#define SOPHIA_TABLE_VALUES 50
+#define SOPHIA_TABLE_TYPES 10
+typedef struct sophia_table {
+  int32_t n_tables;                   /* number of blocks read */
+  int32_t n_values;                   /* values over all blocks */
+  int32_t index[SOPHIA_TABLE_TYPES];  /* first value of the block of p_type t (host_table_index) */
+  int32_t size[SOPHIA_TABLE_TYPES];   /* values in that block (host_table_size) */
+  double T[SOPHIA_TABLE_VALUES];      /* host_Tab_T */
+  double h[SOPHIA_TABLE_VALUES];      /* host_Tab_h */
+  double k[SOPHIA_TABLE_VALUES];      /* host_Tab_k */
+  double mu[SOPHIA_TABLE_VALUES];     /* host_Tab_vis */
+} sophia_table;
+int sophia_read_table(const char* path, sophia_table* out);
+/* one property of |p_type| at x by the reference's interp2
+ * (functions_PROP.cuh:143-175: piecewise linear, extended linearly beyond
+ * both ends): what = SOPHIA_PROP_MU, _K (of temperature), _H (enthalpy of
+ * temperature, Ttoh) or _T (temperature of enthalpy, htoT).  A type with
+ * no block is SOPHIA_E_ARG. */
+#define SOPHIA_PROP_MU 0
+#define SOPHIA_PROP_K 1
+#define SOPHIA_PROP_H 2
+#define SOPHIA_PROP_T 3
+int sophia_table_eval(const sophia_table* t, int32_t p_type, int32_t what, double x, double* y);
+
 /* --- context ------------------------------------------------------------- */
 /* particles in reference order, SPH and DEM mixed; split by p_type <= 1000
  * (read_sph_dem, function_init.cuh:893-911). */
 int sophia_create_from_arrays(const sophia_params* p, const part1* particles,
                               int32_t n, int32_t device, int32_t precision,
                               sophia_ctx** out);
-int sophia_create(const char* solv_path, const char* input_path, int32_t device,
-                  int32_t precision, sophia_ctx** out);
+/* The reference's start-up from its input directory (main,
+ * SOPHIA_gpu.cu:145-151; ISPH.cuh): solv.txt, input.txt, and optionally
+ * p_type.txt (NULL to skip; when given it must list the p_type column of
+ * input.txt, which the reference uses to count boundary particles,
+ * gpu_count_boundary_numbers) and data.txt (NULL to skip; required when
+ * solv.txt asks for property_table YES). */
+int sophia_create(const char* solv_path, const char* input_path, const char* ptype_path,
+                  const char* data_path, int32_t device, int32_t precision, sophia_ctx** out);
+/* install property tables on a context (used when prop_table = 1) */
+int sophia_set_table(sophia_ctx* ctx, const sophia_table* t);
 int sophia_destroy(sophia_ctx* ctx);
 const char* sophia_last_error(void);
 int sophia_get_domain(const sophia_ctx* ctx, sophia_domain* d);

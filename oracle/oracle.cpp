@@ -31,8 +31,37 @@
  *                             tiny system
  *   Hertz-Mindlin             two-sphere head-on: F = 1.33333 E* sqrt(R*) d^1.5,
  *                             pair antisymmetry, tangential cap
- *   Di Felice drag            single sphere at eps = 1: textbook formula
- *   coupling reaction         momentum balance sum_i m_i da_i = -m_a F_a
+ *   Di Felice drag            single sphere at eps -> 1: (1/8) Cd rho pi d^2
+ *                             |u| u with Dallavalle's Cd, Re 1 to 1e4; air and
+ *                             mixture regimes: Rong et al. (2013) porosity
+ *                             function at the reported porosity
+ *   DEM pressure gradient     uniform grad p gives -grad p V_a / m_a
+ *   coupling reaction         momentum balance sum_i m_i da_i = -m_a F_a and
+ *                             energy balance sum_i m_i c_p dT_i = -sum_a Q_sd
+ *   Shepard / delta-SPH       Shepard sum = closed-form lattice quadrature of
+ *                             W (within 1 % of 1); delta-SPH gradient exact
+ *                             for a linear density (KGC)
+ *   advection terms           viscous: 0 for linear shear, mu/rho lap(u) for
+ *                             a quadratic (exact with KGC on a lattice);
+ *                             Eulerian -(u.grad)u exact for linear u;
+ *                             conduction 0 for linear T, Morris lattice sum
+ *                             for quadratic T; artificial viscosity 0 in
+ *                             expansion, dissipative in compression;
+ *                             symmetric pressure gradient exact in a lattice
+ *                             interior
+ *   PPE right-hand side       -rho div(u*)/dt exact for linear u* (KGC); the
+ *                             porosity rate = central difference of eps
+ *   Neumann walls             brute force over all pairs (2 kappa h, +-1
+ *                             cell, W at 2h)
+ *   DEM porosity              brute force; one sphere: 1 - V W(0)/flt_sd
+ *   heat                      Ranz-Marshall Q = Nu k pi d dT on one sphere;
+ *                             contact conduction 2 k a dT (Batchelor-O'Brien)
+ *                             equal and opposite; radiation equal and
+ *                             opposite, ~T^4; wall flux delivers q A
+ *   free surface              lattice labels: the open top only (walls
+ *                             counted), plus wall-adjacent fluid (same-type
+ *                             rule); a column stays at rest for 1000
+ *                             converged steps and drifts without it
  *   predictor/corrector/XSPH  exact restatement, checked against hand-written
  *                             per-particle formulas in the tests
  *   open boundary             uniform interior fluid: the outlet copies it,
@@ -42,9 +71,9 @@
  *                             on the wall, spin unchanged; resting contact:
  *                             normal velocity removed, friction opposes slip;
  *                             all three geometries
- *   parity unpinned           advection Eulerian term and Neumann wall
- *                             interpolation beyond their closed-form special
- *                             cases (uniform field reproduced exactly)
+ *   parity unpinned           the WCSPH interaction's heat and virial
+ *                             terms beyond the gravity-only and uniform-flow
+ *                             checks
  */
 #include "oracle.h"
 
@@ -92,6 +121,9 @@ struct orc_state {
   bool after_corrector = false;
   // fluid Shepard sum of each flux wall (sorted SPH order; 0 elsewhere)
   std::vector<double> flux_S;
+  // data.txt property tables (orc_set_table)
+  sophia_table table{};
+  bool has_table = false;
   // PPE system of the last assembly
   std::vector<int32_t> row_mask;
   std::vector<PPERow> rows;
@@ -213,8 +245,47 @@ static double knl_dw(int kt, double tA, double tH, double rr) {
 /* material properties with the reference constants as parameters           */
 /* (functions_PROP.cuh:228-514)                                             */
 /* ------------------------------------------------------------------------ */
-static double viscosity(const sophia_params& p) { return p.mu_fluid; }
-static double conductivity(const sophia_params& p, int ptype) {
+/* interp2 (functions_PROP.cuh:143-175): piecewise linear over ascending
+ * xs, the end segments extended beyond both ends */
+static double interp2(const double* xs, const double* ys, int n, double x) {
+  int a;
+  if (xs[n - 1] < x) {
+    a = n - 2;
+  } else if (x <= xs[0]) {
+    a = 0;
+  } else {
+    a = n - 2;
+    for (int i = 0; i < n - 1; i++)
+      if (xs[i] < x && x <= xs[i + 1]) {
+        a = i;
+        break;
+      }
+  }
+  double y = ys[a + 1] - ys[a];
+  y /= (xs[a + 1] - xs[a]);
+  y *= (x - xs[a]);
+  y += ys[a];
+  return y;
+}
+static const sophia_table* g_no_table = nullptr;
+/* viscosity(temp, p_type) and conductivity(temp, p_type) (functions_PROP.cuh:
+ * 228-264): the data.txt block of |p_type| when property_table is on and the
+ * type has one, else the constants (the reference's VISCOSITY_AIR for every
+ * type, 0.0518 for fluid, 2.5 for spheres, as parameters) */
+static const double* table_block(const sophia_params& p, const sophia_table* t, int ptype, int& n) {
+  const int a = std::abs(ptype);
+  if (!p.prop_table || !t || a >= t->n_tables || t->size[a] < 2) return nullptr;
+  n = t->size[a];
+  return t->T + t->index[a];
+}
+static double viscosity(const sophia_params& p, const sophia_table* t, int ptype, double T) {
+  int n = 0;
+  if (const double* xs = table_block(p, t, ptype, n)) return interp2(xs, t->mu + (xs - t->T), n, T);
+  return p.mu_fluid;
+}
+static double conductivity(const sophia_params& p, const sophia_table* t, int ptype, double T) {
+  int n = 0;
+  if (const double* xs = table_block(p, t, ptype, n)) return interp2(xs, t->k + (xs - t->T), n, T);
   return std::abs(ptype) > 1000 ? p.k_dem : p.k_fluid;
 }
 static double heat_capacity(const sophia_params& p, int ptype) {
@@ -427,6 +498,10 @@ orc_state* orc_create(const sophia_params* p, const part1* particles, int32_t n)
 }
 
 void orc_destroy(orc_state* s) { delete s; }
+void orc_set_table(orc_state* s, const sophia_table* t) {
+  s->table = *t;
+  s->has_table = true;
+}
 void orc_get_domain(const orc_state* s, sophia_domain* d) { *d = s->d; }
 void orc_get_clock(const orc_state* s, int64_t* count, double* time) {
   if (count) *count = s->count;
@@ -510,7 +585,7 @@ static void mass_init_sph(orc_state& s) {
     q.vol0 = std::pow(q.h / s.p.h_over_dx, s.p.dim);
     q.vol = q.vol0;
     sp.SP2[i].rho0 = q.rho;
-    q.cond = conductivity(s.p, q.p_type);
+    q.cond = conductivity(s.p, s.has_table ? &s.table : g_no_table, q.p_type, q.temp);
   }
 }
 
@@ -799,8 +874,9 @@ void orc_interaction(orc_state* s) {
     double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy, uzi = qi.uz;
     double hi = qi.h, pi = qi.pres, mi = qi.m, rhoi = qi.rho, pori = qi.DEMpor, tempi = qi.temp;
     double A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
-    double kci = conductivity(p, qi.p_type), cpi = heat_capacity(p, qi.p_type);
-    double visi = viscosity(p) + sp.P3[i].vis_t;
+    const sophia_table* tab = s->has_table ? &s->table : g_no_table;
+    double kci = conductivity(p, tab, qi.p_type, qi.temp), cpi = heat_capacity(p, qi.p_type);
+    double visi = viscosity(p, tab, qi.p_type, qi.temp) + sp.P3[i].vis_t;
     double tx = 0, ty = 0, tz = 0, Rc = 0, px = 0, py = 0, pz = 0;
     double ex = 0, ey = 0, ez = 0, et = 0, fl = 0;
     const bool flux = p.con_solve == 1 && p.heat_flux != 0.0 && (int)s->flux_S.size() == sp.n();
@@ -826,7 +902,7 @@ void orc_interaction(orc_state* s) {
         pz += rhoi / pori * Cp * gz;
       }
       if (p.fv_solve) {
-        double visj = viscosity(p) + sp.P3[j].vis_t;
+        double visj = viscosity(p, tab, qj.p_type, qj.temp) + sp.P3[j].vis_t;
         double Cv = dx * gx + dy * gy + dz * gz;
         Cv *= (visi * visj) / (visi + visj);
         Cv *= 4 * pori * (mj / (rhoi * rhoj));
@@ -861,7 +937,7 @@ void orc_interaction(orc_state* s) {
         }
       }
       if (p.con_solve) {
-        double kcj = conductivity(p, qj.p_type);
+        double kcj = conductivity(p, tab, qj.p_type, qj.temp);
         double H = 4.0 * pori * mj * kcj * kci * (tempi - qj.temp) * dw;
         H /= (r + 1e-10) * rhoi * rhoj * (kci + kcj);
         Rc += H;
@@ -1612,11 +1688,64 @@ static void neumann_walls(orc_state& s) {
   }
 }
 
-static bool ppe_row(const orc_state& s, const part1& q) {
+/* fluid particle that takes a pressure force (KERNEL_pressureforce3D_sph's
+ * filter, functions_PPE2.cuh:2154-2335) and, unless it is a free-surface
+ * particle, a PPE row (KERNEL_PPE3D_sph, :542-698) */
+static bool ppe_fluid(const orc_state& s, const part1& q) {
   if (q.i_type > kITypeCrt) return false;
   if (s.p.open_boundary > 0 && q.buffer_type > 0) return false;
   if (q.p_type >= 1000 || q.p_type <= 0) return false;
   return true;
+}
+
+static bool ppe_row(const orc_state& s, int i) {
+  const part1& q = s.sph.SP1[i];
+  if (!ppe_fluid(s, q)) return false;
+  return !(s.p.free_surface > 0 && s.sph.P3[i].lbl_surf != 0.0);
+}
+
+/* KERNEL_clc_surface_detect3D (functions_PST.cuh:263-320), run before the
+ * PPE as at ISPH_Calc_seperate.cuh:289-296: the mean unit vector from the
+ * neighbours within kappa*h, |sum (x_i-x_j)/(r+1e-10)| / (non+1e-10) >
+ * fs_threshold, or non < 10, marks a surface particle.  free_surface = 2
+ * counts neighbours of the particle's own p_type as the reference does;
+ * free_surface = 1 counts every SPH particle (walls included), so fluid next
+ * to a wall is not taken for surface.  Surface particles take the Dirichlet
+ * value p_ref (the commented-out rows of functions_PPE2.cuh:533-534). */
+static void surface_detect(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int n = sp.n();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& qi = sp.SP1[i];
+    sp.P3[i].lbl_surf = 0.0;
+    if (!ppe_fluid(s, qi)) continue;
+    const double R = p.kappa * qi.h;
+    double tx = 0, ty = 0, tz = 0;
+    int non = 0;
+    for_candidates(s, sp, qi.x, qi.y, qi.z, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+      double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (!(r < R)) return;
+      bool same = p.free_surface == 2 ? qj.p_type == qi.p_type : qj.p_type < 1000;
+      if (!same) return;
+      double rd = 1.0 / (r + 1e-10);
+      tx += dx * rd;
+      ty += dy * rd;
+      tz += dz * rd;
+      non++;
+    });
+    double rn = 1.0 / (non + 1e-10);
+    tx *= rn;
+    ty *= rn;
+    tz *= rn;
+    bool surf = std::sqrt(tx * tx + ty * ty + tz * tz) > p.fs_threshold || non < 10;
+    sp.P3[i].lbl_surf = surf ? 1.0 : 0.0;
+  }
+  for (int i = 0; i < n; i++)
+    if (sp.P3[i].lbl_surf != 0.0) sp.SP1[i].pres = p.p_ref;
 }
 
 static void ppe_assemble(orc_state& s, double tdt) {
@@ -1629,7 +1758,7 @@ static void ppe_assemble(orc_state& s, double tdt) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) {
     part1& qi = sp.SP1[i];
-    if (!ppe_row(s, qi)) continue;
+    if (!ppe_row(s, i)) continue;
     s.row_mask[i] = 1;
     PPERow& row = s.rows[i];
     row.i = i;
@@ -1782,7 +1911,7 @@ static void pressure_force(orc_state& s) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < sp.n(); i++) {
     const part1& qi = sp.SP1[i];
-    if (!ppe_row(s, qi)) continue;
+    if (!ppe_fluid(s, qi)) continue;
     double hi = qi.h, A = knl_tmpA(kt, dim, hi), R = p.kappa * hi;
     double xi = qi.x, yi = qi.y, zi = qi.z, pi = qi.pres, rhoi = qi.rho, pori = qi.DEMpor;
     double tx = 0, ty = 0, tz = 0;
@@ -1819,6 +1948,7 @@ int orc_ppe(orc_state* s, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
   def.method = SOPHIA_PPE_JACOBI;
   if (!o) o = &def;
   if (s->count == 0) neumann_walls(*s);
+  if (s->p.free_surface > 0) surface_detect(*s);
   ppe_assemble(*s, stride * s->p.dt);
   int nrows = 0;
   for (int v : s->row_mask) nrows += v;
@@ -2082,6 +2212,7 @@ static void wc_eos(orc_state& s) {
  * Eulerian advection term (interface, repulsive, surface-tension,
  * Boussinesq and concentration terms off) */
 static void wc_interaction(orc_state& s) {
+  const sophia_table* tab = s.has_table ? &s.table : g_no_table;
   Species& sp = s.sph;
   const sophia_params& p = s.p;
   const int kt = p.kernel_type, dim = p.dim;
@@ -2095,10 +2226,10 @@ static void wc_interaction(orc_state& s) {
     double pi = qi.pres, mi = qi.m, rhoi = qi.rho, pori = qi.DEMpor, tempi = qi.temp;
     double eta = 0.001 * h, kci = 0, cpi = 1;
     if (p.con_solve) {
-      kci = conductivity(p, pti);
+      kci = conductivity(p, tab, pti, qi.temp);
       cpi = heat_capacity(p, pti);
     }
-    double visi = viscosity(p) + sp.P3[i].vis_t;
+    double visi = viscosity(p, tab, pti, qi.temp) + sp.P3[i].vis_t;
     double tx = 0, ty = 0, tz = 0, Rc = 0, pgx = 0, pgy = 0, pgz = 0;
     double ex = 0, ey = 0, ez = 0, et = 0;
     const bool euler = qi.elix < 1.0 || qi.eliy < 1.0;
@@ -2125,7 +2256,7 @@ static void wc_interaction(orc_state& s) {
         pgz += rhoi / pori * Cp * gz;
       }
       if (p.fv_solve) {
-        double visj = viscosity(p) + sp.P3[j].vis_t;
+        double visj = viscosity(p, tab, qj.p_type, qj.temp) + sp.P3[j].vis_t;
         double Cv = dx * gx + dy * gy + dz * gz;
         Cv *= (visi * visj) / (visi + visj);
         Cv *= 4 * pori * (mj / (rhoi * rhoj));
@@ -2151,7 +2282,7 @@ static void wc_interaction(orc_state& s) {
         }
       }
       if (p.con_solve) {
-        double kcj = conductivity(p, ptj);
+        double kcj = conductivity(p, tab, ptj, qj.temp);
         double H = 4.0 * mj * kcj * kci * (tempi - tempj) * dw;
         H /= cpi * (r + eta * eta) * rhoi * rhoj * (kci + kcj);
         Rc += H;

@@ -23,6 +23,8 @@ typedef struct orc_state orc_state;
 
 orc_state* orc_create(const sophia_params* p, const part1* particles, int32_t n);
 void orc_destroy(orc_state* s);
+/* data.txt property tables, used when p->prop_table = 1 */
+void orc_set_table(orc_state* s, const sophia_table* t);
 void orc_get_domain(const orc_state* s, sophia_domain* d);
 void orc_get_clock(const orc_state* s, int64_t* count, double* time);
 int orc_count(const orc_state* s, int32_t species);

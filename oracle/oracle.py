@@ -23,6 +23,7 @@ _VP = C.c_void_p
 _SIG = {
     "orc_create": (_VP, [_P(Params), _VP, C.c_int32]),
     "orc_destroy": (None, [_VP]),
+    "orc_set_table": (None, [_VP, _VP]),
     "orc_get_domain": (None, [_VP, _P(Domain)]),
     "orc_get_clock": (None, [_VP, _P(C.c_int64), _P(C.c_double)]),
     "orc_count": (C.c_int, [_VP, C.c_int32]),
@@ -85,6 +86,9 @@ class Oracle:
         self._s = self.lib.orc_create(C.byref(params), ptr(parts), len(parts))
         if not self._s:
             raise RuntimeError("orc_create failed")
+
+    def set_table(self, table):
+        self.lib.orc_set_table(self._s, C.byref(table))
 
     def close(self):
         if self._s:

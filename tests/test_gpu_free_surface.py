@@ -1,0 +1,88 @@
+"""Free-surface PPE rows on the GPU against the oracle: surface labels
+(KERNEL_clc_surface_detect3D, functions_PST.cuh:263-320) bit-exact, the PPE
+stage with Dirichlet surface rows, and a column at rest over 300 steps of
+converged solves with the reference's symmetric pressure gradient."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fluidized_bed_master_amd import abi, configs
+from tests.helpers import SPH_STAGE_FIELDS, compare_fields, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {abi.PREC_FP64: 1e-9, abi.PREC_MIXED: 1e-5}
+PPE = {abi.PREC_FP64: abi.PPE_FP64, abi.PREC_MIXED: abi.PPE_MIXED_IR}
+
+
+def _report(name, data):
+    path = os.environ.get("SOPHIA_REPORT")
+    if not path:
+        return
+    try:
+        old = json.load(open(path)) if os.path.exists(path) else {}
+    except Exception:
+        old = {}
+    old[name] = data
+    with open(path, "w") as f:
+        json.dump(old, f, indent=1, sort_keys=True)
+
+
+@pytest.mark.parametrize("fs", [1, 2], ids=["walls_counted", "same_type"])
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_surface_labels_and_ppe_match_oracle(prec, fs):
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.small_isph_dem(seed=17)
+    p.free_surface = fs
+    g, o = Sophia(p, parts, precision=prec), Oracle(p, parts)
+    for s in (g, o):
+        s.nnps(); s.prep(); s.interaction(); s.dem_interaction(); s.sph_dem_coupling(); s.predict()
+    g.ppe(abi.ppe_opts(method=abi.PPE_JACOBI))
+    o.ppe(abi.ppe_opts(method=abi.PPE_JACOBI))
+    a1, _, a3 = g.get_particles(abi.SPH)
+    b1, _, b3 = o.get_particles(abi.SPH)
+    assert b3["lbl_surf"].sum() > 0
+    assert np.array_equal(a3["lbl_surf"], b3["lbl_surf"])
+    _, _, rg = g.get_ppe_system()
+    _, _, ro = o.get_ppe_system()
+    assert np.array_equal(rg, ro)
+    rep = {}
+    compare_fields(a1, b1, SPH_STAGE_FIELDS["ppe"], TOL[prec], "ppe+surface", report=rep)
+    _report(f"surface_{'fp64' if prec == 0 else 'mixed'}_fs{fs}", rep)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_column_at_rest_300_converged_steps(prec):
+    """A jittered 8x8x10 column with Dirichlet surface rows and the
+    reference's symmetric pressure gradient, BiCGStab to rtol 1e-8 on the GPU
+    and 1e-12 on the oracle for 300 steps: pressure and velocity against the
+    oracle, and max |u| stays bounded (no drift of the pressure level)."""
+    from fluidized_bed_master_amd.sophia import Sophia
+    from oracle.oracle import Oracle
+    p, parts = configs.isph_dem_column(8, 8, 10, 1e-3, seed=5, jitter=0.02, dt=2e-6)
+    p.free_surface = 1
+    p.pgrad_form = 0
+    g, o = Sophia(p, parts, precision=prec), Oracle(p, parts)
+    go = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=PPE[prec], guess=abi.GUESS_PREVIOUS, rtol=1e-8)
+    oo = abi.ppe_opts(method=abi.PPE_BICGSTAB, guess=abi.GUESS_PREVIOUS, rtol=1e-12)
+    errs = {}
+    for s in range(300):
+        sg = g.step(go)
+        o.step(oo)
+        assert sg["converged"], (s, sg)
+        if (s + 1) % 100 == 0:
+            a1, _, _ = g.get_particles(abi.SPH)
+            b1, _, _ = o.get_particles(abi.SPH)
+            f = b1["p_type"] == abi.PT_FLUID
+            umax = float(np.sqrt(a1["ux"] ** 2 + a1["uy"] ** 2 + a1["uz"] ** 2)[f].max())
+            e = {k: rel_err(a1[k][f], b1[k][f]) for k in ("pres", "ux", "uy", "uz")}
+            errs[s + 1] = dict(e, umax=umax)
+            assert umax < 1e-3, (s, umax)
+    _report(f"column_300_{'fp64' if prec == 0 else 'mixed'}", errs)
+    for s, e in errs.items():
+        assert e["pres"] <= 1e-6, (s, e)
+        for k in ("ux", "uy", "uz"):
+            assert e[k] <= 1e-4, (s, k, e)

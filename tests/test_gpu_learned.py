@@ -109,24 +109,35 @@ def test_adam_steps_reduce_loss():
     assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
 
 
-def test_trained_guess_beats_previous_with_fallback():
+def test_trained_guess_improves_its_base_on_an_unseen_column():
+    """Train on one seeded column, evaluate on another.  What the fallback
+    cannot supply: the network's own guess has a smaller initial residual
+    than the extrapolation it corrects on most steps (it is kept), and on
+    average over every step (fallbacks included).  The iteration counts of
+    all four guesses are reported; whether the learned one is faster end to
+    end is the benchmark's question."""
     from fluidized_bed_master_amd.sophia import Sophia
-    p, parts = configs.small_isph_dem(n=12, seed=21, u_noise=0.2)
-    p.pgrad_form = 1  # converged solves: the stable pressure-gradient pairing
 
-    def make():
-        return Sophia(p, parts, precision=abi.PREC_MIXED)
+    def column(seed):
+        p, parts = configs.small_isph_dem(n=12, seed=seed, u_noise=0.2)
+        configs.heated(p, parts)  # free-surface rows: converged solves stay at rest
 
-    sim = make()
+        def make():
+            return Sophia(p, parts, precision=abi.PREC_MIXED)
+        return make
+
+    sim = column(21)()
     w, hist, secs = learned.train(sim, snapshots=150, lr=2e-3)
     sim.close()
-    ev = learned.evaluate(make, w, steps=12, warm=0)
+    ev = learned.evaluate(column(33), w, steps=12, warm=3)
+    lr = ev["learned"]
     _report("mlp_ab", {k: {kk: v[kk] for kk in ("mean_iterations", "mean_r0_rel", "fallbacks")}
                        for k, v in ev.items()} | {"train_loss_first": hist[0]["loss"],
-                                                  "train_loss_last": hist[-1]["loss"]})
-    # the fallback guarantees the learned path never starts worse than its
-    # base, the extrapolated guess; training must also lower the loss
-    assert ev["learned"]["mean_r0_rel"] <= ev["extrapolate"]["mean_r0_rel"] * 1.01
+                                                  "train_loss_last": hist[-1]["loss"],
+                                                  "kept_frac": lr["kept_frac"],
+                                                  "mean_r0_ratio": lr["mean_r0_ratio_learned_to_base"]})
     losses = np.array([h["loss"] for h in hist])
     assert np.all(np.isfinite(losses))
     assert losses[-20:].mean() < losses[:20].mean(), (losses[:20].mean(), losses[-20:].mean())
+    assert lr["kept_frac"] >= 0.5, lr["kept_frac"]
+    assert lr["mean_r0_ratio_learned_to_base"] < 0.95, lr["mean_r0_ratio_learned_to_base"]
