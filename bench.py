@@ -27,6 +27,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# compulsory bytes per particle-step of the whole step (FP32 fields, FP64
+# positions): 981 B outside the PPE, 232 B per mixed BiCGStab iteration
+# (two matrix-free matvecs of 72 B and the Krylov vectors) or 344 B in FP64,
+# 1500 B per sphere (its 16-slot contact history read and written)
+STEP_BYTES_BASE, STEP_BYTES_ITER_MIXED, STEP_BYTES_ITER_FP64, STEP_BYTES_SPHERE = 981.0, 232.0, 344.0, 1500.0
+MF_MATVEC_BYTES = 72.0
 
 
 def _matvec_bytes(layout: dict, n_sph: int, fp32: bool) -> float:
@@ -41,7 +47,8 @@ def _matvec_bytes(layout: dict, n_sph: int, fp32: bool) -> float:
     return layout["nnz"] * (coef + layout["index_bytes"]) + n_sph * (28.0 + 3 * vec)
 
 
-def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp_weights=None, keep_pressure=False):
+def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp_weights=None, keep_pressure=False,
+            probe=None):
     """warmup untimed steps, then `steps` timed ones between two device
     synchronisations.  With keep_pressure the final pressure of the PPE rows
     (fluid particles, in the particles' original numbering) is returned in
@@ -66,6 +73,14 @@ def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp
     rep = g.timers_report() if timers else {}
     layout = g.ppe_layout()
     n_sph, n_dem = g.count(abi.SPH), g.count(abi.DEM)
+    if probe is not None:
+        # the inner FP32 matvec on this step's operator: the solver's
+        # stored-operator kernel against the matrix-free one
+        _, _, rows = g.get_ppe_system()
+        x = np.random.default_rng(1).normal(0.0, 1.0, len(rows)) * (rows == 1)
+        _, us_stored = g.matvec_probe(0, x, reps=30)
+        _, us_mf = g.matvec_probe(1, x, reps=30)
+        probe.update({"stored_us": us_stored, "matrix_free_us": us_mf, "reps": 30})
     pres = None
     if keep_pressure:
         a, _, a3 = g.get_particles(abi.SPH)
@@ -97,6 +112,16 @@ def _pmc_traffic():
                      + kv["write_kib_per_dispatch"] * 1024 * kv["write_factor"])
     except KeyError:
         return None
+
+
+def _step_roofline(n_sph: int, n_dem: int, iters: float, ms_step: float, per_iter: float) -> dict:
+    """The whole step against HBM: compulsory bytes per step (the per-stage
+    figures above times the particles and the measured iterations) over the
+    measured step time."""
+    b = n_sph * (STEP_BYTES_BASE + iters * per_iter) + n_dem * STEP_BYTES_SPHERE
+    ach = b / (ms_step * 1e-3) / 1e9
+    return {"bound": "hbm", "bytes_per_step": round(b), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(params, parts, target_seconds: float = 15.0):
@@ -208,8 +233,10 @@ def main(argv=None) -> int:
     # operator and vectors, previous-step guess
     fp64_opts = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=abi.PPE_FP64, guess=abi.GUESS_PREVIOUS, rtol=1e-8)
 
+    probe = {}
     dt, st, rep, layout, n_sph, n_dem, pres_head = run_gpu(params, parts, abi.PREC_MIXED, mixed_opts, a.steps, a.warmup,
-                                                     timers=True, mlp_weights=weights, keep_pressure=True)
+                                                     timers=True, mlp_weights=weights, keep_pressure=True,
+                                                     probe=probe)
     value = n_total * a.steps / dt
     ms_step = dt / a.steps * 1e3
     ppe_ms = float(np.mean([s["ms"] for s in st]))
@@ -241,12 +268,19 @@ def main(argv=None) -> int:
                         "launches": real, "launches_incl_early_exit": allc, "launches_timed": cnt,
                         "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": ab, "nnz": layout["nnz"],
                         "index_bytes": layout["index_bytes"]}
+            # the same launches against the compulsory traffic of a
+            # matrix-free matvec (72 B per particle: no stored operator)
+            mf = MF_MATVEC_BYTES * n_total
+            roofline["matrix_free_bytes_per_launch"] = mf
+            roofline["frac_vs_matrix_free_bytes"] = round(mf / avg_s / 1e9 / HBM_PEAK_GBS, 4)
 
     fp64 = None
     if "fp64" in legs:
         dt64, st64, _, _, _, _, pres64 = run_gpu(params, parts, abi.PREC_FP64, fp64_opts, a.steps, a.warmup,
                                                  keep_pressure=True)
         fp64 = {"value": n_total * a.steps / dt64, "ms_per_step": dt64 / a.steps * 1e3,
+                "step_roofline": _step_roofline(n_sph, n_dem, float(np.mean([s["iterations"] for s in st64])),
+                                                dt64 / a.steps * 1e3, STEP_BYTES_ITER_FP64),
                 "ppe_ms": float(np.mean([s["ms"] for s in st64])),
                 "ppe_iters": float(np.mean([s["iterations"] for s in st64])),
                 "ppe_r0_rel": float(np.mean([s["r0_rel"] for s in st64])),
@@ -375,6 +409,8 @@ def main(argv=None) -> int:
         "speedup_vs_fp64_same_precond": (value / fp64_pc["value"]) if fp64_pc else None,
         "kernel_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in sorted(rep.items())},
         "roofline": roofline,
+        "step_roofline": _step_roofline(n_sph, n_dem, iters, ms_step, STEP_BYTES_ITER_MIXED),
+        "matvec_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -158,6 +158,7 @@ SIGNATURES = {
     "sophia_create_from_arrays": (C.c_int, [_P(Params), _VP, C.c_int32, C.c_int32, C.c_int32, _P(_ctx)]),
     "sophia_create": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, _P(_ctx)]),
     "sophia_set_table": (C.c_int, [_ctx, _VP]),
+    "sophia_ppe_matvec_probe": (C.c_int, [_ctx, C.c_int32, C.c_int32, _VP, _VP, C.c_int32, _P(C.c_double)]),
     "sophia_read_table": (C.c_int, [C.c_char_p, _VP]),
     "sophia_table_eval": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_double, _P(C.c_double)]),
     "sophia_destroy": (C.c_int, [_ctx]),

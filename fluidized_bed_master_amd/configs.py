@@ -214,6 +214,44 @@ def wcsph_column(nx: int = 10, ny: int = 10, nz: int = 20, dx: float = 1.0e-3, *
     return p, parts
 
 
+def wcsph_dem_column(nx: int = 10, ny: int = 10, nz: int = 16, dx: float = 1.0e-3, *, d_dem: float = 2.0e-3,
+                     dem_height: float = 6.0e-3, phi: float = 0.6, seed: int = 2, jitter: float = 0.02,
+                     u_noise: float = 0.02, dem_u_noise: float = 0.02, dem_w_noise: float = 5.0, con: int = 0,
+                     dt: float | None = None):
+    """The WCSPH column of wcsph_column with DEM spheres on a seeded jittered
+    lattice in its bottom region (phi > pi/6 puts lattice neighbours in
+    contact), run by the WCSPH step with Hertz-Mindlin contact and the WCSPH
+    drag coupling."""
+    p, parts = wcsph_column(nx, ny, nz, dx, seed=seed, jitter=jitter, u_noise=u_noise, dt=dt, con=con)
+    _, with_dem = isph_dem_column(nx, ny, nz, dx, d_dem=d_dem, dem_height=dem_height, phi=phi, seed=seed + 100,
+                                  dem_u_noise=dem_u_noise, dem_w_noise=dem_w_noise)
+    dem = with_dem[with_dem["p_type"] == PT_DEM].copy()
+    if con:
+        dem["temp"] = 330.0
+    return p, np.concatenate([parts, dem])
+
+
+def c2(seed: int = 2):
+    """C2: WCSPH-DEM fluidised bed, 0.05 x 0.05 column filled to 0.15 at
+    dx = 1 mm (375,000 fluid), 8000 glass spheres d = 3 mm rho = 2500 on a
+    seeded jittered lattice in the bottom 8.25 cm (phi = 0.56, the packing
+    8000 such spheres need there), Hertz-Mindlin
+    E = 1e7, nu = 0.3, mu = 0.3, Di Felice/Rong drag, a 0.3 m/s inlet
+    through the bottom 3 fluid layers and an outlet through the top 3,
+    dt = 5e-7."""
+    dx = 1.0e-3
+    p, parts = wcsph_column(50, 50, 150, dx, seed=seed, dt=5.0e-7)
+    _, with_dem = isph_dem_column(50, 50, 150, dx, d_dem=3.0e-3, dem_height=0.0825, phi=0.56, max_dem=8000,
+                                  seed=seed)
+    dem = with_dem[with_dem["p_type"] == PT_DEM].copy()
+    p.open_boundary = 1
+    for k, v in enumerate([-1.0, 3.0e-3, 147.0e-3, 1.0]):
+        p.ob_band[k] = v
+    p.ob_inlet_u = 0.3
+    p.ob_inlet_temp = 293.0
+    return p, np.concatenate([parts, dem])
+
+
 def c4_train(seed: int = 4):
     """Warm-start training column: the C5 sphere size and packing (d = 2 mm,
     phi = 0.35) in a 0.06 x 0.06 box filled to 0.06 (216,000 fluid) at

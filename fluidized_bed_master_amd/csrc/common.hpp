@@ -74,6 +74,7 @@ struct DevParams {
   double heat_flux_box[6];
   // property tables (prop_table): per |p_type| < tab_types the block
   // tab[k*50 + tab_index[t] ...] of size tab_size[t], k = T, h, k, mu
+  int wcsph;              // solver_type 0: the WCSPH variants of the coupling kernels
   int prop_table, tab_types;
   int tab_index[SOPHIA_TABLE_TYPES], tab_size[SOPHIA_TABLE_TYPES];
   const double* tab;
@@ -190,6 +191,9 @@ struct DemView {
   double *test6;
   double *dtemp, *temp0;     // temperature rate (P3.dtemp) and start-of-step temperature (P2.temp0)
   double *hrad, *hcon;       // radiative and contact parts of dtemp (P1.test1, P1.test2)
+  // WCSPH: start-of-step position and spin (P2.x0, P2.wx0), saved by the
+  // half-step predictor before the spheres are re-sorted
+  double *x0, *y0, *z0, *wx0, *wy0, *wz0;
 };
 
 struct CellView {
@@ -567,6 +571,9 @@ template <typename R> int dem_coupling(hipStream_t s, const DevParams& P, const 
 template <typename R> int sph_dem_reaction(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dcells);
 int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
+// WCSPH half-step predictor and corrector of the spheres
+int wc_dem_predictor(hipStream_t s, const DevParams& P, const DemView& d);
+int wc_dem_corrector(hipStream_t s, const DevParams& P, const DemView& d);
 // open boundaries (sophia_params.open_boundary = 1, ob_*)
 struct OpenBoundary {
   double L[4];  // z bands: inlet [L1, L2), outlet [L3, L4)

@@ -109,6 +109,12 @@ struct sophia_ctx {
   double time = 0;
   bool nnps_done = false;
   bool after_corrector = false;
+  // The SPH cell order, ranges, neighbour list, position records and surface
+  // labels stay valid while no SPH position changes: ISPH with Eulerian SPH
+  // (eulerian_sph = 1) never moves an SPH particle after the first sort, so
+  // a later sort of the already sorted keys is the identity and is skipped.
+  bool sph_sorted = false;
+  bool sph_order_identity = false;
   virtual int nnps() = 0;
   virtual int prep() = 0;
   virtual int interaction() = 0;
@@ -122,6 +128,7 @@ struct sophia_ctx {
   virtual int set_pressure(const double* p, int n) = 0;
   virtual int get_ppe_system(double* diag, double* rhs, int32_t* row) const = 0;
   virtual int ppe_apply(const double* x, double* y, int n) = 0;
+  virtual int matvec_probe(int variant, int reps, const double* x, double* y, int n, float* ms) = 0;
   virtual int ppe_layout(int64_t* nnz, int32_t* ib, int32_t* cb) const = 0;
   virtual int ppe_coarse(int32_t* dims, float* E, float* X) const = 0;
   virtual int monitors(int species, double* out) = 0;
@@ -183,6 +190,10 @@ struct Ctx : public sophia_ctx {
   // DEM persistent
   PField<double> dx_, dy_, dz_, dux, duy, duz, dwx, dwy, dwz, dm, dri, drad, dh, drho, dtemp, dfdx, dfdy, dfdz, dqsd;
   PField<int> dptype, dbtype, ditype, didx;
+  // WCSPH with spheres: start-of-step x0, y0, z0, ux0, uy0, uz0, wx0, wy0,
+  // wz0, temp0, written before the re-sort and permuted with the spheres
+  PField<double> dws[10];
+  bool wc_dem = false;
   int* hist_idx[2] = {nullptr, nullptr};
   double* hist_ds[2] = {nullptr, nullptr};
   SphView<R> sv{};
@@ -281,6 +292,7 @@ struct Ctx : public sophia_ctx {
     P.dem_mat_radius = p.dem_mat_radius;
     P.heat_flux = p.con_solve == 1 ? p.heat_flux : 0.0;
     for (int k = 0; k < 6; k++) P.heat_flux_box[k] = p.heat_flux_box[k];
+    P.wcsph = p.solver_type == 0 ? 1 : 0;
     P.prop_table = (p.prop_table && dtab) ? 1 : 0;
     P.tab_types = has_table ? SOPHIA_TABLE_TYPES : 0;
     for (int k = 0; k < SOPHIA_TABLE_TYPES; k++) {
@@ -310,6 +322,12 @@ struct Ctx : public sophia_ctx {
     w.qsd = dqsd.cur; w.ptype = dptype.cur; w.btype = dbtype.cur; w.itype = ditype.cur; w.dem_idx = didx.cur;
     w.hist_idx = hist_idx[0]; w.hist_ds = hist_ds[0];
     w.hist_idx_new = hist_idx[1]; w.hist_ds_new = hist_ds[1];
+    if (wc_dem) {
+      w.x0 = dws[0].cur; w.y0 = dws[1].cur; w.z0 = dws[2].cur;
+      w.ux0 = dws[3].cur; w.uy0 = dws[4].cur; w.uz0 = dws[5].cur;
+      w.wx0 = dws[6].cur; w.wy0 = dws[7].cur; w.wz0 = dws[8].cur;
+      w.temp0 = dws[9].cur;
+    }
   }
 
   int alloc_nbr(int cap) {
@@ -361,6 +379,7 @@ struct Ctx : public sophia_ctx {
     TRY(arena.alloc(&v.por, NS));
     TRY(arena.alloc(&v.flux_s, NS));
     TRY(arena.alloc(&v.surf, NS));
+    if (NS) TRY(fill_u32(stream, (uint32_t*)v.surf, 0u, NS));
     TRY(arena.alloc(&v.visp, NS));
     TRY(arena.alloc(&v.condp, NS));
     for (double** f : {&v.rho_ref, &v.rho0, &v.vol, &v.vol0, &v.x0, &v.y0, &v.z0}) TRY(arena.alloc(f, NS));
@@ -377,6 +396,12 @@ struct Ctx : public sophia_ctx {
                     &dfdy, &dfdz, &dqsd})
       TRY(f->alloc(arena, ND));
     for (auto* f : {&dptype, &dbtype, &ditype, &didx}) TRY(f->alloc(arena, ND));
+    wc_dem = p.solver_type == 0 && nd > 0;
+    if (wc_dem)
+      for (auto& f : dws) {
+        TRY(f.alloc(arena, ND));
+        TRY(fill_u32(stream, (uint32_t*)f.cur, 0u, ND * 2));
+      }
     for (int b = 0; b < 2; b++) {
       TRY(arena.alloc(&hist_idx[b], (size_t)kMaxHist * ND));
       TRY(arena.alloc(&hist_ds[b], (size_t)kMaxHist * 3 * ND));
@@ -541,14 +566,37 @@ struct Ctx : public sophia_ctx {
     return 0;
   }
 
+  // P3 reset (ISPH_Calc_seperate.cuh:102-105).  pgf and the XSPH velocities
+  // live in P1 in the reference and are rewritten each step for the
+  // particles their kernels compute; the others (walls, skipped types) keep
+  // their input value, which is zero (the DEM drag reads pgf of walls).
+  // PPE1..4 (P1) are written for PPE rows only; the rest keep zero.
+  int reset_sph_accumulators() {
+    std::vector<FillSpan> z;
+    for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz, sv.xsx,
+                 sv.xsy, sv.xsz, sv.xst})
+      z.push_back({(uint32_t*)a, (size_t)ns * sizeof(R) / 4, 0u});
+    z.push_back({(uint32_t*)sv.cm, (size_t)6 * ns * sizeof(R) / 4, 0u});
+    for (double* a : {sv.ppe1, sv.ppe2, sv.ppe3, sv.ppe4}) z.push_back({(uint32_t*)a, (size_t)ns * 2, 0u});
+    return fill_u32_many(stream, z.data(), (int)z.size());
+  }
+
   int nnps() override {
     // a side-stream coarse build left by a PPE stage that returned early
     // finishes before the sort rewrites its inputs
     if (ppe_w.coarse) TRY(coarse_settle(stream, *ppe_w.coarse));
     const bool on = sph_on();
     DevParams P = devparams();
-    if (on && ns > 0) {
+    const bool reuse = sph_sorted && p.solver_type == 1 && p.eulerian_sph;
+    if (on && ns > 0 && reuse) {
+      // positions unchanged since the last sort: same keys, identity order,
+      // same ranges and neighbour list; only the per-step accumulators reset
+      sph_order_identity = true;
+      TRY(reset_sph_accumulators());
+      ppe_assembled = false;
+    } else if (on && ns > 0) {
       TimeScope t0(tm(), "nnps_sort_permute_sph");
+      sph_order_identity = false;
       TRY(sort_species(ns, sx.cur, sy.cur, sz.cur, sitype.cur, ssort, &skeys, &sorder, scs, sce));
       std::vector<PermuteField> f;
       auto add = [&](auto& fld, int bytes) { f.push_back(PermuteField{fld.cur, fld.spare, bytes}); };
@@ -567,22 +615,7 @@ struct Ctx : public sophia_ctx {
       for (auto* fl : {&sux, &suy, &suz, &stemp, &sflt, &sdpor}) fl->swap();
       for (auto* fl : {&sptype, &sbtype, &sitype}) fl->swap();
       refresh_views();
-      // P3 reset (ISPH_Calc_seperate.cuh:102-105)
-      // pgf and the XSPH velocities live in P1 in the reference and are
-      // rewritten each step for the particles their kernels compute; the
-      // others (walls, skipped types) keep their input value, which is zero
-      // (the DEM drag reads pgf of walls)
-      // PPE1..4 (P1) are written for PPE rows only; the rest keep zero
-      {
-        std::vector<FillSpan> z;
-        for (R* a : {sv.fx, sv.fy, sv.fz, sv.fmag, sv.fpx, sv.fpy, sv.fpz, sv.dtemp, sv.pgx, sv.pgy, sv.pgz, sv.xsx,
-                     sv.xsy, sv.xsz, sv.xst})
-          z.push_back({(uint32_t*)a, (size_t)ns * sizeof(R) / 4, 0u});
-        z.push_back({(uint32_t*)sv.cm, (size_t)6 * ns * sizeof(R) / 4, 0u});
-        for (double* a : {sv.ppe1, sv.ppe2, sv.ppe3, sv.ppe4}) z.push_back({(uint32_t*)a, (size_t)ns * 2, 0u});
-        z.push_back({(uint32_t*)sv.surf, (size_t)ns, 0u});
-        TRY(fill_u32_many(stream, z.data(), (int)z.size()));
-      }
+      TRY(reset_sph_accumulators());
       // neighbour list with capacity check
       for (int attempt = 0; attempt < 2; attempt++) {
         TRY(fill_u32(stream, (uint32_t*)maxcnt, 0u, 1));
@@ -602,8 +635,24 @@ struct Ctx : public sophia_ctx {
         TRY(alloc_nbr(((mc + 15) / 16) * 16));
         TRY(mlp_alloc(mlp, ns, sv.nbr_cap));
       }
+      // surface labels follow from positions and types alone
+      // (ISPH_Calc_seperate.cuh:289-296)
+      if (p.free_surface > 0) {
+        TimeScope t1(tm(), "surface");
+        TRY(sph_surface<R>(stream, P, sv));
+      }
       ppe_assembled = false;
+      sph_sorted = true;
     }
+    TRY(nnps_dem());
+    nnps_done = true;
+    after_corrector = false;
+    return 0;
+  }
+
+  // sort the spheres, permute their state and reset their per-step
+  // accumulators (the DEM half of NNPS, every step: spheres move)
+  int nnps_dem() {
     if (nd > 0) {
       TimeScope t2(tm(), "nnps_sort_permute_dem");
       TRY(sort_species(nd, dx_.cur, dy_.cur, dz_.cur, ditype.cur, dsort, &dkeys, &dorder, dcs, dce));
@@ -613,6 +662,8 @@ struct Ctx : public sophia_ctx {
                        &dfdx, &dfdy, &dfdz, &dqsd})
         add(*fl, 8);
       for (auto* fl : {&dptype, &dbtype, &ditype, &didx}) add(*fl, 4);
+      if (wc_dem)
+        for (auto& fl : dws) add(fl, 8);
       for (int k = 0; k < kMaxHist; k++) {
         f.push_back(PermuteField{hist_idx[0] + (size_t)k * nd, hist_idx[1] + (size_t)k * nd, 4});
         for (int c = 0; c < 3; c++)
@@ -623,6 +674,8 @@ struct Ctx : public sophia_ctx {
                        &dfdx, &dfdy, &dfdz, &dqsd})
         fl->swap();
       for (auto* fl : {&dptype, &dbtype, &ditype, &didx}) fl->swap();
+      if (wc_dem)
+        for (auto& fl : dws) fl.swap();
       std::swap(hist_idx[0], hist_idx[1]);
       std::swap(hist_ds[0], hist_ds[1]);
       refresh_views();
@@ -635,8 +688,6 @@ struct Ctx : public sophia_ctx {
     } else {
       TRY(fill_u32(stream, dcs, kEmpty, (size_t)d.num_cells));
     }
-    nnps_done = true;
-    after_corrector = false;
     return 0;
   }
 
@@ -718,6 +769,22 @@ struct Ctx : public sophia_ctx {
     return 0;
   }
 
+  // DEM walls after the time update (ISPH_Calc_seperate.cuh:439-441)
+  int apply_dem_walls(const DevParams& P) {
+    if (!p.dem_walls || nd == 0) return 0;
+    DemWallParams W;
+    W.mask = p.dem_walls;
+    std::memcpy(W.cyl, p.wall_cyl, sizeof(W.cyl));
+    std::memcpy(W.box, p.wall_box, sizeof(W.box));
+    std::memcpy(W.cone, p.wall_cone, sizeof(W.cone));
+    W.rest_n = p.wall_rest[0];
+    W.rest_s = p.wall_rest[1];
+    W.mu_s = p.wall_mu[0];
+    W.mu_rs = p.wall_mu[1];
+    TimeScope t(tm(), "dem_walls");
+    return dem_walls(stream, P, dv, W);
+  }
+
   int time_stage(int phase) override {
     TRY(need_nnps());
     DevParams P = devparams();
@@ -742,20 +809,7 @@ struct Ctx : public sophia_ctx {
         ob.inlet_temp = p.ob_inlet_temp;
         TRY(sph_open_boundary<R>(stream, P, sv, CellView{scs, sce}, ob));
       }
-      if (p.dem_walls) {
-        // DEM walls after the time update (ISPH_Calc_seperate.cuh:439-441)
-        DemWallParams W;
-        W.mask = p.dem_walls;
-        std::memcpy(W.cyl, p.wall_cyl, sizeof(W.cyl));
-        std::memcpy(W.box, p.wall_box, sizeof(W.box));
-        std::memcpy(W.cone, p.wall_cone, sizeof(W.cone));
-        W.rest_n = p.wall_rest[0];
-        W.rest_s = p.wall_rest[1];
-        W.mu_s = p.wall_mu[0];
-        W.mu_rs = p.wall_mu[1];
-        TimeScope t(tm(), "dem_walls");
-        TRY(dem_walls(stream, P, dv, W));
-      }
+      TRY(apply_dem_walls(P));
       after_corrector = true;
     }
     return 0;
@@ -784,11 +838,6 @@ struct Ctx : public sophia_ctx {
     if (count == 0) {
       TimeScope t(tm(), "neumann");
       TRY(sph_neumann<R>(stream, P, sv, sc));
-    }
-    if (p.free_surface > 0) {
-      // surface labels and their Dirichlet pressure (ISPH_Calc_seperate.cuh:289-296)
-      TimeScope t(tm(), "surface");
-      TRY(sph_surface<R>(stream, P, sv));
     }
     {
       TimeScope t(tm(), "ppe_assemble");
@@ -912,8 +961,12 @@ struct Ctx : public sophia_ctx {
       std::memcpy(keys, t.data(), n * 4);
     }
     if (order && n) {
-      TRY(get(t, S ? sorder : dorder, n));
-      std::memcpy(order, t.data(), n * 4);
+      if (S && sph_order_identity) {
+        for (int i = 0; i < n; i++) order[i] = i;  // the sort of sorted keys that was skipped
+      } else {
+        TRY(get(t, S ? sorder : dorder, n));
+        std::memcpy(order, t.data(), n * 4);
+      }
     }
     if (cs) {
       TRY(get(t, S ? scs : dcs, d.num_cells));
@@ -1013,8 +1066,12 @@ struct Ctx : public sophia_ctx {
     std::vector<double> a[NF];
     for (int k = 0; k < NF; k++) TRY(get(a[k], src[k], n));
     std::vector<int> pt, bt, it, ix, hi;
-    std::vector<double> hd;
+    std::vector<double> hd, ws[6];
     TRY(get(pt, dv.ptype, n)); TRY(get(bt, dv.btype, n)); TRY(get(it, dv.itype, n)); TRY(get(ix, dv.dem_idx, n));
+    if (wc_dem) {
+      const double* w0[6] = {dv.x0, dv.y0, dv.z0, dv.wx0, dv.wy0, dv.wz0};
+      for (int k = 0; k < 6; k++) TRY(get(ws[k], w0[k], n));
+    }
     TRY(get(hi, (const int*)hist_idx[0], (size_t)kMaxHist * n));
     TRY(get(hd, (const double*)hist_ds[0], (size_t)kMaxHist * 3 * n));
     for (int i = 0; i < n; i++) {
@@ -1041,6 +1098,10 @@ struct Ctx : public sophia_ctx {
         std::memset(&q, 0, sizeof(q));
         q.ux0 = A(35); q.uy0 = A(36); q.uz0 = A(37);
         q.temp0 = A(42);
+        if (wc_dem) {
+          q.x0 = ws[0][i]; q.y0 = ws[1][i]; q.z0 = ws[2][i];
+          q.wx0 = ws[3][i]; q.wy0 = ws[4][i]; q.wz0 = ws[5][i];
+        }
       }
       if (p3) {
         part3& q = p3[i];
@@ -1085,9 +1146,8 @@ struct Ctx : public sophia_ctx {
       set_error("WCSPH stage on a context with solver_type %d", p.solver_type);
       return SOPHIA_E_STATE;
     }
-    if (nd > 0 || p.dim != 3 || p.rho_type != 1) {
-      set_error("WCSPH: supported set is 3D fluid only with continuity density (spheres %d, dim %d, rho_type %d)",
-                nd, p.dim, p.rho_type);
+    if (p.dim != 3 || p.rho_type != 1) {
+      set_error("WCSPH: supported set is 3D with continuity density (dim %d, rho_type %d)", p.dim, p.rho_type);
       return SOPHIA_E_ARG;
     }
     if (ns == 0) return 0;
@@ -1095,6 +1155,13 @@ struct Ctx : public sophia_ctx {
     CellView sc{scs, sce};
     switch (stage) {
       case 0:
+        // spheres: half step from last step's accelerations, then a fresh
+        // sort every step (the reference's WCSPH driver sorts once, which
+        // only its Eulerian fluid allows)
+        if (nd > 0 && p.time_type == 1) {
+          TimeScope t(tm(), "predictor");
+          TRY(wc_dem_predictor(stream, P, dv));
+        }
         if (count == 0) {
           // NNPS once (Eulerian); P3 starts at zero, so the predictor may
           // follow the sort: it is per particle
@@ -1109,6 +1176,7 @@ struct Ctx : public sophia_ctx {
             TRY(refresh_pos4<R>(stream, sv));
           }
           TRY(wc_reset_p3<R>(stream, sv));
+          TRY(nnps_dem());
         }
         after_corrector = false;
         break;
@@ -1124,6 +1192,11 @@ struct Ctx : public sophia_ctx {
           TimeScope t(tm(), "porosity_sph");
           CellView dc{dcs, dce};
           TRY(sph_porosity<R>(stream, P, sv, dv, dc));
+        }
+        if (nd > 0) {
+          // KERNEL_clc_prep3D_coupling (functions_PREP2.cuh:3048), sphere side
+          TimeScope t(tm(), "porosity_dem");
+          TRY(dem_porosity<R>(stream, P, dv, sv, CellView{scs, sce}, CellView{dcs, dce}));
         }
         {
           TimeScope t(tm(), "prep");
@@ -1147,11 +1220,34 @@ struct Ctx : public sophia_ctx {
         TimeScope t(tm(), "wc_interaction");
         if (P.prop_table) TRY(sph_props<R>(stream, P, sv));
         TRY(wc_interaction<R>(stream, P, sv));
+        if (nd > 0) {
+          // Hertz-Mindlin contact (commented out in the reference's WCSPH
+          // driver, WCSPH_Calc.cuh:206), then KERNEL_DEM_coupling3D and
+          // KERNEL_SPH_coupling3D (:225, 236)
+          TRY(dem_interaction());
+          TRY(dem_coupling<R>(stream, P, dv, sv, CellView{scs, sce}));
+          TRY(sph_dem_reaction<R>(stream, P, sv, dv, CellView{dcs, dce}));
+        }
         break;
       }
       case 7:
         TRY(wc_corrector<R>(stream, P, sv));
         TRY(refresh_pos4<R>(stream, sv));
+        if (nd > 0) {
+          TRY(wc_dem_corrector(stream, P, dv));
+          TRY(apply_dem_walls(P));
+        }
+        if (p.open_boundary == 1) {
+          // inlet/outlet buffer state from the interior; the reference's
+          // WCSPH driver computes it (WCSPH_Calc.cuh:282) but never applies
+          // it (no _calc call), so here it is applied as in the ISPH step
+          TimeScope t(tm(), "open_boundary");
+          OpenBoundary ob;
+          std::memcpy(ob.L, p.ob_band, sizeof(ob.L));
+          ob.inlet_u = p.ob_inlet_u;
+          ob.inlet_temp = p.ob_inlet_temp;
+          TRY(sph_open_boundary<R>(stream, P, sv, CellView{scs, sce}, ob));
+        }
         after_corrector = true;
         break;
       case 8:
@@ -1242,6 +1338,32 @@ struct Ctx : public sophia_ctx {
     if (cb) *cb = ppe_w.mixed ? 4 : 8;
     return 0;
   }
+  int ppe_matvec_probe_host(int variant, int reps, const double* x, double* y, int n, float* ms) {
+    if (!ppe_assembled || n != ns) {
+      set_error("matvec probe: operator not assembled or n mismatch");
+      return SOPHIA_E_STATE;
+    }
+    double *dx = nullptr, *dy = nullptr;
+    float2* pk = nullptr;
+    TRY(arena.alloc(&dx, n));
+    TRY(arena.alloc(&dy, n));
+    TRY(arena.alloc(&pk, n));
+    HIP_TRY(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, stream));
+    int rc = ppe_matvec_probe<R>(stream, devparams(), sv, ppe_w, variant, reps, dx, dy, pk, ms);
+    if (!rc) {
+      hipError_t e = hipMemcpyAsync(y, dy, (size_t)n * 8, hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) rc = hip_fail(e, "matvec probe copy", __FILE__, __LINE__);
+    }
+    arena.drop(dx);
+    arena.drop(dy);
+    arena.drop(pk);
+    return rc;
+  }
+  int matvec_probe(int variant, int reps, const double* x, double* y, int n, float* ms) override {
+    return ppe_matvec_probe_host(variant, reps, x, y, n, ms);
+  }
+
   int ppe_apply(const double* x, double* y, int n) override {
     if (!ppe_assembled || n != ns) {
       set_error("ppe_apply: operator not assembled or n mismatch");
@@ -2122,6 +2244,15 @@ int sophia_get_ppe_system(const sophia_ctx* c, double* diag, double* rhs, int32_
   return c->get_ppe_system(diag, rhs, row);
 }
 int sophia_ppe_apply(sophia_ctx* c, const double* x, double* y, int32_t n) { CHECK_CTX(c); return c->ppe_apply(x, y, n); }
+int sophia_ppe_matvec_probe(sophia_ctx* c, int32_t variant, int32_t reps, const double* x, double* y, int32_t n,
+                            double* us_per_launch) {
+  CHECK_CTX(c);
+  if (!x || !y || reps < 0 || (variant != 0 && variant != 1)) return SOPHIA_E_ARG;
+  float ms = 0.f;
+  int rc = c->matvec_probe(variant, reps, x, y, n, &ms);
+  if (!rc && us_per_launch) *us_per_launch = 1e3 * (double)ms;
+  return rc;
+}
 int sophia_monitors(sophia_ctx* c, int32_t species, double* out) {
   CHECK_CTX(c);
   if (!out || (species != SOPHIA_SPH && species != SOPHIA_DEM)) return SOPHIA_E_ARG;

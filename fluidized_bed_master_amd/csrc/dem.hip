@@ -312,9 +312,12 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
     double sq = 0.63 + 4.8 / sqrt(Re);
     double Cd = sq * sq;
     double lg = 1.5 - log10(Re);
-    double beta = 3.7 - 0.65 * exp(-lg * lg / 2.0);
+    // ISPH (KERNEL_DEM_coupling3D_dem, :979-1009): Di Felice, eps^(2-beta);
+    // WCSPH (KERNEL_DEM_coupling3D, :298-310): the Rong form, eps^(-1-beta)
+    double beta = P.wcsph ? 2.65 * (1.0 + pori) - (5.3 - 3.5 * pori) * pori * pori * exp(-lg * lg / 2.0)
+                          : 3.7 - 0.65 * exp(-lg * lg / 2.0);
     double k = (1.0 / 8.0) * Cd * P.rho_fluid * 3.14159 * di * di;
-    double e = pow(pori, 2 - beta);
+    double e = pow(pori, P.wcsph ? -1 - beta : 2 - beta);
     dfx = k * rfx * magf * e / mi;
     dfy = k * rfy * magf * e / mi;
     dfz = k * rfz * magf * e / mi;
@@ -353,9 +356,11 @@ __global__ void k_dem_coupling(DevParams P, DemView d, SphView<R> v, CellView sc
     const double Nu = 2.0 + 0.6 * pow(Re, 0.5) * (int_third ? 1.0 : cbrt(P.pr_air));
     dq_vol = Nu * ki / di * a_ht * ((tf + ta) / sm - Ti);
   }
-  d.fx[i] = d.fx[i] + dfx / pori + tx;
-  d.fy[i] = d.fy[i] + dfy / pori + ty;
-  d.fz[i] = d.fz[i] + dfz / pori + tz;
+  // the ISPH kernel divides the fluid drag by eps (:1099-1101), the WCSPH one does not (:338-340)
+  const double dscale = P.wcsph ? 1.0 : pori;
+  d.fx[i] = d.fx[i] + dfx / dscale + tx;
+  d.fy[i] = d.fy[i] + dfy / dscale + ty;
+  d.fz[i] = d.fz[i] + dfz / dscale + tz;
   d.fax[i] = dax;
   d.fay[i] = day;
   d.faz[i] = daz;
@@ -384,6 +389,9 @@ __global__ void k_sph_reaction(DevParams P, SphView<R> v, DemView d, CellView dc
   if (d.n > 0 && t != SOPHIA_PT_MOVING && t != -3 && t != 0 && (t == 1 || t == 3)) {
     const double A = knl_tmpA<double>(P.kernel_type, P.dim, hd);
     const double rhoi = v.rho[i];
+    // the WCSPH reaction carries the fluid particle's porosity
+    // (KERNEL_SPH_coupling3D, :1677-1679); the ISPH one does not
+    const double pfac = P.wcsph ? v.por[i] : 1.0;
     // heat_capacity(ptypei, tempi) at :1860 swaps its arguments: the
     // temperature is read as the particle type (SOPHIA_COMPAT_CP_ARGS)
     const double cpi = (P.compat & SOPHIA_COMPAT_CP_ARGS)
@@ -397,9 +405,10 @@ __global__ void k_sph_reaction(DevParams P, SphView<R> v, DemView d, CellView dc
       if (!(flt > 0.0)) return;
       double w = knl_w<double>(P.kernel_type, A, hd, r);
       double c = w / (flt + 1.0e-20);
-      tx += -(d.m[j] / rhoi) * d.fdx[j] * c;
-      ty += -(d.m[j] / rhoi) * d.fdy[j] * c;
-      tz += -(d.m[j] / rhoi) * d.fdz[j] * c;
+      const double mr = P.wcsph ? d.m[j] * pfac / rhoi : d.m[j] / rhoi;
+      tx += -mr * d.fdx[j] * c;
+      ty += -mr * d.fdy[j] * c;
+      tz += -mr * d.fdz[j] * c;
       tt += -d.qsd[j] / (rhoi * cpi) * c;
       hit = true;
     });
@@ -469,6 +478,62 @@ __global__ void k_dem_corrector(DevParams P, DemView d) {
     d.wz[i] += d.tqz[i] * bz * P.dt_dem;
   }
   if (P.con_solve == 1 && t > 0) d.temp[i] = d.temp0[i] + d.dtemp[i] * P.dt_dem * (double)(t != -1);
+}
+
+// KERNEL_clc_predictor (functions_TIME.cuh:77-292) for the spheres in the
+// WCSPH step: half step of position, velocity and spin from the previous
+// step's accelerations; the start-of-step state goes to x0, ux0, wx0, temp0.
+// Runs before the spheres are re-sorted.
+__global__ void k_wc_dem_predictor(DevParams P, DemView d) {
+  int i = particle_index();
+  if (i >= d.n) return;
+  if (d.itype[i] == 3) return;
+  const double bz = (d.btype[i] == 0), hdt = P.dt_dem * 0.5;
+  const double x0 = d.x[i], y0 = d.y[i], z0 = d.z[i], ux0 = d.ux[i], uy0 = d.uy[i], uz0 = d.uz[i];
+  const double wx0 = d.wx[i], wy0 = d.wy[i], wz0 = d.wz[i];
+  d.x[i] = x0 + ux0 * hdt;
+  d.y[i] = y0 + uy0 * hdt;
+  d.z[i] = z0 + uz0 * hdt;
+  d.ux[i] = ux0 + d.fx[i] * bz * hdt;
+  d.uy[i] = uy0 + d.fy[i] * bz * hdt;
+  d.uz[i] = uz0 + d.fz[i] * bz * hdt;
+  d.wx[i] = wx0 + d.tqx[i] * bz * hdt;
+  d.wy[i] = wy0 + d.tqy[i] * bz * hdt;
+  d.wz[i] = wz0 + d.tqz[i] * bz * hdt;
+  d.x0[i] = x0; d.y0[i] = y0; d.z0[i] = z0;
+  d.ux0[i] = ux0; d.uy0[i] = uy0; d.uz0[i] = uz0;
+  d.wx0[i] = wx0; d.wy0[i] = wy0; d.wz0[i] = wz0;
+  if (P.con_solve == 1) {
+    d.temp0[i] = d.temp[i];
+    d.temp[i] = d.temp[i] + d.dtemp[i] * hdt;
+  }
+}
+
+// KERNEL_time_update_single (functions_TIME.cuh:676-877) for the spheres:
+// u = u0 + a dt (held at u0 past u_limit), w = w0 + torque/inertia dt,
+// x = x0 + u dt, T = T0 + dT dt.  The reference moves nothing when XSPH is
+// on (:788-792), which freezes the spheres; here they always move.
+__global__ void k_wc_dem_corrector(DevParams P, DemView d) {
+  int i = particle_index();
+  if (i >= d.n) return;
+  if (d.itype[i] == 3) return;
+  const double bz = (d.btype[i] == 0), dt = P.dt_dem;
+  double uxc = d.ux0[i] + d.fx[i] * bz * dt, uyc = d.uy0[i] + d.fy[i] * bz * dt, uzc = d.uz0[i] + d.fz[i] * bz * dt;
+  if (uxc * uxc + uyc * uyc + uzc * uzc >= P.u_limit * P.u_limit) {
+    uxc = d.ux0[i];
+    uyc = d.uy0[i];
+    uzc = d.uz0[i];
+  }
+  d.x[i] = d.x0[i] + uxc * dt;
+  d.y[i] = d.y0[i] + uyc * dt;
+  d.z[i] = d.z0[i] + uzc * dt;
+  d.ux[i] = uxc;
+  d.uy[i] = uyc;
+  d.uz[i] = uzc;
+  d.wx[i] = d.wx0[i] + d.tqx[i] * bz * dt;
+  d.wy[i] = d.wy0[i] + d.tqy[i] * bz * dt;
+  d.wz[i] = d.wz0[i] + d.tqz[i] * bz * dt;
+  if (P.con_solve == 1) d.temp[i] = d.temp0[i] + d.dtemp[i] * dt;
 }
 
 // --- DEM wall boundaries (functions_DEM_BC.cuh) -----------------------------
@@ -669,6 +734,18 @@ int dem_predictor(hipStream_t s, const DevParams& P, const DemView& d) {
   return 0;
 }
 
+int wc_dem_predictor(hipStream_t s, const DevParams& P, const DemView& d) {
+  if (d.n <= 0) return 0;
+  k_wc_dem_predictor<<<grid_xcd(d.n), kBlock, 0, s>>>(P, d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+int wc_dem_corrector(hipStream_t s, const DevParams& P, const DemView& d) {
+  if (d.n <= 0) return 0;
+  k_wc_dem_corrector<<<grid_xcd(d.n), kBlock, 0, s>>>(P, d);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
 int dem_corrector(hipStream_t s, const DevParams& P, const DemView& d) {
   if (d.n <= 0) return 0;
   k_dem_corrector<<<grid_xcd(d.n), kBlock, 0, s>>>(P, d);

@@ -31,6 +31,7 @@
 // fixed order by the last block to finish).  The host checks convergence
 // once per chunk of iterations.
 #include "ppe.hpp"
+#include "nbr.hpp"
 
 #include "coarse.hpp"
 
@@ -296,12 +297,15 @@ __device__ inline int pick9(const int (&a)[9], int r) {
 }
 
 __global__ void k_rowmask(DevParams P, int n, const int* ptype, const int* btype, const int* itype, const int* surf,
-                          int* row) {
+                          double* pres, int* row) {
   int i = particle_index();
   if (i >= n) return;
   int t = ptype[i];
-  bool on = itype[i] <= 2 && !(P.open_boundary > 0 && btype[i] > 0) && t < 1000 && t > 0 &&
-            !(P.free_surface > 0 && surf[i] != 0);
+  const bool fluid = itype[i] <= 2 && !(P.open_boundary > 0 && btype[i] > 0) && t < 1000 && t > 0;
+  const bool dirichlet = fluid && P.free_surface > 0 && surf[i] != 0;
+  // a free-surface particle is a known value p_ref, not an unknown
+  if (dirichlet) pres[i] = P.p_ref;
+  bool on = fluid && !dirichlet;
   row[i] = on ? 1 : 0;
 }
 
@@ -1035,7 +1039,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
   AsmOut o{v.n, P.NI, P.NJ, P.NK, keys, cell_off, w.row, w.ccnt, w.runc, w.idx, w.hi, w.lo, w.c64,
            w.diag, w.dinv, w.rhs, w.flags};
   const int g = grid_xcd(v.n);
-  k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, v.surf, w.row);
+  k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, v.surf, v.pres, w.row);
   int fmt = P.z_index ? kFmtIdx32 : kFmtRun8;
   for (;;) {
     k_zero_u32<<<1, 64, 0, s>>>(w.flags, 4);
@@ -1296,11 +1300,110 @@ int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o,
   return 0;
 }
 
+// --- matrix-free matvec (timing probe) -----------------------------------
+// y_i = d_i x_i - sum_{j row} a_ij x_j with a_ij = c_ij (eps_i V_j + m_j eps_j
+// / rho_i), c_ij = -(x_ij . Cm_i grad W_ij) / r_ij^2, recomputed per launch
+// in FP32 from the neighbour list, the position records and one float2 per
+// column (V_j, m_j eps_j; zero for non-row columns).  Probe only: it reads
+// the 4-byte neighbour list where the stored operator reads 5 bytes per
+// entry, and adds the kernel gradient per pair.
+template <typename R>
+__global__ void k_mf_pack(int n, const SphView<R> v, const int* row, float2* pk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pk[i] = row[i] ? make_float2((float)(v.m[i] / v.rho[i]), (float)(v.m[i] * v.por[i])) : make_float2(0.f, 0.f);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_mf_matvec(DevParams P, SphView<R> v, const int* __restrict__ row,
+                                                      const double* __restrict__ diag, const float2* __restrict__ pk,
+                                                      const float* __restrict__ x, float* __restrict__ y) {
+  const int i = particle_index();
+  if (i >= v.n) return;
+  if (!row[i]) {
+    y[i] = 0.f;
+    return;
+  }
+  const int n = v.n;
+  const float h = (float)v.h[i], A = knl_tmpA<float>(P.kernel_type, P.dim, h);
+  const float epsi = (float)v.por[i], irhoi = (float)(1.0 / v.rho[i]);
+  float acc = 0.f;
+  for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
+    const float dx = (float)q.dx, dy = (float)q.dy, dz = (float)q.dz;
+    const float r = sqrtf(dx * dx + dy * dy + dz * dz) + 1e-20f;
+    const float dw = knl_dw<float>(P.kernel_type, A, h, r);
+    float gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+    kgc_apply(P, v.cm, n, i, gx, gy, gz);
+    const float c = -(dx * gx + dy * gy + dz * gz) / (r * r);
+    const float2 pj = pk[q.j];
+    const float a = c * (epsi * pj.x + pj.y * irhoi);
+    acc += on ? a * x[q.j] : 0.f;
+  });
+  y[i] = (float)diag[i] * x[i] - acc;
+}
+
+__global__ void k_to_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_to_f64(int n, const float* __restrict__ a, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+template <typename R>
+int ppe_matvec_probe(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork& w, int variant, int reps,
+                     const double* x64, double* y64, float2* pk, float* ms) {
+  if (!w.mixed || w.fmt != kFmtRun8) {
+    set_error("matvec probe: needs the mixed-precision operator with run offsets");
+    return SOPHIA_E_STATE;
+  }
+  const int n = w.n;
+  float* x = (float*)w.vec[4];
+  float* y = (float*)w.vec[3];
+  const float* rh = (const float*)w.vec[1];
+  k_to_f32<<<grid_for(n), kBlock, 0, s>>>(n, x64, x);
+  BiState* st = (BiState*)w.state;
+  if (variant == 1) {
+    k_mf_pack<R><<<grid_for(n), kBlock, 0, s>>>(n, v, w.row, pk);
+  } else {
+    // a live solver state: no convergence, no breakdown
+    BiState h{};
+    h.max_iter = 1 << 30;
+    std::memcpy(w.host_state, &h, sizeof(h));
+    HIP_TRY(hipMemcpyAsync(st, w.host_state, sizeof(BiState), hipMemcpyHostToDevice, s));
+  }
+  PpeOp o = ppe_op(w);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  for (int k = 0; k <= reps; k++) {
+    if (k == 1) HIP_TRY(hipEventRecord(e0, s));  // the first launch warms the caches, untimed
+    if (variant == 1) k_mf_matvec<R><<<grid_xcd(n), kBlock, 0, s>>>(P, v, w.row, w.diag, pk, x, y);
+    else k_bi_mv_staged<0, float><<<w.nbg, kRedThreads, 0, s>>>(o, x, y, rh, w.partials, st);
+  }
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  *ms = reps > 0 ? t / reps : 0.f;
+  HIP_TRY(hipEventDestroy(e0));
+  HIP_TRY(hipEventDestroy(e1));
+  k_to_f64<<<grid_for(n), kBlock, 0, s>>>(n, y, y64);
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
 template int ppe_assemble<float>(hipStream_t, const DevParams&, const SphView<float>&, PpeWork&, const uint32_t*,
                                  const uint32_t*);
 template int ppe_assemble<double>(hipStream_t, const DevParams&, const SphView<double>&, PpeWork&, const uint32_t*,
                                   const uint32_t*);
 template int ppe_jacobi<float>(hipStream_t, const SphView<float>&, PpeWork&);
 template int ppe_jacobi<double>(hipStream_t, const SphView<double>&, PpeWork&);
+
+template int ppe_matvec_probe<float>(hipStream_t, const DevParams&, const SphView<float>&, PpeWork&, int, int,
+                                     const double*, double*, float2*, float*);
+template int ppe_matvec_probe<double>(hipStream_t, const DevParams&, const SphView<double>&, PpeWork&, int, int,
+                                      const double*, double*, float2*, float*);
 
 }  // namespace sophia

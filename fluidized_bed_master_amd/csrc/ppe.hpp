@@ -84,6 +84,13 @@ int ppe_jacobi(hipStream_t s, const SphView<R>& v, PpeWork& w);
 // y = A x with the full-precision operator (FP64, or hi + lo); residual form
 // y = rhs - A x when residual; ||y||^2 over rows into *norm2 (synchronises)
 int ppe_apply64(hipStream_t s, PpeWork& w, const double* x, double* y, bool residual, double* norm2);
+// timing probe of the inner FP32 matvec: variant 0 the stored operator
+// (the solver's kernel), 1 matrix-free from the neighbour list; reps timed
+// launches after one untimed; y = A x for the device vector x; pk is n
+// float2 of scratch for variant 1
+template <typename R>
+int ppe_matvec_probe(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork& w, int variant, int reps,
+                     const double* x64, double* y64, float2* pk, float* ms);
 // known_rr >= 0: w.r64 already holds b - A guess over the rows, with this
 // squared norm (the caller has just evaluated the guess)
 int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o, sophia_ppe_stats& st,

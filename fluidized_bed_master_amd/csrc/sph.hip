@@ -481,10 +481,11 @@ __global__ void k_open_boundary(DevParams P, SphView<R> v, CellView cv, OpenBoun
 // fs_threshold or non < 10.  free_surface = 2 counts neighbours of the
 // particle's own p_type, as the reference does; free_surface = 1 counts every
 // SPH particle, walls included, so that fluid against a wall is not surface.
-// Surface particles take the Dirichlet pressure p_ref (the commented-out rows
-// of functions_PPE2.cuh:533-534) and leave the PPE rows.  FP64 sums in the
-// list's (the reference's) order with no contraction, so labels match the
-// oracle's exactly.
+// Surface particles leave the PPE rows and take the Dirichlet pressure p_ref
+// (the commented-out rows of functions_PPE2.cuh:533-534) when the rows are
+// marked.  FP64 sums in the list's (the reference's) order with no
+// contraction, so labels match the oracle's exactly.  Labels depend on
+// positions and types only and are computed after every neighbour-list build.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_surface(DevParams P, SphView<R> v) {
 #pragma clang fp contract(off)
@@ -514,7 +515,6 @@ __global__ __launch_bounds__(kBlock) void k_surface(DevParams P, SphView<R> v) {
   tz = tz * rn;
   const bool surf = dist_exact(tx, ty, tz) > P.fs_threshold || non < 10;
   v.surf[i] = surf ? 1 : 0;
-  if (surf) v.pres[i] = P.p_ref;
 }
 
 template <typename R>

@@ -46,6 +46,16 @@ class Sophia:
         self.precision = precision
         return self
 
+    def matvec_probe(self, variant: int, x: np.ndarray, reps: int = 20):
+        """(y, mean microseconds per launch) of the inner FP32 matvec: variant 0
+        stored operator, 1 matrix-free."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        us = C.c_double()
+        check(self.lib.sophia_ppe_matvec_probe(self._ctx, variant, reps, ptr(x), ptr(y), len(x), C.byref(us)),
+              "sophia_ppe_matvec_probe")
+        return y, us.value
+
     def set_table(self, table: "Table") -> None:
         check(self.lib.sophia_set_table(self._ctx, C.byref(table)), "sophia_set_table")
 

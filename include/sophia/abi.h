@@ -432,6 +432,15 @@ int sophia_get_ppe_system(const sophia_ctx* ctx, double* diag, double* rhs,
                           int32_t* row_mask);
 /* y = A x with the assembled operator (FP64), sorted order, SPH count */
 int sophia_ppe_apply(sophia_ctx* ctx, const double* x, double* y, int32_t n);
+/* Timing probe of the inner FP32 matvec of the mixed-precision solve (no
+ * reference counterpart): variant 0 runs the solver's kernel on the stored
+ * operator, variant 1 a matrix-free kernel that recomputes every a_ij from
+ * the neighbour list and the position records.  y = A x (FP32 arithmetic,
+ * sorted order, SPH count) and the mean device time of `reps` launches after
+ * one untimed one.  Needs a mixed-precision context with an assembled
+ * operator in run-offset format (row-major cell keys). */
+int sophia_ppe_matvec_probe(sophia_ctx* ctx, int32_t variant, int32_t reps, const double* x, double* y,
+                            int32_t n, double* us_per_launch);
 /* storage of the assembled operator: stored entries, bytes per column index
  * (1: u8 offset inside a cell run, 4: u32 index) and per coefficient as read
  * by the inner iterations (4: FP32, 8: FP64).  Any may be NULL. */

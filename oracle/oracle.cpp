@@ -1216,9 +1216,15 @@ static void dem_coupling(orc_state& s) {
     if (magf > 1.0e-8 && (flt_fluid + flt_air) > 0.00001 && f_fluid > p.drag_switch) {
       double Re = di * magf * pori / (nu_f + 1.0e-15);
       double Cd = (0.63 + 4.8 / std::sqrt(Re)) * (0.63 + 4.8 / std::sqrt(Re));
-      double beta = 3.7 - 0.65 * std::exp(-(1.5 - std::log10(Re)) * (1.5 - std::log10(Re)) / 2.0);
+      // ISPH (KERNEL_DEM_coupling3D_dem, :979-1009): Di Felice, eps^(2-beta);
+      // WCSPH (KERNEL_DEM_coupling3D, :298-310): the Rong form of the other
+      // regimes, eps^(-1-beta)
+      const bool wc = p.solver_type == 0;
+      double beta = wc ? 2.65 * (1.0 + pori) - (5.3 - 3.5 * pori) * pori * pori *
+                             std::exp(-(1.5 - std::log10(Re)) * (1.5 - std::log10(Re)) / 2.0)
+                       : 3.7 - 0.65 * std::exp(-(1.5 - std::log10(Re)) * (1.5 - std::log10(Re)) / 2.0);
       double k = (1.0 / 8.0) * Cd * rho_f * 3.14159 * di * di;
-      double e = std::pow(pori, 2 - beta);
+      double e = std::pow(pori, wc ? -1 - beta : 2 - beta);
       dfx = k * rfx * magf * e / mi;
       dfy = k * rfy * magf * e / mi;
       dfz = k * rfz * magf * e / mi;
@@ -1256,9 +1262,11 @@ static void dem_coupling(orc_state& s) {
       const double Nu = 2.0 + 0.6 * std::pow(Re, 0.5) * pr_third(p, p.pr_air);
       dq_vol = Nu * ki / di * a_ht * T_m;
     }
-    o.ftotalx = fx0 + dfx / pori + tx;
-    o.ftotaly = fy0 + dfy / pori + ty;
-    o.ftotalz = fz0 + dfz / pori + tz;
+    // the ISPH kernel divides the fluid drag by eps (:1099-1101), the WCSPH one does not (:338-340)
+    const double dscale = p.solver_type == 0 ? 1.0 : pori;
+    o.ftotalx = fx0 + dfx / dscale + tx;
+    o.ftotaly = fy0 + dfy / dscale + ty;
+    o.ftotalz = fz0 + dfz / dscale + tz;
     qi.Fdx_da = dax;
     qi.Fdy_da = day;
     qi.Fdz_da = daz;
@@ -1303,9 +1311,12 @@ static void sph_coupling(orc_state& s) {
         double w = knl_w(kt, A, hi, r);
         double flt = qj.flt_sd_2 + qj.flt_s;
         if ((t == 1 || t == 3) && flt > 0.0) {
-          tx += -(qj.m / rhoi) * qj.Fdx_df * w / (flt + 1.0e-20);
-          ty += -(qj.m / rhoi) * qj.Fdy_df * w / (flt + 1.0e-20);
-          tz += -(qj.m / rhoi) * qj.Fdz_df * w / (flt + 1.0e-20);
+          // the WCSPH reaction carries the fluid particle's porosity
+          // (KERNEL_SPH_coupling3D, :1677-1679); the ISPH one does not
+          const double mr = p.solver_type == 0 ? qj.m * qi.DEMpor / rhoi : qj.m / rhoi;
+          tx += -mr * qj.Fdx_df * w / (flt + 1.0e-20);
+          ty += -mr * qj.Fdy_df * w / (flt + 1.0e-20);
+          tz += -mr * qj.Fdz_df * w / (flt + 1.0e-20);
           tt += -qj.Q_sd / (rhoi * cpi) * w / (flt + 1.0e-20);
         }
       });
@@ -2369,8 +2380,82 @@ static void wc_corrector(orc_state& s) {
   }
 }
 
+/* KERNEL_clc_predictor (functions_TIME.cuh:77-292) for the spheres, which
+ * the reference keeps in the same arrays as the fluid: half step of
+ * position, velocity and spin from the previous step's accelerations, the
+ * start-of-step state saved in P2; runs before the spheres are re-sorted */
+static void wc_dem_predictor(orc_state& s) {
+  Species& dm = s.dem;
+  const sophia_params& p = s.p;
+  const double dt = p.dt;
+  for (int i = 0; i < dm.n(); i++) {
+    part1& q = dm.P1[i];
+    part2& q2 = dm.P2[i];
+    const part3& o = dm.P3[i];
+    if (q.i_type == 3) continue;
+    const double bz = (q.buffer_type == 0);
+    const double x0 = q.x, y0 = q.y, z0 = q.z, ux0 = q.ux, uy0 = q.uy, uz0 = q.uz;
+    const double wx0 = q.wx, wy0 = q.wy, wz0 = q.wz;
+    q.x = x0 + ux0 * (dt * 0.5);
+    q.y = y0 + uy0 * (dt * 0.5);
+    q.z = z0 + uz0 * (dt * 0.5);
+    q.ux = ux0 + o.ftotalx * bz * (dt * 0.5);
+    q.uy = uy0 + o.ftotaly * bz * (dt * 0.5);
+    q.uz = uz0 + o.ftotalz * bz * (dt * 0.5);
+    q.wx = wx0 + o.torqx * bz * (dt * 0.5);
+    q.wy = wy0 + o.torqy * bz * (dt * 0.5);
+    q.wz = wz0 + o.torqz * bz * (dt * 0.5);
+    q2.x0 = x0; q2.y0 = y0; q2.z0 = z0;
+    q2.ux0 = ux0; q2.uy0 = uy0; q2.uz0 = uz0;
+    q2.wx0 = wx0; q2.wy0 = wy0; q2.wz0 = wz0;
+    if (p.con_solve == 1) {
+      q2.temp0 = q.temp;
+      q.temp = q.temp + o.dtemp * (dt * 0.5);
+    }
+  }
+}
+
+/* KERNEL_time_update_single (functions_TIME.cuh:676-877) for the spheres:
+ * u = u0 + a dt (kept at u0 past u_limit), w = w0 + torque/inertia dt,
+ * x = x0 + u dt, T = T0 + dT dt.  The reference moves nothing when XSPH is on
+ * (:788-792), which freezes the spheres; here they always move. */
+static void wc_dem_corrector(orc_state& s) {
+  Species& dm = s.dem;
+  const sophia_params& p = s.p;
+  const double dt = p.dt;
+  for (int i = 0; i < dm.n(); i++) {
+    const part1& q = dm.SP1[i];
+    const part2& q2 = dm.SP2[i];
+    const part3& o = dm.P3[i];
+    part1& T = dm.P1[i];
+    if (q.i_type == 3) continue;
+    const double bz = (q.buffer_type == 0);
+    double uxc = q2.ux0 + o.ftotalx * bz * dt, uyc = q2.uy0 + o.ftotaly * bz * dt, uzc = q2.uz0 + o.ftotalz * bz * dt;
+    if (uxc * uxc + uyc * uyc + uzc * uzc >= p.u_limit * p.u_limit) {
+      uxc = q2.ux0;
+      uyc = q2.uy0;
+      uzc = q2.uz0;
+    }
+    T.x = q2.x0 + uxc * dt;
+    T.y = q2.y0 + uyc * dt;
+    T.z = q2.z0 + uzc * dt;
+    T.ux = uxc; T.uy = uyc; T.uz = uzc;
+    T.wx = q2.wx0 + o.torqx * bz * dt;
+    T.wy = q2.wy0 + o.torqy * bz * dt;
+    T.wz = q2.wz0 + o.torqz * bz * dt;
+    T.temp = p.con_solve == 1 ? q2.temp0 + o.dtemp * dt : q.temp;
+    T.flt_s = q.flt_s; T.flt_sd = q.flt_sd; T.flt_sd_2 = q.flt_sd_2; T.Q_sd = q.Q_sd;
+    T.DEMpor = q.DEMpor;
+    T.Fdx_df = q.Fdx_df; T.Fdy_df = q.Fdy_df; T.Fdz_df = q.Fdz_df;
+    T.Fdx_da = q.Fdx_da; T.Fdy_da = q.Fdy_da; T.Fdz_da = q.Fdz_da;
+    T.Fdx_b = q.Fdx_b; T.Fdy_b = q.Fdy_b; T.Fdz_b = q.Fdz_b;
+    T.test1 = q.test1; T.test2 = q.test2; T.test6 = q.test6;
+    dm.P2[i] = q2;
+  }
+}
+
 static int wc_check(const orc_state& s) {
-  if (s.dem.n() > 0 || s.p.dim != 3 || s.p.rho_type != 1) return -1;
+  if (s.p.dim != 3 || s.p.rho_type != 1) return -1;
   return 0;
 }
 
@@ -2392,6 +2477,13 @@ int orc_wcsph_stage(orc_state* s, int32_t stage) {
         sp.SP1 = sp.P1;
         sp.SP2 = sp.P2;
       }
+      if (s->dem.n() > 0) {
+        // spheres move: half step, then a fresh sort every step (the
+        // reference's WCSPH driver sorts once, which only its Eulerian
+        // fluid allows)
+        if (s->p.time_type == 1) wc_dem_predictor(*s);
+        nnps_species(*s, s->dem, false);
+      }
       s->after_corrector = false;
       break;
     case 1:
@@ -2401,6 +2493,7 @@ int orc_wcsph_stage(orc_state* s, int32_t stage) {
     case 2:
       if (s->p.kgc_solve > 0 || s->p.delSPH_solve == 2) kgc_3d(*s);
       coupling_prep_sph(*s);
+      if (s->dem.n() > 0) coupling_prep_dem(*s);  // KERNEL_clc_prep3D_coupling (functions_PREP2.cuh:3048)
       prep3d_sph(*s);
       break;
     case 3:
@@ -2414,9 +2507,24 @@ int orc_wcsph_stage(orc_state* s, int32_t stage) {
       break;
     case 6:
       wc_interaction(*s);
+      if (s->dem.n() > 0) {
+        // Hertz-Mindlin contact (commented out in the reference's WCSPH
+        // driver, WCSPH_Calc.cuh:206), then KERNEL_DEM_coupling3D and
+        // KERNEL_SPH_coupling3D (:225, 236)
+        orc_dem_interaction(s);
+        dem_coupling(*s);
+        sph_coupling(*s);
+      }
       break;
     case 7:
       wc_corrector(*s);
+      if (s->dem.n() > 0) {
+        wc_dem_corrector(*s);
+        if (s->p.dem_walls) dem_walls(*s);
+      }
+      // buffer state applied after the time update (computed but not applied
+      // by the reference's WCSPH driver, WCSPH_Calc.cuh:282)
+      if (s->p.open_boundary == 1) open_boundary(*s);
       s->after_corrector = true;
       break;
     case 8:

@@ -82,7 +82,10 @@ def test_column_at_rest_300_converged_steps(prec):
             errs[s + 1] = dict(e, umax=umax)
             assert umax < 1e-3, (s, umax)
     _report(f"column_300_{'fp64' if prec == 0 else 'mixed'}", errs)
+    # measured worst over the 300 steps: pressure 6e-9 / 1.4e-7 and velocity
+    # 1.7e-8 / 7.7e-7 (FP64 / mixed)
+    bar_p, bar_u = (1e-7, 1e-7) if prec == abi.PREC_FP64 else (1e-6, 1e-5)
     for s, e in errs.items():
-        assert e["pres"] <= 1e-6, (s, e)
+        assert e["pres"] <= bar_p, (s, e)
         for k in ("ux", "uy", "uz"):
-            assert e[k] <= 1e-4, (s, k, e)
+            assert e[k] <= bar_u, (s, k, e)
