@@ -192,11 +192,12 @@ def main(argv=None) -> int:
                     help="preconditioner of the mixed-precision solve")
     ap.add_argument("--inner-rtol", type=float, default=0.001,
                     help="mixed solve: fold into the FP64 solution each time the FP32 residual drops by this factor")
-    ap.add_argument("--legs", default="fp64,fp64_pc,prev,extrap,bc",
+    ap.add_argument("--legs", default="fp64,fp64_pc,prev,extrap,bc,c3",
                     help="comparison legs to run: fp64 (plain FP64 path), fp64_pc (FP64, same preconditioner), "
                          "prev (mixed, previous-step guess), extrap (mixed, extrapolated guess without the "
                          "learned correction), bc (open boundaries and a DEM wall on the same column: "
-                         "cost of the two boundary kernels)")
+                         "cost of the two boundary kernels), c3 (the periodic Taylor-Green box at 100^3: "
+                         "zero-mean PPE, FP64 path vs mixed)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -355,6 +356,28 @@ def main(argv=None) -> int:
                       "buffer_particles": int(((parts["z"] < z0 + 2e-3) | (parts["z"] >= z1 - 2e-3))[fl].sum()),
                       "steps": steps_bc}
 
+    # C3: the periodic Taylor-Green box (1,000,000 fluid particles, no walls
+    # or spheres), Lagrangian ISPH with a zero-mean PPE, Jacobi-preconditioned
+    # BiCGStab to rtol 1e-8 from the previous pressure: FP64 path vs mixed
+    c3 = None
+    if "c3" in legs:
+        p3, parts3 = configs.c3()
+        c3 = {"workload": "c3: periodic Taylor-Green box [0, 2 pi)^3, 100^3 = 1,000,000 fluid, h = 1.3 dx, "
+                          f"Wendland C2, dt = {p3.dt:g}, zero-mean PPE, Jacobi preconditioner, previous-step guess",
+              "particles": len(parts3)}
+        steps3 = max(1, min(a.steps, 5))
+        for name, prec, pp in (("fp64", abi.PREC_FP64, abi.PPE_FP64), ("mixed", abi.PREC_MIXED, abi.PPE_MIXED_IR)):
+            o3 = abi.ppe_opts(method=abi.PPE_BICGSTAB, precision=pp, guess=abi.GUESS_PREVIOUS, rtol=1e-8,
+                              zero_mean=1, inner_rtol=a.inner_rtol)
+            dt3, st3, _, _, _, _, _ = run_gpu(p3, parts3, prec, o3, steps3, 1)
+            c3[name] = {"value": len(parts3) * steps3 / dt3, "ms_per_step": dt3 / steps3 * 1e3,
+                        "ppe_ms": float(np.mean([s["ms"] for s in st3])),
+                        "ppe_iters": float(np.mean([s["iterations"] for s in st3])),
+                        "ppe_converged_all": bool(all(s["converged"] for s in st3)),
+                        "ppe_r_rel_max": float(np.max([s["r_rel"] for s in st3]))}
+        c3["steps"] = steps3
+        c3["ppe_speedup_mixed_vs_fp64"] = c3["fp64"]["ppe_ms"] / c3["mixed"]["ppe_ms"]
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(params, parts, a.cpu_seconds)
@@ -405,6 +428,7 @@ def main(argv=None) -> int:
         "mixed_previous_guess": prev,
         "mixed_extrapolated_guess": extrap,
         "boundaries": boundaries,
+        "c3_periodic": c3,
         "speedup_vs_fp64_path": (value / fp64["value"]) if fp64 else None,
         "speedup_vs_fp64_same_precond": (value / fp64_pc["value"]) if fp64_pc else None,
         "kernel_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in sorted(rep.items())},

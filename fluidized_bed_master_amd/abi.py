@@ -99,7 +99,7 @@ class Params(C.Structure):
         ("dem_alpha_mat", C.c_double * 2), ("dem_mat_radius", C.c_double),
         ("heat_flux", C.c_double), ("heat_flux_box", C.c_double * 6),
         ("free_surface", C.c_int32), ("fs_threshold", C.c_double), ("prop_table", C.c_int32),
-        ("reserved_i", C.c_int32), ("reserved", C.c_double * 7),
+        ("noslip_bc", C.c_int32), ("reserved", C.c_double * 7),
     ]
 
     def as_dict(self):

@@ -24,7 +24,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libsophia.so"
 ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
 
-SOURCES = ["nnps.hip", "sph.hip", "dem.hip", "wcsph.hip", "ppe.hip", "coarse.hip", "learned.hip", "context.cpp"]
+SOURCES = ["nnps.hip", "sph.hip", "dem.hip", "wcsph.hip", "wcsph2d.hip", "ppe.hip", "coarse.hip", "learned.hip", "context.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ROCM_LIB = str(Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib")

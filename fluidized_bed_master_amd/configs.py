@@ -231,6 +231,131 @@ def wcsph_dem_column(nx: int = 10, ny: int = 10, nz: int = 16, dx: float = 1.0e-
     return p, np.concatenate([parts, dem])
 
 
+def dam_break_2d(nx: int = 40, ny: int = 80, dx: float = 0.01, *, tank: tuple = (160, 100), wall_layers: int = 3,
+                 seed: int = 1, jitter: float = 0.01, h_ratio: float = 1.3, rho0: float = 1000.0,
+                 mu: float = 1.0e-3, gamma: float = 7.0, dt: float = 1.0e-5, delsph: int = 2, kgc: int = 1,
+                 noslip: int = 1, xsph: int = 1, fva: int = 1, con: int = 0, u_noise: float = 0.0):
+    """C1: 2D WCSPH dam break.  A water column of nx x ny particles at
+    ((i + 1/2) dx, (j + 1/2) dx) +- U(-jitter dx, jitter dx) in the left
+    corner of a tank of tank = (cells wide, cells high) with `wall_layers`
+    wall layers below and on both sides (no lid): 40 x 80 = 3,200 fluid and
+    1,098 wall particles at the defaults (a 1.6 x 1.0 m tank, 0.4 x 0.8 m
+    column).  Wendland C2 h = 1.3 dx, m = rho0 dx^2, Tait gamma = 7 with
+    c0 = 10 sqrt(2 g H) and B = rho0 c0^2 / gamma (eos_b_factor = rho0),
+    continuity density, delta-SPH delta = 0.1, predictor-corrector at
+    dt = 1e-5.  h_over_dx = h_ratio makes the 2D reference density
+    m / (h / h_over_dx)^2 equal rho0.  con = 1 adds conduction with a warm
+    lower half; u_noise adds seeded velocity noise to the fluid."""
+    NX, NY = tank
+    h = h_ratio * dx
+    wi = np.arange(-wall_layers, NX + wall_layers)
+    wj = np.arange(-wall_layers, NY)
+    I, J = np.meshgrid(wi, wj, indexing="ij")
+    inside = (I >= 0) & (I < NX) & (J >= 0)
+    I, J = I[~inside], J[~inside]
+    walls = _records(I.size)
+    walls["x"] = (I + 0.5) * dx
+    walls["y"] = (J + 0.5) * dx
+    walls["p_type"] = PT_BOUNDARY
+    fi, fj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    nf = fi.size
+    fluid = _records(nf)
+    fluid["x"] = (fi.ravel() + 0.5) * dx
+    fluid["y"] = (fj.ravel() + 0.5) * dx
+    if jitter > 0:
+        fluid["x"] += uniform(seed * 3 + 0, nf, -jitter, jitter) * dx
+        fluid["y"] += uniform(seed * 3 + 1, nf, -jitter, jitter) * dx
+    fluid["p_type"] = PT_FLUID
+    if u_noise > 0:
+        fluid["ux"] = uniform(seed * 11 + 1, nf, -u_noise, u_noise)
+        fluid["uy"] = uniform(seed * 11 + 2, nf, -u_noise, u_noise)
+    parts = np.concatenate([walls, fluid])
+    parts["h"] = h
+    parts["m"] = rho0 * dx * dx
+    parts["rho"] = rho0
+    parts["temp"] = 293.0
+    if con:
+        parts["temp"] = 293.0 + 50.0 * (parts["y"] < 0.5 * ny * dx)
+    H = ny * dx
+    p = config_params(dx, dt, rho_f=rho0, mu_f=mu, h_ratio=h_ratio, con=con)
+    p.solver_type = 0
+    p.dim = 2
+    p.time_type = 1
+    p.rho_type = 1
+    p.delSPH_solve = delsph
+    p.kgc_solve = kgc
+    p.delta_sph = 0.1
+    p.soundspeed = 10.0 * math.sqrt(2.0 * 9.81 * H)
+    p.gamma = gamma
+    p.eos_b_factor = rho0
+    p.noslip_bc = noslip
+    p.xsph_solve = xsph
+    p.fva_solve = fva
+    p.sph_max_neighbors = 48
+    return p, parts
+
+
+def taylor_green(n: int = 100, *, seed: int = 3, noise: float = 0.01, jitter: float = 0.0, rho0: float = 1000.0,
+                 mu: float = 0.1, h_ratio: float = 1.3, dt: float = 1.0e-3, kgc: int = 0,
+                 drift: tuple = (0.0, 0.0, 0.0)):
+    """C3: triply periodic Taylor-Green box [0, 2 pi)^3 of n^3 fluid
+    particles at ((i + 1/2) dx, ...) +- U(-jitter dx, jitter dx), dx = 2 pi / n,
+    u = (sin x cos y cos z, -cos x sin y cos z, 0) (1 + noise U(-1, 1)) per
+    component, rho0 = 1000, Wendland C2 h = 1.3 dx; Lagrangian ISPH with a
+    zero-mean PPE (the operator of a periodic box without walls is singular
+    with the constant null space; without KGC it is symmetric, so a zero-mean
+    right-hand side is consistent).  drift adds a uniform velocity (a
+    Galilean shift that carries particles through the faces; the Taylor-Green
+    field itself has no flux through them).  n = 100: 1,000,000 particles."""
+    dx = 2.0 * math.pi / n
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    parts = _records(i.size)
+    parts["x"] = (i.ravel() + 0.5) * dx
+    parts["y"] = (j.ravel() + 0.5) * dx
+    parts["z"] = (k.ravel() + 0.5) * dx
+    N = i.size
+    if jitter > 0:
+        parts["x"] += uniform(seed * 3 + 0, N, -jitter, jitter) * dx
+        parts["y"] += uniform(seed * 3 + 1, N, -jitter, jitter) * dx
+        parts["z"] += uniform(seed * 3 + 2, N, -jitter, jitter) * dx
+    x, y, z = parts["x"], parts["y"], parts["z"]
+    f = [1.0 + noise * uniform(seed * 7 + c, N, -1.0, 1.0) for c in range(3)] if noise > 0 else [1.0, 1.0, 1.0]
+    parts["ux"] = np.sin(x) * np.cos(y) * np.cos(z) * f[0] + drift[0]
+    parts["uy"] = -np.cos(x) * np.sin(y) * np.cos(z) * f[1] + drift[1]
+    parts["uz"] = drift[2]
+    parts["p_type"] = PT_FLUID
+    parts["h"] = h_ratio * dx
+    parts["m"] = rho0 * dx ** 3
+    parts["rho"] = rho0
+    parts["temp"] = 293.0
+    p = config_params(dx, dt, rho_f=rho0, mu_f=mu, h_ratio=h_ratio)
+    p.solver_type = 1
+    p.eulerian_sph = 0
+    p.kgc_solve = kgc
+    p.delSPH_solve = 0
+    p.fg_solve = 0
+    p.fva_solve = 0
+    p.xsph_solve = 0
+    p.con_solve = 0
+    p.domain_fixed = 1
+    p.periodic_mask = 7
+    for a in range(3):
+        p.domain_min[a] = 0.0
+        p.domain_max[a] = 2.0 * math.pi
+    p.u_limit = 100.0
+    return p, parts
+
+
+def c3(seed: int = 3):
+    """C3 at its full size: 100^3 = 1,000,000 particles."""
+    return taylor_green(100, seed=seed)
+
+
+def c1(seed: int = 1):
+    """C1 at its full size: 3,200 fluid + 1,098 wall particles."""
+    return dam_break_2d(seed=seed)
+
+
 def c2(seed: int = 2):
     """C2: WCSPH-DEM fluidised bed, 0.05 x 0.05 column filled to 0.15 at
     dx = 1 mm (375,000 fluid), 8000 glass spheres d = 3 mm rho = 2500 on a

@@ -75,6 +75,9 @@ struct DevParams {
   // property tables (prop_table): per |p_type| < tab_types the block
   // tab[k*50 + tab_index[t] ...] of size tab_size[t], k = T, h, k, mu
   int wcsph;              // solver_type 0: the WCSPH variants of the coupling kernels
+  int noslip;             // 2D WCSPH: KERNEL_boundary2D velocity mirroring
+  int pmask;              // periodic axes (bit 0 x, 1 y, 2 z)
+  double plen[3];         // their periods
   int prop_table, tab_types;
   int tab_index[SOPHIA_TABLE_TYPES], tab_size[SOPHIA_TABLE_TYPES];
   const double* tab;
@@ -138,6 +141,7 @@ struct SphView {
   double *rho_ref, *rho0;  // reference / start-of-step density (P2; the Tait EOS amplifies their rounding)
   double *vol, *vol0;      // WCSPH particle volume and its initial value (P1.vol, P1.vol0)
   double *x0, *y0, *z0;    // WCSPH start-of-step position (P2.x0)
+  double *xp, *yp, *rhop;  // 2D WCSPH with XSPH: predicted x, y, rho kept past the corrector
   R *cm;              // 6 * n, symmetric KGC inverse (xx, xy, xz, yy, yz, zz)
   R *grx, *gry, *grz; // delta-SPH density gradient
   R *fx, *fy, *fz, *fmag;   // ftotal (P3)
@@ -160,7 +164,26 @@ struct SphView {
   int nbr_cap;
   double pg0x, pg0y, pg0z;
   float pgrid;
+  // periodic axes (bit 0 x, 1 y, 2 z) and their periods; on those axes the
+  // pos4 grid has exactly 1024 cells per period
+  int pmask;
+  double plx, ply, plz;
 };
+
+// A position moved by one step back into [mn, mn + L) on a periodic axis.
+__host__ __device__ inline double per_pos(double x, double mn, double L) {
+  if (x >= mn + L) return x - L;
+  if (x < mn) return x + L;
+  return x;
+}
+
+// Periodic minimum image of a coordinate difference (the two positions lie
+// within one period of each other): one subtraction, as the oracle does.
+__host__ __device__ inline double per_wrap(double d, double L) {
+  if (d > 0.5 * L) return d - L;
+  if (d < -0.5 * L) return d + L;
+  return d;
+}
 
 // x_a - x_b from two pos4 records: the integer cell difference times the
 // power-of-two spacing (exact) plus the FP32 offset difference, one rounding
@@ -171,6 +194,34 @@ __device__ inline float3 rel_diff(float4 a, float4 b, float g) {
   const float cy = (float)((int)((ca >> 10) & 1023u) - (int)((cb >> 10) & 1023u));
   const float cz = (float)((int)(ca >> 20) - (int)(cb >> 20));
   return make_float3(fmaf(cx, g, a.x - b.x), fmaf(cy, g, a.y - b.y), fmaf(cz, g, a.z - b.z));
+}
+// the same on periodic axes (pmask bits): the grid has 1024 cells per
+// period there, so the minimum image is a wrap of the integer cell
+// difference into [-512, 512]
+__device__ inline int cell_wrap(int d) { return d > 512 ? d - 1024 : (d < -512 ? d + 1024 : d); }
+__device__ inline float3 rel_diff(float4 a, float4 b, float g, int pmask) {
+  if (!pmask) return rel_diff(a, b, g);
+  const uint32_t ca = __float_as_uint(a.w), cb = __float_as_uint(b.w);
+  int dx = (int)(ca & 1023u) - (int)(cb & 1023u);
+  int dy = (int)((ca >> 10) & 1023u) - (int)((cb >> 10) & 1023u);
+  int dz = (int)(ca >> 20) - (int)(cb >> 20);
+  if (pmask & 1) dx = cell_wrap(dx);
+  if (pmask & 2) dy = cell_wrap(dy);
+  if (pmask & 4) dz = cell_wrap(dz);
+  return make_float3(fmaf((float)dx, g, a.x - b.x), fmaf((float)dy, g, a.y - b.y), fmaf((float)dz, g, a.z - b.z));
+}
+// x_i - x_j in FP64 with the periodic minimum image
+template <class V>
+__device__ inline void pdiff(const V& v, double xi, double yi, double zi, int j, double& dx, double& dy,
+                             double& dz) {
+  dx = xi - v.x[j];
+  dy = yi - v.y[j];
+  dz = zi - v.z[j];
+  if (v.pmask) {
+    if (v.pmask & 1) dx = per_wrap(dx, v.plx);
+    if (v.pmask & 2) dy = per_wrap(dy, v.ply);
+    if (v.pmask & 4) dz = per_wrap(dz, v.plz);
+  }
 }
 
 struct DemView {
@@ -253,20 +304,31 @@ __device__ inline double dist_exact(double dx, double dy, double dz) {
   return sqrt(s);
 }
 
-// Visit the 27 cells around a point in the reference order (z, y, x).
+// Cell index c along an axis of N cells: wrapped on a periodic axis (bit of
+// pmask), else false outside 0..N-1.
+__device__ inline bool cell_in(int pmask, int axis, int& c, int N) {
+  if (pmask >> axis & 1) {
+    c = c < 0 ? c + N : (c > N - 1 ? c - N : c);
+    return true;
+  }
+  return c >= 0 && c <= N - 1;
+}
+
+// Visit the 27 cells around a point in the reference order (z, y, x),
+// wrapping periodic axes.
 template <class F>
 __device__ inline void for_cells(const DevParams& P, double x, double y, double z, const CellView& cv, F&& f) {
   int ic, jc, kc;
   cell_of(P, x, y, z, ic, jc, kc);
   for (int dz = -1; dz <= 1; dz++) {
     int c = kc + dz;
-    if (c < 0 || c > P.NK - 1) continue;
+    if (!cell_in(P.pmask, 2, c, P.NK)) continue;
     for (int dy = -1; dy <= 1; dy++) {
       int b = jc + dy;
-      if (b < 0 || b > P.NJ - 1) continue;
+      if (!cell_in(P.pmask, 1, b, P.NJ)) continue;
       for (int dx = -1; dx <= 1; dx++) {
         int a = ic + dx;
-        if (a < 0 || a > P.NI - 1) continue;
+        if (!cell_in(P.pmask, 0, a, P.NI)) continue;
         int k = cell_key(P, a, b, c);
         uint32_t s = cv.start[k];
         if (s == kEmpty) continue;
@@ -598,6 +660,14 @@ template <typename R> int wc_continuity(hipStream_t s, const DevParams& P, const
 template <typename R> int wc_eos(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int wc_interaction(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int wc_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// 2D WCSPH (SOPHIA_multi's 2D kernels)
+template <typename R> int wc2_kgc(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc2_prep(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc2_boundary(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc2_continuity(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc2_interaction(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc2_time_update(hipStream_t s, const DevParams& P, const SphView<R>& v);
+template <typename R> int wc2_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v);
 // max |u|, rho, ftotal of one species (scratch: 3 * 1024 + 3 doubles)
 template <typename U>
 int monitor_max(hipStream_t s, int n, const int* ptype, const int* itype, const U* ux, const U* uy, const U* uz,

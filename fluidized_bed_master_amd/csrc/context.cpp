@@ -101,7 +101,10 @@ struct sophia_ctx {
   virtual ~sophia_ctx() {}
   sophia_params p{};
   sophia_domain d{};
-  double pgrid = 1.0;  // pos4 grid spacing (power of two)
+  double pgrid = 1.0;  // pos4 grid spacing (power of two; period / 1024 when periodic)
+  // periodic axes (bit 0 x, 1 y, 2 z) of a fixed domain and their periods
+  int pmask = 0;
+  double plen[3] = {0, 0, 0};
   int device = 0;
   int precision = SOPHIA_PREC_MIXED;
   hipStream_t stream = nullptr;
@@ -194,6 +197,12 @@ struct Ctx : public sophia_ctx {
   // wz0, temp0, written before the re-sort and permuted with the spheres
   PField<double> dws[10];
   bool wc_dem = false;
+  // 2D WCSPH re-sorts the fluid every step after the predictor: the
+  // start-of-step state it wrote (x0, y0, z0, rho0, rho_ref; ux0, uy0, uz0,
+  // temp0) is permuted with the particles through these spares
+  bool wc_lag = false;
+  double* lag_d[5] = {};
+  R* lag_r[4] = {};
   int* hist_idx[2] = {nullptr, nullptr};
   double* hist_ds[2] = {nullptr, nullptr};
   SphView<R> sv{};
@@ -239,8 +248,9 @@ struct Ctx : public sophia_ctx {
     P.xsph_solve = p.xsph_solve;
     P.open_boundary = p.open_boundary;
     P.z_index = p.flag_z_index;
-    // WCSPH: KERNEL_set_alpha (functions_ALE.cuh:2-20) makes SPH Eulerian
-    P.eulerian = p.solver_type == 0 ? 1 : p.eulerian_sph;
+    // 3D WCSPH: KERNEL_set_alpha (functions_ALE.cuh:2-20) makes SPH
+    // Eulerian; the 2D step (SOPHIA_multi) has no set_alpha: Lagrangian
+    P.eulerian = p.solver_type == 0 ? (p.dim == 3 ? 1 : 0) : p.eulerian_sph;
     P.fix_xsph = p.fix_xsph_position;
     P.pgrad_form = p.pgrad_form;
     P.NI = d.NI;
@@ -293,6 +303,9 @@ struct Ctx : public sophia_ctx {
     P.heat_flux = p.con_solve == 1 ? p.heat_flux : 0.0;
     for (int k = 0; k < 6; k++) P.heat_flux_box[k] = p.heat_flux_box[k];
     P.wcsph = p.solver_type == 0 ? 1 : 0;
+    P.noslip = p.noslip_bc;
+    P.pmask = pmask;
+    for (int k = 0; k < 3; k++) P.plen[k] = plen[k];
     P.prop_table = (p.prop_table && dtab) ? 1 : 0;
     P.tab_types = has_table ? SOPHIA_TABLE_TYPES : 0;
     for (int k = 0; k < SOPHIA_TABLE_TYPES; k++) {
@@ -314,6 +327,8 @@ struct Ctx : public sophia_ctx {
     v.ptype = sptype.cur; v.btype = sbtype.cur; v.itype = sitype.cur;
     v.pg0x = d.x_min; v.pg0y = d.y_min; v.pg0z = d.z_min;
     v.pgrid = (float)pgrid;
+    v.pmask = pmask;
+    v.plx = plen[0]; v.ply = plen[1]; v.plz = plen[2];
     DemView& w = dv;
     w.n = nd;
     w.x = dx_.cur; w.y = dy_.cur; w.z = dz_.cur; w.ux = dux.cur; w.uy = duy.cur; w.uz = duz.cur;
@@ -383,6 +398,12 @@ struct Ctx : public sophia_ctx {
     TRY(arena.alloc(&v.visp, NS));
     TRY(arena.alloc(&v.condp, NS));
     for (double** f : {&v.rho_ref, &v.rho0, &v.vol, &v.vol0, &v.x0, &v.y0, &v.z0}) TRY(arena.alloc(f, NS));
+    wc_lag = p.solver_type == 0 && p.dim == 2;
+    if (wc_lag) {
+      for (auto& f : lag_d) TRY(arena.alloc(&f, NS));
+      for (auto& f : lag_r) TRY(arena.alloc(&f, NS));
+      for (double** f : {&v.xp, &v.yp, &v.rhop}) TRY(arena.alloc(f, NS));
+    }
     for (R** f : {&v.dpor_prev, &v.grx, &v.gry, &v.grz, &v.fx, &v.fy, &v.fz, &v.fmag,
                   &v.fpx, &v.fpy, &v.fpz, &v.pgx, &v.pgy, &v.pgz, &v.dtemp, &v.temp0, &v.ux0, &v.uy0, &v.uz0,
                   &v.drho, &v.xsx, &v.xsy, &v.xsz, &v.xst})
@@ -467,6 +488,18 @@ struct Ctx : public sophia_ctx {
     d.NI = (int)((d.x_max - d.x_min) / d.dcell) + 1;
     d.NJ = (int)((d.y_max - d.y_min) / d.dcell) + 1;
     d.NK = (int)((d.z_max - d.z_min) / d.dcell) + 1;
+    // periodic axes of a fixed domain: whole cells of width >= dcell tile the
+    // period (the wrapped 27-cell stencil then covers kappa h), and the pos4
+    // grid has 1024 cells per period (the image is an integer wrap)
+    pmask = p.domain_fixed ? (p.periodic_mask & (p.dim == 2 ? 3 : 7)) : 0;
+    int* N[3] = {&d.NI, &d.NJ, &d.NK};
+    const double sp[3] = {d.x_max - d.x_min, d.y_max - d.y_min, d.z_max - d.z_min};
+    for (int a = 0; a < 3; a++)
+      if (pmask >> a & 1) {
+        plen[a] = sp[a];
+        *N[a] = std::max(1, (int)(sp[a] / d.dcell));
+        pgrid = sp[a] / 1024.0;
+      }
     if (p.flag_z_index == 0) {
       d.num_cells = d.NI * d.NJ * d.NK + 1;
     } else {
@@ -607,7 +640,17 @@ struct Ctx : public sophia_ctx {
       add(sux, rb); add(suy, rb); add(suz, rb); add(stemp, rb); add(sflt, rb);
       add(sdpor, rb);
       add(sptype, 4); add(sbtype, 4); add(sitype, 4);
+      double** lag_dc[5] = {&sv.x0, &sv.y0, &sv.z0, &sv.rho0, &sv.rho_ref};
+      R** lag_rc[4] = {&sv.ux0, &sv.uy0, &sv.uz0, &sv.temp0};
+      if (wc_lag) {
+        for (int k = 0; k < 5; k++) f.push_back(PermuteField{*lag_dc[k], lag_d[k], 8});
+        for (int k = 0; k < 4; k++) f.push_back(PermuteField{*lag_rc[k], lag_r[k], rb});
+      }
       TRY(permute_fields(stream, ns, sorder, f.data(), (int)f.size()));
+      if (wc_lag) {
+        for (int k = 0; k < 5; k++) std::swap(*lag_dc[k], lag_d[k]);
+        for (int k = 0; k < 4; k++) std::swap(*lag_rc[k], lag_r[k]);
+      }
       TRY(cell_offsets(stream, ns, skeys, d.num_cells, scoff));
       for (auto* fl : {&sx, &sy, &sz, &sh, &spres, &sm, &srho}) fl->swap();
       if (p_hist > 0) sph1.swap();
@@ -842,6 +885,9 @@ struct Ctx : public sophia_ctx {
     {
       TimeScope t(tm(), "ppe_assemble");
       TRY(ppe_assemble<R>(stream, P, sv, ppe_w, skeys, scoff));
+      // periodic box without walls: a consistent right-hand side for the
+      // singular operator (the oracle's project_mean)
+      if (o->zero_mean && o->method == SOPHIA_PPE_BICGSTAB) TRY(ppe_project_mean(stream, ppe_w, ppe_w.rhs));
     }
     ppe_assembled = true;
     if (o->method == SOPHIA_PPE_BICGSTAB && o->precond == SOPHIA_PRECOND_TWO_LEVEL && o->zero_mean == 0 &&
@@ -1023,8 +1069,9 @@ struct Ctx : public sophia_ctx {
           q.grad_rhox = A(12); q.grad_rhoy = A(13); q.grad_rhoz = A(14);
           q.pgf_x = A(22); q.pgf_y = A(23); q.pgf_z = A(24);
           q.XSPH_ux = A(30); q.XSPH_uy = A(31); q.XSPH_uz = A(32); q.XSPH_temp = A(33);
-          // WCSPH: KERNEL_set_alpha makes every SPH particle Eulerian
-          double e = (pt[i] <= 1000 && (p.eulerian_sph || p.solver_type == 0)) ? 0.0 : 1.0;
+          // 3D WCSPH: KERNEL_set_alpha makes every SPH particle Eulerian
+          const bool eul = p.solver_type == 0 ? p.dim == 3 : p.eulerian_sph != 0;
+          double e = (pt[i] <= 1000 && eul) ? 0.0 : 1.0;
           q.elix = q.eliy = q.eliz = e;
           q.vol = VOLv[i]; q.vol0 = VOL0v[i];
           bool row = !rows.empty() && rows[i];
@@ -1146,11 +1193,12 @@ struct Ctx : public sophia_ctx {
       set_error("WCSPH stage on a context with solver_type %d", p.solver_type);
       return SOPHIA_E_STATE;
     }
-    if (p.dim != 3 || p.rho_type != 1) {
-      set_error("WCSPH: supported set is 3D with continuity density (dim %d, rho_type %d)", p.dim, p.rho_type);
+    if (p.rho_type != 1) {
+      set_error("WCSPH: supported set is continuity density (rho_type %d)", p.rho_type);
       return SOPHIA_E_ARG;
     }
     if (ns == 0) return 0;
+    if (p.dim == 2) return wcsph2d_stage(stage);
     DevParams P = devparams();
     CellView sc{scs, sce};
     switch (stage) {
@@ -1254,6 +1302,74 @@ struct Ctx : public sophia_ctx {
         if (p.xsph_solve) {
           TimeScope t(tm(), "xsph");
           TRY(sph_xsph_only<R>(stream, P, sv));
+        }
+        time += p.dt;
+        count += 1;
+        break;
+      default:
+        set_error("WCSPH stage %d out of range 0..8", stage);
+        return SOPHIA_E_ARG;
+    }
+    return 0;
+  }
+
+  // 2D WCSPH: SOPHIA_multi's stage list (WCSPH_Calc.cuh:368-563) in the
+  // numbering of the 3D stages; 1 and 5 have no 2D work
+  int wcsph2d_stage(int stage) {
+    DevParams P = devparams();
+    switch (stage) {
+      case 0:
+        // half step, then a fresh sort: the particles move
+        if (p.time_type == 1) {
+          TimeScope t(tm(), "predictor");
+          TRY(wc_predictor<R>(stream, P, sv));
+        }
+        TRY(nnps());
+        TRY(wc_reset_p3<R>(stream, sv));
+        after_corrector = false;
+        break;
+      case 1:
+        break;
+      case 2:
+        if (p.kgc_solve > 0 || p.delSPH_solve == 2) {
+          TimeScope t(tm(), "kgc");
+          TRY(wc2_kgc<R>(stream, P, sv));
+        }
+        {
+          TimeScope t(tm(), "prep");
+          TRY(wc2_prep<R>(stream, P, sv));
+        }
+        if (p.noslip_bc == 1) {
+          TimeScope t(tm(), "boundary");
+          TRY(wc2_boundary<R>(stream, P, sv));
+        }
+        break;
+      case 3: {
+        TimeScope t(tm(), "continuity");
+        TRY(wc2_continuity<R>(stream, P, sv));
+        break;
+      }
+      case 4:
+        TRY(wc_eos<R>(stream, P, sv));
+        break;
+      case 5:
+        break;
+      case 6: {
+        TimeScope t(tm(), "wc_interaction");
+        if (P.prop_table) TRY(sph_props<R>(stream, P, sv));
+        TRY(wc2_interaction<R>(stream, P, sv));
+        break;
+      }
+      case 7:
+        TRY(wc2_time_update<R>(stream, P, sv));
+        if (!p.xsph_solve) TRY(refresh_pos4<R>(stream, sv));
+        after_corrector = true;
+        break;
+      case 8:
+        if (p.xsph_solve) {
+          TimeScope t(tm(), "xsph");
+          TRY(wc2_xsph<R>(stream, P, sv));
+          TRY(refresh_pos4<R>(stream, sv));
         }
         time += p.dt;
         count += 1;
@@ -1550,6 +1666,7 @@ void sophia_params_default(sophia_params* p) {
   p->free_surface = 0;
   p->fs_threshold = 0.3;
   p->prop_table = 0;
+  p->noslip_bc = 0;
 }
 
 // read_solv_input (function_init.cuh:216-615): whitespace tokens; a key is
@@ -1614,6 +1731,11 @@ int sophia_read_solv(const char* path, sophia_params* p) {
     } else if (k == "delta-SPH") {
       std::string v = at(i + 2);
       p->delSPH_solve = v == "Molteni" ? 1 : (v == "Antuono" ? 2 : 0);
+    } else if (k == "noslip_boundary") {
+      // function_init.cuh:446-450: the value two words on
+      std::string v = at(i + 2);
+      if (v == "YES") p->noslip_bc = 1;
+      if (v == "NO") p->noslip_bc = 0;
     } else if (k == "reference-pressure") {
       p->p_ref = std::atof(at(i + 2).c_str());
     } else if (k == "sound-speed:") {
@@ -1899,6 +2021,43 @@ int sophia_create_from_arrays(const sophia_params* p, const part1* particles, in
     // the ISPH-DEM stages exist in 3D only (KERNEL_*3D_sph / _dem)
     set_error("create: the ISPH-DEM step is 3D only (dim = %d)", p->dim);
     return SOPHIA_E_ARG;
+  }
+  if (p->periodic_mask & 7) {
+    // periodic boxes: ISPH fluid only, in a fixed domain whose periodic axes
+    // share one period of at least three cells; the wall, sphere and WCSPH
+    // kernels do not wrap
+    const int m = p->periodic_mask & (p->dim == 2 ? 3 : 7);
+    double L = 0;
+    bool ok = p->domain_fixed && p->solver_type == 1 && p->dim == 3;
+    for (int a = 0; ok && a < 3; a++) {
+      const double s = p->domain_max[a] - p->domain_min[a];
+      if (!(s > 0)) ok = false;
+      if (m >> a & 1) {
+        if (L == 0) L = s;
+        if (std::fabs(s - L) > 1e-12 * L || s < 3.0 * 1.1 * p->kappa * particles[0].h) ok = false;
+      }
+    }
+    for (int a = 0; ok && a < 3; a++)
+      if (!(m >> a & 1) && p->domain_max[a] - p->domain_min[a] > L * (1000.0 / 1024.0)) ok = false;
+    if (!ok) {
+      set_error("create: periodic_mask needs the 3D ISPH step, a fixed domain, equal periods of >= 3 cells on "
+                "the periodic axes and no longer span on the others");
+      return SOPHIA_E_ARG;
+    }
+    for (int i = 0; i < n; i++)
+      if (particles[i].p_type <= 0 || particles[i].p_type > 1000) {
+        set_error("create: periodic boxes take fluid only (particle %d has p_type %d)", i, particles[i].p_type);
+        return SOPHIA_E_ARG;
+      }
+  }
+  if (p->solver_type == 0 && p->dim == 2) {
+    // every 2D DEM call of the WCSPH drivers is commented out
+    // (WCSPH_Calc.cuh:205, 224, 235, 503-535)
+    for (int i = 0; i < n; i++)
+      if (particles[i].p_type > 1000) {
+        set_error("create: 2D WCSPH takes no spheres (particle %d has p_type %d)", i, particles[i].p_type);
+        return SOPHIA_E_ARG;
+      }
   }
   if (p->kappa <= 0 || p->dt <= 0) {
     set_error("create: kappa and dt must be positive");

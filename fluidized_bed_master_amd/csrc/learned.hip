@@ -98,7 +98,9 @@ __device__ inline EdgeTile<R> edge_tile(const SphView<R>& v, const float2* nf, i
       if (!(ent & kNbrFlag)) {
         const int j = (int)ent;
         const double ih = 1.0 / v.h[i];
-        const double dx = (v.x[i] - v.x[j]) * ih, dy = (v.y[i] - v.y[j]) * ih, dz = (v.z[i] - v.z[j]) * ih;
+        double ex, ey, ez;
+        pdiff(v, v.x[i], v.y[i], v.z[i], j, ex, ey, ez);
+        const double dx = ex * ih, dy = ey * ih, dz = ez * ih;
         const float2 f = nf[j];
         ein[0] = (float)dx;
         ein[1] = (float)dy;
@@ -369,7 +371,7 @@ __global__ __launch_bounds__(64 * kFwdWaves, 4) void k_mlp_forward(Net net, SphV
           const int j = (int)ent;
           const float4 pj = v.pos4[j];
           const float2 f = nf[j];
-          const float3 d = rel_diff(pi, pj, v.pgrid);
+          const float3 d = rel_diff(pi, pj, v.pgrid, v.pmask);
           const float dx = d.x * ih, dy = d.y * ih, dz = d.z * ih;
           e8[0] = (__bf16)dx;
           e8[1] = (__bf16)dy;

@@ -31,10 +31,10 @@ template <bool REL, typename R>
 __device__ inline Nb<R> load_nb(const SphView<R>& v, int j, const NbOrigin& o) {
   Nb<R> q;
   if constexpr (REL && std::is_same<R, float>::value) {
-    const float3 d = rel_diff(o.p4, v.pos4[j], v.pgrid);
+    const float3 d = rel_diff(o.p4, v.pos4[j], v.pgrid, v.pmask);
     q.dx = d.x; q.dy = d.y; q.dz = d.z;
   } else {
-    q.dx = o.x - v.x[j]; q.dy = o.y - v.y[j]; q.dz = o.z - v.z[j];
+    pdiff(v, o.x, o.y, o.z, j, q.dx, q.dy, q.dz);
   }
   q.m = v.m[j]; q.rho = v.rho[j]; q.pres = v.pres[j]; q.h = v.h[j];
   q.ux = v.ux[j]; q.uy = v.uy[j]; q.uz = v.uz[j]; q.temp = v.temp[j]; q.rref = v.rho_ref[j];

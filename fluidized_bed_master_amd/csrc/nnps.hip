@@ -257,12 +257,14 @@ __global__ void k_neighbors(DevParams P, SphView<R> v, CellView cv, int* maxcnt,
       r[kNbrChunk - 1] = e;
     };
     for_cells(P, xi, yi, zi, cv, [&](int j) {
-      const float3 dd = rel_diff(pi, v.pos4[j], g);
+      const float3 dd = rel_diff(pi, v.pos4[j], g, v.pmask);
       const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
       if (d2 > hi2) return;
       uint32_t e = (uint32_t)j;
       if (!(d2 < lo2)) {
-        const double r = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]);
+        double ex, ey, ez;
+        pdiff(v, xi, yi, zi, j, ex, ey, ez);
+        const double r = dist_exact(ex, ey, ez);
         if (!(r < Rs)) return;
         if (!(r + 1e-20 < Rs)) e |= kNbrFlag;
       }

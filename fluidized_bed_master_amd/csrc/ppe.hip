@@ -418,10 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
       eb[u] = en[u];
       const int j = (int)(eb[u] & ~kNbrFlag);
       if constexpr (REL) {
-        const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid);
+        const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid, v.pmask);
         nb[u].x = d.x; nb[u].y = d.y; nb[u].z = d.z;
       } else {
-        nb[u].x = xi - v.x[j]; nb[u].y = yi - v.y[j]; nb[u].z = zi - v.z[j];
+        pdiff(v, xi, yi, zi, j, nb[u].x, nb[u].y, nb[u].z);
       }
       nb[u].m = v.m[j]; nb[u].rho = v.rho[j]; nb[u].por = v.por[j]; nb[u].pres = v.pres[j];
       nb[u].ux = v.ux[j]; nb[u].uy = v.uy[j]; nb[u].uz = v.uz[j];
@@ -1040,7 +1040,9 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
            w.diag, w.dinv, w.rhs, w.flags};
   const int g = grid_xcd(v.n);
   k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, v.surf, v.pres, w.row);
-  int fmt = P.z_index ? kFmtIdx32 : kFmtRun8;
+  // the x-runs of the stencil are contiguous only in row-major keys without
+  // periodic wrap
+  int fmt = (P.z_index || P.pmask) ? kFmtIdx32 : kFmtRun8;
   for (;;) {
     k_zero_u32<<<1, 64, 0, s>>>(w.flags, 4);
     if (fmt == kFmtRun8) {
@@ -1094,6 +1096,63 @@ int ppe_apply64(hipStream_t s, PpeWork& w, const double* x, double* y, bool resi
     TRY(read_state(s, w, &h));
     *norm2 = h.norm;
   }
+  return 0;
+}
+
+// v -= (mean of v over the rows): the projection that makes the right-hand
+// side of a periodic operator without walls (constant null space, A
+// symmetric without KGC) consistent, and fixes the level of its solution.
+// Grid-stride block partials of (sum, count), summed by one block in a
+// fixed order, then subtracted.
+__global__ __launch_bounds__(kBlock) void k_mean_part(int n, const int* row, const double* v, double* part) {
+  double a = 0, c = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (row[i]) {
+      a += v[i];
+      c += 1.0;
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    c += __shfl_xor(c, off);
+  }
+  __shared__ double sa[kBlock / 64], sc[kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    sa[wv] = a;
+    sc[wv] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0, tc = 0;
+    for (int k = 0; k < kBlock / 64; k++) {
+      ta += sa[k];
+      tc += sc[k];
+    }
+    part[2 * blockIdx.x] = ta;
+    part[2 * blockIdx.x + 1] = tc;
+  }
+}
+__global__ void k_mean_fin(const double* part, int nb, double* mean) {
+  if (threadIdx.x != 0) return;
+  double ta = 0, tc = 0;
+  for (int b = 0; b < nb; b++) {
+    ta += part[2 * b];
+    tc += part[2 * b + 1];
+  }
+  *mean = tc > 0 ? ta / tc : 0.0;
+}
+__global__ void k_mean_sub(int n, const int* row, double* v, const double* mean) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && row[i]) v[i] -= *mean;
+}
+int ppe_project_mean(hipStream_t s, PpeWork& w, double* v) {
+  if (w.n <= 0) return 0;
+  const int nb = std::min(grid_for(w.n), kMaxGrid);
+  k_mean_part<<<nb, kBlock, 0, s>>>(w.n, w.row, v, w.partials);
+  k_mean_fin<<<1, 64, 0, s>>>(w.partials, nb, w.partials + 2 * kMaxGrid);
+  k_mean_sub<<<grid_for(w.n), kBlock, 0, s>>>(w.n, w.row, v, w.partials + 2 * kMaxGrid);
+  HIP_TRY(hipGetLastError());
+  if (v == w.rhs) w.bn = -1;  // the cached norm belongs to the old rhs
   return 0;
 }
 
@@ -1295,6 +1354,7 @@ int ppe_solve(hipStream_t s, PpeWork& w, double* pres, const sophia_ppe_opts& o,
   st.iterations = total;
   st.outer = folds;
   st.converged = st.r_rel <= o.rtol * (1 + 1e-12);
+  if (o.zero_mean) TRY(ppe_project_mean(s, w, w.x64));  // the level of a singular system
   k_scatter_rows<<<g, kBlock, 0, s>>>(n, w.row, w.x64, pres);
   HIP_TRY(hipGetLastError());
   return 0;

@@ -101,5 +101,7 @@ int ppe_residual_of(hipStream_t s, PpeWork& w, const double* x, double* rel, dou
 int ppe_extrapolate(hipStream_t s, const PpeWork& w, const double* p, const double* h1, const double* h2, int depth,
                     double* out);
 int ppe_rhs_norm(hipStream_t s, PpeWork& w, double* bn);
+// v -= mean of v over the rows (zero-mean PPE of a periodic box)
+int ppe_project_mean(hipStream_t s, PpeWork& w, double* v);
 
 }  // namespace sophia

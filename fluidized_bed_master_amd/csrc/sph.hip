@@ -601,9 +601,17 @@ __global__ void k_corrector(DevParams P, SphView<R> v) {
   }
   const double el = (t > 0 && !P.eulerian) ? 1.0 : 0.0;
   if (!P.xsph_solve || t > 1000) {
-    v.x[i] = v.x[i] + ((double)v.ux0[i] + (double)uxc) / 2.0 * dt * el;
-    v.y[i] = v.y[i] + ((double)v.uy0[i] + (double)uyc) / 2.0 * dt * el;
-    v.z[i] = v.z[i] + ((double)v.uz0[i] + (double)uzc) / 2.0 * dt * el;
+    double xn = v.x[i] + ((double)v.ux0[i] + (double)uxc) / 2.0 * dt * el;
+    double yn = v.y[i] + ((double)v.uy0[i] + (double)uyc) / 2.0 * dt * el;
+    double zn = v.z[i] + ((double)v.uz0[i] + (double)uzc) / 2.0 * dt * el;
+    if (P.pmask) {
+      if (P.pmask & 1) xn = per_pos(xn, P.x_min, P.plen[0]);
+      if (P.pmask & 2) yn = per_pos(yn, P.y_min, P.plen[1]);
+      if (P.pmask & 4) zn = per_pos(zn, P.z_min, P.plen[2]);
+    }
+    v.x[i] = xn;
+    v.y[i] = yn;
+    v.z[i] = zn;
   }
   v.ux[i] = uxc;
   v.uy[i] = uyc;
@@ -658,9 +666,17 @@ __global__ void k_xsph_calc(DevParams P, SphView<R> v) {
   if (P.fix_xsph && pti > 0 && !P.eulerian) {
     // positions from the start-of-step position and the smoothed velocity
     const double dt = P.dt_sph;
-    v.x[i] += (double)v.xsx[i] * dt;
-    v.y[i] += (double)v.xsy[i] * dt;
-    v.z[i] += (double)v.xsz[i] * dt;
+    double xn = v.x[i] + (double)v.xsx[i] * dt;
+    double yn = v.y[i] + (double)v.xsy[i] * dt;
+    double zn = v.z[i] + (double)v.xsz[i] * dt;
+    if (P.pmask) {
+      if (P.pmask & 1) xn = per_pos(xn, P.x_min, P.plen[0]);
+      if (P.pmask & 2) yn = per_pos(yn, P.y_min, P.plen[1]);
+      if (P.pmask & 4) zn = per_pos(zn, P.z_min, P.plen[2]);
+    }
+    v.x[i] = xn;
+    v.y[i] = yn;
+    v.z[i] = zn;
   }
 }
 

@@ -153,13 +153,14 @@ __global__ __launch_bounds__(kBlock) void k_continuity(DevParams P, SphView<R> v
 }
 
 // KERNEL_EOS (functions_PROP.cuh:7-28): p = B ((rho / rho_ref)^gamma - 1)
-// with B = 1.186 c0^2 / gamma (the reference applies no rho0 factor)
+// with B = b c0^2 / gamma; the reference's b is 1.186 (an air density: it
+// applies no rho0 factor), eos_b_factor sets it
 template <typename R>
 __global__ void k_eos(DevParams P, SphView<R> v) {
   int i = particle_index();
   if (i >= v.n) return;
   if (v.itype[i] > 2 || v.ptype[i] > 1000 || v.btype[i] != 0) return;
-  const double B = P.soundspeed * P.soundspeed * (1.186) / P.gamma;
+  const double B = P.soundspeed * P.soundspeed * (P.eos_b) / P.gamma;
   v.pres[i] = B * (pow(v.rho[i] / v.rho_ref[i], P.gamma) - 1.0);
 }
 

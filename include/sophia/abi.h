@@ -201,7 +201,10 @@ typedef struct sophia_params {
    * to sophia_create or sophia_set_table.  Types without a table (spheres,
    * |p_type| >= 10) keep the constants. */
   int32_t prop_table;
-  int32_t reserved_i;
+  /* noslip_boundary(YES/NO) of solv.txt (function_init.cuh:446-450,
+   * vii[51]): 2D walls take the mirrored fluid velocity before the
+   * interaction (KERNEL_boundary2D, functions_BC.cuh:1234-1328) */
+  int32_t noslip_bc;
   double reserved[7];
 } sophia_params;
 

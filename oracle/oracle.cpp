@@ -71,7 +71,22 @@
  *                             on the wall, spin unchanged; resting contact:
  *                             normal velocity removed, friction opposes slip;
  *                             all three geometries
- *   parity unpinned           the WCSPH interaction's heat and virial
+ *   2D WCSPH                  KGC_2D: sum_j V_j x_ji (x) C grad W_ij = I for
+ *   (tests/test_wcsph2d_      every particle; prep2D: rho_ref = m/(h/h_over_dx)^2,
+ *    oracle.py)               Shepard sum = lattice quadrature, gradient of a
+ *                             linear density exact (stored with the
+ *                             reference's minus sign); continuity: -rho^2/rho_ref
+ *                             div u for linear u, Molteni = delta h c0 lap(rho)
+ *                             for a quadratic density, the reference's
+ *                             Antuono term = twice Molteni there; boundary2D:
+ *                             walls = minus a uniform fluid velocity;
+ *                             interaction2D: -G/rho in a linear pressure,
+ *                             Morris mu/rho lap(u), k lap(T)/(rho c_p m_i),
+ *                             gravity (0, -g), artificial viscosity off in
+ *                             expansion and dissipative; time update + XSPH:
+ *                             symplectic-Euler free fall, exact translation
+ *                             of a uniform flow
+ *   parity unpinned           the 3D WCSPH interaction's heat and virial
  *                             terms beyond the gravity-only and uniform-flow
  *                             checks
  */
@@ -115,6 +130,9 @@ struct PPERow {
 struct orc_state {
   sophia_params p;
   sophia_domain d;
+  // periodic axes (sophia_params.periodic_mask) and their periods
+  int pmask = 0;
+  double plen[3] = {0, 0, 0};
   Species sph, dem;
   int64_t count = 0;
   double time = 0.0;
@@ -349,6 +367,37 @@ static int idx_cell(const orc_state& s, int ix, int iy, int iz) {
   return (int)morton3((unsigned)ix, (unsigned)iy, (unsigned)iz);
 }
 
+/* periodic minimum image of the coordinate difference a - b along `axis`
+ * (no build of the reference is periodic: its KERNEL_periodic_3D,
+ * functions_BC.cuh:1522, is a velocity copy that is never called) */
+static inline double pd(const orc_state& s, double a, double b, int axis) {
+  double d = a - b;
+  if (s.pmask >> axis & 1) {
+    const double L = s.plen[axis];
+    if (d > 0.5 * L) d -= L;
+    else if (d < -0.5 * L) d += L;
+  }
+  return d;
+}
+/* cell index c along `axis` of N cells: wrapped on a periodic axis, else
+ * false outside 0..N-1 */
+static inline bool cell_in(const orc_state& s, int axis, int& c, int N) {
+  if (s.pmask >> axis & 1) {
+    if (c < 0) c += N;
+    else if (c > N - 1) c -= N;
+    return true;
+  }
+  return c >= 0 && c <= N - 1;
+}
+/* a position moved by one step back into [min, min + L) on a periodic axis */
+static inline double wrap_pos(const orc_state& s, double x, int axis) {
+  if (!(s.pmask >> axis & 1)) return x;
+  const double mn = axis == 0 ? s.d.x_min : axis == 1 ? s.d.y_min : s.d.z_min;
+  if (x >= mn + s.plen[axis]) return x - s.plen[axis];
+  if (x < mn) return x + s.plen[axis];
+  return x;
+}
+
 static void cell_of(const orc_state& s, double x, double y, double z, int& ic, int& jc, int& kc) {
   ic = axis_cell(x, s.d.x_min, s.d.x_max, s.d.NI);
   jc = axis_cell(y, s.d.y_min, s.d.y_max, s.d.NJ);
@@ -365,7 +414,7 @@ static void for_candidates(const orc_state& s, const Species& sp, double x, doub
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++) {
         int a = ic + dx, b = jc + dy, c = kc + dz;
-        if (a < 0 || a > s.d.NI - 1 || b < 0 || b > s.d.NJ - 1 || c < 0 || c > s.d.NK - 1) continue;
+        if (!cell_in(s, 0, a, s.d.NI) || !cell_in(s, 1, b, s.d.NJ) || !cell_in(s, 2, c, s.d.NK)) continue;
         int k = idx_cell(s, a, b, c);
         if (sp.g_str[k] == kEmpty) continue;
         uint32_t e = sp.g_end[k];
@@ -429,6 +478,16 @@ static void setup_domain(orc_state& s, const part1* P, int n) {
   d.NI = (int)((d.x_max - d.x_min) / d.dcell) + 1;
   d.NJ = (int)((d.y_max - d.y_min) / d.dcell) + 1;
   d.NK = (int)((d.z_max - d.z_min) / d.dcell) + 1;
+  // periodic axes (fixed domain): whole cells of width >= dcell tile the
+  // period, so the 27-cell stencil with wrapped indices covers kappa h
+  s.pmask = p.domain_fixed ? (p.periodic_mask & (p.dim == 2 ? 3 : 7)) : 0;
+  int* N[3] = {&d.NI, &d.NJ, &d.NK};
+  const double span[3] = {d.x_max - d.x_min, d.y_max - d.y_min, d.z_max - d.z_min};
+  for (int a = 0; a < 3; a++)
+    if (s.pmask >> a & 1) {
+      s.plen[a] = span[a];
+      *N[a] = std::max(1, (int)(span[a] / d.dcell));
+    }
   if (p.flag_z_index == 0) {
     d.num_cells = d.NI * d.NJ * d.NK + 1;
   } else {
@@ -603,7 +662,7 @@ static void kgc_3d(orc_state& s) {
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (qj.p_type > 1000) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
       double r = dist_of(dx, dy, dz);
       if (!(r > 0 && r < R)) return;
       double dw = knl_dw(kt, A, h, r);
@@ -663,7 +722,7 @@ static void coupling_prep_sph(orc_state& s) {
     double flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
-      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z) + 1e-20;
+      double r = dist_of(pd(s, xi, qj.x, 0), pd(s, yi, qj.y, 1), pd(s, zi, qj.z, 2)) + 1e-20;
       if (!(r < R)) return;
       double w = knl_w(kt, A, h, r);
       if (qi.p_type <= 1000 && qj.p_type <= 1000) flt += qj.m / qj.rho * w * (qj.p_type == 1);
@@ -671,7 +730,7 @@ static void coupling_prep_sph(orc_state& s) {
     if (dm.n() > 0) {
       for_candidates(s, dm, xi, yi, zi, [&](int j) {
         const part1& qj = dm.SP1[j];
-        double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+        double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
         double r = dist_of(dx, dy, dz) + 1e-20;
         if (!(r < R)) return;
         double w = knl_w(kt, A, h, r);
@@ -778,7 +837,7 @@ static void prep3d_sph(orc_state& s) {
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (qj.p_type > 1000) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
       double r = dist_of(dx, dy, dz) + 1e-20;
       if (!(r < R)) return;
       double w = knl_w(kt, A, h, r), dw = knl_dw(kt, A, h, r);
@@ -830,7 +889,7 @@ static void flux_walls(orc_state& s) {
     for_candidates(s, sp, qi.x, qi.y, qi.z, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!flux_receiver(p, qj)) return;
-      double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z);
+      double r = dist_of(pd(s, qi.x, qj.x, 0), pd(s, qi.y, qj.y, 1), pd(s, qi.z, qj.z, 2));
       if (!(r < R)) return;
       S += qj.m / qj.rho * knl_w(kt, A, hi, r);
     });
@@ -883,7 +942,7 @@ void orc_interaction(orc_state* s) {
     for_candidates(*s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type < 1000)) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(*s, xi, qj.x, 0), dy = pd(*s, yi, qj.y, 1), dz = pd(*s, zi, qj.z, 2);
       if (flux && s->flux_S[j] > 0.0) {
         // wall heat flux: q A_w W_iw / S_w (the wall's face area from its own h)
         double rw = dist_of(dx, dy, dz);
@@ -1404,7 +1463,7 @@ static void corrector(orc_state& s, Species& sp, double dt, bool is_sph) {
     double yc = q.y + (q2.uy0 + uyc) / 2.0 * dt * (t > 0) * q.eliy;
     double zc = q.z + (q2.uz0 + uzc) / 2.0 * dt * (t > 0) * q.eliz;
     if (!p.xsph_solve || t > 1000) {
-      T.x = xc; T.y = yc; T.z = zc;
+      T.x = wrap_pos(s, xc, 0); T.y = wrap_pos(s, yc, 1); T.z = wrap_pos(s, zc, 2);
     }
     T.ux = uxc; T.uy = uyc; T.uz = uzc;
     if (!is_sph && t > 1000 && !(p.reference_compat & SOPHIA_COMPAT_NO_SPIN)) {
@@ -1665,7 +1724,7 @@ static void neumann_walls(orc_state& s) {
     double tp = 0, fl = 0, tt = 0, tr = 0;
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
-      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z) + 1e-20;
+      double r = dist_of(pd(s, xi, qj.x, 0), pd(s, yi, qj.y, 1), pd(s, zi, qj.z, 2)) + 1e-20;
       if (!(qj.p_type < 1000)) return;
       if (!(r < 2.0 * R)) return;
       double w = knl_w(kt, A, 2.0 * h, r);
@@ -1737,7 +1796,7 @@ static void surface_detect(orc_state& s) {
     int non = 0;
     for_candidates(s, sp, qi.x, qi.y, qi.z, [&](int j) {
       const part1& qj = sp.SP1[j];
-      double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+      double dx = pd(s, qi.x, qj.x, 0), dy = pd(s, qi.y, qj.y, 1), dz = pd(s, qi.z, qj.z, 2);
       double r = std::sqrt(dx * dx + dy * dy + dz * dz);
       if (!(r < R)) return;
       bool same = p.free_surface == 2 ? qj.p_type == qi.p_type : qj.p_type < 1000;
@@ -1781,7 +1840,7 @@ static void ppe_assemble(orc_state& s, double tdt) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type < 1000)) return;
       if (qj.i_type == 4) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
       double r = dist_of(dx, dy, dz) + 1e-20;
       if (!(r < R)) return;
       double dw = knl_dw(kt, A, hi, r);
@@ -1929,7 +1988,7 @@ static void pressure_force(orc_state& s) {
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type < 1000)) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
       double r = dist_of(dx, dy, dz) + 1e-20;
       if (!(r < R)) return;
       double dw = knl_dw(kt, A, hi, r);
@@ -2011,7 +2070,7 @@ static void xsph_sph(orc_state* s, bool apply) {
     for_candidates(*s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.P1[j];
       if (!(qj.p_type == 1 || qj.p_type == 3)) return;
-      double r = dist_of(xi - qj.x, yi - qj.y, zi - qj.z);
+      double r = dist_of(pd(*s, xi, qj.x, 0), pd(*s, yi, qj.y, 1), pd(*s, zi, qj.z, 2));
       if (!(r < R)) return;
       double w = knl_w(kt, A, h, r);
       double v = qj.m / qj.rho;
@@ -2036,9 +2095,9 @@ static void xsph_sph(orc_state* s, bool apply) {
     q.ux = vx[i]; q.uy = vy[i]; q.uz = vz[i]; q.temp = vt[i];
     if (p.fix_xsph_position && q.p_type > 0 && !p.eulerian_sph) {
       const part2& q2 = sp.SP2[i];
-      q.x = q2.x0 + vx[i] * dt;
-      q.y = q2.y0 + vy[i] * dt;
-      q.z = q2.z0 + vz[i] * dt;
+      q.x = wrap_pos(*s, q2.x0 + vx[i] * dt, 0);
+      q.y = wrap_pos(*s, q2.y0 + vy[i] * dt, 1);
+      q.z = wrap_pos(*s, q2.z0 + vz[i] * dt, 2);
     }
   }
 }
@@ -2134,7 +2193,7 @@ static void wc_volume_update(orc_state& s) {
     for_candidates_r(s, sp, qi.x, qi.y, qi.z, 3, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type < 1000) || qj.i_type == 4) return;
-      double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z) + 1e-20;
+      double r = dist_of(pd(s, qi.x, qj.x, 0), pd(s, qi.y, qj.y, 1), pd(s, qi.z, qj.z, 2)) + 1e-20;
       if (!(r < R)) return;
       double w = knl_w(kt, A, h, r);
       tR += w;
@@ -2177,7 +2236,7 @@ static void wc_continuity(orc_state& s) {
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type <= 1000)) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
       double r = dist_of(dx, dy, dz) + 1e-20;
       if (!(r < R)) return;
       double dw = knl_dw(kt, A, h, r);
@@ -2193,9 +2252,9 @@ static void wc_continuity(orc_state& s) {
       pz += uzi * mj / rhoj * (rhoi / pori - rhoj / porj) * gz;
       if (p.delSPH_solve > 0) {
         double rri = sp.SP2[i].rho_ref, rrj = sp.SP2[j].rho_ref;
-        double phi = (gxi + qj.grad_rhox) * (qj.x - xi);
-        phi += (gyi + qj.grad_rhoy) * (qj.y - yi);
-        phi += (gzi + qj.grad_rhoz) * (qj.z - zi);
+        double phi = (gxi + qj.grad_rhox) * (-pd(s, xi, qj.x, 0));
+        phi += (gyi + qj.grad_rhoy) * (-pd(s, yi, qj.y, 1));
+        phi += (gzi + qj.grad_rhoz) * (-pd(s, zi, qj.z, 2));
         phi = (-0.5 * phi) * (p.delSPH_solve == 2) + (rhoj / rrj - rhoi / rri);
         double tr = -2 * p.delta_sph * h * rri * p.soundspeed * (mj / rhoj);
         tr *= phi * dw / r;
@@ -2206,11 +2265,12 @@ static void wc_continuity(orc_state& s) {
   }
 }
 
-/* KERNEL_EOS (functions_PROP.cuh:7-28): Tait, B = 1.186 c0^2 / gamma */
+/* KERNEL_EOS (functions_PROP.cuh:7-28): Tait, B = b c0^2 / gamma with the
+ * reference's b = 1.186 unless eos_b_factor says otherwise */
 static void wc_eos(orc_state& s) {
   Species& sp = s.sph;
   const sophia_params& p = s.p;
-  const double B = p.soundspeed * p.soundspeed * (1.186) / p.gamma;
+  const double B = p.soundspeed * p.soundspeed * (p.eos_b_factor) / p.gamma;
   for (int i = 0; i < sp.n(); i++) {
     part1& q = sp.SP1[i];
     if (q.i_type > kITypeCrt || q.p_type > 1000 || q.buffer_type != 0) continue;
@@ -2247,7 +2307,7 @@ static void wc_interaction(orc_state& s) {
     for_candidates(s, sp, xi, yi, zi, [&](int j) {
       const part1& qj = sp.SP1[j];
       if (!(qj.p_type <= 1000)) return;
-      double dx = xi - qj.x, dy = yi - qj.y, dz = zi - qj.z;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1), dz = pd(s, zi, qj.z, 2);
       double r = dist_of(dx, dy, dz) + 1e-20;
       if (!(r < R)) return;
       double dw = knl_dw(kt, A, h, r);
@@ -2454,8 +2514,438 @@ static void wc_dem_corrector(orc_state& s) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* 2D WCSPH.  SOPHIA_single comments out KERNEL_interaction2D               */
+/* (WCSPH_Calc.cuh:187), so the 2D step follows SOPHIA_multi                */
+/* (WCSPH_Calc.cuh:368-563) on one GPU's share: predictor, NNPS every step  */
+/* (no KERNEL_set_alpha: Lagrangian), KGC, prep2D, prep2D_coupling,         */
+/* boundary2D, continuity, EOS, interaction2D, KERNEL_time_update, xsph2D.  */
+/* Spheres are refused in 2D (every 2D DEM call is commented out there).    */
+/* ------------------------------------------------------------------------ */
+static void wc2_set_lagrangian(std::vector<part1>& P) {
+  for (part1& q : P)
+    if (q.i_type != 3) q.elix = q.eliy = q.eliz = 1.0;
+}
+
+/* KERNEL_clc_correction_KGC_2D (functions_PREP2.cuh:73-159): inverse of the
+ * 2x2 moment matrix, identity when |det| <= Min_det; the third row and
+ * column stay 0 (P3 is reset every step) */
+static void kgc_2d(orc_state& s) {
+  Species& sp = s.sph;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double xx = 0, yy = 0, xy = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (qj.p_type > 1000) return;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1);
+      double r = std::sqrt(dx * dx + dy * dy);
+      if (!(r > 0 && r < R)) return;
+      double dw = knl_dw(kt, A, h, r);
+      double mtd = qj.m * dw;
+      double rtd = 1.0 / (qj.rho * r);
+      double t;
+      t = -mtd * dx * dx; t *= rtd; xx += t;
+      t = -mtd * dy * dx; t *= rtd; xy += t;
+      t = -mtd * dy * dy; t *= rtd; yy += t;
+    });
+    double det = xx * yy - xy * xy;
+    double (*C)[3] = sp.P3[i].Cm;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) C[a][b] = 0.0;
+    if (std::abs(det) > kMinDet) {
+      double r = 1.0 / det;
+      C[0][0] = yy * r;
+      C[0][1] = -xy * r;
+      C[1][0] = -xy * r;
+      C[1][1] = xx * r;
+    } else {
+      C[0][0] = 1;
+      C[1][1] = 1;
+    }
+  }
+}
+
+/* KERNEL_clc_prep2D (functions_PREP2.cuh:1247-1470): Shepard sum every
+ * freq_filt steps, reference density m / (h / 1.6)^2 every step, and the
+ * Antuono density gradient -(rho_j - rho_i) m_j / rho_j grad~W (rho, not
+ * rho / rho_ref as in 3D); turbulence, p_type switching and the surface-
+ * tension normals are off in this build */
+static void prep2d(orc_state& s) {
+  Species& sp = s.sph;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+  bool filt = (s.count % (int64_t)std::max(1, s.p.freq_filt)) == 0;
+  std::vector<double> fltv(sp.n()), gx(sp.n()), gy(sp.n());
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z, rhoi = qi.rho;
+    double flt = 0, rx = 0, ry = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (qj.p_type > 1000) return;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1);
+      double r = std::sqrt(dx * dx + dy * dy) + 1e-20;
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r), dw = knl_dw(kt, A, h, r);
+      double tx = dw * dx / r, ty = dw * dy / r, tz = 0.0;
+      if (filt) flt += qj.m / qj.rho * w;
+      if (s.p.delSPH_solve == 2) {
+        kgc_apply(s, sp.P3[i].Cm, tx, ty, tz);
+        rx += -(qj.rho - rhoi) * (qj.m / qj.rho) * tx;
+        ry += -(qj.rho - rhoi) * (qj.m / qj.rho) * ty;
+      }
+    });
+    fltv[i] = flt; gx[i] = rx; gy[i] = ry;
+  }
+  for (int i = 0; i < sp.n(); i++) {
+    part1& q = sp.SP1[i];
+    if (q.i_type > kITypeCrt || q.p_type > 1000) continue;
+    double hv = q.h / s.p.h_over_dx;
+    sp.SP2[i].rho_ref = q.m / (hv * hv);
+    if (filt) q.flt_s = fltv[i];
+    if (s.p.delSPH_solve == 2) {
+      q.grad_rhox = gx[i];
+      q.grad_rhoy = gy[i];
+    }
+  }
+}
+
+/* KERNEL_clc_prep2D_coupling (functions_PREP2.cuh:2944-3046): porosity
+ * 1 - sum over spheres of W pi r^2; with no spheres in 2D it is 1 */
+static void prep2d_coupling(orc_state& s) {
+  for (part1& q : s.sph.SP1)
+    if (q.i_type <= kITypeCrt) q.DEMpor = 1.0;
+}
+
+/* KERNEL_boundary2D (functions_BC.cuh:1234-1328), noslip_bc = 1: walls
+ * (p_type 0) and moving walls (9) take 2 u_i [moving] minus the Shepard
+ * average of the other particles' velocity (every p_type but 0, as the
+ * reference's unsigned compare reads it).  All reads see the velocities
+ * before the update. */
+static void boundary2d(orc_state& s) {
+  Species& sp = s.sph;
+  const int kt = s.p.kernel_type, dim = s.p.dim;
+  const int n = sp.n();
+  std::vector<double> nx(n), ny(n);
+  std::vector<char> act(n, 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    const uint32_t pti = (uint32_t)qi.p_type;
+    if (pti > 0 && pti != (uint32_t)kMoving) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = s.p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double tx = 0, ty = 0, fl = 1.0e-10;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (qj.p_type > 1000) return;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1);
+      double r = std::sqrt(dx * dx + dy * dy) + 1e-20;
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r);
+      if ((uint32_t)qj.p_type > 0) {
+        tx += qj.ux * w;
+        ty += qj.uy * w;
+        fl += w;
+      }
+    });
+    nx[i] = 2 * qi.ux * (pti == (uint32_t)kMoving) - tx / fl;
+    ny[i] = 2 * qi.uy * (pti == (uint32_t)kMoving) - ty / fl;
+    act[i] = 1;
+  }
+  for (int i = 0; i < n; i++)
+    if (act[i]) {
+      sp.SP1[i].ux = nx[i];
+      sp.SP1[i].uy = ny[i];
+    }
+}
+
+/* KERNEL_clc_continuity_norm2D (functions_MASS.cuh:670-769): walls too,
+ * velocities of walls taken as 0, the divergence scaled by rho_i / rho_ref_i,
+ * delta-SPH on rho_j - rho_i */
+static void continuity2d(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type > kITypeCrt || qi.p_type > 1000) continue;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double uxi = qi.ux * (qi.p_type > 0), uyi = qi.uy * (qi.p_type > 0);
+    double rhoi = qi.rho, gxi = qi.grad_rhox, gyi = qi.grad_rhoy;
+    double tx = 0, ty = 0, tR = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type <= 1000)) return;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1);
+      double r = std::sqrt(dx * dx + dy * dy) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, h, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = 0.0;
+      if (p.kgc_solve > 0) kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
+      double mj = qj.m, rhoj = qj.rho;
+      double uxj = qj.ux * (qj.p_type > 0), uyj = qj.uy * (qj.p_type > 0);
+      double tmprho = mj * (rhoi / rhoj);
+      tx += (uxi - uxj) * tmprho * gx;
+      ty += (uyi - uyj) * tmprho * gy;
+      if (p.delSPH_solve > 0) {
+        double phi = (gxi + qj.grad_rhox) * (-pd(s, xi, qj.x, 0));
+        phi += (gyi + qj.grad_rhoy) * (-pd(s, yi, qj.y, 1));
+        phi = (-0.5 * phi) * (p.delSPH_solve == 2) + (rhoj - rhoi);
+        double tr = -2 * p.delta_sph * h * p.soundspeed * (mj / rhoj);
+        tr *= phi * dw / r;
+        tR += tr;
+      }
+    });
+    sp.P3[i].drho = (tx + ty) * (rhoi / sp.SP2[i].rho_ref) + tR * (p.delSPH_solve > 0);
+  }
+}
+
+/* KERNEL_interaction2D (functions_INTERACTION.cuh:64-411), i_type 1:
+ * pressure m_j (p_i + p_j) / (eps_i rho_i eps_j rho_j), laminar viscosity
+ * (no eps), artificial viscosity, gravity on y, conduction (divided by m_i
+ * as the reference does); interface, repulsive, surface-tension, Boussinesq
+ * and concentration terms are off in this build */
+static void interaction2d(orc_state& s) {
+  const sophia_table* tab = s.has_table ? &s.table : g_no_table;
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < sp.n(); i++) {
+    part1& qi = sp.SP1[i];
+    if (qi.i_type != 1 || qi.p_type > 1000) continue;
+    const int pti = qi.p_type;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z, uxi = qi.ux, uyi = qi.uy;
+    double pi = qi.pres, mi = qi.m, rhoi = qi.rho, pori = qi.DEMpor, tempi = qi.temp;
+    double visi = viscosity(p, tab, pti, tempi) + sp.P3[i].vis_t;
+    double kci = 0, cpi = 1, eta = 0;
+    if (p.con_solve) {
+      kci = conductivity(p, tab, pti, tempi);
+      cpi = heat_capacity(p, pti);
+      eta = 0.001 * h;
+    }
+    double tx = 0, ty = 0, Rc = 0, pgx = 0, pgy = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (!(qj.p_type <= 1000)) return;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1);
+      double r = std::sqrt(dx * dx + dy * dy) + 1e-20;
+      if (!(r < R)) return;
+      double dw = knl_dw(kt, A, h, r);
+      double gx = dw * dx / r, gy = dw * dy / r, gz = 0.0;
+      if (p.kgc_solve > 0) kgc_apply(s, sp.P3[i].Cm, gx, gy, gz);
+      const int ptj = qj.p_type;
+      double uxj = qj.ux, uyj = qj.uy, mj = qj.m, tempj = qj.temp, rhoj = qj.rho;
+      double pj = qj.pres, hj = qj.h, porj = qj.DEMpor;
+      if (p.fp_solve) {
+        double pp = p.pgrad_form ? pj - pi : pi + pj;
+        double Cp = -mj * pp / (pori * rhoi * porj * rhoj);
+        double tpx = Cp * gx, tpy = Cp * gy;
+        tx += tpx;
+        ty += tpy;
+        pgx += rhoi * pori * tpx;
+        pgy += rhoi * pori * tpy;
+      }
+      if (p.fv_solve) {
+        double visj = viscosity(p, tab, ptj, tempj) + sp.P3[j].vis_t;
+        double Cv = dx * gx + dy * gy;
+        Cv *= 4 * (mj / (rhoi * rhoj));
+        Cv *= (visi * visj) / (visi + visj);
+        Cv /= r;
+        Cv /= r;
+        tx += Cv * (uxi - uxj);
+        ty += Cv * (uyi - uyj);
+      }
+      if (p.fva_solve) {
+        double uij_xij = (uxi - uxj) * dx + (uyi - uyj) * dy;
+        if (uij_xij < 0) {
+          double hij = (h + hj) * 0.5;
+          double phi = hij * uij_xij;
+          phi /= (r * r + 0.01 * hij * hij);
+          double Pij = mi * phi * (-0.1 * p.soundspeed + 0.02 * phi);
+          Pij /= (rhoi + rhoj);
+          Pij *= 0.5;
+          tx += -(Pij)*gx;
+          ty += -(Pij)*gy;
+        }
+      }
+      if (p.con_solve) {
+        double kcj = conductivity(p, tab, ptj, tempj);
+        double H = 4.0 * mj * kcj * kci * (tempi - tempj) * dw;
+        H /= mi * cpi * (r + eta * eta) * rhoi * rhoj * (kci + kcj);
+        Rc += H;
+      }
+    });
+    if (p.fg_solve) ty += -p.gravity;
+    part3& o = sp.P3[i];
+    o.ftotalx = tx;
+    o.ftotaly = ty;
+    o.ftotalz = 0;
+    o.ftotal = std::sqrt(tx * tx + ty * ty);
+    qi.pgf_x = pgx;
+    qi.pgf_y = pgy;
+    if (p.con_solve) o.dtemp = Rc;
+  }
+}
+
+/* KERNEL_time_update (functions_TIME.cuh:523-675), i_type 1: full step from
+ * the start-of-step state; walls and other p_type <= 0 get u = 0, density
+ * rho0 + drho dt for every p_type, temperature kept (the reference's
+ * enthalpy path carries the heat); x written here only without XSPH.
+ * Fields are carried from SP1 (this build keeps one copy of each field; the
+ * reference's TP1 keeps its pre-boundary velocity for p_type 9). */
+static void time_update2d(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const double dt = p.dt;
+  for (int i = 0; i < sp.n(); i++) {
+    const part1& q = sp.SP1[i];
+    const part2& q2 = sp.SP2[i];
+    const part3& o = sp.P3[i];
+    part1& T = sp.P1[i];
+    T = q;
+    if (q.i_type != 1) continue;
+    const int t = q.p_type;
+    if (t == kMoving) {
+      T.x = q2.x0 + q.ux * dt;
+      T.y = q2.y0 + q.uy * dt;
+      T.z = q2.z0 + q.uz * dt;
+    } else {
+      double ux0 = 0, uy0 = 0, uz0 = 0, ax = 0, ay = 0, az = 0;
+      if (t > 0) {
+        ux0 = q2.ux0; uy0 = q2.uy0; uz0 = q2.uz0;
+        ax = o.ftotalx; ay = o.ftotaly; az = o.ftotalz;
+      }
+      double uxc = ux0 + ax * dt, uyc = uy0 + ay * dt, uzc = uz0 + az * dt;
+      if (uxc * uxc + uyc * uyc + uzc * uzc >= p.u_limit * p.u_limit) {
+        uxc = ux0;
+        uyc = uy0;
+        uzc = uz0;
+      }
+      if (!p.xsph_solve) {
+        T.x = q2.x0 + uxc * dt;
+        T.y = q2.y0 + uyc * dt;
+        T.z = q2.z0 + uzc * dt;
+      }
+      T.ux = uxc;
+      T.uy = uyc;
+      T.uz = uzc;
+    }
+    if (p.rho_type == 1) T.rho = q2.rho0 + o.drho * dt;
+  }
+}
+
+/* KERNEL_xsph2D (functions_TIME.cuh:1520-1607): x = x0 + (u + c sum_j
+ * m_j / rho_j (u_j - u_i) W / flt_s_i) dt for fluid, x0 for walls, with
+ * the corrected velocities (P1) and the predicted positions, masses and
+ * densities (SP1); the velocity itself is not smoothed */
+static void xsph2d(orc_state& s) {
+  Species& sp = s.sph;
+  const sophia_params& p = s.p;
+  const int kt = p.kernel_type, dim = p.dim;
+  const double dt = p.dt;
+  const int n = sp.n();
+  std::vector<double> nx(n), ny(n);
+  std::vector<char> act(n, 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const part1& qi = sp.SP1[i];
+    if (qi.i_type != 1 || qi.p_type > 1000) continue;
+    const int pti = qi.p_type;
+    double h = qi.h, A = knl_tmpA(kt, dim, h), R = p.kappa * h;
+    double xi = qi.x, yi = qi.y, zi = qi.z;
+    double uxi = sp.P1[i].ux, uyi = sp.P1[i].uy, fl = qi.flt_s;
+    double tx = 0, ty = 0;
+    for_candidates(s, sp, xi, yi, zi, [&](int j) {
+      const part1& qj = sp.SP1[j];
+      if (qj.p_type > 1000) return;
+      double dx = pd(s, xi, qj.x, 0), dy = pd(s, yi, qj.y, 1);
+      double r = std::sqrt(dx * dx + dy * dy);
+      if (!(r < R)) return;
+      double w = knl_w(kt, A, h, r);
+      double uxj = sp.P1[j].ux, uyj = sp.P1[j].uy;
+      tx += p.c_xsph * qj.m / qj.rho * (-uxi + uxj) * w / fl;
+      ty += p.c_xsph * qj.m / qj.rho * (-uyi + uyj) * w / fl;
+    });
+    const part2& q2 = sp.SP2[i];
+    if (pti == kMoving) {
+      nx[i] = q2.x0 + uxi * dt;
+      ny[i] = q2.y0 + uyi * dt;
+    } else {
+      nx[i] = q2.x0 + (uxi + tx) * dt * (pti > 0);
+      ny[i] = q2.y0 + (uyi + ty) * dt * (pti > 0);
+    }
+    act[i] = 1;
+  }
+  for (int i = 0; i < n; i++)
+    if (act[i]) {
+      sp.P1[i].x = nx[i];
+      sp.P1[i].y = ny[i];
+    }
+}
+
 static int wc_check(const orc_state& s) {
-  if (s.p.dim != 3 || s.p.rho_type != 1) return -1;
+  if (s.p.rho_type != 1) return -1;
+  if (s.p.dim == 2) return s.dem.n() == 0 ? 0 : -1;
+  if (s.p.dim != 3) return -1;
+  return 0;
+}
+
+static int wc2_stage(orc_state* s, int32_t stage) {
+  Species& sp = s->sph;
+  switch (stage) {
+    case 0:
+      wc2_set_lagrangian(sp.P1);
+      if (s->p.time_type == 1) wc_predictor(*s);
+      nnps_species(*s, sp, true);
+      wc2_set_lagrangian(sp.SP1);
+      wc2_set_lagrangian(sp.P1);
+      s->after_corrector = false;
+      break;
+    case 1:
+      break;  // SOPHIA_multi has no volume or mass update
+    case 2:
+      if (s->p.kgc_solve > 0 || s->p.delSPH_solve == 2) kgc_2d(*s);
+      prep2d(*s);
+      prep2d_coupling(*s);
+      if (s->p.noslip_bc == 1) boundary2d(*s);
+      break;
+    case 3:
+      continuity2d(*s);
+      break;
+    case 4:
+      wc_eos(*s);
+      break;
+    case 5:
+      break;  // no Neumann wall extrapolation in 2D
+    case 6:
+      interaction2d(*s);
+      break;
+    case 7:
+      time_update2d(*s);
+      s->after_corrector = true;
+      break;
+    case 8:
+      if (s->p.xsph_solve) xsph2d(*s);
+      s->time += s->p.dt;
+      s->count += 1;
+      break;
+    default:
+      return -1;
+  }
   return 0;
 }
 
@@ -2463,6 +2953,7 @@ static int wc_check(const orc_state& s) {
  * the P1 -> SP1 copy (WCSPH_Calc.cuh:34-69) */
 int orc_wcsph_stage(orc_state* s, int32_t stage) {
   if (wc_check(*s)) return -1;
+  if (s->p.dim == 2) return wc2_stage(s, stage);
   Species& sp = s->sph;
   switch (stage) {
     case 0:
@@ -2578,7 +3069,7 @@ void orc_get_nnps(const orc_state* s, int32_t species, int32_t* keys, int32_t* o
       int c = 0;
       for_candidates(*s, sp, qi.x, qi.y, qi.z, [&](int j) {
         const part1& qj = sp.SP1[j];
-        double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z) + 1e-20;
+        double r = dist_of(pd(*s, qi.x, qj.x, 0), pd(*s, qi.y, qj.y, 1), pd(*s, qi.z, qj.z, 2)) + 1e-20;
         if (r < R) c++;
       });
       nbr_count[i] = c;
@@ -2655,7 +3146,7 @@ void orc_brute_neighbor_counts(const orc_state* s, int32_t* counts) {
     int c = 0;
     for (int j = 0; j < n; j++) {
       const part1& qj = sp.SP1[j];
-      double r = dist_of(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z) + 1e-20;
+      double r = dist_of(pd(*s, qi.x, qj.x, 0), pd(*s, qi.y, qj.y, 1), pd(*s, qi.z, qj.z, 2)) + 1e-20;
       if (r < R) c++;
     }
     counts[i] = c;
