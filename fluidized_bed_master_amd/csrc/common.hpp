@@ -535,6 +535,28 @@ __host__ __device__ inline T knl_dw(int kt, T tA, T tH, T rr) {
   return dw;
 }
 
+// knl_w / knl_dw with 1 / h supplied by the caller (hoisted out of the pair
+// loop): the Wendland C2 branch multiplies where knl_* divides by h, which
+// moves the result by at most an ulp; other kernels take the general path
+template <typename T>
+__host__ __device__ inline T knl_w_ih(int kt, T tA, T tH, T ih, T rr) {
+  if (kt == SOPHIA_KNL_WENDLAND2) {
+    const T tR = rr * ih * (T)0.5;
+    const T o = 1 - tR;
+    return tR < 1 ? tA * o * o * o * o * (1 + 4 * tR) : (T)0;
+  }
+  return knl_w<T>(kt, tA, tH, rr);
+}
+template <typename T>
+__host__ __device__ inline T knl_dw_ih(int kt, T tA, T tH, T ih, T rr) {
+  if (kt == SOPHIA_KNL_WENDLAND2) {
+    const T tR = rr * ih * (T)0.5;
+    const T o = 1 - tR;
+    return tR < 1 ? (ih * (T)0.5) * tA * ((T)(-20) * tR * o * o * o) : (T)0;
+  }
+  return knl_dw<T>(kt, tA, tH, rr);
+}
+
 // KGC-corrected gradient with the symmetric inverse stored as 6 entries
 template <typename R, typename T>
 __device__ inline void kgc_apply(const DevParams& P, const R* cm, int n, int i, T& gx, T& gy, T& gz) {

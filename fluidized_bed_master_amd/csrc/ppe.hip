@@ -356,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
       }
     }
   }
-  const double hd = v.h[i];
+  const double hd = v.h[i], ihd = 1.0 / hd;
   const double A = knl_tmpA<double>(P.kernel_type, P.dim, hd);
   // geometry: FP64 positions, or with FP32 fields the pos4 records (one 16 B
   // gather per neighbour, difference error below 2^-21 of the grid spacing)
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   const double xi = REL ? 0.0 : v.x[i], yi = REL ? 0.0 : v.y[i], zi = REL ? 0.0 : v.z[i];
   const float4 pi4 = REL ? v.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const double uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i];
-  const double rhoi = v.rho[i], pori = v.por[i];
+  const double rhoi = v.rho[i], pori = v.por[i], irhoi = 1.0 / rhoi;
   double bx = 0, by = 0, bz = 0, Aij = 0, AP = 0, lag = 0;
   int kc = 0, run = 0;
   uint64_t rcw = 0;
@@ -438,15 +438,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     if (!(nq.pt < 1000 && nq.it != 4)) continue;
     double dx = nq.x, dy = nq.y, dz = nq.z;
     double r = sqrt(dx * dx + dy * dy + dz * dz) + 1e-20;
-    double dw = knl_dw<double>(P.kernel_type, A, hd, r);
-    double gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+    // two reciprocals per pair (1/r, V_j = m_j / rho_j) in place of the
+    // reference expressions' divisions; the rounding moves by an ulp
+    const double ir = 1.0 / r;
+    double dw = knl_dw_ih<double>(P.kernel_type, A, hd, ihd, r);
+    const double sd = dw * ir;
+    double gx = sd * dx, gy = sd * dy, gz = sd * dz;
     kgc_apply(P, v.cm, n, i, gx, gy, gz);
     double mj = nq.m, rhoj = nq.rho, porj = nq.por;
-    bx += mj / rhoj * (pori * uxi - porj * (double)nq.ux) * gx;
-    by += mj / rhoj * (pori * uyi - porj * (double)nq.uy) * gy;
-    bz += mj / rhoj * (pori * uzi - porj * (double)nq.uz) * gz;
-    double c = -(dx * gx + dy * gy + dz * gz) / r / r;
-    double a = mj * ((pori * rhoi + porj * rhoj) / (rhoj * rhoi)) * c;
+    const double Vj = mj / rhoj;
+    bx += Vj * (pori * uxi - porj * (double)nq.ux) * gx;
+    by += Vj * (pori * uyi - porj * (double)nq.uy) * gy;
+    bz += Vj * (pori * uzi - porj * (double)nq.uz) * gz;
+    double c = -(dx * gx + dy * gy + dz * gz) * (ir * ir);
+    double a = (pori * rhoi + porj * rhoj) * Vj * irhoi * c;
     Aij += a;
     AP += a * nq.pres;
     if (!nq.row) {

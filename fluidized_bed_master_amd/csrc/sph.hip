@@ -40,14 +40,14 @@ __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
     return;
   }
   const double hd = v.h[i];
-  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rs = (T)(P.kappa * hd);
+  const T h = (T)hd, ih = (T)(1.0 / hd), A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rs = (T)(P.kappa * hd);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   T xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
   for_nbrs4<true>(v, i, false, [&](const Nb<R>& q, bool on) {
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz);
     const bool use = on && q.pt <= 1000 && r > 0 && r < Rs;
-    T dw = knl_dw<T>(P.kernel_type, A, h, r);
+    T dw = knl_dw_ih<T>(P.kernel_type, A, h, ih, r);
     T mtd = (T)q.m * dw;
     T rtd = (T)1 / ((T)q.rho * r);
     xx += use ? -mtd * dx * dx * rtd : (T)0;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   if (v.itype[i] > 2 || v.ptype[i] > 1000) return;
   const int n = v.n;
   const double hd = v.h[i];
-  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const T h = (T)hd, ih = (T)(1.0 / hd), A = knl_tmpA<T>(P.kernel_type, P.dim, h);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   // rho / rho_ref differs from 1 by ~drho dt / rho: the difference of two
   // such ratios cancels to a few significant digits, so it is taken in FP64
@@ -163,13 +163,14 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
     const bool use = on && q.pt <= 1000;
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
-    T mj = q.m, rhoj = q.rho;
-    if (P.filt_step) flt += use ? mj / rhoj * knl_w<T>(P.kernel_type, A, h, r) : (T)0;
+    const T Vj = (T)q.m / (T)q.rho;
+    if (P.filt_step) flt += use ? Vj * knl_w_ih<T>(P.kernel_type, A, h, ih, r) : (T)0;
     if (anto) {
-      T dw = knl_dw<T>(P.kernel_type, A, h, r);
-      T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+      T dw = knl_dw_ih<T>(P.kernel_type, A, h, ih, r);
+      const T sd = dw / r;
+      T gx = sd * dx, gy = sd * dy, gz = sd * dz;
       kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      T c = (T)(-(q.rho / q.rref - ratio_i)) * (mj / rhoj);
+      T c = (T)(-(q.rho / q.rref - ratio_i)) * Vj;
       rx += use ? c * gx : (T)0;
       ry += use ? c * gy : (T)0;
       rz += use ? c * gz : (T)0;
@@ -233,8 +234,8 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   if (P.open_boundary > 0 && v.btype[i] > 0) return;
   if (pti >= 1000 || pti <= 0) return;
   const int n = v.n;
-  const double hd = v.h[i];
-  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const double hd = v.h[i], ihd = 1.0 / hd;
+  const T h = (T)hd;
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i];
   const double pid = v.pres[i];
@@ -244,8 +245,15 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   const T cpi = (T)(pti > 1000 ? P.cp_dem : P.cp_fluid);
   const T eli = P.eulerian ? (T)0 : (T)1;
   const bool euler = eli < (T)1;
-  const double rhoid = v.rho[i], porid = v.por[i];
+  const double rhoid = v.rho[i];
   const double Ad = knl_tmpA<double>(P.kernel_type, P.dim, hd);
+  // per-particle factors of the pair terms, hoisted: every neighbour in the
+  // SPH list has p_type <= 1000, so without tables its viscosity and
+  // conductivity are the fluid constants
+  const T visf = (T)P.mu_fluid, kcf = (T)P.k_fluid;
+  const T vis_const = (visi * visf) / (visi + visf);
+  const T cv_i = 4 * pori / rhoi;
+  const T kc_const = (T)4 * pori * kcf * kci / (rhoi * (kci + kcf));
   T tx = 0, ty = 0, tz = 0, Rc = 0, ex = 0, ey = 0, ez = 0, et = 0;
   double px = 0, py = 0, pz = 0, fl = 0;
   auto term = [&](const Nb<R>& q, bool on) {
@@ -255,34 +263,40 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
       const double S = v.flux_s[q.j];
       if (use && S > 0.0) {
         const double rw = sqrt(q.dx * q.dx + q.dy * q.dy + q.dz * q.dz), a = q.h / P.h_over_dx;
-        fl += P.heat_flux * (a * a) * knl_w<double>(P.kernel_type, Ad, hd, rw) / S;
+        fl += P.heat_flux * (a * a) * knl_w_ih<double>(P.kernel_type, Ad, hd, ihd, rw) / S;
       }
     }
     // geometry in FP64 arithmetic: the symmetric pressure-gradient term
-    // cancels a hydrostatic offset of order H/h times its result
+    // cancels a hydrostatic offset of order H/h times its result.  Two
+    // reciprocals per pair (1/r, 1/rho_j) replace the divisions of the
+    // reference's expressions; the rounding moves by an ulp
     const double dxd = q.dx, dyd = q.dy, dzd = q.dz;
     const double rd = sqrt(dxd * dxd + dyd * dyd + dzd * dzd) + 1e-20;
-    const double dwd = knl_dw<double>(P.kernel_type, Ad, hd, rd);
-    double gxd = dwd * dxd / rd, gyd = dwd * dyd / rd, gzd = dwd * dzd / rd;
+    const double ird = 1.0 / rd, irhod = 1.0 / q.rho;
+    const double dwd = knl_dw_ih<double>(P.kernel_type, Ad, hd, ihd, rd);
+    const double sd = dwd * ird;
+    double gxd = sd * dxd, gyd = sd * dyd, gzd = sd * dzd;
     kgc_apply(P, v.cm, n, i, gxd, gyd, gzd);
     if (P.fp_solve) {
+      // (rho_i / eps_i) Cp with Cp = -eps_i m_j pp / (rho_i rho_j)
       const double pp = P.pgrad_form ? q.pres - pid : pid + q.pres;
-      const double Cp = -porid * q.m * pp / (rhoid * q.rho);
-      px += use ? rhoid / porid * Cp * gxd : 0.0;
-      py += use ? rhoid / porid * Cp * gyd : 0.0;
-      pz += use ? rhoid / porid * Cp * gzd : 0.0;
+      const double Vpp = q.m * pp * irhod;
+      px -= use ? Vpp * gxd : 0.0;
+      py -= use ? Vpp * gyd : 0.0;
+      pz -= use ? Vpp * gzd : 0.0;
     }
-    const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd, r = (T)rd, dw = (T)dwd;
+    const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd, r = (T)rd, dw = (T)dwd, ir = (T)ird;
     const T gx = (T)gxd, gy = (T)gyd, gz = (T)gzd;
-    const T mj = (T)q.m, rhoj = (T)q.rho;
+    const T mj = (T)q.m, irhoj = (T)irhod, Vj = mj * irhoj;
     const T uxj = q.ux, uyj = q.uy, uzj = q.uz;
     if (P.fv_solve) {
-      T visj = tab ? v.visp[q.j] : (T)P.mu_fluid;
-      T Cv = dx * gx + dy * gy + dz * gz;
-      Cv *= (visi * visj) / (visi + visj);
-      Cv *= 4 * pori * (mj / (rhoi * rhoj));
-      Cv /= r;
-      Cv /= r;
+      const T visj = tab ? v.visp[q.j] : visf;
+      const T vm = tab ? (visi * visj) / (visi + visj) : vis_const;
+      // (x.g / r) / r: the self pair (x = 0, 1/r = 1e20) stays 0 in FP32,
+      // where a product 1/r^2 would overflow
+      T Cv = (dx * gx + dy * gy + dz * gz) * ir;
+      Cv *= ir;
+      Cv *= vm * cv_i * Vj;
       tx += use ? Cv * (uxi - uxj) : (T)0;
       ty += use ? Cv * (uyi - uyj) : (T)0;
       tz += use ? Cv * (uzi - uzj) : (T)0;
@@ -293,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
       T phi = hij * uij_xij;
       phi /= (r * r + (T)0.01 * hij * hij);
       T Pij = mi * phi * ((T)(-0.1 * P.soundspeed) + (T)0.02 * phi);
-      Pij /= (rhoi + rhoj);
+      Pij /= (rhoi + (T)q.rho);
       Pij *= (T)0.5;
       const bool av = use && uij_xij < 0;
       tx += av ? -Pij * gx : (T)0;
@@ -301,20 +315,24 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
       tz += av ? -Pij * gz : (T)0;
     }
     if (euler) {
-      T c = mj / rhoj;
-      ex += use ? (uxi * (uxj - uxi) * gx + uyi * (uxj - uxi) * gy + uzi * (uxj - uxi) * gz) * c * (1 - eli) : (T)0;
-      ey += use ? (uxi * (uyj - uyi) * gx + uyi * (uyj - uyi) * gy + uzi * (uyj - uyi) * gz) * c * (1 - eli) : (T)0;
-      ez += use ? (uxi * (uzj - uzi) * gx + uyi * (uzj - uzi) * gy + uzi * (uzj - uzi) * gz) * c * (1 - eli) : (T)0;
+      const T c = Vj * (1 - eli);
+      ex += use ? (uxi * (uxj - uxi) * gx + uyi * (uxj - uxi) * gy + uzi * (uxj - uxi) * gz) * c : (T)0;
+      ey += use ? (uxi * (uyj - uyi) * gx + uyi * (uyj - uyi) * gy + uzi * (uyj - uyi) * gz) * c : (T)0;
+      ez += use ? (uxi * (uzj - uzi) * gx + uyi * (uzj - uzi) * gy + uzi * (uzj - uzi) * gz) * c : (T)0;
       if (P.con_solve) {
         T tj = q.temp;
-        et += use ? (uxi * (tj - tempi) * gx + uyi * (tj - tempi) * gy + uzi * (tj - tempi) * gz) * c * (1 - eli)
-                  : (T)0;
+        et += use ? (uxi * (tj - tempi) * gx + uyi * (tj - tempi) * gy + uzi * (tj - tempi) * gz) * c : (T)0;
       }
     }
     if (P.con_solve) {
-      T kcj = tab ? v.condp[q.j] : (T)(q.pt > 1000 ? P.k_dem : P.k_fluid);
-      T H = 4 * pori * mj * kcj * kci * (tempi - (T)q.temp) * dw;
-      H /= (r + (T)1e-10) * rhoi * rhoj * (kci + kcj);
+      // 4 eps_i m_j k_j k_i (T_i - T_j) dW / ((r + 1e-10) rho_i rho_j (k_i + k_j))
+      T kf = kc_const;
+      if (tab) {
+        const T kcj = v.condp[q.j];
+        kf = (T)4 * pori * kcj * kci / (rhoi * (kci + kcj));
+      }
+      const T ir10 = sizeof(T) == 4 ? ir : (T)(1.0 / (rd + 1e-10));
+      const T H = kf * Vj * (tempi - (T)q.temp) * dw * ir10;
       Rc += use ? H : (T)0;
     }
   };
@@ -528,19 +546,23 @@ __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<
   if (pti >= 1000 || pti <= 0) return;
   const int n = v.n;
   const double hd = v.h[i];
-  const T h = hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const T h = hd, ih = 1.0 / hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i], pid = v.pres[i];
   const T rhoi = v.rho[i], pori = v.por[i];
+  // Cp = -eps_i m_j pp / (rho_i rho_j): the per-particle factor hoisted and
+  // 1/r, 1/rho_j taken once per pair (the rounding moves by an ulp)
+  const T ci = -pori / rhoi;
   T tx = 0, ty = 0, tz = 0;
   auto term = [&](const Nb<R>& q, bool on) {
     const bool use = on && q.pt < 1000;
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
-    T dw = knl_dw<T>(P.kernel_type, A, h, r);
-    T gx = dw * dx / r, gy = dw * dy / r, gz = dw * dz / r;
+    T dw = knl_dw_ih<T>(P.kernel_type, A, h, ih, r);
+    const T sd = dw / r;
+    T gx = sd * dx, gy = sd * dy, gz = sd * dz;
     kgc_apply(P, v.cm, n, i, gx, gy, gz);
     const T pp = P.pgrad_form ? q.pres - pid : pid + q.pres;
-    T Cp = -pori * q.m * pp / (rhoi * q.rho);
+    const T Cp = ci * (q.m * pp / q.rho);
     tx += use ? Cp * gx : (T)0;
     ty += use ? Cp * gy : (T)0;
     tz += use ? Cp * gz : (T)0;
@@ -629,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
   const int pti = v.ptype[i];
   if (v.itype[i] != 1 || pti > 1000 || pti == 0 || pti == 9) return;
   const double hd = v.h[i];
-  const T h = (T)hd, A = knl_tmpA<T>(P.kernel_type, P.dim, h);
+  const T h = (T)hd, ih = (T)(1.0 / hd), A = knl_tmpA<T>(P.kernel_type, P.dim, h);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i], ti = v.temp[i], c = (T)P.c_xsph;
   T tx = 0, ty = 0, tz = 0, tt = 0, fl = 0;
@@ -638,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
     const bool use = on && (ptj == 1 || ptj == 3);
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz);
-    T w = knl_w<T>(P.kernel_type, A, h, r);
+    T w = knl_w_ih<T>(P.kernel_type, A, h, ih, r);
     T vj = (T)q.m / (T)q.rho;
     tx += use ? c * vj * (-uxi + (T)q.ux) * w : (T)0;
     ty += use ? c * vj * (-uyi + (T)q.uy) * w : (T)0;
