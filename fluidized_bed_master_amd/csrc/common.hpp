@@ -125,6 +125,14 @@ __device__ inline double prop_k(const DevParams& P, int pt, double T) {
 // Device state.  Persistent fields travel with the particles through the
 // NNPS permutation; scratch fields are recomputed every step.
 // ---------------------------------------------------------------------------
+template <typename R> struct Vec4Of;
+template <> struct Vec4Of<float> { typedef float4 type; };
+template <> struct Vec4Of<double> { typedef double4 type; };
+template <typename R> struct alignas(2 * sizeof(R)) HRec {
+  R h;
+  int pt;
+};
+
 template <typename R>
 struct SphView {
   int n;
@@ -151,6 +159,14 @@ struct SphView {
   R *ux0, *uy0, *uz0;       // velocity at step start (P2.ux0)
   R *drho;                  // WCSPH continuity rate
   double *ppe1, *ppe2, *ppe3, *ppe4;
+  // Packed neighbour records of the pair sums that take FP64 geometry
+  // (the symmetric pressure gradient, and every sum of the FP64 path),
+  // rebuilt before each use: {x, y, z, m}, {rho, p}, and for advection
+  // {ux, uy, uz, T} and {h, p_type}
+  double4* pk;
+  double2* pk2;
+  typename Vec4Of<R>::type* pkv;
+  HRec<R>* pkh;
   R *xsx, *xsy, *xsz, *xst; // XSPH results
   // Positions as FP32 offsets inside a grid of power-of-two spacing pgrid
   // anchored at (pg0x, pg0y, pg0z), the grid cell packed in w (10 bits per

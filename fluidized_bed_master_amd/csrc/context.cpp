@@ -411,6 +411,10 @@ struct Ctx : public sophia_ctx {
     TRY(arena.alloc(&v.cm, 6 * NS));
     for (double** f : {&v.ppe1, &v.ppe2, &v.ppe3, &v.ppe4}) TRY(arena.alloc(f, NS));
     TRY(arena.alloc(&v.nbr_cnt, NS));
+    TRY(arena.alloc(&v.pk, NS));
+    TRY(arena.alloc(&v.pk2, NS));
+    TRY(arena.alloc(&v.pkv, NS));
+    TRY(arena.alloc(&v.pkh, NS));
     TRY(arena.alloc(&v.pos4, NS));
     // DEM persistent
     for (auto* f : {&dx_, &dy_, &dz_, &dux, &duy, &duz, &dwx, &dwy, &dwz, &dm, &dri, &drad, &dh, &drho, &dtemp, &dfdx,

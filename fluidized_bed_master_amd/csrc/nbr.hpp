@@ -81,4 +81,66 @@ __device__ inline void for_nbrs4(const SphView<R>& v, int i, bool eps_rule, F&& 
   }
 }
 
+// One neighbour from the packed records: the members of Nb that the
+// FP64-geometry pair sums read (h and the velocities only with VEL).
+template <typename R>
+struct NbP {
+  double dx, dy, dz, m, rho, pres;
+  R ux, uy, uz, temp, h;
+  int pt;
+  int j;
+};
+
+// The list in batches of four like for_nbrs4, reading the packed records
+// pk = {x, y, z, m} and pk2 = {rho, p}, then with VEL pkv = {ux, uy, uz, T}
+// and pkh = {h, p_type}, else p_type: four or three gathers per neighbour in
+// place of twelve or seven.  dx = x_i - x_j (minimum image on periodic axes),
+// the same bits as pdiff.
+template <bool VEL, typename R, class F>
+__device__ inline void for_nbrs4_pk(const SphView<R>& v, int i, bool eps_rule, F&& f) {
+  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+  if (cnt <= 0) return;
+  const int n = v.n;
+  const double4 pi = v.pk[i];
+  const int last4 = (cnt - 1) & ~3;
+  uint4 en4 = *(const uint4*)(v.nbr + nbr_slot(0, i, n));
+  for (int k = 0; k < cnt; k += 4) {
+    const uint32_t en[4] = {en4.x, en4.y, en4.z, en4.w};
+    NbP<R> q[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = (int)(en[u] & ~kNbrFlag);
+      const double4 a = v.pk[j];
+      const double2 b = v.pk2[j];
+      q[u].dx = pi.x - a.x;
+      q[u].dy = pi.y - a.y;
+      q[u].dz = pi.z - a.z;
+      q[u].m = a.w;
+      q[u].rho = b.x;
+      q[u].pres = b.y;
+      if constexpr (VEL) {
+        const auto w = v.pkv[j];
+        const HRec<R> hr = v.pkh[j];
+        q[u].ux = w.x; q[u].uy = w.y; q[u].uz = w.z; q[u].temp = w.w;
+        q[u].h = hr.h;
+        q[u].pt = hr.pt;
+      } else {
+        q[u].pt = v.ptype[j];
+      }
+      q[u].j = j;
+    }
+    en4 = *(const uint4*)(v.nbr + nbr_slot(min(k + 4, last4), i, n));
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const bool on = (k + u < cnt) && !(eps_rule && (en[u] & kNbrFlag));
+      if (v.pmask) {
+        if (v.pmask & 1) q[u].dx = per_wrap(q[u].dx, v.plx);
+        if (v.pmask & 2) q[u].dy = per_wrap(q[u].dy, v.ply);
+        if (v.pmask & 4) q[u].dz = per_wrap(q[u].dz, v.plz);
+      }
+      f(q[u], on);
+    }
+  }
+}
+
 }  // namespace sophia

@@ -221,6 +221,33 @@ __global__ void k_props(DevParams P, SphView<R> v) {
   v.condp[i] = (R)prop_k(P, v.ptype[i], T);
 }
 
+// The packed neighbour records of the FP64-geometry pair sums (for_nbrs4_pk),
+// rebuilt from the SoA fields before each kernel that reads them.
+template <typename R, bool VEL>
+__global__ void k_pack_records(SphView<R> v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  v.pk[i] = make_double4(v.x[i], v.y[i], v.z[i], v.m[i]);
+  v.pk2[i] = make_double2(v.rho[i], v.pres[i]);
+  if constexpr (VEL) {
+    typename Vec4Of<R>::type w;
+    w.x = v.ux[i]; w.y = v.uy[i]; w.z = v.uz[i]; w.w = v.temp[i];
+    v.pkv[i] = w;
+    HRec<R> hr;
+    hr.h = (R)v.h[i];
+    hr.pt = v.ptype[i];
+    v.pkh[i] = hr;
+  }
+}
+
+// Pair sums in FP64 geometry read the packed records: the symmetric pressure
+// gradient, and every sum of the FP64 path.  The FP32 difference form takes
+// its geometry from the position records instead.
+template <typename R>
+__device__ __host__ inline bool packed_geometry(const DevParams& P) {
+  return !P.pgrad_form || sizeof(R) == 8;
+}
+
 // KERNEL_advection_force3D_sph (functions_PPE2.cuh:1251-1607): viscous,
 // artificial viscosity, Eulerian advection, conduction, gravity and the
 // pressure-gradient force density used by the coupling.
@@ -256,13 +283,13 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   const T kc_const = (T)4 * pori * kcf * kci / (rhoi * (kci + kcf));
   T tx = 0, ty = 0, tz = 0, Rc = 0, ex = 0, ey = 0, ez = 0, et = 0;
   double px = 0, py = 0, pz = 0, fl = 0;
-  auto term = [&](const Nb<R>& q, bool on) {
+  auto term = [&](const auto& q, bool on) {
     const bool use = on && q.pt < 1000;
     if constexpr (FLUX) {
       // wall heat flux from a flux wall j: q A_j W_ij / S_j
       const double S = v.flux_s[q.j];
       if (use && S > 0.0) {
-        const double rw = sqrt(q.dx * q.dx + q.dy * q.dy + q.dz * q.dz), a = q.h / P.h_over_dx;
+        const double rw = sqrt(q.dx * q.dx + q.dy * q.dy + q.dz * q.dz), a = v.h[q.j] / P.h_over_dx;
         fl += P.heat_flux * (a * a) * knl_w_ih<double>(P.kernel_type, Ad, hd, ihd, rw) / S;
       }
     }
@@ -338,9 +365,9 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
   };
   // the difference-form pressure gradient carries no hydrostatic offset, so
   // with FP32 fields the position records give the geometry; the symmetric
-  // form keeps FP64 geometry
-  if (P.pgrad_form) for_nbrs4<true>(v, i, true, term);
-  else for_nbrs4<false>(v, i, true, term);
+  // form keeps FP64 geometry, read from the packed records
+  if (packed_geometry<R>(P)) for_nbrs4_pk<true>(v, i, true, term);
+  else for_nbrs4<true>(v, i, true, term);
   if (P.fg_solve) tz += -(T)P.gravity;
   v.fx[i] = tx - ex;
   v.fy[i] = ty - ey;
@@ -553,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<
   // 1/r, 1/rho_j taken once per pair (the rounding moves by an ulp)
   const T ci = -pori / rhoi;
   T tx = 0, ty = 0, tz = 0;
-  auto term = [&](const Nb<R>& q, bool on) {
+  auto term = [&](const auto& q, bool on) {
     const bool use = on && q.pt < 1000;
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
@@ -569,9 +596,10 @@ __global__ __launch_bounds__(kBlock) void k_pressure_force(DevParams P, SphView<
   };
   if (P.fp_solve) {
     // the difference form carries no hydrostatic offset, so geometry from
-    // the position records is enough; the symmetric form keeps FP64 geometry
-    if (P.pgrad_form) for_nbrs4<true>(v, i, true, term);
-    else for_nbrs4<false>(v, i, true, term);
+    // the position records is enough; the symmetric form keeps FP64
+    // geometry, read from the packed records
+    if (packed_geometry<R>(P)) for_nbrs4_pk<false>(v, i, true, term);
+    else for_nbrs4<true>(v, i, true, term);
   }
   v.fpx[i] = (R)tx;
   v.fpy[i] = (R)ty;
@@ -788,6 +816,7 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_prep, P, v); }
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
+    if (packed_geometry<R>(P)) k_pack_records<R, true><<<grid_for(v.n), kBlock, 0, s>>>(v);
     if (P.heat_flux != 0.0) k_advection<R, true><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     else k_advection<R, false><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
@@ -819,7 +848,13 @@ template <typename R> int sph_props(hipStream_t s, const DevParams& P, const Sph
   }
   return 0;
 }
-template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_pressure_force, P, v); }
+template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  if (v.n <= 0) return 0;
+  if (P.fp_solve && packed_geometry<R>(P)) k_pack_records<R, false><<<grid_for(v.n), kBlock, 0, s>>>(v);
+  k_pressure_force<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
 template <typename R>
 int sph_open_boundary(hipStream_t s, const DevParams& P, const SphView<R>& v, const CellView& cv,
                       const OpenBoundary& ob) {
