@@ -37,14 +37,15 @@ MF_MATVEC_BYTES = 72.0
 
 def _matvec_bytes(layout: dict, n_sph: int, fp32: bool) -> float:
     """Compulsory HBM bytes of one inner PPE matvec y = A x over the compact
-    operator: every stored entry's coefficient and column index once, and per
-    particle the entry count (4), run lengths (8), cell key (4), row flag (4),
-    diagonal (8), own x, y and the fused dot operand (3 vector words); the
-    neighbour gathers of x hit the cache and count once through the own-x
-    read."""
+    operator: every stored entry's coefficient and column index once (on the
+    mixed path one 4-byte word holds both), and per particle the entry count
+    (4), run lengths (8), cell key (4), row flag (4), diagonal (4 FP32 inner
+    diagonal, 8 FP64), own x, y and the fused dot operand (3 vector words);
+    the neighbour gathers of x hit the cache and count once through the
+    own-x read."""
     vec = 4 if fp32 else 8
     coef = layout["coef_bytes"] if fp32 else 8
-    return layout["nnz"] * (coef + layout["index_bytes"]) + n_sph * (28.0 + 3 * vec)
+    return layout["nnz"] * (coef + layout["index_bytes"]) + n_sph * ((24.0 if fp32 else 28.0) + 3 * vec)
 
 
 def run_gpu(params, parts, precision, ppe_opts, steps, warmup, timers=False, mlp_weights=None, keep_pressure=False,

@@ -45,12 +45,6 @@ __device__ inline int agg_of(uint32_t key, const AggGrid& g) {
   return cx / kAgg + g.AX * (cy / kAgg + g.AY * (cz / kAgg));
 }
 
-template <int CM>
-__device__ inline double coef(const PpeOp& o, size_t e) {
-  if (CM == 2) return o.c64[e];
-  return (double)o.hi[e] + (double)o.lo[e];
-}
-
 // Column indices of the four entries of quad q of row i (kFmtRun8: run bases
 // from the row's cell key and run lengths, as in the matvec decoder; entries
 // past cnt get the row's own index and a zero coefficient from the caller).
@@ -103,26 +97,29 @@ __global__ __launch_bounds__(256) void k_coarse_rows(PpeOp o, AggGrid g, double*
     for (int q = 0; q < nq; q++) {
       const size_t qi = (size_t)q * n + (size_t)i;
       double c[4];
+      uint32_t hb[4] = {0u, 0u, 0u, 0u};  // mixed: hi words, kFmtRun8 offset in the low byte
       if (CM == 2) {
         const double2 a = reinterpret_cast<const double2*>(o.c64)[2 * qi];
         const double2 b = reinterpret_cast<const double2*>(o.c64)[2 * qi + 1];
         c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y;
       } else {
-        const float4 a = reinterpret_cast<const float4*>(o.hi)[qi];
+        const uint4 a = reinterpret_cast<const uint4*>(o.hi)[qi];
         const float4 b = reinterpret_cast<const float4*>(o.lo)[qi];
-        c[0] = (double)a.x + (double)b.x; c[1] = (double)a.y + (double)b.y;
-        c[2] = (double)a.z + (double)b.z; c[3] = (double)a.w + (double)b.w;
+        hb[0] = a.x; hb[1] = a.y; hb[2] = a.z; hb[3] = a.w;
+        c[0] = (double)hi_val(hb[0]) + (double)b.x; c[1] = (double)hi_val(hb[1]) + (double)b.y;
+        c[2] = (double)hi_val(hb[2]) + (double)b.z; c[3] = (double)hi_val(hb[3]) + (double)b.w;
       }
       int j[4];
       if (FMT == kFmtRun8) {
-        const uint32_t w = o.idx[qi];
+        const uint32_t w = CM == 2 ? o.idx[qi] : 0u;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int k = 4 * q + u;
           int base = d.B[0];
 #pragma unroll
           for (int r = 0; r < 8; r++) base = k >= d.E[r] ? d.B[r + 1] : base;
-          j[u] = k < cnt ? base + (int)((w >> (8 * u)) & 255u) : i;
+          const uint32_t off = CM == 2 ? (w >> (8 * u)) & 255u : hb[u] & 255u;
+          j[u] = k < cnt ? base + (int)off : i;
         }
       } else {
         const uint4 w = reinterpret_cast<const uint4*>(o.idx)[qi];

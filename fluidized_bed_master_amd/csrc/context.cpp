@@ -1454,7 +1454,7 @@ struct Ctx : public sophia_ctx {
       return SOPHIA_E_STATE;
     }
     if (nnz) *nnz = ppe_w.nnz;
-    if (ib) *ib = ppe_w.fmt == kFmtRun8 ? 1 : 4;
+    if (ib) *ib = ppe_w.fmt == kFmtRun8 ? (ppe_w.mixed ? 0 : 1) : 4;
     if (cb) *cb = ppe_w.mixed ? 4 : 8;
     return 0;
   }

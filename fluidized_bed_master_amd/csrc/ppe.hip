@@ -134,39 +134,43 @@ __device__ void sum_partials(const double* partials, int G, double (&tot)[NV]) {
 }
 
 // ----- row sums over the compact operator ----------------------------------
-// CM: 0 = hi (FP32 accumulation), 1 = hi + lo (FP64), 2 = FP64 coefficients
+// CM: 0 = hi (FP32 accumulation), 1 = hi + lo (FP64), 2 = FP64 coefficients.
+// For CM 0 and 1, b holds the raw hi words: their low byte is the kFmtRun8
+// column offset.
 template <int CM>
 struct CoefQuad;
 template <>
 struct CoefQuad<0> {
   float c[4];
-  __device__ void zero() { c[0] = c[1] = c[2] = c[3] = 0.f; }
+  uint32_t b[4];
   __device__ void load(const PpeOp& o, size_t qi) {
-    float4 a = reinterpret_cast<const float4*>(o.hi)[qi];
-    c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
+    const uint4 a = reinterpret_cast<const uint4*>(o.hi)[qi];
+    b[0] = a.x; b[1] = a.y; b[2] = a.z; b[3] = a.w;
+#pragma unroll
+    for (int u = 0; u < 4; u++) c[u] = hi_val(b[u]);
   }
 };
 template <>
 struct CoefQuad<1> {
   double c[4];
-  __device__ void zero() { c[0] = c[1] = c[2] = c[3] = 0.0; }
+  uint32_t b[4];
   __device__ void load(const PpeOp& o, size_t qi) {
-    float4 a = reinterpret_cast<const float4*>(o.hi)[qi];
-    float4 b = reinterpret_cast<const float4*>(o.lo)[qi];
-    c[0] = (double)a.x + (double)b.x;
-    c[1] = (double)a.y + (double)b.y;
-    c[2] = (double)a.z + (double)b.z;
-    c[3] = (double)a.w + (double)b.w;
+    const uint4 a = reinterpret_cast<const uint4*>(o.hi)[qi];
+    const float4 l = reinterpret_cast<const float4*>(o.lo)[qi];
+    b[0] = a.x; b[1] = a.y; b[2] = a.z; b[3] = a.w;
+    const float lv[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+    for (int u = 0; u < 4; u++) c[u] = (double)hi_val(b[u]) + (double)lv[u];
   }
 };
 template <>
 struct CoefQuad<2> {
   double c[4];
-  __device__ void zero() { c[0] = c[1] = c[2] = c[3] = 0.0; }
+  uint32_t b[4];  // unused: the FP64 path keeps its offsets in idx
   __device__ void load(const PpeOp& o, size_t qi) {
     double2 a = reinterpret_cast<const double2*>(o.c64)[2 * qi];
-    double2 b = reinterpret_cast<const double2*>(o.c64)[2 * qi + 1];
-    c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y;
+    double2 e = reinterpret_cast<const double2*>(o.c64)[2 * qi + 1];
+    c[0] = a.x; c[1] = a.y; c[2] = e.x; c[3] = e.y;
   }
 };
 
@@ -213,8 +217,10 @@ __device__ __forceinline__ AT row_sum(const PpeOp& o, int i, const XT* __restric
       if (r < 8) E[r] = acc;
     }
     const int self = STAGED ? i + adj[4] : i;
+    // mixed operator: the offsets ride in the coefficient words; FP64: idx
+    constexpr bool PK = CM < 2;
     const size_t q1 = (size_t)min(1, last) * n + i;
-    uint32_t wn0 = o.idx[i], wn1 = o.idx[q1];
+    uint32_t wn0 = PK ? 0u : o.idx[i], wn1 = PK ? 0u : o.idx[q1];
     CoefQuad<CM> cn0, cn1;
     cn0.load(o, (size_t)i);
     cn1.load(o, q1);
@@ -228,16 +234,20 @@ __device__ __forceinline__ AT row_sum(const PpeOp& o, int i, const XT* __restric
         int base = B[0];
 #pragma unroll
         for (int r = 0; r < 8; r++) base = k >= E[r] ? B[r + 1] : base;
-        const uint32_t w = u < 4 ? w0 : w1;
-        j[u] = k < cnt ? base + (int)((w >> (8 * (u & 3))) & 255u) : self;
+        uint32_t off;
+        if constexpr (PK) off = (u < 4 ? c0.b[u & 3] : c1.b[u & 3]) & 255u;
+        else off = ((u < 4 ? w0 : w1) >> (8 * (u & 3))) & 255u;
+        j[u] = k < cnt ? base + (int)off : self;
       }
       XT xv[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) xv[u] = x[j[u]];
       const size_t qa = (size_t)min(q + 2, last) * n + i;
       const size_t qb = (size_t)min(q + 3, last) * n + i;
-      wn0 = o.idx[qa];
-      wn1 = o.idx[qb];
+      if constexpr (!PK) {
+        wn0 = o.idx[qa];
+        wn1 = o.idx[qb];
+      }
       cn0.load(o, qa);
       cn1.load(o, qb);
 #pragma unroll
@@ -286,6 +296,7 @@ struct AsmOut {
   float *hi, *lo;
   double* c64;
   double *diag, *dinv, *rhs;
+  float* dgi;
   unsigned* flags;
 };
 
@@ -334,6 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     o.diag[i] = 0;
     o.dinv[i] = 0;
     o.rhs[i] = 0;
+    if (MIXED) o.dgi[i] = 0.f;
     return;
   }
   // runs of the 27-cell stencil (kFmtRun8)
@@ -366,13 +378,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   const double uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i];
   const double rhoi = v.rho[i], pori = v.por[i], irhoi = 1.0 / rhoi;
   double bx = 0, by = 0, bz = 0, Aij = 0, AP = 0, lag = 0;
+  double dcomp = 0;  // sum over stored columns of a - hi16(a) (mixed)
   int kc = 0, run = 0;
   uint64_t rcw = 0;
   uint32_t word = 0;
-  uint32_t qi4[4] = {0u, 0u, 0u, 0u};
-  float qh4[4] = {0.f, 0.f, 0.f, 0.f}, ql4[4] = {0.f, 0.f, 0.f, 0.f};
+  uint32_t qi4[4] = {0u, 0u, 0u, 0u}, qh4[4] = {0u, 0u, 0u, 0u};
+  float ql4[4] = {0.f, 0.f, 0.f, 0.f};
   double qc4[4] = {0.0, 0.0, 0.0, 0.0};
-  auto push_quad = [&](uint32_t jj, float h, float l, double c) {
+  auto push_quad = [&](uint32_t jj, uint32_t h, float l, double c) {
 #pragma unroll
     for (int t = 0; t < 3; t++) {
       qi4[t] = qi4[t + 1];
@@ -389,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     const size_t qs = (size_t)q * nn + i;
     if (FMT == kFmtIdx32) reinterpret_cast<uint4*>(o.idx)[qs] = make_uint4(qi4[0], qi4[1], qi4[2], qi4[3]);
     if (MIXED) {
-      reinterpret_cast<float4*>(o.hi)[qs] = make_float4(qh4[0], qh4[1], qh4[2], qh4[3]);
+      reinterpret_cast<uint4*>(o.hi)[qs] = make_uint4(qh4[0], qh4[1], qh4[2], qh4[3]);
       reinterpret_cast<float4*>(o.lo)[qs] = make_float4(ql4[0], ql4[1], ql4[2], ql4[3]);
     } else {
       reinterpret_cast<double2*>(o.c64)[2 * qs] = make_double2(qc4[0], qc4[1]);
@@ -462,6 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     // store column kc: the quad's values collect in registers (oldest first)
     // and leave as whole 16-byte stores
     const int q = kc >> 2, u = kc & 3;
+    uint32_t off8 = 0;
     if (FMT == kFmtRun8) {
       while (run < 9 && j >= pick9(E, run)) run++;
       int off = run < 9 ? j - pick9(B, run) : -1;
@@ -471,15 +485,22 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
         if ((int)((rcw >> (kRunBits * run)) & kRunMax) == kRunMax) bad |= kFlagRunCount;
         else rcw += 1ull << (kRunBits * run);
       }
-      word |= ((uint32_t)off & 255u) << (8 * u);
-      if (u == 3) {
-        o.idx[(size_t)q * nn + i] = word;
-        word = 0;
+      off8 = (uint32_t)off & 255u;
+      if (!MIXED) {
+        word |= off8 << (8 * u);
+        if (u == 3) {
+          o.idx[(size_t)q * nn + i] = word;
+          word = 0;
+        }
       }
     }
-    {
-      const float h = (float)a;
-      push_quad((uint32_t)j, h, (float)(a - (double)h), a);
+    if (MIXED) {
+      const uint32_t hb = hi16_bits(a);
+      const double hv = (double)hi_val(hb);
+      dcomp += a - hv;
+      push_quad((uint32_t)j, hb | off8, (float)(a - hv), a);
+    } else {
+      push_quad((uint32_t)j, 0u, 0.f, a);
     }
     kc++;
     if ((kc & 3) == 0) store_quad(q);
@@ -488,9 +509,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   // pad the last quad so whole-quad loads read defined values
   if (kc & 3) {
     const int q = kc >> 2;
-    for (int u = kc & 3; u < 4; u++) push_quad((uint32_t)i, 0.f, 0.f, 0.0);
+    for (int u = kc & 3; u < 4; u++) push_quad((uint32_t)i, 0u, 0.f, 0.0);
     store_quad(q);
-    if (FMT == kFmtRun8) o.idx[(size_t)q * nn + i] = word;
+    if (FMT == kFmtRun8 && !MIXED) o.idx[(size_t)q * nn + i] = word;
   }
   if (bad) atomicOr(o.flags, bad);
   o.ccnt[i] = kc;
@@ -502,6 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   v.ppe1[i] = rhoi * (bx + by + bz) / tdt;
   v.ppe2[i] = Aij;
   o.diag[i] = Aij;
+  if (MIXED) o.dgi[i] = (float)(Aij - dcomp);
   o.dinv[i] = Aij != 0 ? 1.0 / Aij : 0.0;
   o.rhs[i] = bi + lag;
 }
@@ -609,7 +631,7 @@ __global__ __launch_bounds__(kRedThreads) void k_bi_mv(PpeOp o, const V* __restr
     AT acc = 0;
     if (o.row[i]) {
       const AT s = row_sum<FMT, CM, AT>(o, i, x);
-      acc = (AT)o.diag[i] * (AT)x[i] - s;
+      acc = (CM == 0 ? (AT)o.dgi[i] : (AT)o.diag[i]) * (AT)x[i] - s;
     }
     const V yi = (V)acc;
     y[i] = yi;
@@ -702,7 +724,7 @@ __global__ __launch_bounds__(kRedThreads, 8) void k_bi_mv_staged(PpeOp o, const 
       if (o.row[i]) {
         const AT s = staged ? row_sum<kFmtRun8, CM, AT, V, true>(o, i, S.x, S.adj)
                             : row_sum<kFmtRun8, CM, AT, V>(o, i, x);
-        acc = (AT)o.diag[i] * (AT)x[i] - s;
+        acc = (CM == 0 ? (AT)o.dgi[i] : (AT)o.diag[i]) * (AT)x[i] - s;
       }
       const V yi = (V)acc;
       y[i] = yi;
@@ -971,10 +993,11 @@ int ppe_workspace_alloc(PpeWork& w, int n, int cap, bool mixed) {
   if (mixed) {
     HIP_TRY(A((void**)&w.hi, E * 4));
     HIP_TRY(A((void**)&w.lo, E * 4));
+    HIP_TRY(A((void**)&w.dgi, N * 4));
     w.c64 = nullptr;
   } else {
     HIP_TRY(A((void**)&w.c64, E * 8));
-    w.hi = w.lo = nullptr;
+    w.hi = w.lo = w.dgi = nullptr;
   }
   return 0;
 }
@@ -988,7 +1011,7 @@ void ppe_workspace_free(PpeWork& w) {
   if (w.host_state) (void)hipHostFree(w.host_state);
   if (w.host_flags) (void)hipHostFree(w.host_flags);
   for (int k = 0; k < 8; k++) F(w.vec[k]);
-  F(w.hi); F(w.lo); F(w.c64);
+  F(w.hi); F(w.lo); F(w.dgi); F(w.c64);
   if (w.coarse) {
     coarse_free(*w.coarse);
     delete w.coarse;
@@ -1014,6 +1037,7 @@ PpeOp ppe_op(const PpeWork& w) {
   o.lo = w.lo;
   o.c64 = w.c64;
   o.diag = w.diag;
+  o.dgi = w.dgi;
   o.dinv = w.dinv;
   o.rhs = w.rhs;
   return o;
@@ -1042,7 +1066,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
   w.ncell = P.NI * P.NJ * P.NK;
   w.rowmajor = P.z_index == 0;
   AsmOut o{v.n, P.NI, P.NJ, P.NK, keys, cell_off, w.row, w.ccnt, w.runc, w.idx, w.hi, w.lo, w.c64,
-           w.diag, w.dinv, w.rhs, w.flags};
+           w.diag, w.dinv, w.rhs, w.dgi, w.flags};
   const int g = grid_xcd(v.n);
   k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, v.surf, v.pres, w.row);
   // the x-runs of the stencil are contiguous only in row-major keys without

@@ -21,6 +21,17 @@ constexpr int kRunMax = (1 << kRunBits) - 1;
 // (wall/buffer columns live in the right-hand side), slot-quad major:
 // entry k of row i sits at ((k / 4) * n + i) * 4 + k % 4, so a wave reads
 // 64 consecutive 16 B (FP32) or 32 B (FP64) quads per four slots.
+// Mixed path: the inner coefficient is fl32(a) rounded to nearest at a
+// 16-bit significand (the low 8 bits of the FP32 word cleared), and with
+// kFmtRun8 the entry's u8 column offset rides in those 8 bits, so an entry
+// is one 4-byte word.  The inner diagonal takes up the rounding of its row,
+// dgi_i = d_i - sum_j (a_ij - hi16(a_ij)): the inner operator then maps a
+// constant vector exactly as the FP64 one does, and differs from it by
+// sum_j (a_ij - hi16(a_ij)) (x_i - x_j) only (relative 2^-16 of a
+// difference).  hi + lo keeps a to about 2^-40 for the FP64 residual.
+__device__ inline uint32_t hi16_bits(double a) { return (__float_as_uint((float)a) + 0x80u) & 0xFFFFFF00u; }
+__device__ inline float hi_val(uint32_t e) { return __uint_as_float(e & 0xFFFFFF00u); }
+
 struct PpeOp {
   int n = 0, nq = 0, fmt = kFmtRun8;
   int NI = 1, NJ = 1, ncell = 1;
@@ -29,11 +40,12 @@ struct PpeOp {
   const int* row = nullptr;
   const int* ccnt = nullptr;           // compact entries per row
   const uint64_t* runc = nullptr;      // kFmtRun8 run lengths
-  const uint32_t* idx = nullptr;       // kFmtRun8: u32 [q][i] (4 offsets); kFmtIdx32: u32 [q][i][4]
-  const float* hi = nullptr;           // mixed: fl32(a)            [q][i][4]
-  const float* lo = nullptr;           // mixed: fl32(a - hi)       [q][i][4]
-  const double* c64 = nullptr;         // FP64 path                 [q][i][4]
+  const uint32_t* idx = nullptr;       // kFmtRun8 FP64 path: u32 [q][i] (4 offsets); kFmtIdx32: u32 [q][i][4]
+  const float* hi = nullptr;           // mixed: hi16(a) | offset    [q][i][4] (hi_bits / hi_val)
+  const float* lo = nullptr;           // mixed: fl32(a - hi16(a))   [q][i][4]
+  const double* c64 = nullptr;         // FP64 path                  [q][i][4]
   const double* diag = nullptr;
+  const float* dgi = nullptr;          // mixed: inner diagonal d_i - sum_j (a_ij - hi16(a_ij))
   const double* dinv = nullptr;
   const double* rhs = nullptr;
 };
@@ -52,6 +64,7 @@ struct PpeWork {
   uint64_t* runc = nullptr;
   uint32_t* idx = nullptr;
   float *hi = nullptr, *lo = nullptr;
+  float* dgi = nullptr;  // mixed: inner diagonal
   double* c64 = nullptr;
   double *diag = nullptr, *dinv = nullptr, *rhs = nullptr, *x64 = nullptr, *r64 = nullptr;
   double* partials = nullptr;

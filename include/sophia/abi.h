@@ -445,8 +445,10 @@ int sophia_ppe_apply(sophia_ctx* ctx, const double* x, double* y, int32_t n);
 int sophia_ppe_matvec_probe(sophia_ctx* ctx, int32_t variant, int32_t reps, const double* x, double* y,
                             int32_t n, double* us_per_launch);
 /* storage of the assembled operator: stored entries, bytes per column index
- * (1: u8 offset inside a cell run, 4: u32 index) and per coefficient as read
- * by the inner iterations (4: FP32, 8: FP64).  Any may be NULL. */
+ * (1: u8 offset inside a cell run, 4: u32 index, 0: the u8 offset rides in
+ * the low byte of the FP32 inner coefficient, whose significand is rounded
+ * to 16 bits) and per coefficient as read by the inner iterations (4: FP32,
+ * 8: FP64).  Any may be NULL. */
 int sophia_ppe_layout(const sophia_ctx* ctx, int64_t* nnz, int32_t* index_bytes, int32_t* coef_bytes);
 /* Coarse operator of the last two-level solve (no reference counterpart;
  * inspection): aggregates per axis into dims[3] (n_coarse = product), and,

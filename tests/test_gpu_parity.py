@@ -215,7 +215,9 @@ def test_operator_matches_oracle(prec, z_index):
         s.nnps(); s.prep(); s.interaction(); s.dem_interaction(); s.sph_dem_coupling(); s.predict()
         s.ppe(abi.ppe_opts(method=abi.PPE_JACOBI))
     lay = g.ppe_layout()
-    assert lay["index_bytes"] == (4 if z_index else 1), lay
+    # row-major keys: u8 run offsets, in the low byte of the FP32 inner
+    # coefficient on the mixed path; Morton keys: u32 indices
+    assert lay["index_bytes"] == (4 if z_index else (0 if prec == abi.PREC_MIXED else 1)), lay
     dg, bg, rg = g.get_ppe_system()
     do, bo, ro = o.get_ppe_system()
     assert np.array_equal(rg, ro)
@@ -443,3 +445,30 @@ def test_matvec_probe_stored_and_matrix_free_agree():
     assert rel_err(y1[r], y_ref[r]) < 1e-4
     assert np.all(y0[~r] == 0) and np.all(y1[~r] == 0)
     assert us0 > 0 and us1 > 0
+
+
+def test_inner_operator_rounding_acts_on_differences():
+    """The inner FP32 operator stores each coefficient at a 16-bit
+    significand (the low byte holds its u8 column offset) and takes each
+    row's rounding into its FP32 diagonal, so it differs from the FP64
+    operator (sophia_ppe_apply) by sum_j (a_ij - hi16(a_ij)) (x_i - x_j)
+    only.  On a constant vector that term is zero, and the two agree to the
+    FP32 rounding of the diagonal and of the row sum; on a random vector
+    they agree within 2^-16 of sum_j |a_ij| |x_i - x_j| <= 2 d_i max|x|."""
+    from fluidized_bed_master_amd.sophia import Sophia
+    p, parts = configs.small_isph_dem(seed=7)
+    g = Sophia(p, parts, precision=abi.PREC_MIXED)
+    g.nnps(); g.prep(); g.interaction(); g.dem_interaction(); g.sph_dem_coupling(); g.predict()
+    g.ppe(abi.ppe_opts(method=abi.PPE_JACOBI))
+    assert g.ppe_layout()["index_bytes"] == 0
+    d, _, rows = g.get_ppe_system()
+    r = rows == 1
+    one = r.astype(np.float64)
+    e1 = np.abs(g.matvec_probe(0, one, reps=1)[0] - g.ppe_apply(one))[r] / d[r]
+    x = np.random.default_rng(11).uniform(-1, 1, len(rows)) * r
+    e2 = np.abs(g.matvec_probe(0, x, reps=1)[0] - g.ppe_apply(x))[r] / d[r]
+    rep = {"const_max_over_d": float(e1.max()), "const_rms_over_d": float(np.sqrt(np.mean(e1 ** 2))),
+           "random_max_over_d": float(e2.max()), "random_rms_over_d": float(np.sqrt(np.mean(e2 ** 2)))}
+    _report("inner_operator_rounding", rep)
+    assert e1.max() <= 2e-6, rep
+    assert e2.max() <= 2 * 2.0 ** -16 + 2e-6, rep
