@@ -133,6 +133,25 @@ template <typename R> struct alignas(2 * sizeof(R)) HRec {
   int pt;
 };
 
+// Packed neighbour records of the FP32 pair kernels (mixed path), rebuilt
+// before each kernel that reads them: one 32 B record per particle (a single
+// cache-line gather) in place of four to five field gathers.  aux: rho/rho_ref
+// - 1 for the prep sums, else 0.
+struct alignas(32) Rec32 {
+  float4 p;  // pos4
+  float m, rho;
+  int pt;
+  float aux;
+};
+// PPE assembly (mixed path): V = m/rho, eps, eps rho and p in FP64 with the
+// FP32 velocity; flags = p_type (low 16 bits) | i_type << 16 | row << 24.
+struct alignas(64) RecAsm {
+  float4 p;  // pos4
+  double vol, por, porrho, pres;
+  float ux, uy, uz;
+  int flags;
+};
+
 template <typename R>
 struct SphView {
   int n;
@@ -167,6 +186,8 @@ struct SphView {
   double2* pk2;
   typename Vec4Of<R>::type* pkv;
   HRec<R>* pkh;
+  Rec32* r32;
+  RecAsm* ra;
   R *xsx, *xsy, *xsz, *xst; // XSPH results
   // Positions as FP32 offsets inside a grid of power-of-two spacing pgrid
   // anchored at (pg0x, pg0y, pg0z), the grid cell packed in w (10 bits per

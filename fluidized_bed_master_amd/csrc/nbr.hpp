@@ -143,4 +143,49 @@ __device__ inline void for_nbrs4_pk(const SphView<R>& v, int i, bool eps_rule, F
   }
 }
 
+// One neighbour from the 32 B records (FP32 pair kernels): the members of Nb
+// those kernels read, the differences from the position records.
+struct NbR {
+  float dx, dy, dz, m, rho, aux;
+  float ux, uy, uz, temp;
+  int pt;
+  int j;
+};
+
+// The list in batches of four like for_nbrs4<true>, reading r32 (and with
+// VEL pkv = {ux, uy, uz, T}): one or two gathers per neighbour in place of
+// four to eight, the same values.
+template <bool VEL, class F>
+__device__ inline void for_nbrs4_r32(const SphView<float>& v, int i, bool eps_rule, F&& f) {
+  const int cnt = min(v.nbr_cnt[i], v.nbr_cap);
+  if (cnt <= 0) return;
+  const int n = v.n;
+  const float4 pi4 = v.pos4[i];
+  const int last4 = (cnt - 1) & ~3;
+  uint4 en4 = *(const uint4*)(v.nbr + nbr_slot(0, i, n));
+  for (int k = 0; k < cnt; k += 4) {
+    const uint32_t en[4] = {en4.x, en4.y, en4.z, en4.w};
+    NbR q[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = (int)(en[u] & ~kNbrFlag);
+      const Rec32 a = v.r32[j];
+      const float3 d = rel_diff(pi4, a.p, v.pgrid, v.pmask);
+      q[u].dx = d.x; q[u].dy = d.y; q[u].dz = d.z;
+      q[u].m = a.m; q[u].rho = a.rho; q[u].aux = a.aux; q[u].pt = a.pt;
+      if constexpr (VEL) {
+        const float4 w = v.pkv[j];
+        q[u].ux = w.x; q[u].uy = w.y; q[u].uz = w.z; q[u].temp = w.w;
+      }
+      q[u].j = j;
+    }
+    en4 = *(const uint4*)(v.nbr + nbr_slot(min(k + 4, last4), i, n));
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const bool on = (k + u < cnt) && !(eps_rule && (en[u] & kNbrFlag));
+      f(q[u], on);
+    }
+  }
+}
+
 }  // namespace sophia

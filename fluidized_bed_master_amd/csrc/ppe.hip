@@ -332,6 +332,26 @@ __global__ void k_cell_offsets(int n, const uint32_t* __restrict__ keys, int nc,
   off[c] = (uint32_t)lo;
 }
 
+// The assembly's 64 B neighbour records (mixed path), after the row mask
+// and the Dirichlet pressures are set
+__global__ void k_pack_asm(SphView<float> v, const int* __restrict__ row) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  RecAsm r;
+  r.p = v.pos4[i];
+  const double m = v.m[i], rho = v.rho[i], por = v.por[i];
+  r.vol = m / rho;
+  r.por = por;
+  r.porrho = por * rho;
+  r.pres = v.pres[i];
+  r.ux = v.ux[i];
+  r.uy = v.uy[i];
+  r.uz = v.uz[i];
+  const int pt = min(max(v.ptype[i], -32768), 32767);
+  r.flags = (pt & 0xFFFF) | ((v.itype[i] & 0xFF) << 16) | ((row[i] ? 1 : 0) << 24);
+  v.ra[i] = r;
+}
+
 template <typename R, int FMT, int MIXED>
 __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, AsmOut o) {
   int i = particle_index();
@@ -416,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
   typedef typename std::conditional<REL, float, double>::type GT;
   struct AsmNb {
     GT x, y, z;  // x_i - x_j
-    double m, rho, por, pres;
+    double vol, por, porrho, pres;  // V_j = m_j / rho_j, eps_j, eps_j rho_j, p_j
     R ux, uy, uz;
     int pt, it, row;
   };
@@ -431,14 +451,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
       eb[u] = en[u];
       const int j = (int)(eb[u] & ~kNbrFlag);
       if constexpr (REL) {
-        const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid, v.pmask);
+        // one 64 B record (k_pack_asm) in place of eleven field gathers
+        const RecAsm ra = v.ra[j];
+        const float3 d = rel_diff(pi4, ra.p, v.pgrid, v.pmask);
         nb[u].x = d.x; nb[u].y = d.y; nb[u].z = d.z;
+        nb[u].vol = ra.vol; nb[u].por = ra.por; nb[u].porrho = ra.porrho; nb[u].pres = ra.pres;
+        nb[u].ux = ra.ux; nb[u].uy = ra.uy; nb[u].uz = ra.uz;
+        nb[u].pt = (int)(int16_t)(ra.flags & 0xFFFF); nb[u].it = (ra.flags >> 16) & 0xFF; nb[u].row = ra.flags >> 24;
       } else {
         pdiff(v, xi, yi, zi, j, nb[u].x, nb[u].y, nb[u].z);
+        const double mj = v.m[j], rhoj = v.rho[j], porj = v.por[j];
+        nb[u].vol = mj / rhoj; nb[u].por = porj; nb[u].porrho = porj * rhoj; nb[u].pres = v.pres[j];
+        nb[u].ux = v.ux[j]; nb[u].uy = v.uy[j]; nb[u].uz = v.uz[j];
+        nb[u].pt = v.ptype[j]; nb[u].it = v.itype[j]; nb[u].row = o.row[j];
       }
-      nb[u].m = v.m[j]; nb[u].rho = v.rho[j]; nb[u].por = v.por[j]; nb[u].pres = v.pres[j];
-      nb[u].ux = v.ux[j]; nb[u].uy = v.uy[j]; nb[u].uz = v.uz[j];
-      nb[u].pt = v.ptype[j]; nb[u].it = v.itype[j]; nb[u].row = o.row[j];
     }
     en4 = *(const uint4*)(v.nbr + nbr_slot(min(k0 + 4, last4), i, (int)n));
 #pragma unroll
@@ -458,13 +484,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble(DevParams P, SphView<R> v, 
     const double sd = dw * ir;
     double gx = sd * dx, gy = sd * dy, gz = sd * dz;
     kgc_apply(P, v.cm, n, i, gx, gy, gz);
-    double mj = nq.m, rhoj = nq.rho, porj = nq.por;
-    const double Vj = mj / rhoj;
+    const double porj = nq.por, Vj = nq.vol;
     bx += Vj * (pori * uxi - porj * (double)nq.ux) * gx;
     by += Vj * (pori * uyi - porj * (double)nq.uy) * gy;
     bz += Vj * (pori * uzi - porj * (double)nq.uz) * gz;
     double c = -(dx * gx + dy * gy + dz * gz) * (ir * ir);
-    double a = (pori * rhoi + porj * rhoj) * Vj * irhoi * c;
+    double a = (pori * rhoi + nq.porrho) * Vj * irhoi * c;
     Aij += a;
     AP += a * nq.pres;
     if (!nq.row) {
@@ -1069,6 +1094,7 @@ int ppe_assemble(hipStream_t s, const DevParams& P, const SphView<R>& v, PpeWork
            w.diag, w.dinv, w.rhs, w.dgi, w.flags};
   const int g = grid_xcd(v.n);
   k_rowmask<<<g, kBlock, 0, s>>>(P, v.n, v.ptype, v.btype, v.itype, v.surf, v.pres, w.row);
+  if constexpr (std::is_same<R, float>::value) k_pack_asm<<<grid_for(v.n), kBlock, 0, s>>>(v, w.row);
   // the x-runs of the stencil are contiguous only in row-major keys without
   // periodic wrap
   int fmt = (P.z_index || P.pmask) ? kFmtIdx32 : kFmtRun8;

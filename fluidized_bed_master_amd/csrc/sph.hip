@@ -28,6 +28,45 @@ __global__ void k_mass_init(DevParams P, SphView<R> v) {
   v.rho0[i] = v.rho[i];
 }
 
+// rho_j / rho_ref_j - rho_i / rho_ref_i (the Antuono difference) in FP64:
+// from the fields, or from the record's rho_j / rho_ref_j - 1 (FP32, exact
+// to 2^-24 of the difference's own size, not of 1)
+template <typename R>
+__device__ inline double ratio_diff(const Nb<R>& q, double ratio_i, double) {
+  return q.rho / q.rref - ratio_i;
+}
+__device__ inline double ratio_diff(const NbR& q, double, double ratio_i1) { return (double)q.aux - ratio_i1; }
+
+// The 32 B records of the FP32 pair kernels (for_nbrs4_r32), and with VEL the
+// velocity records; AUX: rho / rho_ref - 1 for the prep sums
+template <bool AUX, bool VEL>
+__global__ void k_pack_r32(SphView<float> v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  Rec32 r;
+  r.p = v.pos4[i];
+  r.m = (float)v.m[i];
+  r.rho = (float)v.rho[i];
+  r.pt = v.ptype[i];
+  r.aux = AUX ? (float)(v.rho[i] / v.rho_ref[i] - 1.0) : 0.f;
+  v.r32[i] = r;
+  if constexpr (VEL) v.pkv[i] = make_float4(v.ux[i], v.uy[i], v.uz[i], v.temp[i]);
+}
+
+// before an FP32 pair kernel (the FP64 path gathers its fields directly)
+template <typename R>
+int pack_r32(hipStream_t s, const SphView<R>& v, bool aux, bool vel) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (v.n <= 0) return 0;
+    const int g = grid_for(v.n);
+    if (vel) k_pack_r32<false, true><<<g, kBlock, 0, s>>>(v);
+    else if (aux) k_pack_r32<true, false><<<g, kBlock, 0, s>>>(v);
+    else k_pack_r32<false, false><<<g, kBlock, 0, s>>>(v);
+    HIP_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
 // KERNEL_clc_correction_KGC_3D_sph (functions_PREP2.cuh:452-568)
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
@@ -43,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
   const T h = (T)hd, ih = (T)(1.0 / hd), A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rs = (T)(P.kappa * hd);
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   T xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
-  for_nbrs4<true>(v, i, false, [&](const Nb<R>& q, bool on) {
+  auto term = [&](const auto& q, bool on) {
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz);
     const bool use = on && q.pt <= 1000 && r > 0 && r < Rs;
@@ -56,7 +95,9 @@ __global__ __launch_bounds__(kBlock) void k_kgc(DevParams P, SphView<R> v) {
     zx += use ? -mtd * dx * dz * rtd : (T)0;
     yz += use ? -mtd * dy * dz * rtd : (T)0;
     zz += use ? -mtd * dz * dz * rtd : (T)0;
-  });
+  };
+  if constexpr (std::is_same<R, float>::value) for_nbrs4_r32<false>(v, i, false, term);
+  else for_nbrs4<true>(v, i, false, term);
   T det = xx * (yy * zz - yz * yz);
   det -= xy * (xy * zz - yz * zx);
   det += zx * (xy * yz - yy * zx);
@@ -91,14 +132,16 @@ __global__ __launch_bounds__(kBlock) void k_porosity_sph(DevParams P, SphView<R>
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const int pti = v.ptype[i];
   T flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
-  for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
+  auto term = [&](const auto& q, bool on) {
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
     T w = knl_w<T>(P.kernel_type, A, h, r);
     int ptj = q.pt;
     const bool use = on && pti <= 1000 && ptj <= 1000 && ptj == 1;
     flt += use ? (T)q.m / (T)q.rho * w : (T)0;
-  });
+  };
+  if constexpr (std::is_same<R, float>::value) for_nbrs4_r32<false>(v, i, true, term);
+  else for_nbrs4<true>(v, i, true, term);
   if (d.n > 0) {
     for_cells(P, xi, yi, zi, dc, [&](int j) {
       double rd = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]) + 1e-20;
@@ -156,10 +199,10 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   // rho / rho_ref differs from 1 by ~drho dt / rho: the difference of two
   // such ratios cancels to a few significant digits, so it is taken in FP64
-  const double ratio_i = v.rho[i] / v.rho_ref[i];
+  const double ratio_i = v.rho[i] / v.rho_ref[i], ratio_i1 = ratio_i - 1.0;
   const bool anto = P.delsph == 2;
   T flt = 0, rx = 0, ry = 0, rz = 0;
-  for_nbrs4<true>(v, i, true, [&](const Nb<R>& q, bool on) {
+  auto term = [&](const auto& q, bool on) {
     const bool use = on && q.pt <= 1000;
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
     T r = sqrt(dx * dx + dy * dy + dz * dz) + (T)1e-20;
@@ -170,12 +213,14 @@ __global__ __launch_bounds__(kBlock) void k_prep(DevParams P, SphView<R> v) {
       const T sd = dw / r;
       T gx = sd * dx, gy = sd * dy, gz = sd * dz;
       kgc_apply(P, v.cm, n, i, gx, gy, gz);
-      T c = (T)(-(q.rho / q.rref - ratio_i)) * Vj;
+      T c = (T)(-ratio_diff(q, ratio_i, ratio_i1)) * Vj;
       rx += use ? c * gx : (T)0;
       ry += use ? c * gy : (T)0;
       rz += use ? c * gz : (T)0;
     }
-  });
+  };
+  if constexpr (std::is_same<R, float>::value) for_nbrs4_r32<false>(v, i, true, term);
+  else for_nbrs4<true>(v, i, true, term);
   if (P.filt_step) v.flt_s[i] = flt;
   if (anto) {
     v.grx[i] = rx;
@@ -683,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
   const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
   const T uxi = v.ux[i], uyi = v.uy[i], uzi = v.uz[i], ti = v.temp[i], c = (T)P.c_xsph;
   T tx = 0, ty = 0, tz = 0, tt = 0, fl = 0;
-  for_nbrs4<true>(v, i, false, [&](const Nb<R>& q, bool on) {
+  auto term = [&](const auto& q, bool on) {
     int ptj = q.pt;
     const bool use = on && (ptj == 1 || ptj == 3);
     T dx = (T)q.dx, dy = (T)q.dy, dz = (T)q.dz;
@@ -695,7 +740,9 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DevParams P, SphView<R> v) {
     tz += use ? c * vj * (-uzi + (T)q.uz) * w : (T)0;
     tt += use ? (T)0.01 * c * vj * ((T)q.temp - ti) * w : (T)0;
     fl += use ? vj * w : (T)0;
-  });
+  };
+  if constexpr (std::is_same<R, float>::value) for_nbrs4_r32<true>(v, i, false, term);
+  else for_nbrs4<true>(v, i, false, term);
   const T on = (T)(pti > 0);
   v.xsx[i] = (uxi + tx / fl) * on;
   v.xsy[i] = (uyi + ty / fl) * on;
@@ -799,21 +846,29 @@ template int monitor_max<double>(hipStream_t, int, const int*, const int*, const
   } while (0)
 
 template <typename R> int sph_mass_init(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_mass_init, P, v); }
-template <typename R> int sph_kgc(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_kgc, P, v); }
+template <typename R> int sph_kgc(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  TRY(pack_r32(s, v, false, false));
+  LAUNCH(k_kgc, P, v);
+}
 template <typename R>
 int sph_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dc) {
+  TRY(pack_r32(s, v, false, false));
   LAUNCH(k_porosity_sph, P, v, d, dc);
 }
 template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
     k_prep_prep<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
+    TRY(pack_r32(s, v, P.delsph == 2, false));
     k_prep<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
   }
   return 0;
 }
-template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_prep, P, v); }
+template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) {
+  TRY(pack_r32(s, v, P.delsph == 2, false));
+  LAUNCH(k_prep, P, v);
+}
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
     if (packed_geometry<R>(P)) k_pack_records<R, true><<<grid_for(v.n), kBlock, 0, s>>>(v);
@@ -869,6 +924,7 @@ template <typename R> int sph_predictor(hipStream_t s, const DevParams& P, const
 template <typename R> int sph_corrector(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_corrector, P, v); }
 template <typename R> int sph_xsph_only(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
+    TRY(pack_r32(s, v, false, true));
     k_xsph<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
   }
@@ -876,6 +932,7 @@ template <typename R> int sph_xsph_only(hipStream_t s, const DevParams& P, const
 }
 template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
+    TRY(pack_r32(s, v, false, true));
     k_xsph<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
     k_xsph_calc<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
