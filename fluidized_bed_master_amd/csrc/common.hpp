@@ -670,6 +670,8 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
 // KERNEL_clc_prep3D_sph alone, without the ISPH mass/density reset (the WCSPH
 // driver integrates density by continuity)
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// k_prep_prep alone (the ISPH density reset that sph_prep runs before its sums)
+template <typename R> int sph_prep_prep(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v);
 // fluid Shepard sums of the flux walls (no-op without a wall heat flux)
 template <typename R> int sph_flux_walls(hipStream_t s, const DevParams& P, const SphView<R>& v);

@@ -122,6 +122,8 @@ struct sophia_ctx {
   virtual int prep() = 0;
   virtual int interaction() = 0;
   virtual int dem_interaction() = 0;
+  // interaction + dem_interaction with the sphere contact beside advection
+  virtual int interaction_both() = 0;
   virtual int coupling() = 0;
   virtual int time_stage(int phase) = 0;
   virtual int ppe(const sophia_ppe_opts* o, sophia_ppe_stats* st) = 0;
@@ -223,13 +225,21 @@ struct Ctx : public sophia_ctx {
   double* dtab = nullptr;
   Mlp mlp;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream for sphere-side kernels that overlap SPH kernels they do
+  // not share data with (default priority, like the main stream)
+  hipStream_t dem_stream = nullptr;
+  hipEvent_t ev_dem_in = nullptr, ev_dem_out = nullptr;
 
   ~Ctx() override {
+    if (dem_stream) (void)hipStreamSynchronize(dem_stream);
     ppe_workspace_free(ppe_w);
     mlp_free(mlp);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_dem_in) (void)hipEventDestroy(ev_dem_in);
+    if (ev_dem_out) (void)hipEventDestroy(ev_dem_out);
     arena.release();
+    if (dem_stream) (void)hipStreamDestroy(dem_stream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -362,9 +372,12 @@ struct Ctx : public sophia_ctx {
     time = prm.time;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&dem_stream, hipStreamNonBlocking));
     timers.stream = stream;
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventCreateWithFlags(&ev_dem_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_dem_out, hipEventDisableTiming));
     setup_domain(P, n);
     std::vector<part1> tagged;
     if (p.open_boundary == 1) {
@@ -761,6 +774,28 @@ struct Ctx : public sophia_ctx {
     if (p.dim != 3) return 0;
     if (on) TRY(sph_mass_init<R>(stream, P, sv));
     if (on && P.prop_table) TRY(sph_props<R>(stream, P, sv));
+    // The spheres' porosity reads SPH positions, m, rho and p_type only, which
+    // KGC and the SPH porosity leave alone, and writes sphere fields only: it
+    // runs beside them on the second stream and joins before k_prep_prep
+    // rewrites m and rho.
+    const bool side = nd > 0;
+    if (side) {
+      HIP_TRY(hipEventRecord(ev_dem_in, stream));
+      HIP_TRY(hipStreamWaitEvent(dem_stream, ev_dem_in, 0));
+      Timers* T = tm();
+      hipEvent_t ta = nullptr, tb = nullptr;
+      if (T && T->on) {
+        ta = T->get();
+        tb = T->get();
+        HIP_TRY(hipEventRecord(ta, dem_stream));
+      }
+      TRY(dem_porosity<R>(dem_stream, P, dv, sv, sc, dc));
+      if (T && T->on) {
+        HIP_TRY(hipEventRecord(tb, dem_stream));
+        T->add_span("porosity_dem", ta, tb);
+      }
+      HIP_TRY(hipEventRecord(ev_dem_out, dem_stream));
+    }
     if (on && p.kgc_solve > 0) {
       TimeScope t(tm(), "kgc");
       TRY(sph_kgc<R>(stream, P, sv));
@@ -769,15 +804,24 @@ struct Ctx : public sophia_ctx {
       TimeScope t(tm(), "porosity_sph");
       TRY(sph_porosity<R>(stream, P, sv, dv, dc));
     }
-    {
-      TimeScope t(tm(), "porosity_dem");
-      TRY(dem_porosity<R>(stream, P, dv, sv, sc, dc));
-    }
-    if (on) {
+    if (side) HIP_TRY(hipStreamWaitEvent(stream, ev_dem_out, 0));
+    if (on && P.heat_flux != 0.0) {
+      // the flux walls' fluid sums read m and rho after the density reset and
+      // write flux_s only, which the prep sums do not read: they run beside
+      // the prep sums on the second stream
+      TimeScope t(tm(), "prep");
+      TRY(sph_prep_prep<R>(stream, P, sv));
+      HIP_TRY(hipEventRecord(ev_dem_in, stream));
+      HIP_TRY(hipStreamWaitEvent(dem_stream, ev_dem_in, 0));
+      TRY(sph_flux_walls<R>(dem_stream, P, sv));
+      HIP_TRY(hipEventRecord(ev_dem_out, dem_stream));
+      const int rc = sph_prep_sums<R>(stream, P, sv);
+      HIP_TRY(hipStreamWaitEvent(stream, ev_dem_out, 0));
+      TRY(rc);
+    } else if (on) {
       TimeScope t(tm(), "prep");
       TRY(sph_prep<R>(stream, P, sv));
     }
-    if (on && P.heat_flux != 0.0) TRY(sph_flux_walls<R>(stream, P, sv));
     return 0;
   }
 
@@ -786,6 +830,40 @@ struct Ctx : public sophia_ctx {
     if (!sph_on() || p.dim != 3) return 0;
     TimeScope t(tm(), "advection");
     return sph_advection<R>(stream, devparams(), sv);
+  }
+
+  // The sphere contact reads and writes sphere state only and advection SPH
+  // state only: the contact runs on the second stream beside advection, and
+  // the main stream waits for it right after advection is queued (also when
+  // queueing advection fails), so no later call sees it in flight.
+  int interaction_both() override {
+    TRY(need_nnps());
+    if (p.dim != 3 || nd == 0) {
+      TRY(interaction());
+      return dem_interaction();
+    }
+    CellView dc{dcs, dce};
+    HIP_TRY(hipEventRecord(ev_dem_in, stream));
+    HIP_TRY(hipStreamWaitEvent(dem_stream, ev_dem_in, 0));
+    Timers* T = tm();
+    hipEvent_t ta = nullptr, tb = nullptr;
+    if (T && T->on) {
+      ta = T->get();
+      tb = T->get();
+      HIP_TRY(hipEventRecord(ta, dem_stream));
+    }
+    TRY(dem_contact(dem_stream, devparams(), dv, dc));
+    if (T && T->on) {
+      HIP_TRY(hipEventRecord(tb, dem_stream));
+      T->add_span("dem_contact", ta, tb);
+    }
+    HIP_TRY(hipEventRecord(ev_dem_out, dem_stream));
+    std::swap(hist_idx[0], hist_idx[1]);
+    std::swap(hist_ds[0], hist_ds[1]);
+    refresh_views();
+    const int rc = interaction();
+    HIP_TRY(hipStreamWaitEvent(stream, ev_dem_out, 0));
+    return rc;
   }
 
   int dem_interaction() override {
@@ -2377,8 +2455,7 @@ int sophia_step(sophia_ctx* c, const sophia_ppe_opts* o, sophia_ppe_stats* st) {
   }
   TRY(c->nnps());
   TRY(c->prep());
-  TRY(c->interaction());
-  TRY(c->dem_interaction());
+  TRY(c->interaction_both());
   TRY(c->coupling());
   TRY(c->time_stage(SOPHIA_PREDICT));
   TRY(c->ppe(o, st));

@@ -865,6 +865,7 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
   }
   return 0;
 }
+template <typename R> int sph_prep_prep(hipStream_t s, const DevParams& P, const SphView<R>& v) { LAUNCH(k_prep_prep, P, v); }
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   TRY(pack_r32(s, v, P.delsph == 2, false));
   LAUNCH(k_prep, P, v);
@@ -947,6 +948,7 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_porosity<R>(hipStream_t, const DevParams&, const SphView<R>&, const DemView&, const CellView&); \
   template int sph_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                               \
   template int sph_prep_sums<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_prep_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_flux_walls<R>(hipStream_t, const DevParams&, const SphView<R>&);                         \
   template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
