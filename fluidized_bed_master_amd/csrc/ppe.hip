@@ -760,6 +760,70 @@ __global__ __launch_bounds__(kRedThreads, 8) void k_bi_mv_staged(PpeOp o, const 
   block_partials<2>(d, partials);
 }
 
+// The FP64 operator passes (y = A x or rhs - A x with an optional norm, and
+// the true residual after a fold) with x64 staged in LDS per tile like the
+// inner matvec (kFmtRun8).  Each thread sums the same rows in the same order
+// as the unstaged kernels, so the bits are the same.
+template <int CM>
+__global__ __launch_bounds__(kRedThreads) void k_apply_staged(PpeOp o, const double* __restrict__ x,
+                                                              double* __restrict__ y, int residual, int want_norm,
+                                                              double* partials) {
+  __shared__ Stage<double> S;
+  double d[1] = {0.0};
+  const RowIter it = xcd_rows(o.n);
+  const int t = (int)threadIdx.x;
+  for (int i0 = it.i - t; i0 < it.end; i0 += it.stride) {
+    const int i1 = min(i0 + (int)blockDim.x, it.end);
+    const bool staged = stage_tile(o, x, i0, i1, S);
+    const int i = i0 + t;
+    if (i < i1) {
+      double acc = 0;
+      if (o.row[i]) {
+        const double sm = staged ? row_sum<kFmtRun8, CM, double, double, true>(o, i, S.x, S.adj)
+                                 : row_sum<kFmtRun8, CM, double>(o, i, x);
+        acc = o.diag[i] * x[i] - sm;
+        if (residual) acc = o.rhs[i] - acc;
+      }
+      y[i] = acc;
+      d[0] += acc * acc;
+    }
+  }
+  if (want_norm) block_partials<1>(d, partials);
+}
+
+template <int CMF, typename V>
+__global__ __launch_bounds__(kRedThreads) void k_bi_true_staged(PpeOp o, const double* __restrict__ x64,
+                                                                double* __restrict__ r64, V* __restrict__ r,
+                                                                const V* __restrict__ rh, double* partials,
+                                                                const BiState* st) {
+  if (st->done || !st->update) return;
+  const double inv = 1.0 / st->xscale;
+  __shared__ Stage<double> S;
+  double d[3] = {0.0, 0.0, 0.0};
+  const RowIter it = xcd_rows(o.n);
+  const int t = (int)threadIdx.x;
+  for (int i0 = it.i - t; i0 < it.end; i0 += it.stride) {
+    const int i1 = min(i0 + (int)blockDim.x, it.end);
+    const bool staged = stage_tile(o, x64, i0, i1, S);
+    const int i = i0 + t;
+    if (i < i1) {
+      double acc = 0;
+      if (o.row[i]) {
+        const double sm = staged ? row_sum<kFmtRun8, CMF, double, double, true>(o, i, S.x, S.adj)
+                                 : row_sum<kFmtRun8, CMF, double>(o, i, x64);
+        acc = o.rhs[i] - (o.diag[i] * x64[i] - sm);
+      }
+      r64[i] = acc;
+      const V ri = (V)(acc * inv);
+      r[i] = ri;
+      d[0] += (double)ri * (double)ri;
+      d[1] += (double)rh[i] * (double)ri;
+      d[2] += acc * acc;
+    }
+  }
+  block_partials<3>(d, partials);
+}
+
 // s = r - alpha v (in place in r); sh = D^-1 s
 template <typename V>
 __global__ __launch_bounds__(kRedThreads) void k_bi_s(int n, const double* __restrict__ dinv, V* __restrict__ r,
@@ -974,7 +1038,7 @@ template <int CM>
 void launch_apply_full(hipStream_t s, const PpeWork& w, const PpeOp& o, const double* x, double* y, bool residual,
                        bool norm) {
   if (o.fmt == kFmtRun8)
-    k_apply_full<kFmtRun8, CM><<<w.nbg, kRedThreads, 0, s>>>(o, x, y, residual, norm, w.partials);
+    k_apply_staged<CM><<<w.nbg, kRedThreads, 0, s>>>(o, x, y, residual, norm, w.partials);
   else
     k_apply_full<kFmtIdx32, CM><<<w.nbg, kRedThreads, 0, s>>>(o, x, y, residual, norm, w.partials);
   if (norm) launch_fin(s, w, kFinNorm);
@@ -1311,7 +1375,8 @@ int bicgstab_run(hipStream_t s, PpeWork& w, const CoarseWork* cw, double rn, dou
       k_bi_xr<V><<<G, kRedThreads, 0, s>>>(n, xs, ph, sh, r, t, rh, w.partials, st);
       launch_fin(s, w, kFinRho);
       k_bi_fold<V><<<G, kRedThreads, 0, s>>>(n, w.row, w.x64, xs, st);
-      k_bi_true<FMT, CMF, V><<<G, kRedThreads, 0, s>>>(o, w.x64, w.r64, r, rh, w.partials, st);
+      if (FMT == kFmtRun8) k_bi_true_staged<CMF, V><<<G, kRedThreads, 0, s>>>(o, w.x64, w.r64, r, rh, w.partials, st);
+      else k_bi_true<FMT, CMF, V><<<G, kRedThreads, 0, s>>>(o, w.x64, w.r64, r, rh, w.partials, st);
       launch_fin(s, w, kFinTrue);
     }
     HIP_TRY(hipGetLastError());
