@@ -670,6 +670,10 @@ template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphV
 // KERNEL_clc_prep3D_sph alone, without the ISPH mass/density reset (the WCSPH
 // driver integrates density by continuity)
 template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const SphView<R>& v);
+// sph_kgc (when kgc) and sph_porosity in one pass over the list on the mixed path
+template <typename R>
+int sph_kgc_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dc,
+                     bool kgc);
 // k_prep_prep alone (the ISPH density reset that sph_prep runs before its sums)
 template <typename R> int sph_prep_prep(hipStream_t s, const DevParams& P, const SphView<R>& v);
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v);

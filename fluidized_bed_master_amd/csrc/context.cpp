@@ -796,13 +796,9 @@ struct Ctx : public sophia_ctx {
       }
       HIP_TRY(hipEventRecord(ev_dem_out, dem_stream));
     }
-    if (on && p.kgc_solve > 0) {
-      TimeScope t(tm(), "kgc");
-      TRY(sph_kgc<R>(stream, P, sv));
-    }
     if (on) {
-      TimeScope t(tm(), "porosity_sph");
-      TRY(sph_porosity<R>(stream, P, sv, dv, dc));
+      TimeScope t(tm(), "kgc_porosity_sph");
+      TRY(sph_kgc_porosity<R>(stream, P, sv, dv, dc, p.kgc_solve > 0));
     }
     if (side) HIP_TRY(hipStreamWaitEvent(stream, ev_dem_out, 0));
     if (on && P.heat_flux != 0.0) {

@@ -150,6 +150,7 @@ struct NbR {
   float ux, uy, uz, temp;
   int pt;
   int j;
+  bool flag;  // the entry's eps-rule flag (kNbrFlag), for kernels that apply it themselves
 };
 
 // The list in batches of four like for_nbrs4<true>, reading r32 (and with
@@ -178,6 +179,7 @@ __device__ inline void for_nbrs4_r32(const SphView<float>& v, int i, bool eps_ru
         q[u].ux = w.x; q[u].uy = w.y; q[u].uz = w.z; q[u].temp = w.w;
       }
       q[u].j = j;
+      q[u].flag = (en[u] & kNbrFlag) != 0;
     }
     en4 = *(const uint4*)(v.nbr + nbr_slot(min(k + 4, last4), i, n));
 #pragma unroll

@@ -165,6 +165,90 @@ __global__ __launch_bounds__(kBlock) void k_porosity_sph(DevParams P, SphView<R>
   }
 }
 
+// k_kgc and k_porosity_sph in one pass over the neighbour list (mixed path):
+// the same 32 B records, each kernel's own terms, neighbour rules (KGC takes
+// every list entry, the porosity only those without the eps flag) and
+// arithmetic, so the results are those of the two kernels.
+template <bool KGC>
+__global__ __launch_bounds__(kBlock) void k_kgc_porosity(DevParams P, SphView<float> v, DemView d, CellView dc) {
+  typedef float T;
+  const int i = particle_index();
+  if (i >= v.n) return;
+  const int n = v.n;
+  const int iti = v.itype[i], pti = v.ptype[i];
+  const bool krow = KGC && !(iti > 2 || pti > 1000);
+  if (KGC && !krow)
+    for (int c = 0; c < 6; c++) v.cm[c * n + i] = 0;
+  if (iti > 2) return;
+  v.dpor_prev[i] = v.dpor[i];
+  const double hd = v.h[i];
+  const T h = (T)hd, ih = (T)(1.0 / hd), A = knl_tmpA<T>(P.kernel_type, P.dim, h), Rk = (T)(P.kappa * hd);
+  const double Rs = P.kappa * hd;
+  const double xi = v.x[i], yi = v.y[i], zi = v.z[i];
+  T xx = 0, yy = 0, zz = 0, xy = 0, yz = 0, zx = 0;
+  T flt = 0, por = 0, dfx = 0, dfy = 0, dfz = 0;
+  for_nbrs4_r32<false>(v, i, false, [&](const NbR& q, bool on) {
+    const T dx = q.dx, dy = q.dy, dz = q.dz;
+    const T r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (krow) {
+      const bool use = on && q.pt <= 1000 && r > 0 && r < Rk;
+      T dw = knl_dw_ih<T>(P.kernel_type, A, h, ih, r);
+      T mtd = q.m * dw;
+      T rtd = (T)1 / (q.rho * r);
+      xx += use ? -mtd * dx * dx * rtd : (T)0;
+      xy += use ? -mtd * dy * dx * rtd : (T)0;
+      yy += use ? -mtd * dy * dy * rtd : (T)0;
+      zx += use ? -mtd * dx * dz * rtd : (T)0;
+      yz += use ? -mtd * dy * dz * rtd : (T)0;
+      zz += use ? -mtd * dz * dz * rtd : (T)0;
+    }
+    const T rp = r + (T)1e-20;
+    const T w = knl_w<T>(P.kernel_type, A, h, rp);
+    const bool usep = on && !q.flag && pti <= 1000 && q.pt <= 1000 && q.pt == 1;
+    flt += usep ? q.m / q.rho * w : (T)0;
+  });
+  if (krow) {
+    T det = xx * (yy * zz - yz * yz);
+    det -= xy * (xy * zz - yz * zx);
+    det += zx * (xy * yz - yy * zx);
+    T c0, c1, c2, c3, c4, c5;
+    if (fabs((double)det) > 1e-4) {
+      T rr = (T)1 / det;
+      c0 = (yy * zz - yz * yz) * rr;
+      c1 = (zx * yz - xy * zz) * rr;
+      c2 = (xy * yz - zx * yy) * rr;
+      c3 = (xx * zz - zx * zx) * rr;
+      c4 = (zx * xy - xx * yz) * rr;
+      c5 = (xx * yy - xy * xy) * rr;
+    } else {
+      c0 = 1; c1 = 0; c2 = 0; c3 = 1; c4 = 0; c5 = 1;
+    }
+    v.cm[i] = c0; v.cm[n + i] = c1; v.cm[2 * n + i] = c2;
+    v.cm[3 * n + i] = c3; v.cm[4 * n + i] = c4; v.cm[5 * n + i] = c5;
+  }
+  if (d.n > 0) {
+    for_cells(P, xi, yi, zi, dc, [&](int j) {
+      double rd = dist_exact(xi - d.x[j], yi - d.y[j], zi - d.z[j]) + 1e-20;
+      if (!(rd < Rs)) return;
+      if (!(pti <= 1000 && d.ptype[j] > 1000)) return;
+      T dx = (T)(xi - d.x[j]), dy = (T)(yi - d.y[j]), dz = (T)(zi - d.z[j]);
+      T r = (T)rd;
+      T w = knl_w<T>(P.kernel_type, A, h, r);
+      T dw = knl_dw<T>(P.kernel_type, A, h, r);
+      T rj = (T)d.rad[j];
+      T Vj = (T)4.18879 * rj * rj * rj;
+      por += w * Vj;
+      dfx += dw * dx / r * (T)d.ux[j] * Vj;
+      dfy += dw * dy / r * (T)d.uy[j] * Vj;
+      dfz += dw * dz / r * (T)d.uz[j] * Vj;
+    });
+  }
+  if (pti <= 1000) {
+    v.por[i] = (T)1 - (por / (flt + (T)1e-20));
+    v.dpor[i] = (dfx + dfy + dfz) / (flt + (T)1e-20);
+  }
+}
+
 // KERNEL_clc_prep3D_prep_sph (functions_PREP2.cuh:1623-1679)
 template <typename R>
 __global__ void k_prep_prep(DevParams P, SphView<R> v) {
@@ -855,6 +939,21 @@ int sph_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const D
   TRY(pack_r32(s, v, false, false));
   LAUNCH(k_porosity_sph, P, v, d, dc);
 }
+template <typename R>
+int sph_kgc_porosity(hipStream_t s, const DevParams& P, const SphView<R>& v, const DemView& d, const CellView& dc,
+                     bool kgc) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (v.n <= 0) return 0;
+    TRY(pack_r32(s, v, false, false));
+    if (kgc) k_kgc_porosity<true><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, d, dc);
+    else k_kgc_porosity<false><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v, d, dc);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  } else {
+    if (kgc) TRY(sph_kgc(s, P, v));
+    return sph_porosity(s, P, v, d, dc);
+  }
+}
 template <typename R> int sph_prep(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
     k_prep_prep<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
@@ -949,6 +1048,8 @@ template <typename R> int sph_xsph(hipStream_t s, const DevParams& P, const SphV
   template int sph_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                               \
   template int sph_prep_sums<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_prep_prep<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
+  template int sph_kgc_porosity<R>(hipStream_t, const DevParams&, const SphView<R>&, const DemView&,         \
+                                   const CellView&, bool);                                                  \
   template int sph_advection<R>(hipStream_t, const DevParams&, const SphView<R>&);                          \
   template int sph_flux_walls<R>(hipStream_t, const DevParams&, const SphView<R>&);                         \
   template int sph_neumann<R>(hipStream_t, const DevParams&, const SphView<R>&, const CellView&);           \
