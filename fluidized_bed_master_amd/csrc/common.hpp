@@ -152,6 +152,15 @@ struct alignas(64) RecAsm {
   int flags;
 };
 
+// Neumann walls (mixed path): a candidate's pos4, and for those within
+// range V = m/rho (FP32), p and rho/eps (FP64), T and p_type, on one line.
+struct alignas(64) RecNeu {
+  float4 p;
+  double pres, rhopor;
+  float vol, temp;
+  int pt;
+};
+
 template <typename R>
 struct SphView {
   int n;
@@ -188,6 +197,7 @@ struct SphView {
   HRec<R>* pkh;
   Rec32* r32;
   RecAsm* ra;
+  RecNeu* rn;
   R *xsx, *xsy, *xsz, *xst; // XSPH results
   // Positions as FP32 offsets inside a grid of power-of-two spacing pgrid
   // anchored at (pg0x, pg0y, pg0z), the grid cell packed in w (10 bits per

@@ -511,6 +511,20 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
 // KERNEL_Neumann_boundary3D_sph (functions_BC.cuh:139-274): wall particles
 // take a Shepard interpolation of fluid p, T and eps-weighted rho over the
 // 27 cells within 2*kappa*h, with W(2h) normalised by A(h).
+// the Neumann walls' candidate records (mixed path), before each pass
+__global__ void k_pack_neu(SphView<float> v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  RecNeu r;
+  r.p = v.pos4[i];
+  r.pres = v.pres[i];
+  r.rhopor = v.rho[i] / v.por[i];
+  r.vol = (float)v.m[i] / (float)v.rho[i];
+  r.temp = v.temp[i];
+  r.pt = v.ptype[i];
+  v.rn[i] = r;
+}
+
 constexpr int kNeumannLanes = 8;
 
 template <typename R>
@@ -541,24 +555,37 @@ __global__ void k_neumann(DevParams P, SphView<R> v, CellView cv, double tol_abs
   group_for_cells<kNeumannLanes>(P, xi, yi, zi, cv, [&](int j) {
     double rd;
     if constexpr (REL) {
-      const float3 d = rel_diff(pi4, v.pos4[j], v.pgrid);
+      // one 64 B record (k_pack_neu): the range test's pos4 brings the
+      // fields of a candidate in range with it
+      const RecNeu q = v.rn[j];
+      const float3 d = rel_diff(pi4, q.p, v.pgrid);
       const float d2 = d.x * d.x + d.y * d.y + d.z * d.z;
       if (d2 > hi2) return;  // r >= R2 for certain
       rd = d2 < lo2 ? (double)sqrtf(d2) + 1e-20 : dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+      const int ptj = q.pt;
+      if (!(ptj < 1000)) return;
+      if (!(rd < R2)) return;
+      if (!(ptj >= 1)) return;
+      T w = knl_w<T>(P.kernel_type, A, H2, (T)rd);
+      const T wv = (T)q.vol * w;
+      tp += (double)wv * q.pres;
+      fl += (double)wv;
+      tt += wv * (T)q.temp;
+      tr += q.rhopor * (double)wv;
     } else {
       rd = dist_exact(xi - v.x[j], yi - v.y[j], zi - v.z[j]) + 1e-20;
+      int ptj = v.ptype[j];
+      if (!(ptj < 1000)) return;
+      if (!(rd < R2)) return;
+      if (!(ptj >= 1)) return;
+      T w = knl_w<T>(P.kernel_type, A, H2, (T)rd);
+      T vj = (T)v.m[j] / (T)v.rho[j];
+      const T wv = vj * w;
+      tp += (double)wv * v.pres[j];
+      fl += (double)wv;
+      tt += wv * (T)v.temp[j];
+      tr += v.rho[j] / v.por[j] * (double)wv;
     }
-    int ptj = v.ptype[j];
-    if (!(ptj < 1000)) return;
-    if (!(rd < R2)) return;
-    if (!(ptj >= 1)) return;
-    T w = knl_w<T>(P.kernel_type, A, H2, (T)rd);
-    T vj = (T)v.m[j] / (T)v.rho[j];
-    const T wv = vj * w;
-    tp += (double)wv * v.pres[j];
-    fl += (double)wv;
-    tt += wv * (T)v.temp[j];
-    tr += v.rho[j] / v.por[j] * (double)wv;
   });
   tp = group_sum<kNeumannLanes>(tp);
   fl = group_sum<kNeumannLanes>(fl);
@@ -989,6 +1016,7 @@ template <typename R> int sph_neumann(hipStream_t s, const DevParams& P, const S
   const double m = fmax(fmax(fmax(fabs(P.x_min), fabs(P.x_max)), fmax(fabs(P.y_min), fabs(P.y_max))),
                         fmax(fabs(P.z_min), fabs(P.z_max)));
   if (v.n > 0) {
+    if constexpr (std::is_same<R, float>::value) k_pack_neu<<<grid_for(v.n), kBlock, 0, s>>>(v);
     const size_t nt = (size_t)v.n * kNeumannLanes;
     k_neumann<R><<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, v, cv, m * (1.0 / 2097152.0));
     HIP_TRY(hipGetLastError());
