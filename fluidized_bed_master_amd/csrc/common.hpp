@@ -152,6 +152,18 @@ struct alignas(64) RecAsm {
   int flags;
 };
 
+// Advection's neighbour records on the mixed path, after pk = {x, y, z, m}:
+// {rho, p, u, T} and {S (the flux-wall sum), h, p_type}
+struct alignas(32) PkB {
+  double rho, p;
+  float ux, uy, uz, temp;
+};
+struct alignas(16) PkC {
+  double S;
+  float h;
+  int pt;
+};
+
 // Neumann walls (mixed path): a candidate's pos4, and for those within
 // range V = m/rho (FP32), p and rho/eps (FP64), T and p_type, on one line.
 struct alignas(64) RecNeu {
@@ -198,6 +210,8 @@ struct SphView {
   Rec32* r32;
   RecAsm* ra;
   RecNeu* rn;
+  PkB* pkb;
+  PkC* pkc;
   R *xsx, *xsy, *xsz, *xst; // XSPH results
   // Positions as FP32 offsets inside a grid of power-of-two spacing pgrid
   // anchored at (pg0x, pg0y, pg0z), the grid cell packed in w (10 bits per

@@ -431,6 +431,8 @@ struct Ctx : public sophia_ctx {
     TRY(arena.alloc(&v.r32, NS));
     TRY(arena.alloc(&v.ra, NS));
     TRY(arena.alloc(&v.rn, NS));
+    TRY(arena.alloc(&v.pkb, NS));
+    TRY(arena.alloc(&v.pkc, NS));
     TRY(arena.alloc(&v.pos4, NS));
     // DEM persistent
     for (auto* f : {&dx_, &dy_, &dz_, &dux, &duy, &duz, &dwx, &dwy, &dwz, &dm, &dri, &drad, &dh, &drho, &dtemp, &dfdx,

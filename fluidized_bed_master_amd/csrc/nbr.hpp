@@ -89,12 +89,14 @@ struct NbP {
   R ux, uy, uz, temp, h;
   int pt;
   int j;
+  double S;  // flux-wall sum (mixed path with VEL; otherwise flux_s[j])
 };
 
 // The list in batches of four like for_nbrs4, reading the packed records
 // pk = {x, y, z, m} and pk2 = {rho, p}, then with VEL pkv = {ux, uy, uz, T}
 // and pkh = {h, p_type}, else p_type: four or three gathers per neighbour in
-// place of twelve or seven.  dx = x_i - x_j (minimum image on periodic axes),
+// place of twelve or seven.  With VEL on the mixed path pk, pkb = {rho, p,
+// u, T} and pkc = {S, h, p_type}: three.  dx = x_i - x_j (minimum image on periodic axes),
 // the same bits as pdiff.
 template <bool VEL, typename R, class F>
 __device__ inline void for_nbrs4_pk(const SphView<R>& v, int i, bool eps_rule, F&& f) {
@@ -111,11 +113,21 @@ __device__ inline void for_nbrs4_pk(const SphView<R>& v, int i, bool eps_rule, F
     for (int u = 0; u < 4; u++) {
       const int j = (int)(en[u] & ~kNbrFlag);
       const double4 a = v.pk[j];
-      const double2 b = v.pk2[j];
       q[u].dx = pi.x - a.x;
       q[u].dy = pi.y - a.y;
       q[u].dz = pi.z - a.z;
       q[u].m = a.w;
+      if constexpr (VEL && std::is_same<R, float>::value) {
+        // mixed path: two more records, {rho, p, u, T} and {S, h, p_type}
+        const PkB b = v.pkb[j];
+        const PkC c = v.pkc[j];
+        q[u].rho = b.rho; q[u].pres = b.p;
+        q[u].ux = b.ux; q[u].uy = b.uy; q[u].uz = b.uz; q[u].temp = b.temp;
+        q[u].S = c.S; q[u].h = c.h; q[u].pt = c.pt;
+        q[u].j = j;
+        continue;
+      }
+      const double2 b = v.pk2[j];
       q[u].rho = b.x;
       q[u].pres = b.y;
       if constexpr (VEL) {

@@ -353,10 +353,23 @@ __global__ void k_props(DevParams P, SphView<R> v) {
 // The packed neighbour records of the FP64-geometry pair sums (for_nbrs4_pk),
 // rebuilt from the SoA fields before each kernel that reads them.
 template <typename R, bool VEL>
-__global__ void k_pack_records(SphView<R> v) {
+__global__ void k_pack_records(SphView<R> v, bool flux) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= v.n) return;
   v.pk[i] = make_double4(v.x[i], v.y[i], v.z[i], v.m[i]);
+  if constexpr (VEL && std::is_same<R, float>::value) {
+    PkB b;
+    b.rho = v.rho[i];
+    b.p = v.pres[i];
+    b.ux = v.ux[i]; b.uy = v.uy[i]; b.uz = v.uz[i]; b.temp = v.temp[i];
+    v.pkb[i] = b;
+    PkC c;
+    c.S = flux ? v.flux_s[i] : 0.0;
+    c.h = (float)v.h[i];
+    c.pt = v.ptype[i];
+    v.pkc[i] = c;
+    return;
+  }
   v.pk2[i] = make_double2(v.rho[i], v.pres[i]);
   if constexpr (VEL) {
     typename Vec4Of<R>::type w;
@@ -416,7 +429,9 @@ __global__ __launch_bounds__(kBlock) void k_advection(DevParams P, SphView<R> v)
     const bool use = on && q.pt < 1000;
     if constexpr (FLUX) {
       // wall heat flux from a flux wall j: q A_j W_ij / S_j
-      const double S = v.flux_s[q.j];
+      double S;
+      if constexpr (std::is_same<std::decay_t<decltype(q)>, NbP<float>>::value) S = q.S;
+      else S = v.flux_s[q.j];
       if (use && S > 0.0) {
         const double rw = sqrt(q.dx * q.dx + q.dy * q.dy + q.dz * q.dz), a = v.h[q.j] / P.h_over_dx;
         fl += P.heat_flux * (a * a) * knl_w_ih<double>(P.kernel_type, Ad, hd, ihd, rw) / S;
@@ -998,7 +1013,7 @@ template <typename R> int sph_prep_sums(hipStream_t s, const DevParams& P, const
 }
 template <typename R> int sph_advection(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n > 0) {
-    if (packed_geometry<R>(P)) k_pack_records<R, true><<<grid_for(v.n), kBlock, 0, s>>>(v);
+    if (packed_geometry<R>(P)) k_pack_records<R, true><<<grid_for(v.n), kBlock, 0, s>>>(v, P.heat_flux != 0.0);
     if (P.heat_flux != 0.0) k_advection<R, true><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     else k_advection<R, false><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
     HIP_TRY(hipGetLastError());
@@ -1033,7 +1048,7 @@ template <typename R> int sph_props(hipStream_t s, const DevParams& P, const Sph
 }
 template <typename R> int sph_pressure_force(hipStream_t s, const DevParams& P, const SphView<R>& v) {
   if (v.n <= 0) return 0;
-  if (P.fp_solve && packed_geometry<R>(P)) k_pack_records<R, false><<<grid_for(v.n), kBlock, 0, s>>>(v);
+  if (P.fp_solve && packed_geometry<R>(P)) k_pack_records<R, false><<<grid_for(v.n), kBlock, 0, s>>>(v, false);
   k_pressure_force<R><<<grid_xcd(v.n), kBlock, 0, s>>>(P, v);
   HIP_TRY(hipGetLastError());
   return 0;
