@@ -29,7 +29,8 @@ PPE_JACOBI, PPE_BICGSTAB = 0, 1
 PPE_FP64, PPE_MIXED_IR = 0, 1
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_TWO_LEVEL = 0, 1
-VTK_FLUID, VTK_DEM = 0, 1
+VTK_FLUID, VTK_DEM, VTK_AIR, VTK_SOLID, VTK_MOVING = 0, 1, 2, 3, 4
+VTK_FLUID2, VTK_BOUNDARY, VTK_FLUID_PGF, VTK_ACTIVE, VTK_MOVING_UY = 5, 6, 7, 8, 9
 PREDICT, CORRECT = 0, 1
 COMPAT_NO_SPIN, COMPAT_NUSSELT, COMPAT_HEAT_OVERWRITE, COMPAT_CP_ARGS, COMPAT_ALL = 1, 2, 4, 8, 15
 ERRORS = {

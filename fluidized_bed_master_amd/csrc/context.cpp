@@ -2319,26 +2319,176 @@ int fetch(const sophia_ctx* c, int species, Records& r) {
   return c->get_particles(species, r.p1.data(), r.p2.data(), r.p3.data());
 }
 
-// one float field over the selected particles
+// one array "name ncomp count type" over the selected particles; float
+// arrays are rounded to float (FloatSwap of a double or an int), int arrays
+// are written as they are (IntSwap)
+using Sel = bool (*)(const part1&);
 template <class G>
-void vtk_field(FILE* f, const char* name, int count, const Records& r, bool (*sel)(const part1&), G get) {
+void vtk_field(FILE* f, const char* name, int count, const Records& r, Sel sel, G get) {
   std::fprintf(f, "%s\t1\t%d\tfloat\n", name, count);
   for (size_t i = 0; i < r.p1.size(); i++)
     if (sel(r.p1[i])) put_float(f, get(i));
 }
+template <class G>
+void vtk_int(FILE* f, const char* name, int count, const Records& r, Sel sel, G get) {
+  std::fprintf(f, "%s\t1\t%d\tint\n", name, count);
+  for (size_t i = 0; i < r.p1.size(); i++)
+    if (sel(r.p1[i])) put_be32(f, (uint32_t)get(i));
+}
+// three components per particle, interleaved (the fluid2 writer's vectors)
+template <class G>
+void vtk_vec3(FILE* f, const char* name, int count, const Records& r, Sel sel, G get) {
+  std::fprintf(f, "%s\t3\t%d\tfloat\n", name, count);
+  for (size_t i = 0; i < r.p1.size(); i++)
+    if (sel(r.p1[i]))
+      for (int c = 0; c < 3; c++) put_float(f, get(i, c));
+}
 bool sel_fluid(const part1& q) { return q.p_type == 1; }
+bool sel_fluid2(const part1& q) { return q.p_type == 2; }
+bool sel_air(const part1& q) { return q.p_type == 3; }
+bool sel_wall(const part1& q) { return q.p_type == 0; }
+bool sel_moving(const part1& q) { return q.p_type == 9; }
+bool sel_active(const part1& q) { return q.i_type == 1 || q.i_type == 2; }
 bool sel_dem(const part1& q) { return q.p_type > 1000; }
+bool cnt_dem(const part1& q) { return q.dem_idx > 0; }
+
+// one writer of functions_OUTPUT.cuh: the species it reads, the particles
+// it writes, the predicate its point count uses (the sphere writers count
+// dem_idx > 0 but write p_type > 1000) and its number of arrays
+struct VtkKind {
+  int species;
+  Sel sel, cnt;
+  int nfields;
+};
+const VtkKind kVtkKinds[] = {
+    {SOPHIA_SPH, sel_fluid, sel_fluid, 9},     // SOPHIA_VTK_FLUID
+    {SOPHIA_DEM, sel_dem, cnt_dem, 10},        // SOPHIA_VTK_DEM
+    {SOPHIA_SPH, sel_air, sel_air, 10},        // SOPHIA_VTK_AIR
+    {SOPHIA_DEM, sel_dem, cnt_dem, 10},        // SOPHIA_VTK_SOLID
+    {SOPHIA_SPH, sel_moving, sel_moving, 2},   // SOPHIA_VTK_MOVING
+    {SOPHIA_SPH, sel_fluid2, sel_fluid2, 7},   // SOPHIA_VTK_FLUID2
+    {SOPHIA_SPH, sel_wall, sel_wall, 6},       // SOPHIA_VTK_BOUNDARY
+    {SOPHIA_SPH, sel_fluid, sel_fluid, 12},    // SOPHIA_VTK_FLUID_PGF
+    {SOPHIA_SPH, sel_active, sel_active, 3},   // SOPHIA_VTK_ACTIVE
+    {SOPHIA_SPH, sel_moving, sel_moving, 2},   // SOPHIA_VTK_MOVING_UY
+};
+constexpr int kNumVtkKinds = (int)(sizeof(kVtkKinds) / sizeof(kVtkKinds[0]));
+
+void vtk_arrays(FILE* f, int kind, int n, const Records& r, Sel sel) {
+  const auto& P1 = r.p1;
+  const auto& P2 = r.p2;
+  const auto& P3 = r.p3;
+  switch (kind) {
+    case SOPHIA_VTK_FLUID:  // save_plot_fluid_vtk_bin_fluid_sph (:445-659)
+      vtk_field(f, "porosity", n, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+      vtk_field(f, "conv_rate", n, r, sel, [&](size_t i) { return P1[i].Q_sdf; });
+      vtk_field(f, "pressure", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      vtk_field(f, "cond_rate", n, r, sel, [&](size_t i) { return P1[i].Q_f; });
+      vtk_field(f, "dtemp", n, r, sel, [&](size_t i) { return P3[i].dtemp; });
+      vtk_int(f, "buffer_type", n, r, sel, [&](size_t i) { return P1[i].buffer_type; });
+      vtk_field(f, "density", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_field(f, "temperature", n, r, sel, [&](size_t i) { return P1[i].temp; });
+      vtk_field(f, "uz", n, r, sel, [&](size_t i) { return P1[i].uz; });
+      break;
+    case SOPHIA_VTK_DEM:  // save_plot_fluid_vtk_bin_solid_dem (:1135-1446)
+      vtk_field(f, "itype", n, r, sel, [&](size_t i) { return (double)P1[i].i_type; });
+      vtk_field(f, "mag_uijf", n, r, sel, [&](size_t i) { return P1[i].test6; });
+      vtk_field(f, "Euler", n, r, sel, [&](size_t i) { return P1[i].eliz; });
+      vtk_field(f, "porosity", n, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+      vtk_field(f, "uz", n, r, sel, [&](size_t i) { return P1[i].uz; });
+      vtk_field(f, "Q_sd", n, r, sel, [&](size_t i) { return P1[i].Q_sd; });
+      vtk_field(f, "ftotal_z", n, r, sel, [&](size_t i) { return P3[i].ftotalz; });
+      vtk_field(f, "temperature_surface", n, r, sel, [&](size_t i) { return P1[i].temp; });
+      vtk_field(f, "rad", n, r, sel, [&](size_t i) { return P1[i].rad; });
+      vtk_field(f, "dtemp_dt", n, r, sel, [&](size_t i) { return P3[i].dtemp; });
+      break;
+    case SOPHIA_VTK_AIR:  // save_plot_fluid_vtk_bin_air (:660-822)
+      vtk_field(f, "rho_ref", n, r, sel, [&](size_t i) { return P2[i].rho_ref; });
+      vtk_field(f, "buffer_type", n, r, sel, [&](size_t i) { return (double)P1[i].buffer_type; });
+      vtk_field(f, "uz", n, r, sel, [&](size_t i) { return P1[i].uz; });
+      vtk_field(f, "temperature", n, r, sel, [&](size_t i) { return P1[i].temp; });
+      vtk_field(f, "density", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_field(f, "porosity", n, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+      vtk_field(f, "pressure", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      vtk_field(f, "drho_dt", n, r, sel, [&](size_t i) { return P3[i].drho; });
+      vtk_field(f, "dtemp_dt", n, r, sel, [&](size_t i) { return P3[i].dtemp; });
+      vtk_field(f, "ptype", n, r, sel, [&](size_t i) { return (double)P1[i].p_type; });
+      break;
+    case SOPHIA_VTK_SOLID:  // save_plot_fluid_vtk_bin_solid (:824-1134)
+      vtk_field(f, "dem_idx", n, r, sel, [&](size_t i) { return (double)P1[i].dem_idx; });
+      vtk_field(f, "mag_uijf", n, r, sel, [&](size_t i) { return P1[i].test6; });
+      vtk_field(f, "drag_r", n, r, sel, [&](size_t i) { return P1[i].test7; });
+      vtk_field(f, "porosity", n, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+      vtk_field(f, "uz", n, r, sel, [&](size_t i) { return P1[i].uz; });
+      vtk_field(f, "Q_sd", n, r, sel, [&](size_t i) { return P1[i].Q_sd; });
+      vtk_field(f, "ftotal_z", n, r, sel, [&](size_t i) { return P3[i].ftotalz; });
+      vtk_field(f, "temperature_surface", n, r, sel, [&](size_t i) { return P1[i].temp; });
+      vtk_field(f, "rad", n, r, sel, [&](size_t i) { return P1[i].rad; });
+      vtk_field(f, "dtemp_dt", n, r, sel, [&](size_t i) { return P3[i].dtemp; });
+      break;
+    case SOPHIA_VTK_MOVING:  // save_plot_fluid_vtk_bin_moving (:1447-1578)
+      vtk_field(f, "density", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_field(f, "pressure", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      break;
+    case SOPHIA_VTK_FLUID2:  // save_plot_fluid_vtk_bin_fluid2 (:1579-1702)
+      vtk_field(f, "rho", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_field(f, "pres", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      vtk_field(f, "curvature", n, r, sel, [&](size_t i) { return P3[i].curv; });
+      vtk_field(f, "detection", n, r, sel, [&](size_t i) { return P3[i].lbl_surf; });
+      vtk_vec3(f, "surface_tension", n, r, sel, [&](size_t i, int c) {
+        return c == 0 ? P3[i].fsx : c == 1 ? P3[i].fsy : P3[i].fsz;
+      });
+      vtk_vec3(f, "normal_vector_c", n, r, sel, [&](size_t i, int c) {
+        return c == 0 ? P3[i].nx_c : c == 1 ? P3[i].ny_c : P3[i].nz_c;
+      });
+      vtk_vec3(f, "normal_vector", n, r, sel, [&](size_t i, int c) {
+        return c == 0 ? P3[i].nx : c == 1 ? P3[i].ny : P3[i].nz;
+      });
+      break;
+    case SOPHIA_VTK_BOUNDARY:  // save_plot_fluid_vtk_bin_boundary (:1703-1841)
+      vtk_field(f, "uy", n, r, sel, [&](size_t i) { return P1[i].uy; });
+      vtk_field(f, "ux", n, r, sel, [&](size_t i) { return P1[i].ux; });
+      vtk_field(f, "uz", n, r, sel, [&](size_t i) { return P1[i].uz; });
+      vtk_field(f, "temperature", n, r, sel, [&](size_t i) { return P1[i].temp; });
+      vtk_field(f, "pressure", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      vtk_field(f, "density", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      break;
+    case SOPHIA_VTK_FLUID_PGF:  // save_plot_fluid_vtk_bin_fluid (:202-444)
+      vtk_field(f, "porosity", n, r, sel, [&](size_t i) { return P1[i].DEMpor; });
+      vtk_field(f, "pressuregrad_x", n, r, sel, [&](size_t i) { return P1[i].pgf_x; });
+      vtk_field(f, "pressuregrad_y", n, r, sel, [&](size_t i) { return P1[i].pgf_y; });
+      vtk_field(f, "pressuregrad_z", n, r, sel, [&](size_t i) { return P1[i].pgf_z; });
+      vtk_field(f, "ux", n, r, sel, [&](size_t i) { return P1[i].ux; });
+      vtk_field(f, "pressure", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      vtk_field(f, "uy", n, r, sel, [&](size_t i) { return P1[i].uy; });
+      vtk_field(f, "dtemp", n, r, sel, [&](size_t i) { return P3[i].dtemp; });
+      vtk_int(f, "buffer_type", n, r, sel, [&](size_t i) { return P1[i].buffer_type; });
+      vtk_field(f, "density", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_field(f, "temperature", n, r, sel, [&](size_t i) { return P1[i].temp; });
+      vtk_field(f, "uz", n, r, sel, [&](size_t i) { return P1[i].uz; });
+      break;
+    case SOPHIA_VTK_ACTIVE:  // save_plot_fluid_vtk_bin (:126-201)
+      vtk_field(f, "rho", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_int(f, "ptype", n, r, sel, [&](size_t i) { return P1[i].p_type; });
+      vtk_field(f, "pres", n, r, sel, [&](size_t i) { return P1[i].pres; });
+      break;
+    case SOPHIA_VTK_MOVING_UY:  // save_plot_moving_vtk_bin (:1842-1925)
+      vtk_field(f, "rho", n, r, sel, [&](size_t i) { return P1[i].rho; });
+      vtk_field(f, "uy", n, r, sel, [&](size_t i) { return P1[i].uy; });
+      break;
+  }
+}
 
 }  // namespace
 }  // extern "C++"
 
 int sophia_write_vtk(const sophia_ctx* c, const char* path, int32_t kind) {
-  if (!c || !path || (kind != SOPHIA_VTK_FLUID && kind != SOPHIA_VTK_DEM)) return SOPHIA_E_ARG;
+  if (!c || !path || kind < 0 || kind >= kNumVtkKinds) return SOPHIA_E_ARG;
+  const VtkKind& k = kVtkKinds[kind];
   Records r;
-  TRY(fetch(c, kind == SOPHIA_VTK_FLUID ? SOPHIA_SPH : SOPHIA_DEM, r));
-  bool (*sel)(const part1&) = kind == SOPHIA_VTK_FLUID ? sel_fluid : sel_dem;
+  TRY(fetch(c, k.species, r));
   int count = 0;
-  for (const part1& q : r.p1) count += kind == SOPHIA_VTK_FLUID ? (q.p_type == 1) : (q.dem_idx > 0);
+  for (const part1& q : r.p1) count += k.cnt(q);
   FILE* f = std::fopen(path, "wb");
   if (!f) {
     set_error("cannot open %s for writing", path);
@@ -2350,42 +2500,14 @@ int sophia_write_vtk(const sophia_ctx* c, const char* path, int32_t kind) {
   std::fprintf(f, "DATASET POLYDATA\n");
   std::fprintf(f, "POINTS\t%d\tfloat\n", count);
   for (const part1& q : r.p1)
-    if (sel(q)) {
+    if (k.sel(q)) {
       put_float(f, q.x);
       put_float(f, q.y);
       put_float(f, q.z);
     }
   std::fprintf(f, "POINT_DATA\t%d\n", count);
-  const auto& P1 = r.p1;
-  const auto& P3 = r.p3;
-  if (kind == SOPHIA_VTK_FLUID) {
-    // save_plot_fluid_vtk_bin_fluid_sph (:445-659)
-    std::fprintf(f, "FIELD FieldData\t9\n");
-    vtk_field(f, "porosity", count, r, sel, [&](size_t i) { return P1[i].DEMpor; });
-    vtk_field(f, "conv_rate", count, r, sel, [&](size_t i) { return P1[i].Q_sdf; });
-    vtk_field(f, "pressure", count, r, sel, [&](size_t i) { return P1[i].pres; });
-    vtk_field(f, "cond_rate", count, r, sel, [&](size_t i) { return P1[i].Q_f; });
-    vtk_field(f, "dtemp", count, r, sel, [&](size_t i) { return P3[i].dtemp; });
-    std::fprintf(f, "buffer_type\t1\t%d\tint\n", count);
-    for (const part1& q : P1)
-      if (sel(q)) put_be32(f, (uint32_t)q.buffer_type);
-    vtk_field(f, "density", count, r, sel, [&](size_t i) { return P1[i].rho; });
-    vtk_field(f, "temperature", count, r, sel, [&](size_t i) { return P1[i].temp; });
-    vtk_field(f, "uz", count, r, sel, [&](size_t i) { return P1[i].uz; });
-  } else {
-    // save_plot_fluid_vtk_bin_solid_dem (:1135-1446)
-    std::fprintf(f, "FIELD FieldData\t10\n");
-    vtk_field(f, "itype", count, r, sel, [&](size_t i) { return (double)P1[i].i_type; });
-    vtk_field(f, "mag_uijf", count, r, sel, [&](size_t i) { return P1[i].test6; });
-    vtk_field(f, "Euler", count, r, sel, [&](size_t i) { return P1[i].eliz; });
-    vtk_field(f, "porosity", count, r, sel, [&](size_t i) { return P1[i].DEMpor; });
-    vtk_field(f, "uz", count, r, sel, [&](size_t i) { return P1[i].uz; });
-    vtk_field(f, "Q_sd", count, r, sel, [&](size_t i) { return P1[i].Q_sd; });
-    vtk_field(f, "ftotal_z", count, r, sel, [&](size_t i) { return P3[i].ftotalz; });
-    vtk_field(f, "temperature_surface", count, r, sel, [&](size_t i) { return P1[i].temp; });
-    vtk_field(f, "rad", count, r, sel, [&](size_t i) { return P1[i].rad; });
-    vtk_field(f, "dtemp_dt", count, r, sel, [&](size_t i) { return P3[i].dtemp; });
-  }
+  std::fprintf(f, "FIELD FieldData\t%d\n", k.nfields);
+  vtk_arrays(f, kind, count, r, k.sel);
   const bool ok = std::ferror(f) == 0;
   std::fclose(f);
   if (!ok) {

@@ -192,8 +192,11 @@ class Sophia:
         check(self.lib.sophia_wcsph_stage(self._ctx, stage), "sophia_wcsph_stage")
 
     def write_vtk(self, path, kind: int = 0) -> None:
-        """Legacy binary VTK of the fluid (kind 0) or the spheres (kind 1),
-        in the reference writers' layout (functions_OUTPUT.cuh)."""
+        """Legacy binary VTK in the layout of one of the reference's writers
+        (functions_OUTPUT.cuh): abi.VTK_FLUID (0) and abi.VTK_DEM (1) are the
+        two its SPH-DEM drivers call; abi.VTK_AIR ... abi.VTK_MOVING_UY the
+        others (air, solid, moving, fluid2, boundary, pressure-gradient fluid,
+        i_type 1|2 and the second moving writer)."""
         check(self.lib.sophia_write_vtk(self._ctx, str(path).encode(), kind), "sophia_write_vtk")
 
     def write_restart(self, path) -> None:

@@ -398,14 +398,42 @@ int sophia_monitors(sophia_ctx* ctx, int32_t species, double out[3]);
 int sophia_count(const sophia_ctx* ctx, int32_t species, int32_t* n);
 /* Output files of the current state, in sorted order as the reference writes
  * its arrays.  Legacy VTK 3.0 binary POLYDATA, big-endian 4-byte words
- * (FloatSwap / IntSwap, functions_OUTPUT.cuh:68-121):
- *   SOPHIA_VTK_FLUID  save_plot_fluid_vtk_bin_fluid_sph (:445-659): fluid
- *                     particles (p_type 1), points and 9 fields;
- *   SOPHIA_VTK_DEM    save_plot_fluid_vtk_bin_solid_dem (:1135-1446): spheres
- *                     (p_type > 1000; the point count is the spheres with
- *                     dem_idx > 0, as the reference counts), 10 fields. */
+ * (FloatSwap / IntSwap, functions_OUTPUT.cuh:68-121), one kind per writer of
+ * that file (species read, particles written, arrays):
+ *   SOPHIA_VTK_FLUID      save_plot_fluid_vtk_bin_fluid_sph (:445-659): SPH,
+ *                         p_type 1, 9 arrays;
+ *   SOPHIA_VTK_DEM        save_plot_fluid_vtk_bin_solid_dem (:1135-1446):
+ *                         spheres (p_type > 1000; the point count is the
+ *                         spheres with dem_idx > 0, as the reference counts),
+ *                         10 arrays;
+ *   SOPHIA_VTK_AIR        save_plot_fluid_vtk_bin_air (:660-822): SPH,
+ *                         p_type 3, 10 arrays;
+ *   SOPHIA_VTK_SOLID      save_plot_fluid_vtk_bin_solid (:824-1134): spheres,
+ *                         counted as SOPHIA_VTK_DEM, 10 arrays (dem_idx and
+ *                         drag_r where the other has itype and Euler);
+ *   SOPHIA_VTK_MOVING     save_plot_fluid_vtk_bin_moving (:1447-1578): SPH,
+ *                         p_type 9 (MOVING), density and pressure;
+ *   SOPHIA_VTK_FLUID2     save_plot_fluid_vtk_bin_fluid2 (:1579-1702): SPH,
+ *                         p_type 2, 7 arrays, three of them 3-component;
+ *   SOPHIA_VTK_BOUNDARY   save_plot_fluid_vtk_bin_boundary (:1703-1841): SPH,
+ *                         p_type 0, 6 arrays;
+ *   SOPHIA_VTK_FLUID_PGF  save_plot_fluid_vtk_bin_fluid (:202-444): SPH,
+ *                         p_type 1, 12 arrays with the pressure gradient;
+ *   SOPHIA_VTK_ACTIVE     save_plot_fluid_vtk_bin (:126-201): SPH, i_type 1
+ *                         or 2, rho, ptype (int) and pres;
+ *   SOPHIA_VTK_MOVING_UY  save_plot_moving_vtk_bin (:1842-1925): SPH,
+ *                         p_type 9, rho and uy.
+ * Unknown kinds return SOPHIA_E_ARG. */
 #define SOPHIA_VTK_FLUID 0
 #define SOPHIA_VTK_DEM 1
+#define SOPHIA_VTK_AIR 2
+#define SOPHIA_VTK_SOLID 3
+#define SOPHIA_VTK_MOVING 4
+#define SOPHIA_VTK_FLUID2 5
+#define SOPHIA_VTK_BOUNDARY 6
+#define SOPHIA_VTK_FLUID_PGF 7
+#define SOPHIA_VTK_ACTIVE 8
+#define SOPHIA_VTK_MOVING_UY 9
 int sophia_write_vtk(const sophia_ctx* ctx, const char* path, int32_t kind);
 /* restart.txt of save_restart (functions_OUTPUT.cuh:1926-1955): a header of
  * input column codes "1 2 3 4 5 6 7 8 9 10 11 12 13 26" and one line per
