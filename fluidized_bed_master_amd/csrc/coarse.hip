@@ -245,17 +245,51 @@ __global__ __launch_bounds__(256) void k_res_partials(const float* __restrict__ 
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// c[I] = sum of v over the rows of aggregate I (v is zero off the rows).
-// One wave per aggregate: lanes 0..8 fetch the particle ranges of the
-// aggregate's kAgg^2 cell rows, every lane then walks the concatenated ranges
-// with independent loads; the wave sums in a fixed order.
-template <typename V>
-__global__ __launch_bounds__(256) void k_restrict(PpeOp o, AggGrid g, int na, const V* __restrict__ v,
-                                                  double* __restrict__ c, const int* done, const int* brk) {
+// The BiCGStab vector update that precedes each coarse application, done in
+// the restriction's pass over the particles (the update is elementwise, the
+// restriction visits every particle of the grid once):
+//   KIND 1, direction: p = r + beta (p - omega v)  (p = r on the first
+//           iteration), out = p / D;
+//   KIND 2, s:         r = r - alpha v,                     out = r / D;
+// each in FP64 and stored as V, and c[I] = the sum over aggregate I of the
+// updated vector as stored (the vectors are zero off the rows).
+// One 256-thread block per aggregate: every wave fetches the particle ranges
+// of the aggregate's kAgg^2 cell rows (lanes 0..8), the block walks the
+// concatenated ranges with independent loads and sums in a fixed order (lane
+// butterflies, then waves 0..3).  Block na updates the particles past the last
+// cell, which belong to no aggregate.
+template <typename V, int KIND>
+__device__ inline double update_one(const CoarseUpd<V>& u, int i, bool first, double alpha, double beta,
+                                    double omega) {
+  double val;
+  V* dst;
+  if constexpr (KIND == 1) {
+    val = first ? (double)u.r[i] : (double)u.r[i] + beta * ((double)u.p[i] - omega * (double)u.v[i]);
+    dst = u.p;
+  } else {
+    val = (double)u.r[i] - alpha * (double)u.v[i];
+    dst = u.r;
+  }
+  const V st = (V)val;
+  dst[i] = st;
+  u.out[i] = (V)(val * u.dinv[i]);
+  return (double)st;
+}
+
+template <typename V, int KIND>
+__global__ __launch_bounds__(256) void k_update_restrict(PpeOp o, AggGrid g, int na, CoarseUpd<V> u,
+                                                         double* __restrict__ c, const int* done, const int* brk) {
   if (*done || *brk) return;
-  const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (I >= na) return;
+  const bool first = KIND == 1 && *u.first != 0;
+  const double alpha = KIND == 2 ? *u.alpha : 0.0;
+  const double beta = KIND == 1 ? *u.beta : 0.0, omega = KIND == 1 ? *u.omega : 0.0;
+  const int I = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
+  if (I == na) {
+    for (int i = (int)o.cell_off[g.NI * g.NJ * g.NK] + t; i < o.n; i += 256)
+      update_one<V, KIND>(u, i, first, alpha, beta, omega);
+    return;
+  }
   const int ax = I % g.AX, ay = (I / g.AX) % g.AY, az = I / (g.AX * g.AY);
   const int x0 = ax * kAgg, x1 = min(x0 + kAgg, g.NI);
   int p0 = 0, len = 0;
@@ -267,24 +301,27 @@ __global__ __launch_bounds__(256) void k_restrict(PpeOp o, AggGrid g, int na, co
       len = (int)o.cell_off[kr + x1] - p0;
     }
   }
-  int start[kAgg * kAgg], first[kAgg * kAgg];
+  int start[kAgg * kAgg], first_i[kAgg * kAgg];
   int tot = 0;
 #pragma unroll
   for (int r = 0; r < kAgg * kAgg; r++) {
     start[r] = tot;
-    first[r] = __shfl(p0, r);
+    first_i[r] = __shfl(p0, r);
     tot += __shfl(len, r);
   }
   double s = 0;
-#pragma unroll 4
-  for (int e = lane; e < tot; e += 64) {
-    int i = first[0] + e;
+#pragma unroll 2
+  for (int e = t; e < tot; e += 256) {
+    int i = first_i[0] + e;
 #pragma unroll
-    for (int r = 1; r < kAgg * kAgg; r++) i = e >= start[r] ? first[r] + (e - start[r]) : i;
-    s += (double)v[i];
+    for (int r = 1; r < kAgg * kAgg; r++) i = e >= start[r] ? first_i[r] + (e - start[r]) : i;
+    s += update_one<V, KIND>(u, i, first, alpha, beta, omega);
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) c[I] = s;
+  __shared__ double red[4];
+  if (lane == 0) red[t >> 6] = s;
+  __syncthreads();
+  if (t == 0) c[I] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // y = X c: one wave per coarse row, 16 B loads, four in flight per lane
@@ -512,14 +549,15 @@ int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, T
 }
 
 template <typename V>
-int coarse_apply(hipStream_t s, const CoarseWork& c, const PpeWork& w, const V* v, V* out, const int* done,
-                 const int* brk) {
+int coarse_apply_update(hipStream_t s, const CoarseWork& c, const PpeWork& w, const CoarseUpd<V>& u, int kind,
+                        const int* done, const int* brk) {
   const PpeOp o = ppe_op(w);
   const AggGrid g = agg_grid(c);
+  if (kind == 1) k_update_restrict<V, 1><<<c.na + 1, 256, 0, s>>>(o, g, c.na, u, c.c, done, brk);
+  else k_update_restrict<V, 2><<<c.na + 1, 256, 0, s>>>(o, g, c.na, u, c.c, done, brk);
   const int gb = (c.na + 3) / 4;
-  k_restrict<V><<<gb, 256, 0, s>>>(o, g, c.na, v, c.c, done, brk);
   k_coarse_gemv<<<gb, 256, 0, s>>>(c.X, c.na, c.ld, c.c, c.y, done, brk);
-  k_prolong_add<V><<<grid_for(w.n), kBlock, 0, s>>>(o, g, c.y, out, done, brk);
+  k_prolong_add<V><<<grid_for(w.n), kBlock, 0, s>>>(o, g, c.y, u.out, done, brk);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -548,9 +586,9 @@ void coarse_free(CoarseWork& c) {
   c = CoarseWork{};
 }
 
-template int coarse_apply<float>(hipStream_t, const CoarseWork&, const PpeWork&, const float*, float*, const int*,
-                                 const int*);
-template int coarse_apply<double>(hipStream_t, const CoarseWork&, const PpeWork&, const double*, double*, const int*,
-                                  const int*);
+template int coarse_apply_update<float>(hipStream_t, const CoarseWork&, const PpeWork&, const CoarseUpd<float>&,
+                                        int, const int*, const int*);
+template int coarse_apply_update<double>(hipStream_t, const CoarseWork&, const PpeWork&, const CoarseUpd<double>&,
+                                         int, const int*, const int*);
 
 }  // namespace sophia

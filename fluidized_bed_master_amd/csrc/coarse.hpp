@@ -61,10 +61,26 @@ int coarse_settle(hipStream_t s, CoarseWork& c);
 // ||I - E X|| <= kCoarseTol.  Returns 0 with *usable = false when X cannot be
 // brought there (the solve then runs with Jacobi alone).
 int coarse_setup(hipStream_t s, CoarseWork& c, const PpeWork& w, bool* usable, Timers* tm);
-// out += Z X Z^T v over rows (skipped when *done or *brk)
+// The Krylov vector update that precedes a coarse application (kind 1: the
+// BiCGStab direction p = r + beta (p - omega v), or p = r when *first; kind 2:
+// s = r - alpha v, in place in r), with out = D^-1 of the updated vector; the
+// scalars are read on the device.
 template <typename V>
-int coarse_apply(hipStream_t s, const CoarseWork& c, const PpeWork& w, const V* v, V* out, const int* done,
-                 const int* brk);
+struct CoarseUpd {
+  const double* dinv;
+  V* r;
+  V* p;
+  const V* v;
+  V* out;
+  const double *alpha, *beta, *omega;
+  const int* first;
+};
+// The update of kind `kind`, then out += Z X Z^T of the updated vector over
+// rows; the update and the restriction Z^T are one pass (skipped when *done or
+// *brk).
+template <typename V>
+int coarse_apply_update(hipStream_t s, const CoarseWork& c, const PpeWork& w, const CoarseUpd<V>& u, int kind,
+                        const int* done, const int* brk);
 void coarse_free(CoarseWork& c);
 // copy E and X (row-major na x na, FP32) to the host where non-null (synchronises)
 int coarse_export(hipStream_t s, const CoarseWork& c, float* E, float* X);

@@ -1356,16 +1356,24 @@ int bicgstab_run(hipStream_t s, PpeWork& w, const CoarseWork* cw, double rn, dou
   double rr0 = 1.0;
   for (;;) {
     for (int c = 0; c < chunk; c++) {
-      k_bi_dir<V><<<G, kRedThreads, 0, s>>>(n, w.dinv, r, p, v, ph, st);
-      if (cw) TRY(coarse_apply<V>(s, *cw, w, p, ph, &st->done, &st->brk));
+      if (cw) {
+        const CoarseUpd<V> u{w.dinv, r, p, v, ph, &st->alpha, &st->beta, &st->omega, &st->first};
+        TRY(coarse_apply_update<V>(s, *cw, w, u, 1, &st->done, &st->brk));
+      } else {
+        k_bi_dir<V><<<G, kRedThreads, 0, s>>>(n, w.dinv, r, p, v, ph, st);
+      }
       {
         TimeScope ts(mv_timer(), mv_name);
         if (FMT == kFmtRun8) k_bi_mv_staged<CM, V><<<G, kRedThreads, 0, s>>>(o, ph, v, rh, w.partials, st);
         else k_bi_mv<FMT, CM, V><<<G, kRedThreads, 0, s>>>(o, ph, v, rh, w.partials, st);
       }
       launch_fin(s, w, kFinAlpha);
-      k_bi_s<V><<<G, kRedThreads, 0, s>>>(n, w.dinv, r, v, sh, st);
-      if (cw) TRY(coarse_apply<V>(s, *cw, w, r, sh, &st->done, &st->brk));
+      if (cw) {
+        const CoarseUpd<V> u{w.dinv, r, p, v, sh, &st->alpha, &st->beta, &st->omega, &st->first};
+        TRY(coarse_apply_update<V>(s, *cw, w, u, 2, &st->done, &st->brk));
+      } else {
+        k_bi_s<V><<<G, kRedThreads, 0, s>>>(n, w.dinv, r, v, sh, st);
+      }
       {
         TimeScope ts(mv_timer(), mv_name);
         if (FMT == kFmtRun8) k_bi_mv_staged<CM, V><<<G, kRedThreads, 0, s>>>(o, sh, t, r, w.partials, st);
